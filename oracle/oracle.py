@@ -1,0 +1,198 @@
+"""ctypes wrapper of the oracle library (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Fields are
+numpy float64 arrays of shape (6, ld, ld) = (panel, j, i) with halo width hw,
+i.e. reference indices i,j in [1-hw, n+hw] map to array index [.., j-1+hw, i-1+hw].
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            from oracle import build as _b  # noqa: F401  (local import: build on demand)
+            _b.build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.ora_create.restype = ctypes.c_void_p
+        L.ora_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.ora_destroy.argtypes = [ctypes.c_void_p]
+        for f in ("ora_n", "ora_hw", "ora_ld"):
+            getattr(L, f).restype = ctypes.c_int
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.ora_field_len.restype = ctypes.c_long
+        L.ora_field_len.argtypes = [ctypes.c_void_p]
+        L.ora_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.ora_sync_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.ora_quad_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.ora_halo_map.argtypes = [ctypes.c_void_p, _ip, _ip]
+        L.ora_exchange.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        L.ora_exchange_vec.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.ora_sbp_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char, _dp, _dp]
+        L.ora_div.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.ora_grad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.ora_curl.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.ora_scalar_sync.argtypes = [ctypes.c_void_p, _dp]
+        L.ora_vec_sync_covariant.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ora_rhs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ora_rk4.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int]
+        L.ora_diagnostics.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.ora_quad_mass.restype = ctypes.c_double
+        L.ora_quad_mass.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.ora_l2norm.restype = ctypes.c_double
+        L.ora_l2norm.argtypes = [ctypes.c_void_p, _dp]
+        L.ora_maxabs.restype = ctypes.c_double
+        L.ora_maxabs.argtypes = [ctypes.c_void_p, _dp]
+        L.ora_project_vector.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.ora_rk4_iomega.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _pn(a):
+    return None if a is None else _p(a)
+
+
+GEO = {"J": 0, "Qi11": 1, "Qi12": 2, "Qi22": 3, "f": 4, "r": 5, "a1": 8, "a2": 11,
+       "b1": 14, "b2": 17, "Q11": 20, "Q12": 21, "Q22": 22}
+
+
+class Grid:
+    """One equiangular cubed sphere with Ah (corner-point) storage."""
+
+    def __init__(self, N, grad_order=42, div_kind=42, scale=1.0, omega=1.0, grav=1.0):
+        self.N = N
+        self.grad_order = grad_order
+        self.div_kind = div_kind
+        self.h = lib().ora_create(N, grad_order, div_kind, scale, omega, grav)
+        if not self.h:
+            raise RuntimeError("ora_create failed")
+        self.n = lib().ora_n(self.h)
+        self.hw = lib().ora_hw(self.h)
+        self.ld = lib().ora_ld(self.h)
+        self.shape = (6, self.ld, self.ld)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_destroy(self.h)
+            self.h = None
+
+    def zeros(self):
+        return np.zeros(self.shape)
+
+    def geometry(self, name):
+        w = GEO[name]
+        if name in ("r", "a1", "a2", "b1", "b2"):
+            out = [self.zeros() for _ in range(3)]
+            for c in range(3):
+                lib().ora_geometry(self.h, w + c, _p(out[c]))
+            return out
+        out = self.zeros()
+        lib().ora_geometry(self.h, w, _p(out))
+        return out
+
+    def sync_weights(self):
+        out = {}
+        for k, name in enumerate(("qb", "qt", "ql", "qr")):
+            a = np.zeros(self.n)
+            lib().ora_sync_weights(self.h, k, _p(a))
+            out[name] = a
+        return out
+
+    def quad_weights(self, order):
+        a = np.zeros(self.n)
+        lib().ora_quad_weights(self.h, order, _p(a))
+        return a
+
+    def halo_map(self):
+        src = np.zeros(self.shape, dtype=np.int32)
+        vec = np.zeros(self.shape, dtype=np.int32)
+        lib().ora_halo_map(self.h, src.ctypes.data_as(_ip), vec.ctypes.data_as(_ip))
+        return src, vec
+
+    def interior(self, f):
+        """(6, n, n) view of the mesh points."""
+        hw, n = self.hw, self.n
+        return f[:, hw:hw + n, hw:hw + n]
+
+    def embed(self, a):
+        """(6, n, n) mesh values -> stored field with zero halo."""
+        f = self.zeros()
+        self.interior(f)[...] = a
+        return f
+
+    # ---- operators ----
+    def sbp(self, order, direction, f):
+        out = self.zeros()
+        lib().ora_sbp_apply(self.h, order, direction.encode(), _p(f), _p(out))
+        return out
+
+    def div(self, u, v):
+        out = self.zeros()
+        lib().ora_div(self.h, _p(u), _p(v), _p(out))
+        return out
+
+    def grad(self, f):
+        gx, gy = self.zeros(), self.zeros()
+        lib().ora_grad(self.h, _p(f), _p(gx), _p(gy))
+        return gx, gy
+
+    def curl(self, u, v):
+        out = self.zeros()
+        lib().ora_curl(self.h, _p(u), _p(v), _p(out))
+        return out
+
+    def rhs(self, h, u, v, hs=None):
+        dh, du, dv = self.zeros(), self.zeros(), self.zeros()
+        lib().ora_rhs(self.h, _p(h), _p(u), _p(v), _pn(hs), _p(dh), _p(du), _p(dv))
+        return dh, du, dv
+
+    def rk4(self, h, u, v, dt, nsteps, hs=None):
+        lib().ora_rk4(self.h, _p(h), _p(u), _p(v), _pn(hs), dt, nsteps)
+
+    def diagnostics(self, quad_order, h, u, v, hs=None):
+        out = np.zeros(7)
+        lib().ora_diagnostics(self.h, quad_order, _p(h), _p(u), _p(v), _pn(hs), _p(out))
+        return out
+
+    def quad_mass(self, order, f):
+        return lib().ora_quad_mass(self.h, order, _p(f))
+
+    def l2norm(self, f):
+        return lib().ora_l2norm(self.h, _p(f))
+
+    def maxabs(self, f):
+        return lib().ora_maxabs(self.h, _p(f))
+
+    def project(self, kind, vx, vy, vz):
+        """kind 'covariant' or 'contravariant'; returns (u, v) over the stored extent."""
+        u, v = self.zeros(), self.zeros()
+        lib().ora_project_vector(self.h, 0 if kind == "covariant" else 1,
+                                 _p(np.ascontiguousarray(vx)), _p(np.ascontiguousarray(vy)),
+                                 _p(np.ascontiguousarray(vz)), _p(u), _p(v))
+        return u, v
+
+
+def rk4_iomega(w, f, dt):
+    w = np.asarray(w, dtype=np.complex128)
+    f = np.asarray(f, dtype=np.complex128)
+    wr, wi = np.ascontiguousarray(w.real), np.ascontiguousarray(w.imag)
+    fr, fi = np.ascontiguousarray(f.real), np.ascontiguousarray(f.imag)
+    lib().ora_rk4_iomega(len(w), _p(wr), _p(wi), _p(fr), _p(fi), dt)
+    return fr + 1j * fi

@@ -1,0 +1,961 @@
+/*
+ * swm_oracle.c -- CPU FP64 restatement of the reference shallow-water hot path
+ * (vector-invariant Ah-grid SWM with SBP operators, cube-edge syncs, RK4).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the
+ * timed CPU baseline.  Nothing in cubedsphere_sbp_swm_amd links or calls it.
+ *
+ * Compiled with -ffp-contract=off so that every a*b+c is rounded twice, as in
+ * the reference's debug build (-fp-model strict).  Fortran expressions are
+ * evaluated left to right as written; SUM() over an array section is taken as
+ * a left-to-right accumulation starting from the first product.
+ *
+ * Citations are to files under the reference's src/ directory.
+ */
+#include "swm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <complex.h>
+
+#define HW 3 /* stored halo width: the Ah path reads at most depth 2 */
+
+/* ---- cube topology tables: topology/cubed_sphere_topology_mod.f90:29-55 ---- */
+static const int EX[6][3] = {{-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 0}};
+static const int EY[6][3] = {{0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, -1}};
+static const int NV[6][3] = {{0, 0, 1}, {-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+static const int RV[6][3] = {{1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 1, 1}};
+
+/* ---- SBP tables: differential_operators/sbp_operators/sbp_operators_collection_mod.f90 ---- */
+typedef struct {
+    int nw, ne;          /* rows of weights, number of edge rows */
+    double Ql[9 * 6];    /* Ql[(m-1) + nw*(i-1)] = W_edge_l(m,i) */
+    int last[6];         /* edge_last_l */
+    double Qr[9 * 6];    /* W_edge_r, built as in sbp_factory_mod.f90:162-165 */
+    int rshift[6];       /* edge_first_shift_r */
+    int win;             /* inner stencil width */
+    double Qin[7];
+    int inshift;
+} sbp_op;
+
+/* sbp_operators_collection_mod.f90:15-20 (Q21, Da2_in) */
+static void sbp_d21(sbp_op *o) {
+    o->nw = 2; o->ne = 1;
+    o->Ql[0] = -1.0; o->Ql[1] = 1.0;
+    o->last[0] = 2;
+    o->win = 3; o->Qin[0] = -0.5; o->Qin[1] = 0.0; o->Qin[2] = 0.5; o->inshift = -1;
+}
+/* sbp_operators_collection_mod.f90:23-30,84-85 (Q42, Da4_in) */
+static void sbp_d42(sbp_op *o) {
+    static const double q[24] = {
+        -1.4117647058823528, 1.7352941176470589, -0.2352941176470588, -0.08823529411764705, 0.0, 0.0,
+        -0.5, 0.0, 0.5, 0.0, 0.0, 0.0,
+        0.09302325581395347, -0.686046511627907, 0.0, 0.686046511627907, -0.09302325581395347, 0.0,
+        0.030612244897959186, -0.0, -0.6020408163265307, 0.0, 0.653061224489796, -0.0816326530612245};
+    o->nw = 6; o->ne = 4;
+    memcpy(o->Ql, q, sizeof q);
+    o->last[0] = 4; o->last[1] = 3; o->last[2] = 5; o->last[3] = 6;
+    o->win = 5;
+    o->Qin[0] = 1.0 / 12.0; o->Qin[1] = -2.0 / 3.0; o->Qin[2] = 0.0;
+    o->Qin[3] = 2.0 / 3.0; o->Qin[4] = -1.0 / 12.0;
+    o->inshift = -2;
+}
+/* sbp_operators_collection_mod.f90:73-81 (Q43) with Da4_in */
+static void sbp_d43(sbp_op *o) {
+    static const double q[24] = {
+        -1.8280236842718398, 2.978054584697369, -1.4653148294044123, 0.3078538227474199, 0.008136925288119662, -0.0007068190566565205,
+        -0.37814159126691327, -0.31480210500706657, 0.707290999364066, 0.04835554461933453, -0.06866771096803422, 0.005964863258613564,
+        0.1162478575921292, -0.8028500807354999, 0.21558841368737428, 0.5078566674295846, -0.03231754093993847, -0.004525317033649745,
+        -0.010287398573128293, 0.12592344801071614, -0.7341531396449149, 0.04979271660173103, 0.6506171865540596, -0.08189281294846368};
+    o->nw = 6; o->ne = 4;
+    memcpy(o->Ql, q, sizeof q);
+    for (int i = 0; i < 4; i++) o->last[i] = 6;
+    o->win = 5;
+    o->Qin[0] = 1.0 / 12.0; o->Qin[1] = -2.0 / 3.0; o->Qin[2] = 0.0;
+    o->Qin[3] = 2.0 / 3.0; o->Qin[4] = -1.0 / 12.0;
+    o->inshift = -2;
+}
+/* sbp_operators_collection_mod.f90:33-67 (Q63_A, q-constants, Q63, Da6_in) */
+static void sbp_d63(sbp_op *o) {
+    const double A1 = 0.13649e5 / 0.43200e5, A2 = 0.12013e5 / 0.8640e4, A3 = 0.2711e4 / 0.4320e4,
+                 A4 = 0.5359e4 / 0.4320e4, A5 = 0.7877e4 / 0.8640e4, A6 = 0.43801e5 / 0.43200e5;
+    const double q56 = 5591070156686698065364559.0 / 7931626489314500743872000.0;
+    const double q11 = -1.0 / 2.0;
+    const double q12 = -0.953e3 / 0.16200e5 + q56;
+    const double q13 = 0.715489e6 / 0.259200e6 - (4.0 * q56);
+    const double q14 = -0.62639e5 / 0.14400e5 + (6.0 * q56);
+    const double q15 = 0.147127e6 / 0.51840e5 - (4.0 * q56);
+    const double q16 = -0.89387e5 / 0.129600e6 + q56;
+    const double q23 = -0.57139e5 / 0.8640e4 + (10.0 * q56);
+    const double q24 = 0.745733e6 / 0.51840e5 - (20.0 * q56);
+    const double q25 = -0.18343e5 / 0.1728e4 + (15.0 * q56);
+    const double q26 = 0.240569e6 / 0.86400e5 - (4.0 * q56);
+    const double q34 = -0.176839e6 / 0.12960e5 + (20.0 * q56);
+    const double q35 = 0.242111e6 / 0.17280e5 - (20.0 * q56);
+    const double q36 = -0.182261e6 / 0.43200e5 + (6.0 * q56);
+    const double q45 = -0.165041e6 / 0.25920e5 + (10.0 * q56);
+    const double q46 = 0.710473e6 / 0.259200e6 - (4.0 * q56);
+    const double q47 = 1.0 / 6.0e1, q57 = -3.0 / 2.0e1, q58 = 1.0 / 6.0e1;
+    const double q67 = 3.0 / 4.0, q68 = -3.0 / 2.0e1, q69 = 1.0 / 6.0e1;
+    const double q[54] = {
+        q11 / A1, q12 / A1, q13 / A1, q14 / A1, q15 / A1, q16 / A1, 0.0, 0.0, 0.0,
+        -q12 / A2, 0.0, q23 / A2, q24 / A2, q25 / A2, q26 / A2, 0.0, 0.0, 0.0,
+        -q13 / A3, -q23 / A3, 0.0, q34 / A3, q35 / A3, q36 / A3, 0.0, 0.0, 0.0,
+        -q14 / A4, -q24 / A4, -q34 / A4, 0.0, q45 / A4, q46 / A4, q47 / A4, 0.0, 0.0,
+        -q15 / A5, -q25 / A5, -q35 / A5, -q45 / A5, 0.0, q56 / A5, q57 / A5, q58 / A5, 0.0,
+        -q16 / A6, -q26 / A6, -q36 / A6, -q46 / A6, -q56 / A6, 0.0, q67 / A6, q68 / A6, q69 / A6};
+    o->nw = 9; o->ne = 6;
+    memcpy(o->Ql, q, sizeof q);
+    const int last[6] = {6, 6, 6, 7, 8, 9};
+    memcpy(o->last, last, sizeof last);
+    o->win = 7;
+    o->Qin[0] = -1.0 / 6.0e1; o->Qin[1] = 3.0 / 2.0e1; o->Qin[2] = -3.0 / 4.0; o->Qin[3] = 0.0;
+    o->Qin[4] = 3.0 / 4.0; o->Qin[5] = -3.0 / 2.0e1; o->Qin[6] = 1.0 / 6.0e1;
+    o->inshift = -3;
+}
+
+/* create_sbp_operator: differential_operators/sbp_operators/sbp_factory_mod.f90:52-75,162-165 */
+static void sbp_make(sbp_op *o, int order) {
+    memset(o, 0, sizeof *o);
+    switch (order) {
+    case 21: sbp_d21(o); break;
+    case 42: sbp_d42(o); break;
+    case 43: sbp_d43(o); break;
+    case 63: sbp_d63(o); break;
+    default: abort();
+    }
+    /* W_edge_r = -W_edge_l(nw:1:-1, ne:1:-1);  edge_first_shift_r = -edge_last_l(ne:1:-1)+1 */
+    for (int ir = 1; ir <= o->ne; ir++) {
+        for (int m = 1; m <= o->nw; m++)
+            o->Qr[(m - 1) + o->nw * (ir - 1)] = -1.0 * o->Ql[(o->nw - m) + o->nw * (o->ne - ir)];
+        o->rshift[ir - 1] = -o->last[o->ne - ir] + 1;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+struct ora_grid {
+    int N, n, hw, ld;
+    long flen;
+    double hx, scale, omega, grav, axis[3];
+    int grad_order, div_kind;
+    sbp_op grad_sbp, div_sbp;
+    double *geo[23];                 /* see ora_geometry */
+    double *qb, *qt, *ql, *qr;       /* covariant sync weights, 1..n */
+    int *hsrc;                       /* halo map, width 2 */
+    int *hvec;
+    double *work[16];
+};
+
+#define IDX(g, p, i, j) (((long)(p) * (g)->ld + ((j) - 1 + (g)->hw)) * (g)->ld + ((i) - 1 + (g)->hw))
+
+int ora_n(const ora_grid *g) { return g->n; }
+int ora_hw(const ora_grid *g) { return g->hw; }
+int ora_ld(const ora_grid *g) { return g->ld; }
+long ora_field_len(const ora_grid *g) { return g->flen; }
+
+/* ---- ECS metric: equiang_cs/ecs_metric_mod.f90 ---- */
+static double pi_(void) { return acos(-1.0); } /* const_mod.f90:3 */
+
+/* ecs_point_r_proto, ecs_metric_mod.f90:326-337 */
+static void ecs_r_proto(double a, double b, double r[3]) {
+    double ta = tan(a), tb = tan(b);
+    double d = sqrt(1.0 + ta * ta + tb * tb);
+    r[0] = ta / d; r[1] = tb / d; r[2] = 1.0 / d;
+}
+/* ecs_a1_proto, ecs_metric_mod.f90:339-351 (real exponents -> pow) */
+static void ecs_a1_proto(double a, double b, double v[3]) {
+    double ta = tan(a), tb = tan(b);
+    double s = pow(1.0 + pow(ta, 2.0) + pow(tb, 2.0), 3.0 / 2.0);
+    v[0] = (1.0 + pow(ta, 2.0)) * (1.0 + pow(tb, 2.0)) / s;
+    v[1] = -ta * tb * (1.0 + pow(ta, 2.0)) / s;
+    v[2] = -ta * (1.0 + pow(ta, 2.0)) / s;
+}
+/* ecs_a2_proto, ecs_metric_mod.f90:353-365 */
+static void ecs_a2_proto(double a, double b, double v[3]) {
+    double ta = tan(a), tb = tan(b);
+    double s = pow(1.0 + pow(ta, 2.0) + pow(tb, 2.0), 3.0 / 2.0);
+    v[0] = -ta * tb * (1.0 + pow(tb, 2.0)) / s;
+    v[1] = (1.0 + pow(ta, 2.0)) * (1.0 + pow(tb, 2.0)) / s;
+    v[2] = -tb * (1.0 + pow(tb, 2.0)) / s;
+}
+/* ecs_b1_proto, ecs_metric_mod.f90:367-380 */
+static void ecs_b1_proto(double a, double b, double v[3]) {
+    double ta = tan(a), tb = tan(b);
+    double sg = sqrt(1.0 + ta * ta + tb * tb);
+    v[0] = (1.0 + tb * tb - (ta * tb) * (ta * tb) / (1.0 + ta * ta)) / sg;
+    v[1] = 0.0;
+    v[2] = -ta * (1.0 + tb * tb / (1 + ta * ta)) / sg;
+}
+/* ecs_b2_proto, ecs_metric_mod.f90:382-395 */
+static void ecs_b2_proto(double a, double b, double v[3]) {
+    double ta = tan(a), tb = tan(b);
+    double sg = sqrt(1.0 + ta * ta + tb * tb);
+    v[0] = 0.0;
+    v[1] = ((1 + ta * ta) - (ta * tb) * (ta * tb) / (1 + tb * tb)) / sg;
+    v[2] = -tb * (ta * ta / (1 + tb * tb) + 1.0) / sg;
+}
+/* ecs_proto2realface with the identity rotation matrix, ecs_metric_mod.f90:303-324 */
+static void proto2real(int p, const double r[3], double out[3]) {
+    double rt[3];
+    for (int k = 0; k < 3; k++)
+        rt[k] = (double)EX[p][k] * r[0] + (double)EY[p][k] * r[1] - (double)NV[p][k] * r[2];
+    for (int c = 0; c < 3; c++) {
+        /* sum(rotation_matrix(1:3,c)*rtemp(1:3)) with rotation_matrix = I */
+        double s = (c == 0 ? 1.0 : 0.0) * rt[0];
+        s = s + (c == 1 ? 1.0 : 0.0) * rt[1];
+        s = s + (c == 2 ? 1.0 : 0.0) * rt[2];
+        out[c] = s;
+    }
+}
+
+static double *alloc_field(const ora_grid *g) { return (double *)calloc((size_t)g->flen, sizeof(double)); }
+
+/* mesh geometry: mesh_factory_mod.f90:34-47,108-134, mesh_mod.f90:83-92,
+ * ecs_metric_mod.f90:397-478, coriolis_factory_mod.f90:151-172 */
+static void build_geometry(ora_grid *g) {
+    const double pi = pi_();
+    const double alpha0 = -0.25 * pi;
+    for (int w = 0; w < 23; w++) g->geo[w] = alloc_field(g);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1 - g->hw; j <= g->n + g->hw; j++) {
+            double beta = alpha0 + ((double)(j - 1) + 0.0) * g->hx;
+            for (int i = 1 - g->hw; i <= g->n + g->hw; i++) {
+                double alpha = alpha0 + ((double)(i - 1) + 0.0) * g->hx;
+                long id = IDX(g, p, i, j);
+                double ta = tan(alpha), tb = tan(beta);
+                double sg = sqrt(1.0 + ta * ta + tb * tb);
+                /* ecs_J */
+                g->geo[0][id] = (1.0 + ta * ta) * (1.0 + tb * tb) / (sg * sg * sg);
+                /* ecs_QI */
+                g->geo[1][id] = sg * sg / (1.0 + ta * ta);
+                g->geo[2][id] = sg * sg * ta * tb / ((1 + ta * ta) * (1 + tb * tb));
+                g->geo[3][id] = sg * sg / (1.0 + tb * tb);
+                double r0[3], r[3], v[3], w3[3];
+                ecs_r_proto(alpha, beta, r0);
+                proto2real(p, r0, r);
+                for (int k = 0; k < 3; k++) g->geo[5 + k][id] = r[k];
+                /* calc_coriolis_parameter: f = 2*omega*(axis . n)/|n| */
+                double dn = g->axis[0] * r[0] + g->axis[1] * r[1] + g->axis[2] * r[2];
+                double pr = dn / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                g->geo[4][id] = 2 * g->omega * pr;
+                ecs_a1_proto(alpha, beta, v); proto2real(p, v, w3);
+                for (int k = 0; k < 3; k++) g->geo[8 + k][id] = w3[k];
+                ecs_a2_proto(alpha, beta, v); proto2real(p, v, w3);
+                for (int k = 0; k < 3; k++) g->geo[11 + k][id] = w3[k];
+                ecs_b1_proto(alpha, beta, v); proto2real(p, v, w3);
+                for (int k = 0; k < 3; k++) g->geo[14 + k][id] = w3[k];
+                ecs_b2_proto(alpha, beta, v); proto2real(p, v, w3);
+                for (int k = 0; k < 3; k++) g->geo[17 + k][id] = w3[k];
+                /* ecs_Q, ecs_metric_mod.f90:397-413 (sigm**4 = (s*s)*(s*s)) */
+                double s4 = (sg * sg) * (sg * sg);
+                g->geo[20][id] = (1.0 + ta * ta) * (1.0 + ta * ta) * (1.0 + tb * tb) / s4;
+                g->geo[21][id] = -(1.0 + ta * ta) * (1.0 + tb * tb) * ta * tb / s4;
+                g->geo[22][id] = (1.0 + ta * ta) * (1.0 + tb * tb) * (1.0 + tb * tb) / s4;
+            }
+        }
+}
+
+/* covariant weights qb,qt,ql,qr: equiang_cs/ecs_Ah_vec_sync_factory_mod.f90:101-149 */
+static double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void build_sync_weights(ora_grid *g) {
+    const double pi = pi_();
+    int n = g->n;
+    g->qb = calloc(n + 1, sizeof(double)); g->qt = calloc(n + 1, sizeof(double));
+    g->ql = calloc(n + 1, sizeof(double)); g->qr = calloc(n + 1, sizeof(double));
+    double a[3], b[3], t[3];
+    for (int i = 1; i <= n; i++) {
+        double al = -0.25 * pi + (i - 1) * g->hx;
+        ecs_a2_proto(al, -0.25 * pi, a);
+        ecs_b1_proto(al, 0.25 * pi, t); b[0] = t[0]; b[1] = -t[2]; b[2] = t[1];
+        g->qb[i] = dot3(a, b);
+        ecs_a2_proto(al, 0.25 * pi, a);
+        ecs_b1_proto(al, -0.25 * pi, t); b[0] = t[0]; b[1] = t[2]; b[2] = -t[1];
+        g->qt[i] = dot3(a, b);
+    }
+    for (int j = 1; j <= n; j++) {
+        double be = -0.25 * pi + (j - 1) * g->hx;
+        ecs_a1_proto(-0.25 * pi, be, a);
+        ecs_b2_proto(0.25 * pi, be, t); b[0] = -t[2]; b[1] = t[1]; b[2] = t[0];
+        g->ql[j] = dot3(a, b);
+        ecs_a1_proto(0.25 * pi, be, a);
+        ecs_b2_proto(-0.25 * pi, be, t); b[0] = t[2]; b[1] = t[1]; b[2] = -t[0];
+        g->qr[j] = dot3(a, b);
+    }
+}
+
+/* ---- topology: cubed_sphere_topology_mod.f90:80-148 (transform_index) ---- */
+static int idot(const int a[3], const int b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static void transform_index(int ps, int is, int js, int pt, int *it, int *jt, int nx, int ny) {
+    if (ps == pt) { *it = is; *jt = js; return; }
+    int exl[3], eyl[3], rl[3], sh[2];
+    int is_right = idot(EX[pt], NV[ps]) == -1;
+    int is_left = idot(EX[pt], NV[ps]) == 1;
+    int is_bottom = idot(EY[pt], NV[ps]) == 1;
+    int is_top = idot(EY[pt], NV[ps]) == -1;
+    int is_opp = idot(NV[pt], NV[ps]) == -1;
+    for (int k = 0; k < 3; k++) { exl[k] = eyl[k] = rl[k] = 0; }
+    sh[0] = sh[1] = 0;
+    if (is_right) {
+        for (int k = 0; k < 3; k++) { exl[k] = NV[pt][k]; eyl[k] = EY[pt][k]; rl[k] = RV[pt][k] + EX[pt][k]; }
+        sh[0] = nx; sh[1] = 0;
+    } else if (is_left) {
+        for (int k = 0; k < 3; k++) { exl[k] = -NV[pt][k]; eyl[k] = EY[pt][k]; }
+        for (int k = 0; k < 3; k++) rl[k] = RV[pt][k] - exl[k];
+        sh[0] = -nx; sh[1] = 0;
+    } else if (is_bottom) {
+        for (int k = 0; k < 3; k++) { exl[k] = EX[pt][k]; eyl[k] = -NV[pt][k]; rl[k] = RV[pt][k] + NV[pt][k]; }
+        sh[0] = 0; sh[1] = -ny;
+    } else if (is_top) {
+        for (int k = 0; k < 3; k++) { exl[k] = EX[pt][k]; eyl[k] = NV[pt][k]; rl[k] = RV[pt][k] + EY[pt][k]; }
+        sh[0] = 0; sh[1] = ny;
+    } else if (is_opp) {
+        for (int k = 0; k < 3; k++) { exl[k] = -EX[pt][k]; eyl[k] = EY[pt][k]; rl[k] = RV[pt][k] + EX[pt][k] + NV[pt][k]; }
+        sh[0] = 2 * nx; sh[1] = 0;
+    }
+    int d[3];
+    for (int k = 0; k < 3; k++) d[k] = RV[ps][k] - rl[k];
+    int ex_ex = idot(EX[ps], exl), ex_ey = idot(EY[ps], exl), ex_r = idot(d, exl);
+    int ey_ex = idot(EX[ps], eyl), ey_ey = idot(EY[ps], eyl), ey_r = idot(d, eyl);
+    *it = (is - 1) * ex_ex + (js - 1) * ex_ey + (nx - 1) * ex_r + 1 + sh[0];
+    *jt = (is - 1) * ey_ex + (js - 1) * ey_ey + (ny - 1) * ey_r + 1 + sh[1];
+}
+
+/* find_basis_orientation, cubed_sphere_topology_mod.f90:180-239; first = 'i' -> 0, 'j' -> 1 */
+static void find_basis_orientation(int ps, int pt, int *istep, int *jstep, int *first) {
+    if (ps == pt) { *first = 0; *istep = 1; *jstep = 1; return; }
+    int exl[3] = {0, 0, 0}, eyl[3] = {0, 0, 0};
+    int is_right = idot(EX[pt], NV[ps]) == -1;
+    int is_left = idot(EX[pt], NV[ps]) == 1;
+    int is_bottom = idot(EY[pt], NV[ps]) == 1;
+    int is_top = idot(EY[pt], NV[ps]) == -1;
+    int is_opp = idot(NV[pt], NV[ps]) == -1;
+    if (is_right) { for (int k = 0; k < 3; k++) { exl[k] = NV[pt][k]; eyl[k] = EY[pt][k]; } }
+    else if (is_left) { for (int k = 0; k < 3; k++) { exl[k] = -NV[pt][k]; eyl[k] = EY[pt][k]; } }
+    else if (is_bottom) { for (int k = 0; k < 3; k++) { exl[k] = EX[pt][k]; eyl[k] = -NV[pt][k]; } }
+    else if (is_top) { for (int k = 0; k < 3; k++) { exl[k] = EX[pt][k]; eyl[k] = NV[pt][k]; } }
+    else if (is_opp) { for (int k = 0; k < 3; k++) { exl[k] = -EX[pt][k]; eyl[k] = EY[pt][k]; } }
+    int ex_ex = idot(EX[ps], exl), ex_ey = idot(EY[ps], exl);
+    int ey_ex = idot(EX[ps], eyl), ey_ey = idot(EY[ps], eyl);
+    if (ex_ex != 0) { *first = 0; *istep = ex_ex; *jstep = ey_ey; }
+    else { *first = 1; *istep = ex_ey; *jstep = ey_ex; }
+}
+
+typedef struct { int is, ie, js, je; } tile2;
+
+/* transform_tile_coords, cubed_sphere_topology_mod.f90:149-178 */
+static tile2 transform_tile(int ps, tile2 s, int pt, int n) {
+    int is, js, ie, je;
+    transform_index(ps, s.is, s.js, pt, &is, &js, n, n);
+    transform_index(ps, s.ie, s.je, pt, &ie, &je, n, n);
+    tile2 t = {is < ie ? is : ie, is > ie ? is : ie, js < je ? js : je, js > je ? js : je};
+    return t;
+}
+/* find_tiles_halo_intersection ('full'), parallel/exchange_factory_mod.f90:618-640 */
+static int halo_intersect(tile2 t1, int hw, tile2 t2, tile2 *out) {
+    out->is = (t1.is - hw > t2.is) ? t1.is - hw : t2.is;
+    out->ie = (t1.ie + hw < t2.ie) ? t1.ie + hw : t2.ie;
+    out->js = (t1.js - hw > t2.js) ? t1.js - hw : t2.js;
+    out->je = (t1.je + hw < t2.je) ? t1.je + hw : t2.je;
+    return out->is <= out->ie && out->js <= out->je;
+}
+
+/* Exchange map of create_xy_xyz_points_halo_exchange (exchange_factory_mod.f90:188-299) for one
+ * tile per panel, simulated message by message: the remote panel packs its send tile in the order
+ * given by pack_to_buf(_vec) (parallel/buffer_mod.f90:48-277) and the local panel unpacks it over
+ * its receive tile in natural order (buffer_mod.f90:24-46,147-170). */
+static void build_halo_map(ora_grid *g, int width) {
+    int n = g->n;
+    g->hsrc = malloc(sizeof(int) * g->flen);
+    g->hvec = malloc(sizeof(int) * g->flen);
+    for (long k = 0; k < g->flen; k++) { g->hsrc[k] = -1; g->hvec[k] = 0; }
+    tile2 full = {1, n, 1, n};
+    int *buf_src = malloc(sizeof(int) * (size_t)(width * (n + 2 * width) * 2 + 16));
+    int *buf_vec = malloc(sizeof(int) * (size_t)(width * (n + 2 * width) * 2 + 16));
+    for (int pl = 0; pl < 6; pl++)
+        for (int pr = 0; pr < 6; pr++) {
+            if (pl == pr) continue;
+            tile2 tmp = transform_tile(pr, full, pl, n), recv;
+            if (!halo_intersect(full, width, tmp, &recv)) continue;
+            /* the remote panel's send side for this message */
+            tile2 tmp2 = transform_tile(pl, full, pr, n), snd;
+            if (!halo_intersect(tmp2, width, full, &snd)) abort();
+            int istep, jstep, first;
+            find_basis_orientation(pl, pr, &istep, &jstep, &first);
+            int cnt = 0;
+            int svec = 0;
+            if (first == 0) {
+                /* u_buf = istep*u, v_buf = jstep*v */
+                svec = (istep < 0 ? 2 : 0) | (jstep < 0 ? 4 : 0);
+                for (int jj = 0; jj <= snd.je - snd.js; jj++) {
+                    int j = jstep == 1 ? snd.js + jj : snd.je - jj;
+                    for (int ii = 0; ii <= snd.ie - snd.is; ii++) {
+                        int i = istep == 1 ? snd.is + ii : snd.ie - ii;
+                        buf_src[cnt] = (int)IDX(g, pr, i, j); buf_vec[cnt] = svec; cnt++;
+                    }
+                }
+            } else {
+                /* u_buf = jstep*v, v_buf = istep*u */
+                svec = 1 | (jstep < 0 ? 2 : 0) | (istep < 0 ? 4 : 0);
+                for (int ii = 0; ii <= snd.ie - snd.is; ii++) {
+                    int i = istep == 1 ? snd.is + ii : snd.ie - ii;
+                    for (int jj = 0; jj <= snd.je - snd.js; jj++) {
+                        int j = jstep == 1 ? snd.js + jj : snd.je - jj;
+                        buf_src[cnt] = (int)IDX(g, pr, i, j); buf_vec[cnt] = svec; cnt++;
+                    }
+                }
+            }
+            int cnt2 = 0;
+            for (int j = recv.js; j <= recv.je; j++)
+                for (int i = recv.is; i <= recv.ie; i++) {
+                    long id = IDX(g, pl, i, j);
+                    g->hsrc[id] = buf_src[cnt2];
+                    g->hvec[id] = buf_vec[cnt2];
+                    cnt2++;
+                }
+            if (cnt2 != cnt) abort();
+        }
+    free(buf_src); free(buf_vec);
+}
+
+void ora_halo_map(const ora_grid *g, int *src, int *vec) {
+    memcpy(src, g->hsrc, sizeof(int) * g->flen);
+    memcpy(vec, g->hvec, sizeof(int) * g->flen);
+}
+
+static int in_halo_ring(const ora_grid *g, int i, int j, int width) {
+    int n = g->n;
+    int ii = i < 1 || i > n, jj = j < 1 || j > n;
+    if (!(ii || jj)) return 0;
+    return i >= 1 - width && i <= n + width && j >= 1 - width && j <= n + width;
+}
+
+/* exchange_2D_halo_t%do (parallel/exchange_halo_mod.f90:42-102): every receive of this call uses
+ * the pre-exchange interior values of the remote panels */
+void ora_exchange(const ora_grid *g, double *f, int width) {
+    for (int p = 0; p < 6; p++)
+        for (int j = 1 - width; j <= g->n + width; j++)
+            for (int i = 1 - width; i <= g->n + width; i++) {
+                if (!in_halo_ring(g, i, j, width)) continue;
+                long id = IDX(g, p, i, j);
+                if (g->hsrc[id] >= 0) f[id] = f[g->hsrc[id]];
+            }
+}
+/* exchange_2D_halo_t%do_vec (parallel/exchange_halo_mod.f90:103-167) with the component
+ * permutation and sign flips of pack_to_buf_vec */
+void ora_exchange_vec(const ora_grid *g, double *u, double *v, int width) {
+    for (int p = 0; p < 6; p++)
+        for (int j = 1 - width; j <= g->n + width; j++)
+            for (int i = 1 - width; i <= g->n + width; i++) {
+                if (!in_halo_ring(g, i, j, width)) continue;
+                long id = IDX(g, p, i, j);
+                int s = g->hsrc[id];
+                if (s < 0) continue;
+                int c = g->hvec[id];
+                double us = u[s], vs = v[s];
+                double a = (c & 1) ? vs : us, b = (c & 1) ? us : vs;
+                u[id] = (c & 2) ? -a : a;
+                v[id] = (c & 4) ? -b : b;
+            }
+}
+
+/* ---- SBP apply: sbp_operator_mod.f90:152-228 (sbp_apply) over the whole panel ---- */
+static void sbp_apply_panel(const ora_grid *g, const sbp_op *o, char dir, const double *f, double *df) {
+    int n = g->n;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                int t = dir == 'x' ? i : j; /* index along the direction */
+#define F(k) (dir == 'x' ? f[IDX(g, p, (k), j)] : f[IDX(g, p, i, (k))])
+                double s;
+                if (t <= o->ne) {
+                    int mx = o->last[t - 1];
+                    s = o->Ql[0 + o->nw * (t - 1)] * F(1);
+                    for (int m = 2; m <= mx; m++) s = s + o->Ql[(m - 1) + o->nw * (t - 1)] * F(m);
+                } else if (t <= n - o->ne) {
+                    s = o->Qin[0] * F(t + o->inshift);
+                    for (int k = 1; k < o->win; k++) s = s + o->Qin[k] * F(t + o->inshift + k);
+                } else {
+                    int ir = t - (n - o->ne);
+                    int mx = n + o->rshift[ir - 1];
+                    int nwst = o->nw + o->rshift[ir - 1];
+                    s = o->Qr[(nwst - 1) + o->nw * (ir - 1)] * F(mx);
+                    for (int m = nwst + 1; m <= o->nw; m++)
+                        s = s + o->Qr[(m - 1) + o->nw * (ir - 1)] * F(mx + (m - nwst));
+                }
+#undef F
+                df[IDX(g, p, i, j)] = s;
+            }
+}
+
+void ora_sbp_apply(const ora_grid *g, int order, char dir, const double *f, double *df) {
+    sbp_op o;
+    sbp_make(&o, order);
+    sbp_apply_panel(g, &o, dir, f, df);
+}
+
+/* ---- Ah scalar sync: halo/halo_Ah_scalar_sync_mod.f90:20-93 ---- */
+void ora_scalar_sync(const ora_grid *g, double *f) {
+    int n = g->n;
+    ora_exchange(g, f, 1);
+#define P(i, j) f[IDX(g, p, (i), (j))]
+    for (int p = 0; p < 6; p++) {
+        /* is == 1 */
+        P(1, 1) = (P(1, 1) + P(0, 1) + P(1, 0)) / 3.0;
+        for (int j = 2; j <= n - 1; j++) P(1, j) = 0.5 * (P(0, j) + P(1, j));
+        P(1, n) = (P(1, n) + P(0, n) + P(1, n + 1)) / 3.0;
+        /* ie == nx */
+        P(n, 1) = (P(n, 1) + P(n + 1, 1) + P(n, 0)) / 3.0;
+        for (int j = 2; j <= n - 1; j++) P(n, j) = 0.5 * (P(n, j) + P(n + 1, j));
+        P(n, n) = (P(n, n) + P(n + 1, n) + P(n, n + 1)) / 3.0;
+        /* js == 1 */
+        for (int i = 2; i <= n - 1; i++) P(i, 1) = 0.5 * (P(i, 1) + P(i, 0));
+        /* je == ny */
+        for (int i = 2; i <= n - 1; i++) P(i, n) = 0.5 * (P(i, n) + P(i, n + 1));
+    }
+#undef P
+}
+
+/* ---- Ah vector sync: equiang_cs/ecs_Ah_vec_sync_mod.f90:34-125, covariant ordering ---- */
+static void sync_grad_on_edges(const ora_grid *g, int p, double *u, double *v) {
+    int n = g->n;
+#define U(i, j) u[IDX(g, p, (i), (j))]
+#define V(i, j) v[IDX(g, p, (i), (j))]
+    const double *qb = g->qb, *qt = g->qt, *ql = g->ql, *qr = g->qr;
+    U(1, 1) = (U(1, 1) + U(0, 1) + U(1, 0) + qb[1] * V(1, 0)) / 3.0;
+    V(1, 1) = (V(1, 1) + V(0, 1) + V(1, 0) + ql[1] * U(0, 1)) / 3.0;
+    U(n, 1) = (U(n, 1) + U(n + 1, 1) + U(n, 0) + qb[n] * V(n, 0)) / 3.0;
+    V(n, 1) = (V(n, 1) + V(n + 1, 1) + V(n, 0) + qr[1] * U(n + 1, 1)) / 3.0;
+    U(1, n) = (U(1, n) + U(0, n) + U(1, n + 1) + qt[1] * V(1, n + 1)) / 3.0;
+    V(1, n) = (V(1, n) + V(0, n) + V(1, n + 1) + ql[n] * U(0, n)) / 3.0;
+    U(n, n) = (U(n, n) + U(n + 1, n) + U(n, n + 1) + qt[n] * V(n, n + 1)) / 3.0;
+    V(n, n) = (V(n, n) + V(n + 1, n) + V(n, n + 1) + qr[n] * U(n + 1, n)) / 3.0;
+    for (int j = 2; j <= n - 1; j++) {
+        U(1, j) = 0.5 * (U(0, j) + U(1, j));
+        V(1, j) = 0.5 * (V(1, j) + V(0, j) + ql[j] * U(0, j));
+    }
+    for (int j = 2; j <= n - 1; j++) {
+        U(n, j) = 0.5 * (U(n, j) + U(n + 1, j));
+        V(n, j) = 0.5 * (V(n, j) + V(n + 1, j) + qr[j] * U(n + 1, j));
+    }
+    for (int i = 2; i <= n - 1; i++) {
+        V(i, 1) = 0.5 * (V(i, 0) + V(i, 1));
+        U(i, 1) = 0.5 * (U(i, 0) + U(i, 1) + qb[i] * V(i, 0));
+    }
+    for (int i = 2; i <= n - 1; i++) {
+        V(i, n) = 0.5 * (V(i, n) + V(i, n + 1));
+        U(i, n) = 0.5 * (U(i, n) + U(i, n + 1) + qt[i] * V(i, n + 1));
+    }
+#undef U
+#undef V
+}
+void ora_vec_sync_covariant(const ora_grid *g, double *gx, double *gy) {
+    ora_exchange_vec(g, gx, gy, 1);
+    /* covariant components: syncronize_grad_on_edges(v, u, ...) (ecs_Ah_vec_sync_mod.f90:51-53) */
+    for (int p = 0; p < 6; p++) sync_grad_on_edges(g, p, gy, gx);
+}
+
+/* ---- divergence_ah{42,43,63}_sbp: div_ah_sbp_mod.f90:24-98 ---- */
+static void div_ah_sbp(const ora_grid *g, const double *u, const double *v, double *div) {
+    int n = g->n;
+    double *Gu = g->work[0], *Gv = g->work[1], *Dx = g->work[2], *Dy = g->work[3];
+    const double *J = g->geo[0];
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                Gu[id] = J[id] * u[id];
+                Gv[id] = J[id] * v[id];
+            }
+    sbp_apply_panel(g, &g->div_sbp, 'x', Gu, Dx);
+    sbp_apply_panel(g, &g->div_sbp, 'y', Gv, Dy);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                div[id] = (Dx[id] + Dy[id]) / (J[id] * g->hx * g->scale);
+            }
+    ora_scalar_sync(g, div);
+}
+
+/* ---- divergence_ah2: div_ah2_mod.f90:20-139 (width-2 vector exchange, no sync) ---- */
+static void div_ah2(const ora_grid *g, double *u, double *v, double *div) {
+    int nx = g->n, ny = g->n;
+    const double *Jg = g->geo[0];
+    const double hx = g->hx, sc = g->scale;
+    ora_exchange_vec(g, u, v, 2);
+    for (int p = 0; p < 6; p++) {
+#define U(i, j) u[IDX(g, p, (i), (j))]
+#define V(i, j) v[IDX(g, p, (i), (j))]
+#define J(i, j) Jg[IDX(g, p, (i), (j))]
+#define D(i, j) div[IDX(g, p, (i), (j))]
+        int i, j, n;
+        D(1, 1) = J(2, 1) * (U(2, 1) - U(-1, 1) + V(1, 2) - V(1, -1) + V(0, 2) + U(2, 0)) /
+                  (3.0 * J(1, 1) * hx * sc);
+        n = nx - 1;
+        D(n + 1, 1) = J(n, 1) * (U(n + 3, 1) - U(n, 1) + V(n + 1, 2) - V(n + 1, -1) + V(n + 2, 2) - U(n, 0)) /
+                      (3.0 * J(n + 1, 1) * hx * sc);
+        i = nx; j = ny;
+        D(i, j) = J(i - 1, j) * (-U(i - 1, j + 1) + V(i, j + 2) + -U(i - 1, j) - V(i, j - 1) + U(i + 2, j) - V(i + 1, j - 1)) /
+                  (3.0 * J(i, j) * hx * sc);
+        i = 1; j = ny;
+        D(i, j) = J(i + 1, j) * (U(i + 1, j + 1) + V(i, j + 2) + U(i + 1, j) - V(i, j - 1) + -U(i - 2, j) - V(i - 1, j - 1)) /
+                  (3.0 * J(i, j) * hx * sc);
+        j = 1;
+        for (i = 2; i <= nx - 1; i++)
+            D(i, j) = (0.5 * J(i + 1, j) * (U(i + 1, j) + U(i + 1, j - 1)) - 0.5 * J(i - 1, j) * (U(i - 1, j) + U(i - 1, j - 1)) +
+                       J(i, j + 1) * (V(i, j + 1) - V(i, j - 2))) / (2.0 * J(i, j) * hx * sc);
+        n = ny - 1;
+        j = n + 1;
+        for (i = 2; i <= nx - 1; i++)
+            D(i, j) = (0.5 * J(i + 1, j) * (U(i + 1, j) + U(i + 1, j + 1)) - 0.5 * J(i - 1, j) * (U(i - 1, j) + U(i - 1, j + 1)) +
+                       J(i, j - 1) * (V(i, j + 2) - V(i, j - 1))) / (2.0 * J(i, j) * hx * sc);
+        i = 1;
+        for (j = 2; j <= ny - 1; j++)
+            D(i, j) = (0.5 * J(i, j + 1) * (V(i, j + 1) + V(i - 1, j + 1)) - 0.5 * J(i, j - 1) * (V(i, j - 1) + V(i - 1, j - 1)) +
+                       J(i + 1, j) * (U(i + 1, j) - U(i - 2, j))) / (2.0 * J(i, j) * hx * sc);
+        n = nx - 1;
+        i = n + 1;
+        for (j = 2; j <= ny - 1; j++)
+            D(i, j) = (0.5 * J(i, j + 1) * (V(i, j + 1) + V(i + 1, j + 1)) - 0.5 * J(i, j - 1) * (V(i, j - 1) + V(i + 1, j - 1)) +
+                       J(i - 1, j) * (U(i + 2, j) - U(i - 1, j))) / (2.0 * J(i, j) * hx * sc);
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                D(i, j) = (J(i + 1, j) * U(i + 1, j) - J(i - 1, j) * U(i - 1, j) + J(i, j + 1) * V(i, j + 1) -
+                           J(i, j - 1) * V(i, j - 1)) / (2.0 * J(i, j) * hx * sc);
+#undef U
+#undef V
+#undef J
+#undef D
+    }
+}
+
+void ora_div(const ora_grid *g, double *u, double *v, double *div) {
+    if (g->div_kind == 2) div_ah2(g, u, v, div);
+    else div_ah_sbp(g, u, v, div);
+}
+
+/* ---- gradient_ah{21,42,43,63}_sbp_ecs: grad_ah_sbp_mod.f90:26-90 ---- */
+void ora_grad(const ora_grid *g, double *f, double *gx, double *gy) {
+    int n = g->n;
+    double *Dx = g->work[2], *Dy = g->work[3];
+    sbp_apply_panel(g, &g->grad_sbp, 'x', f, Dx);
+    sbp_apply_panel(g, &g->grad_sbp, 'y', f, Dy);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                gx[id] = Dx[id] / (g->hx * g->scale);
+                gy[id] = Dy[id] / (g->hx * g->scale);
+            }
+    ora_vec_sync_covariant(g, gx, gy);
+}
+
+/* ---- curl_divergence_*: curl_div_based_mod.f90:26-66 ---- */
+void ora_curl(const ora_grid *g, double *u, double *v, double *curl) {
+    int n = g->n;
+    double *ut = g->work[4], *vt = g->work[5];
+    const double *J = g->geo[0];
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                ut[id] = v[id] / J[id];
+                vt[id] = -u[id] / J[id];
+            }
+    ora_div(g, ut, vt, curl);
+}
+
+/* ---- operator_swm_t%apply: models/shallow_water/operator/operator_swm_mod.f90:61-113 ---- */
+void ora_rhs(const ora_grid *g, double *h, double *u, double *v, const double *hs,
+             double *dh, double *du, double *dv) {
+    int n = g->n;
+    double *ut = g->work[6], *vt = g->work[7], *hu = g->work[8], *hv = g->work[9];
+    double *KE = g->work[10], *gx = g->work[11], *gy = g->work[12], *curl = g->work[13];
+    double *div = g->work[14];
+    const double *J = g->geo[0], *Qi1 = g->geo[1], *Qi2 = g->geo[2], *Qi3 = g->geo[3], *fc = g->geo[4];
+    const double grav = g->grav;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                /* co2contra_colocated_mod.f90:55-56 */
+                ut[id] = Qi1[id] * u[id] + Qi2[id] * v[id];
+                vt[id] = Qi2[id] * u[id] + Qi3[id] * v[id];
+                /* massflux_colocated_mod.f90:26-27 -> assign_prod(1.0,h,ut) */
+                hu[id] = 1.0 * h[id] * ut[id];
+                hv[id] = 1.0 * h[id] * vt[id];
+                /* KE_colocated_mod.f90:44 */
+                KE[id] = (u[id] * ut[id] + v[id] * vt[id]) / 2.0;
+                /* KE%update(grav,h,grav,h_surf) (operator_swm_mod.f90:85) */
+                KE[id] = KE[id] + grav * h[id] + grav * (hs ? hs[id] : 0.0);
+            }
+    ora_grad(g, KE, gx, gy);
+    ora_curl(g, u, v, curl);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                /* coriolis_colocated_mod.f90:97-98 */
+                double cor_u = (fc[id] + curl[id]) * hv[id] / h[id] * J[id];
+                double cor_v = -(fc[id] + curl[id]) * hu[id] / h[id] * J[id];
+                /* vout%u%assign(-1,grad_x,1,cor_u) */
+                du[id] = -1.0 * gx[id] + 1.0 * cor_u;
+                dv[id] = -1.0 * gy[id] + 1.0 * cor_v;
+            }
+    ora_div(g, hu, hv, div);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                dh[id] = -1.0 * div[id];
+            }
+}
+
+/* ---- rk4_t%step: time_schemes/rk4_mod.f90:20-42 with stvec_swm_mod.f90:38-106 ---- */
+void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *hs, double dt, int nsteps) {
+    int n = g->n;
+    long L = g->flen;
+    double *k[4][3], *y[3];
+    for (int s = 0; s < 4; s++)
+        for (int c = 0; c < 3; c++) k[s][c] = alloc_field(g);
+    for (int c = 0; c < 3; c++) y[c] = alloc_field(g);
+    double *v0[3] = {h, u, v};
+    const double c1 = 0.5 * dt, c2 = dt / 6.0, c3 = dt / 3.0;
+    for (int step = 0; step < nsteps; step++) {
+        ora_rhs(g, v0[0], v0[1], v0[2], hs, k[0][0], k[0][1], k[0][2]);
+        for (int c = 0; c < 3; c++)
+            for (int p = 0; p < 6; p++)
+                for (int j = 1; j <= n; j++)
+                    for (int i = 1; i <= n; i++) {
+                        long id = IDX(g, p, i, j);
+                        y[c][id] = 1.0 * v0[c][id] + c1 * k[0][c][id];
+                    }
+        ora_rhs(g, y[0], y[1], y[2], hs, k[1][0], k[1][1], k[1][2]);
+        for (int c = 0; c < 3; c++)
+            for (int p = 0; p < 6; p++)
+                for (int j = 1; j <= n; j++)
+                    for (int i = 1; i <= n; i++) {
+                        long id = IDX(g, p, i, j);
+                        y[c][id] = 1.0 * v0[c][id] + c1 * k[1][c][id];
+                    }
+        ora_rhs(g, y[0], y[1], y[2], hs, k[2][0], k[2][1], k[2][2]);
+        for (int c = 0; c < 3; c++)
+            for (int p = 0; p < 6; p++)
+                for (int j = 1; j <= n; j++)
+                    for (int i = 1; i <= n; i++) {
+                        long id = IDX(g, p, i, j);
+                        y[c][id] = 1.0 * v0[c][id] + dt * k[2][c][id];
+                    }
+        ora_rhs(g, y[0], y[1], y[2], hs, k[3][0], k[3][1], k[3][2]);
+        for (int c = 0; c < 3; c++)
+            for (int p = 0; p < 6; p++)
+                for (int j = 1; j <= n; j++)
+                    for (int i = 1; i <= n; i++) {
+                        long id = IDX(g, p, i, j);
+                        v0[c][id] = v0[c][id] + c2 * k[0][c][id] + c3 * k[1][c][id];
+                    }
+        for (int c = 0; c < 3; c++)
+            for (int p = 0; p < 6; p++)
+                for (int j = 1; j <= n; j++)
+                    for (int i = 1; i <= n; i++) {
+                        long id = IDX(g, p, i, j);
+                        v0[c][id] = v0[c][id] + c3 * k[2][c][id] + c2 * k[3][c][id];
+                    }
+    }
+    (void)L;
+    for (int s = 0; s < 4; s++)
+        for (int c = 0; c < 3; c++) free(k[s][c]);
+    for (int c = 0; c < 3; c++) free(y[c]);
+}
+
+/* ---- quadrature: quadrature/quadrature_factory_mod.f90:39-86,158-177, sbp_quadrature_mod.f90:21-40 ---- */
+void ora_quad_weights(const ora_grid *g, int order, double *A) {
+    int n = g->n, ne;
+    double Ae[8];
+    if (order == 21) { ne = 2; Ae[0] = 0.5; Ae[1] = 1.0; }
+    else if (order == 42) { ne = 4; Ae[0] = 17.0 / 48.0; Ae[1] = 59.0 / 48.0; Ae[2] = 43.0 / 48.0; Ae[3] = 49.0 / 48.0; }
+    else if (order == 63) {
+        ne = 6;
+        Ae[0] = 0.13649e5 / 0.43200e5; Ae[1] = 0.12013e5 / 0.8640e4; Ae[2] = 0.2711e4 / 0.4320e4;
+        Ae[3] = 0.5359e4 / 0.4320e4; Ae[4] = 0.7877e4 / 0.8640e4; Ae[5] = 0.43801e5 / 0.43200e5;
+    } else abort();
+    for (int i = 1; i <= n; i++) A[i - 1] = 1.0;
+    for (int i = 1; i <= ne && i <= n; i++) A[i - 1] = Ae[i - 1];
+    for (int i = (n - ne + 1 > 1 ? n - ne + 1 : 1); i <= n; i++) A[i - 1] = Ae[n - i];
+}
+
+double ora_quad_mass(const ora_grid *g, int order, const double *f) {
+    int n = g->n;
+    double *A = malloc(sizeof(double) * n);
+    ora_quad_weights(g, order, A);
+    const double *J = g->geo[0];
+    double mass = 0.0;
+    for (int p = 0; p < 6; p++) {
+        double out = 0.0;
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                out = out + f[id] * J[id] * g->hx * g->hx * A[i - 1] * A[j - 1];
+            }
+        mass = mass + out;
+    }
+    free(A);
+    return mass;
+}
+
+/* vec_math_mod.f90:50-90 */
+double ora_l2norm(const ora_grid *g, const double *f) {
+    int n = g->n;
+    const double *J = g->geo[0];
+    double acc = 0.0;
+    for (int p = 0; p < 6; p++) {
+        double out = 0.0;
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                out = out + f[id] * f[id] * J[id] * g->hx * g->hx;
+            }
+        acc = acc + out;
+    }
+    return sqrt(acc);
+}
+/* grid_field_mod.f90:597-632 */
+double ora_maxabs(const ora_grid *g, const double *f) {
+    int n = g->n;
+    double m = 0.0;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                double a = fabs(f[IDX(g, p, i, j)]);
+                if (a > m) m = a;
+            }
+    return m;
+}
+
+/* ---- get_diagnostics, calc_energy, calc_enstrophy: operator_swm_mod.f90:115-260 ---- */
+void ora_diagnostics(const ora_grid *g, int qo, double *h, double *u, double *v, const double *hs, double out[7]) {
+    int n = g->n;
+    double hmin = 1e300, hmax = -1e300;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                double x = h[IDX(g, p, i, j)];
+                if (x < hmin) hmin = x;
+                if (x > hmax) hmax = x;
+            }
+    double *KEu = alloc_field(g), *KEv = alloc_field(g), *PE = alloc_field(g), *cu = alloc_field(g);
+    const double *Qi1 = g->geo[1], *Qi2 = g->geo[2], *Qi3 = g->geo[3], *fc = g->geo[4];
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                double ut = Qi1[id] * u[id] + Qi2[id] * v[id];
+                double vt = Qi2[id] * u[id] + Qi3[id] * v[id];
+                double hu = 1.0 * h[id] * ut, hv = 1.0 * h[id] * vt;
+                KEu[id] = 0.5 * hu * u[id];
+                KEv[id] = 0.5 * hv * v[id];
+                double pe = 1.0 * h[id] + 1.0 * (hs ? hs[id] : 0.0);
+                PE[id] = 0.5 * g->grav * pe * pe;
+            }
+    double ke = ora_quad_mass(g, qo, KEu) + ora_quad_mass(g, qo, KEv);
+    double pe = ora_quad_mass(g, qo, PE);
+    ora_curl(g, u, v, cu);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                double c = cu[id] + 1.0 * fc[id];
+                c = 0.5 * c * c;
+                cu[id] = 1.0 * c / h[id];
+            }
+    out[0] = hmin; out[1] = hmax;
+    out[2] = ora_quad_mass(g, qo, h);
+    out[3] = ke + pe; out[4] = ke; out[5] = pe;
+    out[6] = ora_quad_mass(g, qo, cu);
+    free(KEu); free(KEv); free(PE); free(cu);
+}
+
+/* set_vector_test_field_1tile_1comp, test_fields/test_fields_mod.f90:397-433:
+ * u = sum([vx,vy,vz]*a1) (covariant) or *b1 (contravariant), incl. the halo */
+void ora_project_vector(const ora_grid *g, int kind, const double *vx, const double *vy,
+                        const double *vz, double *u, double *v) {
+    const double *e1 = kind == 0 ? g->geo[8] : g->geo[14];
+    const double *e2 = kind == 0 ? g->geo[11] : g->geo[17];
+    long L = (long)g->ld * g->ld;
+    for (long k = 0; k < 6 * L; k++) {
+        u[k] = vx[k] * e1[k] + vy[k] * e1[k + g->flen] + vz[k] * e1[k + 2 * g->flen];
+        v[k] = vx[k] * e2[k] + vy[k] * e2[k + g->flen] + vz[k] * e2[k + 2 * g->flen];
+    }
+}
+
+void ora_geometry(const ora_grid *g, int which, double *out) {
+    memcpy(out, g->geo[which], sizeof(double) * g->flen);
+}
+void ora_sync_weights(const ora_grid *g, int which, double *out) {
+    const double *q = which == 0 ? g->qb : which == 1 ? g->qt : which == 2 ? g->ql : g->qr;
+    memcpy(out, q + 1, sizeof(double) * g->n);
+}
+
+ora_grid *ora_create(int N, int grad_order, int div_kind, double scale, double omega, double grav) {
+    ora_grid *g = calloc(1, sizeof *g);
+    g->N = N; g->n = N + 1; g->hw = HW; g->ld = g->n + 2 * HW;
+    g->flen = 6L * g->ld * g->ld;
+    double pi = pi_();
+    /* create_ecs_metric: alpha0 = -0.25*pi, alpha1 = 0.25*pi (ecs_metric_factory_mod.f90:38-41);
+     * hx = (alpha1-alpha0)/nh (mesh_factory_mod.f90:45) */
+    g->hx = (0.25 * pi - (-0.25 * pi)) / (double)N;
+    g->scale = scale; g->omega = omega; g->grav = grav;
+    g->axis[0] = 0.0; g->axis[1] = 0.0; g->axis[2] = 1.0;
+    g->grad_order = grad_order; g->div_kind = div_kind;
+    sbp_make(&g->grad_sbp, grad_order);
+    if (div_kind != 2) sbp_make(&g->div_sbp, div_kind);
+    /* geometry arrays are laid out so that r, a1, a2, b1, b2 components are consecutive fields */
+    build_geometry(g);
+    /* regroup vector components contiguously: geo[5..7] r etc. are separate allocations; copy into
+     * one block per vector for ora_project_vector's strided access */
+    for (int base = 5; base <= 17; base += 3) {
+        double *blk = malloc(sizeof(double) * 3 * g->flen);
+        for (int c = 0; c < 3; c++) { memcpy(blk + c * g->flen, g->geo[base + c], sizeof(double) * g->flen); free(g->geo[base + c]); }
+        for (int c = 0; c < 3; c++) g->geo[base + c] = blk + c * g->flen;
+    }
+    build_sync_weights(g);
+    build_halo_map(g, 2);
+    for (int w = 0; w < 16; w++) g->work[w] = alloc_field(g);
+    return g;
+}
+
+void ora_destroy(ora_grid *g) {
+    if (!g) return;
+    for (int w = 0; w < 5; w++) free(g->geo[w]);
+    for (int base = 5; base <= 17; base += 3) free(g->geo[base]);
+    for (int w = 20; w < 23; w++) free(g->geo[w]);
+    free(g->qb); free(g->qt); free(g->ql); free(g->qr);
+    free(g->hsrc); free(g->hvec);
+    for (int w = 0; w < 16; w++) free(g->work[w]);
+    free(g);
+}
+
+/* ---- RK4 on the iomega mock model: models/iomega_model/operator_iomega_mod.f90:36-57 and
+ * time_schemes/rk4_mod.f90:20-42 on complex vectors ---- */
+void ora_rk4_iomega(int n, const double *wr, const double *wi, double *fr, double *fi, double dt) {
+    double complex *v0 = malloc(sizeof(double complex) * n), *y = malloc(sizeof(double complex) * n);
+    double complex *k[4];
+    for (int s = 0; s < 4; s++) k[s] = malloc(sizeof(double complex) * n);
+    for (int i = 0; i < n; i++) v0[i] = fr[i] + I * fi[i];
+#define APPLY(out, in) for (int i = 0; i < n; i++) out[i] = (wr[i] + I * wi[i]) * in[i]
+    APPLY(k[0], v0);
+    for (int i = 0; i < n; i++) y[i] = 1.0 * v0[i] + 0.5 * dt * k[0][i];
+    APPLY(k[1], y);
+    for (int i = 0; i < n; i++) y[i] = 1.0 * v0[i] + 0.5 * dt * k[1][i];
+    APPLY(k[2], y);
+    for (int i = 0; i < n; i++) y[i] = 1.0 * v0[i] + dt * k[2][i];
+    APPLY(k[3], y);
+    for (int i = 0; i < n; i++) v0[i] = v0[i] + dt / 6.0 * k[0][i] + dt / 3.0 * k[1][i];
+    for (int i = 0; i < n; i++) v0[i] = v0[i] + dt / 3.0 * k[2][i] + dt / 6.0 * k[3][i];
+#undef APPLY
+    for (int i = 0; i < n; i++) { fr[i] = creal(v0[i]); fi[i] = cimag(v0[i]); }
+    free(v0); free(y);
+    for (int s = 0; s < 4; s++) free(k[s]);
+}

@@ -1,0 +1,88 @@
+/*
+ * swm_oracle.h -- CPU FP64 restatement of the reference shallow-water hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product path (cubedsphere_sbp_swm_amd)
+ * never links, imports or calls it.
+ *
+ * Every function restates one reference routine; the citation of the file and
+ * lines it follows is given next to each definition in swm_oracle.c.  Paths are
+ * relative to the reference's src/ directory.
+ *
+ * Storage: one field = 6 panels x (n+2*hw) x (n+2*hw) doubles, n = N+1 Ah
+ * points per panel side, halo width hw; 1-based panel-local indices
+ * i,j in [1-hw, n+hw] as in the reference's tile arrays p(i,j,k).
+ */
+#ifndef SWM_ORACLE_H
+#define SWM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_grid ora_grid;
+
+/* grad_order in {21,42,43,63}; div_kind in {2,42,43,63} (2 = divergence_ah2) */
+ora_grid *ora_create(int N, int grad_order, int div_kind, double scale,
+                     double omega, double grav);
+void ora_destroy(ora_grid *g);
+
+int ora_n(const ora_grid *g);      /* points per panel side (N+1) */
+int ora_hw(const ora_grid *g);     /* halo width of stored fields */
+int ora_ld(const ora_grid *g);     /* leading dimension n+2*hw */
+long ora_field_len(const ora_grid *g); /* 6*ld*ld */
+
+/* geometry accessors: which = 0 J, 1 Qi11, 2 Qi12, 3 Qi22, 4 f (Coriolis),
+ * 5..7 r, 8..10 a1, 11..13 a2, 14..16 b1, 17..19 b2, 20..22 Q11,Q12,Q22 */
+void ora_geometry(const ora_grid *g, int which, double *out);
+/* q weights of the covariant Ah vector sync: which 0 qb, 1 qt, 2 ql, 3 qr; n values */
+void ora_sync_weights(const ora_grid *g, int which, double *out);
+/* SBP quadrature edge weights A(1..n) of the given order (21/42/63) */
+void ora_quad_weights(const ora_grid *g, int order, double *out);
+
+/* halo exchange map of the xy-point exchange (width 2):
+ * out[panel][j][i] over the stored extent: encoded source or -1.
+ * code = ((src_panel*ld + (js-1+hw))*ld + (is-1+hw)); vec = swap | (su<0)<<1 | (sv<0)<<2 */
+void ora_halo_map(const ora_grid *g, int *src, int *vec);
+
+/* fill halos of stored fields (xy-point exchange semantics) */
+void ora_exchange(const ora_grid *g, double *f, int width);
+void ora_exchange_vec(const ora_grid *g, double *u, double *v, int width);
+
+/* operators (all fields in stored layout; halos of inputs are overwritten) */
+void ora_sbp_apply(const ora_grid *g, int order, char dir, const double *f, double *df);
+void ora_div(const ora_grid *g, double *u, double *v, double *div);
+void ora_grad(const ora_grid *g, double *f, double *gx, double *gy);
+void ora_curl(const ora_grid *g, double *u, double *v, double *curl);
+void ora_scalar_sync(const ora_grid *g, double *f);
+void ora_vec_sync_covariant(const ora_grid *g, double *gx, double *gy);
+
+/* full RHS: out = RHS(in); in/out are [h,u,v] each a stored field; hs may be NULL */
+void ora_rhs(const ora_grid *g, double *in_h, double *in_u, double *in_v,
+             const double *hs, double *dh, double *du, double *dv);
+/* classical RK4 steps in place on (h,u,v) */
+void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *hs,
+             double dt, int nsteps);
+/* diagnostics: hmin, hmax, mass, TE, KE, PE, enstrophy */
+void ora_diagnostics(const ora_grid *g, int quad_order, double *h, double *u,
+                     double *v, const double *hs, double out[7]);
+/* quadrature mass of a field, and l2norm / maxabs of the reference's vec_math */
+double ora_quad_mass(const ora_grid *g, int quad_order, const double *f);
+double ora_l2norm(const ora_grid *g, const double *f);
+double ora_maxabs(const ora_grid *g, const double *f);
+
+/* test-field projection: cartesian vectors (vx,vy,vz) sampled at r -> covariant
+ * (kind 0) or contravariant (kind 1) components; points incl. halo */
+void ora_project_vector(const ora_grid *g, int kind, const double *vx,
+                        const double *vy, const double *vz, double *u, double *v);
+
+/* generic classical RK4 on a diagonal complex linear ODE df/dt = w f (the
+ * reference's iomega mock model); returns f after one step */
+void ora_rk4_iomega(int n, const double *w_re, const double *w_im,
+                    double *f_re, double *f_im, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
