@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark: RK4 steps of the Ah-grid vector-invariant shallow-water model on one MI355X.
+
+Workload (default): C384 (6 x 384 x 384 cells, 6 x 385^2 Ah points), sbp42 operators
+(divergence_ah42_sbp, gradient_ah42_sbp_ecs, curl_divergence_ah42_sbp), FP64, RK4 with
+dt = 150 s, Earth radius / gravity / rotation, seeded spherical-harmonic initial state
+(h = 1e4 m + perturbation of degree <= 32 with max 100 m, max |V| = 40 m/s).
+
+A "step" is one RK4 time step (4 fused stage kernels) over the whole sphere; the metric
+is cell-steps/s = 6 N^2 * steps / time, inputs resident in HBM before the timed region.
+The dominant kernel (RK stages 2/3) is timed separately with HIP events on its stream
+and priced against the HBM roofline with its algorithmic bytes per launch.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s at C384 sbp42 vs HBM roofline; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic traffic per Ah point of one RK stage-2/3 launch: read y(3) + metric J, Qi11,
+# Qi12, Qi22, f (5) + v0(3) + acc(3), write y(3) + acc(3) = 20 values.
+STAGE23_VALUES = 20
+STEP_VALUES = 71  # per point per RK4 step (14 + 20 + 20 + 17)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--N", type=int, default=384)
+    ap.add_argument("--order", type=int, default=42)
+    ap.add_argument("--dt", type=float, default=150.0)
+    ap.add_argument("--lmax", type=int, default=32)
+    ap.add_argument("--precision", type=int, default=64, choices=(32, 64))
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(N, order, dt, lmax, budget_s, op):
+    """The oracle (single-threaded FP64 C restatement) on the host, bounded sample."""
+    from oracle.oracle import Grid
+    import cubedsphere_sbp_swm_amd as swm
+    G = Grid(N, order, order, swm.EARTH["radius"], swm.EARTH["omega"], swm.EARTH["grav"])
+    h, u, v = op.initial_state(seed=0, lmax=lmax, halo_width=G.hw)
+    t0 = time.perf_counter()
+    G.rk4(h, u, v, dt, 1)
+    one = time.perf_counter() - t0
+    steps = 1
+    extra = max(0, int(budget_s / max(one, 1e-6)) - 1)
+    if extra:
+        t1 = time.perf_counter()
+        G.rk4(h, u, v, dt, extra)
+        steps += extra
+        total = one + (time.perf_counter() - t1)
+    else:
+        total = one
+    cells = 6 * N * N
+    return {"value": cells * steps / total, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"C{N} sbp{order} FP64 RK4 dt={dt:g}s, {steps} steps ({total:.1f} s) of the oracle "
+                      f"(single-threaded C restatement, gcc -O2) on the GPU box host"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the panel-sharded multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import cubedsphere_sbp_swm_amd as swm
+    from cubedsphere_sbp_swm_amd import _capi
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    N, order, dt = args.N, args.order, args.dt
+    op = swm.create_swm_operator(N, **swm.operator_names(order), scale=swm.EARTH["radius"],
+                                 omega=swm.EARTH["omega"], grav=swm.EARTH["grav"], precision=args.precision)
+    stream = torch.cuda.Stream(dev)  # default priority; the library's kernels and the events share it
+    op.set_stream(stream.cuda_stream)
+    h, u, v = op.initial_state(seed=0, lmax=args.lmax)
+    x = swm.create_stvec_swm(op)
+    x.upload(h, u, v)
+    ts = swm.create_timescheme(op, "rk4")
+    ts.steps(x, dt, args.warmup)
+    torch.cuda.synchronize()
+
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    ts.steps(x, dt, args.steps)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end)
+    cells = 6 * N * N
+    points = 6 * (N + 1) ** 2
+    value = cells * args.steps / (ms / 1e3)
+    diag = op.get_diagnostics(x)
+
+    # dominant kernel: RK stage 2/3, back-to-back launches timed with events on the same stream
+    y = swm.create_stvec_swm(op)
+    y.upload(h, u, v)
+    L = _capi.lib()
+    import ctypes
+    _capi.check(L.swm_bench_stage(op.ctx, ctypes.c_void_p(y.ptr), 2, 5))
+    torch.cuda.synchronize()
+    k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    k0.record(stream)
+    _capi.check(L.swm_bench_stage(op.ctx, ctypes.c_void_p(y.ptr), 2, args.kernel_reps))
+    k1.record(stream)
+    torch.cuda.synchronize()
+    k_us = k0.elapsed_time(k1) * 1e3 / args.kernel_reps
+    eb = op.elem_bytes
+    stage_bytes = STAGE23_VALUES * eb * points
+    achieved = stage_bytes / (k_us * 1e-6) / 1e9
+    step_bytes = STEP_VALUES * eb * points
+    step_gbs = step_bytes / ((ms / 1e3) / args.steps) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(N, order, dt, args.lmax, args.cpu_seconds, op)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 64 else "f32",
+        "data": "synthetic (seeded spherical-harmonic initial state)",
+        "config": {"workload": f"C{N} sbp{order} RK4 dt={dt:g}s, {'FP64' if args.precision == 64 else 'FP32'}, "
+                               f"one RK4 step per step", "N": N, "points": points, "cells": cells,
+                   "parallelism": "single GPU, all six panels resident"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage_kernel<42, double, MODE_STAGE23> (RK stages 2 and 3)",
+                     "kernel_avg_us": k_us, "algorithmic_bytes_per_launch": stage_bytes,
+                     "step_effective_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS},
+        "cpu_baseline": cpu,
+        "diagnostics_after": {k: diag[k] for k in ("mass", "TE", "hmin", "hmax")},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,193 @@
+"""MI355X (gfx950) shallow-water dynamical core on the equiangular cubed sphere.
+
+Host-side mirror of the reference's operator / stvec / timescheme interface for the
+Ah-grid vector-invariant shallow-water model, backed by the HIP library libswm_gpu.so
+(C ABI in include/swm_gpu.h):
+
+    op = create_swm_operator(N, **operator_names(42), scale=a, omega=om, grav=g)
+    v = create_stvec_swm(op); v.upload(h, u, v_)          # stvec_swm_t
+    k = v.create_similar(); op.apply(k, v)                # operator_t%apply
+    ts = create_timescheme(op, "rk4"); ts.step(v, dt)     # timescheme_t%step
+    op.get_diagnostics(v)                                 # operator_t%get_diagnostics
+
+Host arrays use the reference tile layout: shape (6, n+2*hw, n+2*hw), index [panel, j, i]
+with reference indices i,j = 1..n stored at [.., j-1+hw, i-1+hw], n = N+1.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import SwmError, check
+
+__all__ = ["SwmError", "operator_names", "create_swm_operator", "create_stvec_swm", "create_timescheme",
+           "OperatorSWM", "StvecSWM", "RK4", "EARTH"]
+
+# const_mod.f90:6-9
+EARTH = {"radius": 0.6371229e7, "grav": 9.80616, "omega": 2.0 * np.pi / (23 * 60 * 60 + 56 * 60 + 4)}
+
+
+def operator_names(order):
+    """Operator names of the reference's sbp{21,42,43,63} run configurations
+    (models/shallow_water/test/scripts/ts2_ah/gen_namelist.sh:1-32)."""
+    table = {
+        21: ("divergence_ah2", "gradient_ah21_sbp_ecs", "SBP_Ah21_quadrature"),
+        42: ("divergence_ah42_sbp", "gradient_ah42_sbp_ecs", "SBP_Ah42_quadrature"),
+        43: ("divergence_ah43_sbp", "gradient_ah43_sbp_ecs", "SBP_Ah63_quadrature"),
+        63: ("divergence_ah63_sbp", "gradient_ah63_sbp_ecs", "SBP_Ah63_quadrature"),
+    }
+    d, g, q = table[order]
+    return {"div_op_name": d, "grad_op_name": g, "curl_op_name": "curl_" + d, "quadrature_name": q}
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class OperatorSWM:
+    """Domain + vector-invariant SWM operator (create_domain + create_swm_operator)."""
+
+    def __init__(self, N, div_op_name, grad_op_name, curl_op_name, quadrature_name,
+                 scale=1.0, omega=1.0, grav=1.0, precision=64):
+        L = _capi.lib()
+        h = ctypes.c_void_p()
+        check(L.swm_ctx_create(ctypes.byref(h), int(N), div_op_name.encode(), grad_op_name.encode(),
+                               curl_op_name.encode(), quadrature_name.encode(), float(scale), float(omega),
+                               float(grav), int(precision)))
+        self.ctx = h
+        self.N = N
+        self.precision = precision
+        n, pitch, fe, eb = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
+        check(L.swm_ctx_info(self.ctx, ctypes.byref(n), ctypes.byref(pitch), ctypes.byref(fe), ctypes.byref(eb)))
+        self.n, self.pitch, self.field_elems, self.elem_bytes = n.value, pitch.value, fe.value, eb.value
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            _capi.lib().swm_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(_capi.lib().swm_ctx_synchronize(self.ctx))
+
+    def set_stream(self, hip_stream_handle):
+        check(_capi.lib().swm_ctx_set_stream(self.ctx, ctypes.c_void_p(hip_stream_handle)))
+
+    # operator_t%apply (operator_swm_mod.f90:61-113)
+    def apply(self, vout, vin):
+        check(_capi.lib().swm_rhs(self.ctx, ctypes.c_void_p(vin.ptr), ctypes.c_void_p(vout.ptr)))
+
+    # operator_t%get_diagnostics (operator_swm_mod.f90:115-155)
+    def get_diagnostics(self, v):
+        out = np.zeros(7)
+        check(_capi.lib().swm_diagnostics(self.ctx, ctypes.c_void_p(v.ptr), _dp(out)))
+        return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))
+
+    def set_orography(self, hs, halo_width=0):
+        if hs is None:
+            check(_capi.lib().swm_set_orography(self.ctx, None, 0))
+            return
+        hs = np.ascontiguousarray(hs, dtype=np.float64)
+        self._check_host(hs, halo_width)
+        check(_capi.lib().swm_set_orography(self.ctx, _dp(hs), int(halo_width)))
+
+    def host_shape(self, halo_width=0):
+        ld = self.n + 2 * halo_width
+        return (6, ld, ld)
+
+    def _check_host(self, a, halo_width):
+        if a.shape != self.host_shape(halo_width):
+            raise ValueError(f"host array shape {a.shape} != {self.host_shape(halo_width)}")
+
+    def initial_state(self, seed, lmax, h0=1.0e4, h_amp=100.0, v_max=40.0, halo_width=0):
+        """Seeded spherical-harmonic state (h, u, v) as host arrays (swm_ic_spherical_harmonics)."""
+        shp = self.host_shape(halo_width)
+        h, u, v = np.zeros(shp), np.zeros(shp), np.zeros(shp)
+        check(_capi.lib().swm_ic_spherical_harmonics(self.ctx, int(seed), int(lmax), float(h0), float(h_amp),
+                                                     float(v_max), _dp(h), _dp(u), _dp(v), int(halo_width)))
+        return h, u, v
+
+    def debug_geometry(self, which, halo_width=0):
+        out = np.zeros(self.host_shape(halo_width))
+        check(_capi.lib().swm_debug_geometry(self.ctx, int(which), _dp(out), int(halo_width)))
+        return out
+
+
+class StvecSWM:
+    """Device-resident state vector (h, u, v) of one operator's domain (stvec_swm_t)."""
+
+    def __init__(self, op):
+        self.op = op
+        p = ctypes.c_void_p()
+        check(_capi.lib().swm_state_alloc(op.ctx, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def free(self):
+        if getattr(self, "ptr", None) and getattr(self.op, "ctx", None):
+            _capi.lib().swm_state_free(self.op.ctx, ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def create_similar(self):
+        return StvecSWM(self.op)
+
+    def upload(self, h, u, v, halo_width=0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (h, u, v)]
+        for a in arrs:
+            self.op._check_host(a, halo_width)
+        check(_capi.lib().swm_state_upload(self.op.ctx, ctypes.c_void_p(self.ptr), *(_dp(a) for a in arrs),
+                                           int(halo_width)))
+
+    def download(self, halo_width=0):
+        shp = self.op.host_shape(halo_width)
+        h, u, v = np.zeros(shp), np.zeros(shp), np.zeros(shp)
+        check(_capi.lib().swm_state_download(self.op.ctx, ctypes.c_void_p(self.ptr), _dp(h), _dp(u), _dp(v),
+                                             int(halo_width)))
+        return h, u, v
+
+    # stvec_t%assign / update (stvec_swm_mod.f90:38-80)
+    def assign(self, s1, v1, s2, v2):
+        check(_capi.lib().swm_assign(self.op.ctx, ctypes.c_void_p(self.ptr), float(s1), ctypes.c_void_p(v1.ptr),
+                                     float(s2), ctypes.c_void_p(v2.ptr)))
+
+    def update(self, s1, v1, s2, v2):
+        check(_capi.lib().swm_update(self.op.ctx, ctypes.c_void_p(self.ptr), float(s1), ctypes.c_void_p(v1.ptr),
+                                     float(s2), ctypes.c_void_p(v2.ptr)))
+
+
+class RK4:
+    """Classical RK4 (time_schemes/rk4_mod.f90) with fused stage kernels."""
+
+    def __init__(self, op):
+        self.op = op
+
+    def step(self, v0, dt):
+        self.steps(v0, dt, 1)
+
+    def steps(self, v0, dt, nsteps):
+        check(_capi.lib().swm_rk4_steps(self.op.ctx, ctypes.c_void_p(v0.ptr), float(dt), int(nsteps)))
+
+
+def create_swm_operator(N, div_op_name, grad_op_name, curl_op_name, quadrature_name, scale=1.0, omega=1.0,
+                        grav=1.0, precision=64):
+    return OperatorSWM(N, div_op_name, grad_op_name, curl_op_name, quadrature_name, scale, omega, grav, precision)
+
+
+def create_stvec_swm(op):
+    return StvecSWM(op)
+
+
+def create_timescheme(op, name="rk4"):
+    if name != "rk4":
+        raise SwmError(f"unknown or unsupported time scheme on the device path: {name}")
+    return RK4(op)

@@ -1,0 +1,78 @@
+"""ctypes binding of libswm_gpu.so (declarations in include/swm_gpu.h).
+
+The library is built in-tree by cubedsphere_sbp_swm_amd/build.py.  There is no CPU
+fallback: if the library is missing or the device call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libswm_gpu.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+# name -> (restype, argtypes); the full exported C ABI of include/swm_gpu.h
+SIGNATURES = {
+    "swm_last_error": (ctypes.c_char_p, []),
+    "swm_ctx_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                            ctypes.c_char_p, _d, _d, _d, _i]),
+    "swm_ctx_destroy": (_i, [_vp]),
+    "swm_ctx_info": (_i, [_vp, _ip, _ip, ctypes.POINTER(ctypes.c_longlong), _ip]),
+    "swm_ctx_set_stream": (_i, [_vp, _vp]),
+    "swm_ctx_synchronize": (_i, [_vp]),
+    "swm_state_alloc": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "swm_state_free": (_i, [_vp, _vp]),
+    "swm_state_upload": (_i, [_vp, _vp, _dp, _dp, _dp, _i]),
+    "swm_state_download": (_i, [_vp, _vp, _dp, _dp, _dp, _i]),
+    "swm_rhs": (_i, [_vp, _vp, _vp]),
+    "swm_assign": (_i, [_vp, _vp, _d, _vp, _d, _vp]),
+    "swm_update": (_i, [_vp, _vp, _d, _vp, _d, _vp]),
+    "swm_rk4_steps": (_i, [_vp, _vp, _d, _i]),
+    "swm_diagnostics": (_i, [_vp, _vp, _dp]),
+    "swm_set_orography": (_i, [_vp, _dp, _i]),
+    "swm_ic_spherical_harmonics": (_i, [_vp, ctypes.c_ulonglong, _i, _d, _d, _d, _dp, _dp, _dp, _i]),
+    "swm_bench_stage": (_i, [_vp, _vp, _i, _i]),
+    "swm_debug_geometry": (_i, [_vp, _i, _dp, _i]),
+    "swm_debug_halo_table": (_i, [_vp, _ip]),
+}
+
+# host set-up routines (geometry.h), exported by the same library; they need no device
+HOST_SIGNATURES = {
+    "swmh_geometry": (None, [_i, _i, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "swmh_sync_weights": (None, [_i, _dp, _dp, _dp, _dp]),
+    "swmh_quad_weights": (_i, [_i, _i, _dp]),
+    "swmh_halo_table": (None, [_i, _i, _ip]),
+    "swmh_ic_sh": (_i, [_i, _i, ctypes.c_ulonglong, _i, _d, _d, _d, _dp, _dp, _dp, _i]),
+}
+
+_lib = None
+
+
+class SwmError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SwmError(f"{LIB_PATH} is missing: build it with cubedsphere_sbp_swm_amd/build.py "
+                           "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in {**SIGNATURES, **HOST_SIGNATURES}.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().swm_last_error()
+        raise SwmError(f"swm status {rc}: {msg.decode() if msg else ''}")
+    return rc

@@ -1,0 +1,370 @@
+// Host FP64 set-up for the device path (see geometry.h).  Built with the host compiler and
+// -ffp-contract=off: every metric term is evaluated with the same operation sequence as the
+// reference's Fortran expressions, so device results can be compared bit for bit with an
+// FP64 CPU evaluation of the same algorithm.
+#include "geometry.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// Panel frames of the reference's cube topology (topology/cubed_sphere_topology_mod.f90:29-55):
+// index axes ex, ey, inner normal n and lattice origin r of panels 1..6.
+const int EX[6][3] = {{-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 0}};
+const int EY[6][3] = {{0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, -1}};
+const int NV[6][3] = {{0, 0, 1}, {-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+const int RV[6][3] = {{1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 1, 1}};
+
+double pi_() { return std::acos(-1.0); }
+
+// grid coordinate of Ah point i (1-based): alpha0 + (i-1)*hx, hx = (alpha1-alpha0)/N
+double grid_hx(int N) { return (0.25 * pi_() - (-0.25 * pi_())) / (double)N; }
+double grid_coord(int N, int i) { return -0.25 * pi_() + ((double)(i - 1) + 0.0) * grid_hx(N); }
+
+// prototype-face vectors (ecs_metric_mod.f90:326-395)
+void r_proto(double a, double b, double r[3]) {
+    const double ta = std::tan(a), tb = std::tan(b);
+    const double d = std::sqrt(1.0 + ta * ta + tb * tb);
+    r[0] = ta / d;
+    r[1] = tb / d;
+    r[2] = 1.0 / d;
+}
+void a1_proto(double a, double b, double v[3]) {
+    const double ta = std::tan(a), tb = std::tan(b);
+    const double s = std::pow(1.0 + std::pow(ta, 2.0) + std::pow(tb, 2.0), 3.0 / 2.0);
+    v[0] = (1.0 + std::pow(ta, 2.0)) * (1.0 + std::pow(tb, 2.0)) / s;
+    v[1] = -ta * tb * (1.0 + std::pow(ta, 2.0)) / s;
+    v[2] = -ta * (1.0 + std::pow(ta, 2.0)) / s;
+}
+void a2_proto(double a, double b, double v[3]) {
+    const double ta = std::tan(a), tb = std::tan(b);
+    const double s = std::pow(1.0 + std::pow(ta, 2.0) + std::pow(tb, 2.0), 3.0 / 2.0);
+    v[0] = -ta * tb * (1.0 + std::pow(tb, 2.0)) / s;
+    v[1] = (1.0 + std::pow(ta, 2.0)) * (1.0 + std::pow(tb, 2.0)) / s;
+    v[2] = -tb * (1.0 + std::pow(tb, 2.0)) / s;
+}
+void b1_proto(double a, double b, double v[3]) {
+    const double ta = std::tan(a), tb = std::tan(b);
+    const double sg = std::sqrt(1.0 + ta * ta + tb * tb);
+    v[0] = (1.0 + tb * tb - (ta * tb) * (ta * tb) / (1.0 + ta * ta)) / sg;
+    v[1] = 0.0;
+    v[2] = -ta * (1.0 + tb * tb / (1 + ta * ta)) / sg;
+}
+void b2_proto(double a, double b, double v[3]) {
+    const double ta = std::tan(a), tb = std::tan(b);
+    const double sg = std::sqrt(1.0 + ta * ta + tb * tb);
+    v[0] = 0.0;
+    v[1] = ((1 + ta * ta) - (ta * tb) * (ta * tb) / (1 + tb * tb)) / sg;
+    v[2] = -tb * (ta * ta / (1 + tb * tb) + 1.0) / sg;
+}
+// prototype face -> panel p with the identity rotation (ecs_metric_mod.f90:303-324)
+void to_panel(int p, const double r[3], double out[3]) {
+    double t[3];
+    for (int k = 0; k < 3; ++k)
+        t[k] = (double)EX[p][k] * r[0] + (double)EY[p][k] * r[1] - (double)NV[p][k] * r[2];
+    for (int c = 0; c < 3; ++c) {
+        double s = (c == 0 ? 1.0 : 0.0) * t[0];
+        s = s + (c == 1 ? 1.0 : 0.0) * t[1];
+        s = s + (c == 2 ? 1.0 : 0.0) * t[2];
+        out[c] = s;
+    }
+}
+double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// integer lattice position of Ah point (i,j) (1-based, may lie outside 1..n) of panel p on
+// the cube surface of edge length n-1; panels sharing a point give the same lattice position.
+void lattice(int p, int n, int i, int j, int L[3]) {
+    for (int k = 0; k < 3; ++k) L[k] = (i - 1) * EX[p][k] + (j - 1) * EY[p][k] + (n - 1) * RV[p][k];
+}
+int idot(const int a[3], const int b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+}  // namespace
+
+extern "C" void swmh_geometry(int N, int pitch, double omega, const double axis[3], double *J, double *Qi11,
+                              double *Qi12, double *Qi22, double *f, double *r, double *a1, double *a2) {
+    const int n = N + 1;
+    const long plane = 6L * n * pitch;
+    for (int p = 0; p < 6; ++p)
+        for (int j = 1; j <= n; ++j) {
+            const double beta = grid_coord(N, j);
+            for (int i = 1; i <= n; ++i) {
+                const double alpha = grid_coord(N, i);
+                const long id = ((long)p * n + (j - 1)) * pitch + (i - 1);
+                const double ta = std::tan(alpha), tb = std::tan(beta);
+                const double sg = std::sqrt(1.0 + ta * ta + tb * tb);
+                if (J) J[id] = (1.0 + ta * ta) * (1.0 + tb * tb) / (sg * sg * sg);
+                if (Qi11) Qi11[id] = sg * sg / (1.0 + ta * ta);
+                if (Qi12) Qi12[id] = sg * sg * ta * tb / ((1 + ta * ta) * (1 + tb * tb));
+                if (Qi22) Qi22[id] = sg * sg / (1.0 + tb * tb);
+                double rp[3], rr[3], v[3], w[3];
+                r_proto(alpha, beta, rp);
+                to_panel(p, rp, rr);
+                if (r)
+                    for (int k = 0; k < 3; ++k) r[id + k * plane] = rr[k];
+                if (f) {
+                    const double pr = (axis[0] * rr[0] + axis[1] * rr[1] + axis[2] * rr[2]) /
+                                      std::sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+                    f[id] = 2 * omega * pr;
+                }
+                if (a1) {
+                    a1_proto(alpha, beta, v);
+                    to_panel(p, v, w);
+                    for (int k = 0; k < 3; ++k) a1[id + k * plane] = w[k];
+                }
+                if (a2) {
+                    a2_proto(alpha, beta, v);
+                    to_panel(p, v, w);
+                    for (int k = 0; k < 3; ++k) a2[id + k * plane] = w[k];
+                }
+            }
+        }
+}
+
+extern "C" void swmh_sync_weights(int N, double *qb, double *qt, double *ql, double *qr) {
+    const int n = N + 1;
+    const double pi = pi_();
+    const double hx = grid_hx(N);
+    double a[3], b[3], t[3];
+    for (int i = 1; i <= n; ++i) {
+        const double al = -0.25 * pi + (i - 1) * hx;
+        a2_proto(al, -0.25 * pi, a);
+        b1_proto(al, 0.25 * pi, t);
+        b[0] = t[0]; b[1] = -t[2]; b[2] = t[1];
+        qb[i - 1] = dot3(a, b);
+        a2_proto(al, 0.25 * pi, a);
+        b1_proto(al, -0.25 * pi, t);
+        b[0] = t[0]; b[1] = t[2]; b[2] = -t[1];
+        qt[i - 1] = dot3(a, b);
+    }
+    for (int j = 1; j <= n; ++j) {
+        const double be = -0.25 * pi + (j - 1) * hx;
+        a1_proto(-0.25 * pi, be, a);
+        b2_proto(0.25 * pi, be, t);
+        b[0] = -t[2]; b[1] = t[1]; b[2] = t[0];
+        ql[j - 1] = dot3(a, b);
+        a1_proto(0.25 * pi, be, a);
+        b2_proto(-0.25 * pi, be, t);
+        b[0] = t[2]; b[1] = t[1]; b[2] = -t[0];
+        qr[j - 1] = dot3(a, b);
+    }
+}
+
+extern "C" int swmh_quad_weights(int N, int order, double *A) {
+    const int n = N + 1;
+    int ne = 0;
+    double e[6] = {0, 0, 0, 0, 0, 0};
+    if (order == 21) { ne = 2; e[0] = 0.5; e[1] = 1.0; }
+    else if (order == 42) { ne = 4; e[0] = 17.0 / 48.0; e[1] = 59.0 / 48.0; e[2] = 43.0 / 48.0; e[3] = 49.0 / 48.0; }
+    else if (order == 63) {
+        ne = 6;
+        e[0] = 0.13649e5 / 0.43200e5; e[1] = 0.12013e5 / 0.8640e4; e[2] = 0.2711e4 / 0.4320e4;
+        e[3] = 0.5359e4 / 0.4320e4; e[4] = 0.7877e4 / 0.8640e4; e[5] = 0.43801e5 / 0.43200e5;
+    } else {
+        return -1;
+    }
+    for (int i = 0; i < n; ++i) A[i] = 1.0;
+    for (int i = 0; i < ne && i < n; ++i) A[i] = e[i];
+    for (int i = std::max(n - ne, 0); i < n; ++i) A[i] = e[n - 1 - i];
+    return 0;
+}
+
+// Neighbour table derived from the cube lattice: a halo point beyond an edge of panel p lies,
+// in the unfolded plane, d steps past the edge; on the cube it is the point d steps inside the
+// adjacent face, i.e. lattice(edge point) + d * n_p (n_p = inner normal of p).  The adjacent
+// face q across the i=1 edge has n_q = ex_p (i=n: -ex_p, j=1: ey_p, j=n: -ey_p).  Vector
+// components follow the index directions: the local +i / +j directions at the halo point are
+// lattice directions which coincide (up to sign) with q's ex or ey.
+extern "C" void swmh_halo_table(int N, int ndepth, int *tab) {
+    const int n = N + 1;
+    for (int p = 0; p < 6; ++p)
+        for (int e = 0; e < 4; ++e)
+            for (int k = 0; k < n; ++k)
+                for (int d = 0; d < ndepth; ++d) {
+                    int L[3], nq[3], di[3], dj[3];
+                    // edge point (1-based) and the lattice directions of local +i / +j there
+                    int ie = 0, je = 0;
+                    for (int c = 0; c < 3; ++c) {
+                        switch (e) {
+                        case 0: nq[c] = EX[p][c]; di[c] = -NV[p][c]; dj[c] = EY[p][c]; break;
+                        case 1: nq[c] = -EX[p][c]; di[c] = NV[p][c]; dj[c] = EY[p][c]; break;
+                        case 2: nq[c] = EY[p][c]; di[c] = EX[p][c]; dj[c] = -NV[p][c]; break;
+                        default: nq[c] = -EY[p][c]; di[c] = EX[p][c]; dj[c] = NV[p][c]; break;
+                        }
+                    }
+                    if (e == 0) { ie = 1; je = k + 1; }
+                    if (e == 1) { ie = n; je = k + 1; }
+                    if (e == 2) { ie = k + 1; je = 1; }
+                    if (e == 3) { ie = k + 1; je = n; }
+                    lattice(p, n, ie, je, L);
+                    for (int c = 0; c < 3; ++c) L[c] += d * NV[p][c];
+                    int q = -1;
+                    for (int s = 0; s < 6; ++s)
+                        if (NV[s][0] == nq[0] && NV[s][1] == nq[1] && NV[s][2] == nq[2]) q = s;
+                    int rel[3];
+                    for (int c = 0; c < 3; ++c) rel[c] = L[c] - (n - 1) * RV[q][c];
+                    const int iq = idot(rel, EX[q]), jq = idot(rel, EY[q]);
+                    // local axis +i maps to q's ex or ey
+                    const int swap = idot(di, EX[q]) == 0 ? 1 : 0;
+                    const int si = swap ? idot(di, EY[q]) : idot(di, EX[q]);
+                    const int sj = swap ? idot(dj, EX[q]) : idot(dj, EY[q]);
+                    int *t = tab + ((((long)p * 4 + e) * n + k) * ndepth + d) * 4;
+                    t[0] = q;
+                    t[1] = iq;
+                    t[2] = jq;
+                    t[3] = swap | (si < 0 ? 2 : 0) | (sj < 0 ? 4 : 0);
+                }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Seeded spherical-harmonic initial state
+namespace {
+
+struct Rng {
+    uint64_t s;
+    uint64_t next() {  // splitmix64
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    bool have = false;
+    double spare = 0.0;
+    double normal() {  // Box-Muller
+        if (have) { have = false; return spare; }
+        const double u1 = uniform(), u2 = uniform();
+        const double rad = std::sqrt(-2.0 * std::log(u1));
+        spare = rad * std::sin(2.0 * pi_() * u2);
+        have = true;
+        return rad * std::cos(2.0 * pi_() * u2);
+    }
+};
+
+// Fully normalised associated Legendre functions without the Condon-Shortley phase,
+// Pb[l][m] at mu = sin(phi); Qb[l][m] = Pb[l][m]/cos(phi) for m >= 1 (regular at the poles).
+struct Legendre {
+    int L;
+    std::vector<double> P, Q;  // (L+1)*(L+1)
+    explicit Legendre(int L_) : L(L_), P((L_ + 1) * (L_ + 1)), Q((L_ + 1) * (L_ + 1)) {}
+    double &p(int l, int m) { return P[l * (L + 1) + m]; }
+    double &q(int l, int m) { return Q[l * (L + 1) + m]; }
+    void eval(double mu, double c) {
+        // m = 0 column
+        p(0, 0) = 1.0;
+        if (L >= 1) p(1, 0) = std::sqrt(3.0) * mu;
+        for (int l = 2; l <= L; ++l) {
+            const double a = std::sqrt((4.0 * l * l - 1.0) / (double)(l * l));
+            const double b = std::sqrt(((l - 1.0) * (l - 1.0)) * (2.0 * l + 1.0) / ((2.0 * l - 3.0) * (double)(l * l)));
+            p(l, 0) = a * mu * p(l - 1, 0) - b * p(l - 2, 0);
+        }
+        // m >= 1 columns on Q = P / cos(phi)
+        double qmm = 0.0;
+        for (int m = 1; m <= L; ++m) {
+            qmm = (m == 1) ? std::sqrt(3.0) : std::sqrt((2.0 * m + 1.0) / (2.0 * m)) * c * qmm;
+            q(m, m) = qmm;
+            if (m + 1 <= L) q(m + 1, m) = std::sqrt(2.0 * m + 3.0) * mu * qmm;
+            for (int l = m + 2; l <= L; ++l) {
+                const double a = std::sqrt((4.0 * l * l - 1.0) / (double)(l * l - m * m));
+                const double b = std::sqrt(((l - 1.0) * (l - 1.0) - m * m) * (2.0 * l + 1.0) /
+                                           ((2.0 * l - 3.0) * (double)(l * l - m * m)));
+                q(l, m) = a * mu * q(l - 1, m) - b * q(l - 2, m);
+            }
+            for (int l = m; l <= L; ++l) p(l, m) = c * q(l, m);
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" int swmh_ic_sh(int N, int pitch, unsigned long long seed, int lmax, double h0, double h_amp,
+                          double v_max, double *h, double *u, double *v, int nthreads) {
+    if (N < 1 || lmax < 1 || pitch < N + 1) return -1;
+    const int n = N + 1;
+    const long plane = 6L * n * pitch;
+    const int K = (lmax + 1) * (lmax + 1);
+    std::vector<double> ch(2 * K, 0.0), cp(2 * K, 0.0);  // [l][m] cos, sin coefficients
+    Rng rng{seed * 0x2545F4914F6CDD1Dull + 0x1234567ull};
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<double> &c = pass == 0 ? ch : cp;
+        for (int l = 1; l <= lmax; ++l)
+            for (int m = 0; m <= l; ++m) {
+                const double amp = 1.0 / ((double)l * (double)l);
+                c[2 * (l * (lmax + 1) + m)] = rng.normal() * amp;
+                c[2 * (l * (lmax + 1) + m) + 1] = (m > 0) ? rng.normal() * amp : 0.0;
+            }
+    }
+    std::vector<double> r(3 * plane), a1(3 * plane), a2(3 * plane);
+    const double axis[3] = {0.0, 0.0, 1.0};
+    swmh_geometry(N, pitch, 0.0, axis, nullptr, nullptr, nullptr, nullptr, nullptr, r.data(), a1.data(), a2.data());
+    std::vector<double> hp(plane, 0.0), vx(plane, 0.0), vy(plane, 0.0), vz(plane, 0.0);
+    auto work = [&](int t0, int tstride) {
+        Legendre Lg(lmax);
+        std::vector<double> cm(lmax + 1), sm(lmax + 1);
+        for (long row = t0; row < 6L * n; row += tstride) {
+            for (int i = 0; i < n; ++i) {
+                const long id = row * pitch + i;
+                const double x = r[id], y = r[id + plane], z = r[id + 2 * plane];
+                const double rho = std::sqrt(x * x + y * y);
+                const double nrm = std::sqrt(rho * rho + z * z);
+                const double mu = z / nrm, c = rho / nrm;
+                const double lam = std::atan2(y, x);
+                Lg.eval(mu, c);
+                for (int m = 0; m <= lmax; ++m) { cm[m] = std::cos(m * lam); sm[m] = std::sin(m * lam); }
+                double hv = 0.0, dpsi_phi = 0.0, dpsi_lam_c = 0.0;
+                for (int l = 1; l <= lmax; ++l) {
+                    for (int m = 0; m <= l; ++m) {
+                        const int kk = 2 * (l * (lmax + 1) + m);
+                        const double cosm = cm[m], sinm = sm[m];
+                        hv += Lg.p(l, m) * (ch[kk] * cosm + ch[kk + 1] * sinm);
+                        const double ap = cp[kk], bp = cp[kk + 1];
+                        double dphi;
+                        if (m == 0) {
+                            dphi = std::sqrt(l * (l + 1.0) / 2.0) * Lg.p(l, 1);
+                        } else {
+                            const double e = std::sqrt((2.0 * l + 1.0) * (double)(l * l - m * m) / (2.0 * l - 1.0));
+                            dphi = -l * mu * Lg.q(l, m) + (l - 1 >= m ? e * Lg.q(l - 1, m) : 0.0);
+                        }
+                        dpsi_phi += dphi * (ap * cosm + bp * sinm);
+                        if (m > 0) dpsi_lam_c += Lg.q(l, m) * m * (-ap * sinm + bp * cosm);
+                    }
+                }
+                // V = r x grad psi = -dpsi/dphi e_lam + (1/cos phi) dpsi/dlam e_phi
+                const double sl = std::sin(lam), cl = std::cos(lam);
+                const double el[3] = {-sl, cl, 0.0};
+                const double ep[3] = {-mu * cl, -mu * sl, c};
+                hp[id] = hv;
+                vx[id] = -dpsi_phi * el[0] + dpsi_lam_c * ep[0];
+                vy[id] = -dpsi_phi * el[1] + dpsi_lam_c * ep[1];
+                vz[id] = -dpsi_phi * el[2] + dpsi_lam_c * ep[2];
+            }
+        }
+    };
+    const int nt = std::max(1, nthreads);
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, t, nt);
+    work(0, nt);
+    for (auto &t : th) t.join();
+    double hmax = 0.0, vmax = 0.0;
+    for (long row = 0; row < 6L * n; ++row)
+        for (int i = 0; i < n; ++i) {
+            const long id = row * pitch + i;
+            hmax = std::max(hmax, std::fabs(hp[id]));
+            vmax = std::max(vmax, std::sqrt(vx[id] * vx[id] + vy[id] * vy[id] + vz[id] * vz[id]));
+        }
+    const double sh = hmax > 0 ? h_amp / hmax : 0.0, sv = vmax > 0 ? v_max / vmax : 0.0;
+    for (long row = 0; row < 6L * n; ++row)
+        for (int i = 0; i < n; ++i) {
+            const long id = row * pitch + i;
+            const double Vx = vx[id] * sv, Vy = vy[id] * sv, Vz = vz[id] * sv;
+            if (h) h[id] = h0 + hp[id] * sh;
+            // covariant projection sum([vx,vy,vz]*a) (test_fields_mod.f90:428)
+            if (u) u[id] = Vx * a1[id] + Vy * a1[id + plane] + Vz * a1[id + 2 * plane];
+            if (v) v[id] = Vx * a2[id] + Vy * a2[id + plane] + Vz * a2[id + 2 * plane];
+        }
+    return 0;
+}

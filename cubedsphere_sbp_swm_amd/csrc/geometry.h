@@ -1,0 +1,44 @@
+// Host-side FP64 set-up of the equiangular cubed sphere for the device path:
+// metric terms, Coriolis parameter, edge-sync weights, quadrature weights,
+// cube-edge neighbour tables and the seeded spherical-harmonic initial state.
+// Compiled by the host compiler with -ffp-contract=off.
+#pragma once
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Layout of every per-point array: [panel 0..5][j 0..n-1][i 0..pitch-1], n = N+1.
+
+// Metric terms at Ah points of the unit sphere (ecs_metric_mod.f90:303-478) and the
+// Coriolis parameter f = 2*omega*(axis.r)/|r| (coriolis_factory_mod.f90:151-172).
+// Any output pointer may be NULL.
+void swmh_geometry(int N, int pitch, double omega, const double axis[3],
+                   double *J, double *Qi11, double *Qi12, double *Qi22, double *f,
+                   double *r /*3 planes*/, double *a1 /*3 planes*/, double *a2 /*3 planes*/);
+
+// Covariant Ah vector-sync weights qb, qt, ql, qr (n values each)
+// (ecs_Ah_vec_sync_factory_mod.f90:101-149).
+void swmh_sync_weights(int N, double *qb, double *qt, double *ql, double *qr);
+
+// SBP quadrature weights A(1..n) for order 21/42/63 (quadrature_factory_mod.f90:158-177).
+int swmh_quad_weights(int N, int order, double *A);
+
+// Cube-edge neighbour table.  For panel p, edge e (0: i=1 side, 1: i=n side, 2: j=1 side,
+// 3: j=n side), position k (0..n-1 along the edge) and halo depth d (0: the neighbour's copy
+// of the shared edge point, 1: its first interior line), entry
+//   tab[(((p*4+e)*n+k)*ndepth+d)*4 + {0,1,2,3}] = {q, iq, jq, code}
+// with 0-based (iq,jq) on panel q and code bit 0 = components swapped, bit 1 = first
+// component negated, bit 2 = second component negated (vector halo convention).
+void swmh_halo_table(int N, int ndepth, int *tab);
+
+// Seeded initial state: h = h0 + h' (max|h'| = h_amp), V = r x grad(psi) (max|V| = v_max),
+// h', psi random real spherical-harmonic series of degree 1..lmax with N(0,1)*l^-2
+// coefficients drawn from a splitmix64/Box-Muller stream of `seed`; covariant components
+// u = V.a1, v = V.a2 (test_fields_mod.f90:397-433).  Output arrays in device layout.
+int swmh_ic_sh(int N, int pitch, unsigned long long seed, int lmax, double h0, double h_amp,
+               double v_max, double *h, double *u, double *v, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif

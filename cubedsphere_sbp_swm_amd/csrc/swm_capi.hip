@@ -1,0 +1,505 @@
+// C ABI of the device path: context set-up (geometry, neighbour tables, work buffers),
+// kernel launches, HIP-graph captured RK4 stepping and FP64 diagnostics.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/swm_gpu.h"
+#include "geometry.h"
+#include "swm_stage.hip"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(SWM_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kTX = 64, kTY = 8, kRPT = 2;
+constexpr int NDEPTH = 2;
+
+}  // namespace
+
+struct swm_ctx {
+    int N = 0, n = 0, pitch = 0;
+    long long fs = 0;
+    int grad_order = 42, div_kind = 42, quad_order = 42, prec = 64;
+    double scale = 1, omega = 1, grav = 1, hx = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // device geometry in the context precision
+    void *J = nullptr, *Qi11 = nullptr, *Qi12 = nullptr, *Qi22 = nullptr, *f = nullptr;
+    void *q[4] = {nullptr, nullptr, nullptr, nullptr};
+    int4 *nbr = nullptr;
+    void *hs = nullptr;
+    // FP64 copies used by the diagnostics
+    double *J64 = nullptr, *Qi64[3] = {nullptr, nullptr, nullptr}, *f64 = nullptr, *A64 = nullptr;
+    double *hs64 = nullptr;
+    // RK4 work buffers (3 fields each) and diagnostics scratch
+    void *yA = nullptr, *yB = nullptr, *acc = nullptr, *curl = nullptr;
+    double *red = nullptr;
+    int red_blocks = 0;
+    // captured RK4 step
+    hipGraphExec_t graph = nullptr;
+    void *graph_state = nullptr;
+    double graph_dt = 0.0;
+    std::vector<int> host_nbr;
+
+    size_t esz() const { return prec == 64 ? 8 : 4; }
+    size_t field_bytes() const { return (size_t)fs * esz(); }
+};
+
+namespace {
+
+template <typename T>
+swm::DevGeo<T> make_geo(const swm_ctx *c) {
+    swm::DevGeo<T> g;
+    g.J = (const T *)c->J;
+    g.Qi11 = (const T *)c->Qi11;
+    g.Qi12 = (const T *)c->Qi12;
+    g.Qi22 = (const T *)c->Qi22;
+    g.f = (const T *)c->f;
+    g.qb = (const T *)c->q[0];
+    g.qt = (const T *)c->q[1];
+    g.ql = (const T *)c->q[2];
+    g.qr = (const T *)c->q[3];
+    g.nbr = c->nbr;
+    g.n = c->n;
+    g.pitch = c->pitch;
+    g.ndepth = NDEPTH;
+    g.fs = c->fs;
+    g.hxs = (T)(c->hx * c->scale);
+    g.hx = (T)c->hx;
+    g.scale = (T)c->scale;
+    g.grav = (T)c->grav;
+    return g;
+}
+
+template <int ORD, typename T, int MODE>
+int launch_stage_t(swm_ctx *c, swm::StageArgs<T> a) {
+    using C = swm::TileCfg<ORD, T, kTX, kTY, kRPT>;
+    const int ntx = (c->n + kTX - 1) / kTX, nty = (c->n + kTY - 1) / kTY;
+    const int nblocks = 6 * ntx * nty;
+    hipLaunchKernelGGL((swm::stage_kernel<ORD, T, MODE, kTX, kTY, kRPT>), dim3(nblocks), dim3(C::THREADS), 0, c->stream,
+                       make_geo<T>(c), a);
+    HIP_TRY(hipGetLastError());
+    return SWM_OK;
+}
+
+template <typename T, int MODE>
+int launch_stage(swm_ctx *c, swm::StageArgs<T> a) {
+    switch (c->grad_order) {
+    case 42: return launch_stage_t<42, T, MODE>(c, a);
+    case 43: return launch_stage_t<43, T, MODE>(c, a);
+    case 63: return launch_stage_t<63, T, MODE>(c, a);
+    default: return fail(SWM_ENAME, "operator set not available on the device path");
+    }
+}
+
+template <typename T>
+int stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, void *acc, double cy, double cacc) {
+    swm::StageArgs<T> a;
+    a.y = (const T *)y;
+    a.out = (T *)out;
+    a.v0 = (const T *)v0;
+    a.acc = (T *)acc;
+    a.hs = (const T *)c->hs;
+    a.cy = (T)cy;
+    a.cacc = (T)cacc;
+    switch (mode) {
+    case swm::MODE_RHS: return launch_stage<T, swm::MODE_RHS>(c, a);
+    case swm::MODE_STAGE1: return launch_stage<T, swm::MODE_STAGE1>(c, a);
+    case swm::MODE_STAGE23: return launch_stage<T, swm::MODE_STAGE23>(c, a);
+    case swm::MODE_STAGE4: return launch_stage<T, swm::MODE_STAGE4>(c, a);
+    case swm::MODE_CURL: return launch_stage<T, swm::MODE_CURL>(c, a);
+    }
+    return fail(SWM_EARG, "bad stage mode");
+}
+
+int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, void *acc, double cy, double cacc) {
+    return c->prec == 64 ? stage<double>(c, mode, y, out, v0, acc, cy, cacc)
+                         : stage<float>(c, mode, y, out, v0, acc, cy, cacc);
+}
+
+// one RK4 step = 4 fused stage launches (rk4_mod.f90:28-40)
+int enqueue_rk4_step(swm_ctx *c, void *state, double dt) {
+    const double c1 = 0.5 * dt, c2 = dt / 6.0, c3 = dt / 3.0;
+    int rc;
+    if ((rc = run_stage(c, swm::MODE_STAGE1, state, c->yA, nullptr, c->acc, c1, c2))) return rc;
+    if ((rc = run_stage(c, swm::MODE_STAGE23, c->yA, c->yB, state, c->acc, c1, c3))) return rc;
+    if ((rc = run_stage(c, swm::MODE_STAGE23, c->yB, c->yA, state, c->acc, dt, c3))) return rc;
+    if ((rc = run_stage(c, swm::MODE_STAGE4, c->yA, state, nullptr, c->acc, c2, 0.0))) return rc;
+    return SWM_OK;
+}
+
+template <typename T>
+int upload_plane(swm_ctx *c, void *dst, const double *src, int hw) {
+    const int n = c->n, ld = n + 2 * hw;
+    std::vector<T> buf((size_t)c->fs, T(0));
+    for (int p = 0; p < 6; ++p)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                buf[((size_t)p * n + j) * c->pitch + i] = (T)src[((size_t)p * ld + (j + hw)) * ld + (i + hw)];
+    HIP_TRY(hipMemcpyAsync(dst, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SWM_OK;
+}
+
+template <typename T>
+int download_plane(swm_ctx *c, const void *src, double *dst, int hw) {
+    const int n = c->n, ld = n + 2 * hw;
+    std::vector<T> buf((size_t)c->fs);
+    HIP_TRY(hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int p = 0; p < 6; ++p)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                dst[((size_t)p * ld + (j + hw)) * ld + (i + hw)] = (double)buf[((size_t)p * n + j) * c->pitch + i];
+    return SWM_OK;
+}
+
+template <typename T>
+int upload_device_plane(swm_ctx *c, void **dst, const std::vector<double> &src, size_t count) {
+    std::vector<T> buf(count);
+    for (size_t k = 0; k < count; ++k) buf[k] = (T)src[k];
+    HIP_TRY(hipMalloc(dst, count * sizeof(T)));
+    HIP_TRY(hipMemcpy(*dst, buf.data(), count * sizeof(T), hipMemcpyHostToDevice));
+    return SWM_OK;
+}
+
+int upload_f64(double **dst, const std::vector<double> &src) {
+    HIP_TRY(hipMalloc((void **)dst, src.size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+    return SWM_OK;
+}
+
+int parse_names(swm_ctx *c, const char *div, const char *grad, const char *curl, const char *quad) {
+    std::string d = div ? div : "", gr = grad ? grad : "", cu = curl ? curl : "", q = quad ? quad : "";
+    int dk = 0, go = 0, qo = 0;
+    if (d == "divergence_ah42_sbp") dk = 42;
+    else if (d == "divergence_ah43_sbp") dk = 43;
+    else if (d == "divergence_ah63_sbp") dk = 63;
+    else if (d == "divergence_ah2") dk = 2;
+    else return fail(SWM_ENAME, "unknown divergence operator: " + d);
+    if (gr == "gradient_ah42_sbp_ecs") go = 42;
+    else if (gr == "gradient_ah43_sbp_ecs") go = 43;
+    else if (gr == "gradient_ah63_sbp_ecs") go = 63;
+    else if (gr == "gradient_ah21_sbp_ecs") go = 21;
+    else return fail(SWM_ENAME, "unknown gradient operator: " + gr);
+    // curl_factory_mod.f90:17-18: "curl_div..." names wrap the named divergence
+    if (cu.rfind("curl_", 0) != 0 || cu.substr(5) != d)
+        return fail(SWM_ENAME, "curl operator must be curl_ + divergence name: " + cu);
+    if (q == "SBP_Ah21_quadrature") qo = 21;
+    else if (q == "SBP_Ah42_quadrature") qo = 42;
+    else if (q == "SBP_Ah63_quadrature") qo = 63;
+    else return fail(SWM_ENAME, "unknown quadrature: " + q);
+    // the fused stage kernel pairs gradient and divergence of the same SBP family
+    const bool ok = (dk == go) || (dk == 2 && go == 21);
+    if (!ok) return fail(SWM_ENAME, "unsupported gradient/divergence pair: " + gr + " + " + d);
+    if (dk == 2) return fail(SWM_ENAME, "divergence_ah2 (sbp21 family) is not built on the device path yet");
+    c->div_kind = dk;
+    c->grad_order = go;
+    c->quad_order = qo;
+    return SWM_OK;
+}
+
+void free_ctx(swm_ctx *c) {
+    if (!c) return;
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    void *ptrs[] = {c->J, c->Qi11, c->Qi12, c->Qi22, c->f, c->q[0], c->q[1], c->q[2], c->q[3], c->nbr, c->hs,
+                    c->J64, c->Qi64[0], c->Qi64[1], c->Qi64[2], c->f64, c->A64, c->hs64, c->yA, c->yB, c->acc,
+                    c->curl, c->red};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+extern "C" const char *swm_last_error(void) { return g_err.c_str(); }
+
+extern "C" int swm_ctx_create(swm_ctx **out, int N, const char *div, const char *grad, const char *curl,
+                              const char *quad, double scale, double omega, double grav, int precision_bits) {
+    if (!out) return fail(SWM_EARG, "ctx pointer is NULL");
+    *out = nullptr;
+    if (N < 8 || N > 8192) return fail(SWM_EARG, "N must be in [8, 8192]");
+    if (precision_bits != 64 && precision_bits != 32) return fail(SWM_EARG, "precision_bits must be 64 or 32");
+    if (!(scale > 0) || !(grav > 0)) return fail(SWM_EARG, "scale and grav must be positive");
+    swm_ctx *c = new swm_ctx();
+    int rc = parse_names(c, div, grad, curl, quad);
+    if (rc) { delete c; return rc; }
+    c->N = N;
+    c->n = N + 1;
+    c->pitch = (c->n + 7) / 8 * 8;
+    c->fs = 6LL * c->n * c->pitch;
+    c->prec = precision_bits;
+    c->scale = scale;
+    c->omega = omega;
+    c->grav = grav;
+    c->hx = (0.25 * std::acos(-1.0) - (-0.25 * std::acos(-1.0))) / (double)N;
+    auto bail = [&](int code) { free_ctx(c); return code; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SWM_EHIP, "hipStreamCreate failed"));
+    c->own_stream = true;
+
+    const size_t F = (size_t)c->fs;
+    std::vector<double> J(F, 0.0), Qi11(F, 0.0), Qi12(F, 0.0), Qi22(F, 0.0), fc(F, 0.0);
+    const double axis[3] = {0.0, 0.0, 1.0};
+    swmh_geometry(N, c->pitch, omega, axis, J.data(), Qi11.data(), Qi12.data(), Qi22.data(), fc.data(), nullptr,
+                  nullptr, nullptr);
+    std::vector<double> qw[4];
+    for (auto &v : qw) v.assign(c->n, 0.0);
+    swmh_sync_weights(N, qw[0].data(), qw[1].data(), qw[2].data(), qw[3].data());
+    std::vector<double> A(c->n);
+    swmh_quad_weights(N, c->quad_order, A.data());
+    c->host_nbr.assign((size_t)6 * 4 * c->n * NDEPTH * 4, 0);
+    swmh_halo_table(N, NDEPTH, c->host_nbr.data());
+
+    if (c->prec == 64) {
+        if ((rc = upload_device_plane<double>(c, &c->J, J, F)) || (rc = upload_device_plane<double>(c, &c->Qi11, Qi11, F)) ||
+            (rc = upload_device_plane<double>(c, &c->Qi12, Qi12, F)) || (rc = upload_device_plane<double>(c, &c->Qi22, Qi22, F)) ||
+            (rc = upload_device_plane<double>(c, &c->f, fc, F)))
+            return bail(rc);
+        for (int k = 0; k < 4; ++k)
+            if ((rc = upload_device_plane<double>(c, &c->q[k], qw[k], c->n))) return bail(rc);
+    } else {
+        if ((rc = upload_device_plane<float>(c, &c->J, J, F)) || (rc = upload_device_plane<float>(c, &c->Qi11, Qi11, F)) ||
+            (rc = upload_device_plane<float>(c, &c->Qi12, Qi12, F)) || (rc = upload_device_plane<float>(c, &c->Qi22, Qi22, F)) ||
+            (rc = upload_device_plane<float>(c, &c->f, fc, F)))
+            return bail(rc);
+        for (int k = 0; k < 4; ++k)
+            if ((rc = upload_device_plane<float>(c, &c->q[k], qw[k], c->n))) return bail(rc);
+    }
+    if ((rc = upload_f64(&c->J64, J)) || (rc = upload_f64(&c->Qi64[0], Qi11)) || (rc = upload_f64(&c->Qi64[1], Qi12)) ||
+        (rc = upload_f64(&c->Qi64[2], Qi22)) || (rc = upload_f64(&c->f64, fc)) || (rc = upload_f64(&c->A64, A)))
+        return bail(rc);
+    if (hipMalloc((void **)&c->nbr, c->host_nbr.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpy(c->nbr, c->host_nbr.data(), c->host_nbr.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SWM_EHIP, "neighbour table upload failed"));
+    const size_t sb = 3 * c->field_bytes();
+    if (hipMalloc(&c->yA, sb) != hipSuccess || hipMalloc(&c->yB, sb) != hipSuccess || hipMalloc(&c->acc, sb) != hipSuccess ||
+        hipMalloc(&c->curl, c->field_bytes()) != hipSuccess)
+        return bail(fail(SWM_EHIP, "work buffer allocation failed"));
+    // zero the padding columns once so that every buffer holds defined values
+    if (hipMemset(c->yA, 0, sb) != hipSuccess || hipMemset(c->yB, 0, sb) != hipSuccess ||
+        hipMemset(c->acc, 0, sb) != hipSuccess || hipMemset(c->curl, 0, c->field_bytes()) != hipSuccess)
+        return bail(fail(SWM_EHIP, "work buffer clear failed"));
+    c->red_blocks = 1024;
+    if (hipMalloc((void **)&c->red, sizeof(double) * 16 * (c->red_blocks + 1)) != hipSuccess)
+        return bail(fail(SWM_EHIP, "reduction scratch allocation failed"));
+    *out = c;
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_destroy(swm_ctx *c) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_info(const swm_ctx *c, int *n, int *pitch, long long *fe, int *eb) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (n) *n = c->n;
+    if (pitch) *pitch = c->pitch;
+    if (fe) *fe = c->fs;
+    if (eb) *eb = (int)c->esz();
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_set_stream(swm_ctx *c, void *s) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    if (s) {
+        if (c->own_stream) { (void)hipStreamDestroy(c->stream); c->own_stream = false; }
+        c->stream = (hipStream_t)s;
+    } else if (!c->own_stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_synchronize(swm_ctx *c) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SWM_OK;
+}
+
+extern "C" int swm_state_alloc(swm_ctx *c, void **state) {
+    if (!c || !state) return fail(SWM_EARG, "NULL argument");
+    const size_t sb = 3 * c->field_bytes();
+    HIP_TRY(hipMalloc(state, sb));
+    HIP_TRY(hipMemset(*state, 0, sb));
+    return SWM_OK;
+}
+
+extern "C" int swm_state_free(swm_ctx *c, void *state) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (!state) return SWM_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->graph_state == state && c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; c->graph_state = nullptr; }
+    HIP_TRY(hipFree(state));
+    return SWM_OK;
+}
+
+extern "C" int swm_state_upload(swm_ctx *c, void *state, const double *h, const double *u, const double *v, int hw) {
+    if (!c || !state || !h || !u || !v) return fail(SWM_EARG, "NULL argument");
+    if (hw < 0 || hw > 64) return fail(SWM_EARG, "halo_width out of range");
+    const double *src[3] = {h, u, v};
+    for (int k = 0; k < 3; ++k) {
+        char *dst = (char *)state + k * c->field_bytes();
+        int rc = c->prec == 64 ? upload_plane<double>(c, dst, src[k], hw) : upload_plane<float>(c, dst, src[k], hw);
+        if (rc) return rc;
+    }
+    return SWM_OK;
+}
+
+extern "C" int swm_state_download(swm_ctx *c, const void *state, double *h, double *u, double *v, int hw) {
+    if (!c || !state || !h || !u || !v) return fail(SWM_EARG, "NULL argument");
+    if (hw < 0 || hw > 64) return fail(SWM_EARG, "halo_width out of range");
+    double *dst[3] = {h, u, v};
+    for (int k = 0; k < 3; ++k) {
+        const char *src = (const char *)state + k * c->field_bytes();
+        int rc = c->prec == 64 ? download_plane<double>(c, src, dst[k], hw) : download_plane<float>(c, src, dst[k], hw);
+        if (rc) return rc;
+    }
+    return SWM_OK;
+}
+
+extern "C" int swm_rhs(swm_ctx *c, const void *vin, void *vout) {
+    if (!c || !vin || !vout) return fail(SWM_EARG, "NULL argument");
+    if (vin == vout) return fail(SWM_EARG, "vin and vout must differ");
+    return run_stage(c, swm::MODE_RHS, vin, vout, nullptr, nullptr, 0.0, 0.0);
+}
+
+template <typename T>
+static int axpby(swm_ctx *c, bool update, void *out, double s1, const void *v1, double s2, const void *v2) {
+    const int threads = 256, blocks = 2048;
+    if (update)
+        hipLaunchKernelGGL((swm::axpby_kernel<T, true>), dim3(blocks), dim3(threads), 0, c->stream, (T *)out, (T)s1,
+                           (const T *)v1, (T)s2, (const T *)v2, c->n, c->pitch, c->fs);
+    else
+        hipLaunchKernelGGL((swm::axpby_kernel<T, false>), dim3(blocks), dim3(threads), 0, c->stream, (T *)out, (T)s1,
+                           (const T *)v1, (T)s2, (const T *)v2, c->n, c->pitch, c->fs);
+    HIP_TRY(hipGetLastError());
+    return SWM_OK;
+}
+
+extern "C" int swm_assign(swm_ctx *c, void *out, double s1, const void *v1, double s2, const void *v2) {
+    if (!c || !out || !v1 || !v2) return fail(SWM_EARG, "NULL argument");
+    return c->prec == 64 ? axpby<double>(c, false, out, s1, v1, s2, v2) : axpby<float>(c, false, out, s1, v1, s2, v2);
+}
+
+extern "C" int swm_update(swm_ctx *c, void *out, double s1, const void *v1, double s2, const void *v2) {
+    if (!c || !out || !v1 || !v2) return fail(SWM_EARG, "NULL argument");
+    return c->prec == 64 ? axpby<double>(c, true, out, s1, v1, s2, v2) : axpby<float>(c, true, out, s1, v1, s2, v2);
+}
+
+extern "C" int swm_rk4_steps(swm_ctx *c, void *state, double dt, int nsteps) {
+    if (!c || !state) return fail(SWM_EARG, "NULL argument");
+    if (nsteps < 0) return fail(SWM_EARG, "nsteps must be >= 0");
+    if (nsteps == 0) return SWM_OK;
+    if (!c->graph || c->graph_state != state || c->graph_dt != dt) {
+        if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_rk4_step(c, state, dt);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(SWM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) { c->graph = nullptr; return fail(SWM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+        c->graph_state = state;
+        c->graph_dt = dt;
+    }
+    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    return SWM_OK;
+}
+
+// Measurement helper: `reps` back-to-back launches of one fused stage kernel on the context
+// stream (stage 1, 2 (= 2 and 3) or 4) reading `state`; results go to the work buffers.
+extern "C" int swm_bench_stage(swm_ctx *c, void *state, int stage_id, int reps) {
+    if (!c || !state || reps < 0) return fail(SWM_EARG, "bad argument");
+    for (int r = 0; r < reps; ++r) {
+        int rc;
+        if (stage_id == 1) rc = run_stage(c, swm::MODE_STAGE1, state, c->yA, nullptr, c->acc, 1.0, 1.0);
+        else if (stage_id == 2) rc = run_stage(c, swm::MODE_STAGE23, state, c->yB, state, c->acc, 1.0, 0.0);
+        else if (stage_id == 4) rc = run_stage(c, swm::MODE_STAGE4, state, c->yB, nullptr, c->acc, 1.0, 0.0);
+        else return fail(SWM_EARG, "stage_id must be 1, 2 or 4");
+        if (rc) return rc;
+    }
+    return SWM_OK;
+}
+
+extern "C" int swm_set_orography(swm_ctx *c, const double *hs, int hw) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    if (c->hs) { HIP_TRY(hipFree(c->hs)); c->hs = nullptr; }
+    if (c->hs64) { HIP_TRY(hipFree(c->hs64)); c->hs64 = nullptr; }
+    if (!hs) return SWM_OK;
+    HIP_TRY(hipMalloc(&c->hs, c->field_bytes()));
+    HIP_TRY(hipMalloc((void **)&c->hs64, (size_t)c->fs * sizeof(double)));
+    int rc = c->prec == 64 ? upload_plane<double>(c, c->hs, hs, hw) : upload_plane<float>(c, c->hs, hs, hw);
+    if (rc) return rc;
+    return upload_plane<double>(c, c->hs64, hs, hw);
+}
+
+extern "C" int swm_ic_spherical_harmonics(swm_ctx *c, unsigned long long seed, int lmax, double h0, double h_amp,
+                                          double v_max, double *h, double *u, double *v, int hw) {
+    if (!c || !h || !u || !v) return fail(SWM_EARG, "NULL argument");
+    if (lmax < 1 || lmax > 512) return fail(SWM_EARG, "lmax out of range");
+    const int n = c->n, ld = n + 2 * hw;
+    std::vector<double> H((size_t)c->fs), U((size_t)c->fs), V((size_t)c->fs);
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > 16) nt = 16;
+    if (swmh_ic_sh(c->N, c->pitch, seed, lmax, h0, h_amp, v_max, H.data(), U.data(), V.data(), (int)nt))
+        return fail(SWM_EARG, "initial-state generator failed");
+    for (int p = 0; p < 6; ++p)
+        for (int j = 0; j < ld; ++j)
+            for (int i = 0; i < ld; ++i) {
+                const size_t o = ((size_t)p * ld + j) * ld + i;
+                const int jj = j - hw, ii = i - hw;
+                const bool in = ii >= 0 && ii < n && jj >= 0 && jj < n;
+                const size_t s = in ? ((size_t)p * n + jj) * c->pitch + ii : 0;
+                h[o] = in ? H[s] : 0.0;
+                u[o] = in ? U[s] : 0.0;
+                v[o] = in ? V[s] : 0.0;
+            }
+    return SWM_OK;
+}
+
+extern "C" int swm_debug_geometry(swm_ctx *c, int which, double *out, int hw) {
+    if (!c || !out) return fail(SWM_EARG, "NULL argument");
+    void *src = which == 0 ? c->J : which == 1 ? c->Qi11 : which == 2 ? c->Qi12 : which == 3 ? c->Qi22 : which == 4 ? c->f : nullptr;
+    if (!src) return fail(SWM_EARG, "which out of range");
+    return c->prec == 64 ? download_plane<double>(c, src, out, hw) : download_plane<float>(c, src, out, hw);
+}
+
+extern "C" int swm_debug_halo_table(swm_ctx *c, int *table) {
+    if (!c || !table) return fail(SWM_EARG, "NULL argument");
+    std::memcpy(table, c->host_nbr.data(), c->host_nbr.size() * sizeof(int));
+    return SWM_OK;
+}
+
+#include "swm_diag.hip"

@@ -1,0 +1,184 @@
+// Device kernels of the Ah-grid vector-invariant shallow-water step (gfx950).
+//
+// One launch per RK stage evaluates the whole right-hand side of
+// operator_swm_t%apply (models/shallow_water/operator/operator_swm_mod.f90:61-113) for a
+// tile of points and applies the RK4 stage update (time_schemes/rk4_mod.f90:20-42):
+//   * the five stencil inputs K = (u.ut+v.vt)/2 + g h + g hs, J*hu, J*hv, J*(v/J), J*(-u/J)
+//     are formed once per point in LDS from the state and the metric;
+//   * SBP derivative rows (interior stencil or boundary closure) are applied along x and y;
+//   * at cube edges the neighbour panel's one-sided derivative values at the shared point
+//     are recomputed from global memory and combined with the reference's scalar and
+//     covariant-vector edge averaging (halo_Ah_scalar_sync_mod.f90:52-93,
+//     ecs_Ah_vec_sync_mod.f90:77-124) -- this replaces the MPI halo exchange;
+//   * the Coriolis term, tendencies and the RK stage combination finish in registers.
+// FP64 builds use -ffp-contract=off and the reference's operation order, so results agree
+// bit for bit with an FP64 CPU evaluation of the same algorithm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sbp_tables.h"
+
+namespace swm {
+
+// ----------------------------------------------------------------------------------------
+// Geometry and per-launch arguments
+template <typename T>
+struct DevGeo {
+    const T *J, *Qi11, *Qi12, *Qi22, *f;   // [panel][j][i], row pitch `pitch`
+    const T *qb, *qt, *ql, *qr;            // covariant sync weights, n values each
+    const int4 *nbr;                       // neighbour table [6][4][n][ndepth]
+    int n, pitch, ndepth;
+    long long fs;                          // elements per field (6*n*pitch)
+    T hxs;                                 // hx*scale
+    T hx, scale, grav;
+};
+
+enum StageMode : int {
+    MODE_RHS = 0,     // out = RHS(y)
+    MODE_STAGE1 = 1,  // out = y + cy*k ; acc = y + cacc*k          (y is the step's v0)
+    MODE_STAGE23 = 2, // out = v0 + cy*k ; acc = acc + cacc*k
+    MODE_STAGE4 = 3,  // out = acc + cy*k                           (out is v0)
+    MODE_CURL = 4     // out(h plane) = synced relative vorticity of y
+};
+
+template <typename T>
+struct StageArgs {
+    const T *y;    // input state: h at y, u at y+fs, v at y+2fs
+    T *out;
+    const T *v0;
+    T *acc;
+    const T *hs;   // orography, may be null
+    T cy, cacc;
+};
+
+// ----------------------------------------------------------------------------------------
+// Pointwise stencil inputs of one point (co2contra_colocated_mod.f90:55-56,
+// massflux_colocated_mod.f90:26-27, KE_colocated_mod.f90:44, operator_swm_mod.f90:85,
+// div_ah_sbp_mod.f90:80-91, curl_div_based_mod.f90:58-59)
+template <typename T>
+struct Derived {
+    T K, Gxd, Gyd, Gxc, Gyc;
+};
+
+template <typename T>
+__device__ __forceinline__ Derived<T> make_derived(T h, T u, T v, T J, T q11, T q12, T q22, T hsv, T grav) {
+    Derived<T> d;
+    const T ut = q11 * u + q12 * v;
+    const T vt = q12 * u + q22 * v;
+    const T hu = h * ut;
+    const T hv = h * vt;
+    T K = (u * ut + v * vt) / T(2);
+    K = K + grav * h + grav * hsv;
+    d.K = K;
+    d.Gxd = J * hu;
+    d.Gyd = J * hv;
+    d.Gxc = J * (v / J);
+    d.Gyc = J * (-u / J);
+    return d;
+}
+
+template <typename T>
+__device__ __forceinline__ long long pidx(const DevGeo<T> &g, int p, int i, int j) {
+    return ((long long)p * g.n + j) * g.pitch + i;
+}
+
+template <typename T>
+__device__ __forceinline__ Derived<T> derived_global(const DevGeo<T> &g, const T *y, const T *hs, int p, int i, int j) {
+    const long long id = pidx(g, p, i, j);
+    return make_derived<T>(y[id], y[id + g.fs], y[id + 2 * g.fs], g.J[id], g.Qi11[id], g.Qi12[id], g.Qi22[id],
+                           hs ? hs[id] : T(0), g.grav);
+}
+
+// ----------------------------------------------------------------------------------------
+// SBP rows (sbp_operator_mod.f90:187-223).  `get(a)` returns the input at 0-based index a
+// along the line; t is the 0-based output index, n the line length.
+template <int ORD, int R, typename T, typename G>
+__device__ __forceinline__ T sbp_left_row(const G &get) {
+    using S = Sbp<ORD>;
+    T s = T(S::Ql[R][0]) * get(0);
+#pragma unroll
+    for (int m = 1; m < S::last[R]; ++m) s = s + T(S::Ql[R][m]) * get(m);
+    return s;
+}
+// right-boundary row ir (0-based): the left row RR = NE-1-ir reflected, sign flipped
+template <int ORD, int IR, typename T, typename G>
+__device__ __forceinline__ T sbp_right_row(int n, const G &get) {
+    using S = Sbp<ORD>;
+    constexpr int RR = S::NE - 1 - IR;
+    constexpr int LAST = S::last[RR];
+    const int base = n - LAST;
+    T s = T(-1.0 * S::Ql[RR][LAST - 1]) * get(base);
+#pragma unroll
+    for (int c = 1; c < LAST; ++c) s = s + T(-1.0 * S::Ql[RR][LAST - 1 - c]) * get(base + c);
+    return s;
+}
+
+template <int ORD, typename T, typename G, int... Is>
+__device__ __forceinline__ T sbp_left_dispatch(int t, const G &get, std::integer_sequence<int, Is...>) {
+    T r = T(0);
+    ((t == Is ? (void)(r = sbp_left_row<ORD, Is, T>(get)) : (void)0), ...);
+    return r;
+}
+template <int ORD, typename T, typename G, int... Is>
+__device__ __forceinline__ T sbp_right_dispatch(int ir, int n, const G &get, std::integer_sequence<int, Is...>) {
+    T r = T(0);
+    ((ir == Is ? (void)(r = sbp_right_row<ORD, Is, T>(n, get)) : (void)0), ...);
+    return r;
+}
+
+template <int ORD, typename T, typename G>
+__device__ __forceinline__ T sbp_row(int t, int n, const G &get) {
+    using S = Sbp<ORD>;
+    if (t >= S::NE && t < n - S::NE) {
+        T s = T(S::Qin[0]) * get(t + S::INSHIFT);
+#pragma unroll
+        for (int k = 1; k < S::WIN; ++k) s = s + T(S::Qin[k]) * get(t + S::INSHIFT + k);
+        return s;
+    }
+    if (t < S::NE) return sbp_left_dispatch<ORD, T>(t, get, std::make_integer_sequence<int, S::NE>{});
+    return sbp_right_dispatch<ORD, T>(t - (n - S::NE), n, get, std::make_integer_sequence<int, S::NE>{});
+}
+
+// Raw (pre-sync) grad/div/curl at point (p,i,j) from global memory; used for the neighbour
+// panel's copy of a shared edge point.  Same arithmetic as the LDS path.
+template <int ORD, typename T>
+struct Raw4 {
+    T gx, gy, dv, cu;
+};
+
+template <int ORD, typename T>
+__device__ __noinline__ Raw4<ORD, T> raw_global(const DevGeo<T> &g, const T *y, const T *hs, int p, int i, int j) {
+    const int n = g.n;
+    auto gxK = [&](int a) { return derived_global(g, y, hs, p, a, j).K; };
+    auto gyK = [&](int a) { return derived_global(g, y, hs, p, i, a).K; };
+    auto gxD = [&](int a) { return derived_global(g, y, hs, p, a, j).Gxd; };
+    auto gyD = [&](int a) { return derived_global(g, y, hs, p, i, a).Gyd; };
+    auto gxC = [&](int a) { return derived_global(g, y, hs, p, a, j).Gxc; };
+    auto gyC = [&](int a) { return derived_global(g, y, hs, p, i, a).Gyc; };
+    const T dxK = sbp_row<ORD, T>(i, n, gxK);
+    const T dyK = sbp_row<ORD, T>(j, n, gyK);
+    const T dxD = sbp_row<ORD, T>(i, n, gxD);
+    const T dyD = sbp_row<ORD, T>(j, n, gyD);
+    const T dxC = sbp_row<ORD, T>(i, n, gxC);
+    const T dyC = sbp_row<ORD, T>(j, n, gyC);
+    const long long id = pidx(g, p, i, j);
+    const T Jd = g.J[id] * g.hx * g.scale;
+    Raw4<ORD, T> r;
+    r.gx = dxK / g.hxs;
+    r.gy = dyK / g.hxs;
+    r.dv = (dxD + dyD) / Jd;
+    r.cu = (dxC + dyC) / Jd;
+    return r;
+}
+
+// Vector halo convention: neighbour components (a,b) -> local (u,v)
+template <typename T>
+__device__ __forceinline__ void map_vec(int code, T a, T b, T &u, T &v) {
+    const T x = (code & 1) ? b : a;
+    const T z = (code & 1) ? a : b;
+    u = (code & 2) ? -x : x;
+    v = (code & 4) ? -z : z;
+}
+
+}  // namespace swm

@@ -1,0 +1,113 @@
+/*
+ * swm_gpu.h -- C ABI of the MI355X (gfx950) shallow-water dynamical-core path.
+ *
+ * The reference model (vvshashkin/cubedsphere_sbp_swm, Fortran 2008 + MPI) has no C
+ * interface; its hot path is reached through Fortran type-bound procedures.  Each entry
+ * point below replaces one of them (file:line relative to the reference's src/) and is
+ * bound from Fortran with ISO_C_BINDING (cubedsphere_sbp_swm_amd/fortran/swm_gpu_mod.f90)
+ * or from Python with ctypes (cubedsphere_sbp_swm_amd/__init__.py).
+ *
+ * Conventions
+ *  - Every function returns SWM_OK (0) or a negative status; swm_last_error() gives the
+ *    message of the last failure on the calling thread (the reference aborts through
+ *    parcomm_global%abort; the Fortran binding turns a negative status into that abort).
+ *  - Device state vectors are opaque device pointers holding h, u, v of all six panels:
+ *    3 fields x field_elems elements of the context precision (8 or 4 bytes).
+ *  - Host arrays use the reference tile layout p(1-hw:n+hw, 1-hw:n+hw) per panel (i
+ *    fastest), panels 1..6 consecutive, FP64, n = N+1 Ah points per panel side.
+ *  - Host pointers are borrowed for the duration of a call; device buffers allocated by
+ *    swm_state_alloc are owned by the caller until swm_state_free.
+ *  - One HIP stream per context; calls on one context are not thread-safe.
+ */
+#ifndef SWM_GPU_H
+#define SWM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWM_OK 0
+#define SWM_EARG (-1)    /* invalid argument or shape */
+#define SWM_ENAME (-2)   /* unknown or unsupported operator name */
+#define SWM_EHIP (-3)    /* HIP runtime error */
+#define SWM_ESTATE (-4)  /* object in the wrong state (e.g. wrong context) */
+
+typedef struct swm_ctx swm_ctx;
+
+const char *swm_last_error(void);
+
+/* create_domain + create_swm_operator (+ create_timescheme 'rk4'):
+ *   domain/domain_factory_mod.f90:17-136, models/shallow_water/operator/operator_swm_factory_mod.f90:43-131,
+ *   time_schemes/timescheme_factory_mod.f90:42-59.
+ * Operator names as in the &shallow_water_model namelist (config_swm_mod.f90:70-79):
+ *   div  "divergence_ah42_sbp" | "divergence_ah43_sbp" | "divergence_ah63_sbp" | "divergence_ah2"
+ *   grad "gradient_ah42_sbp_ecs" | "gradient_ah43_sbp_ecs" | "gradient_ah63_sbp_ecs" | "gradient_ah21_sbp_ecs"
+ *   curl "curl_" + div name
+ *   quadrature "SBP_Ah21_quadrature" | "SBP_Ah42_quadrature" | "SBP_Ah63_quadrature"
+ * scale = sphere radius, omega = rotation rate (axis z), grav = gravity.
+ * precision_bits = 64 (FP64 state and stencils) or 32 (FP32 state and stencils, metric
+ * precomputed in FP64, compensated FP64 integrals). */
+int swm_ctx_create(swm_ctx **ctx, int N, const char *div_op_name, const char *grad_op_name,
+                   const char *curl_op_name, const char *quadrature_name, double scale,
+                   double omega, double grav, int precision_bits);
+int swm_ctx_destroy(swm_ctx *ctx);
+/* n = N+1, row pitch, elements per field, bytes per element */
+int swm_ctx_info(const swm_ctx *ctx, int *n, int *pitch, long long *field_elems, int *elem_bytes);
+/* run subsequent work of this context on a caller-provided hipStream_t (NULL = own stream) */
+int swm_ctx_set_stream(swm_ctx *ctx, void *hip_stream);
+int swm_ctx_synchronize(swm_ctx *ctx);
+
+/* create_stvec_swm (models/shallow_water/stvec/stvec_swm_factory_mod.f90:7-28) and
+ * stvec_t%create_similar (stvec_swm_mod.f90:23-36): device h,u,v */
+int swm_state_alloc(swm_ctx *ctx, void **state);
+int swm_state_free(swm_ctx *ctx, void *state);
+/* host <-> device copies of h,u,v (reference tile layout with halo width halo_width) */
+int swm_state_upload(swm_ctx *ctx, void *state, const double *h, const double *u,
+                     const double *v, int halo_width);
+int swm_state_download(swm_ctx *ctx, const void *state, double *h, double *u, double *v,
+                       int halo_width);
+
+/* operator_t%apply (operator_mod.f90:11-25; operator_swm_mod.f90:61-113): vout = RHS(vin) */
+int swm_rhs(swm_ctx *ctx, const void *vin, void *vout);
+
+/* stvec_t%assign_s1v1s2v2 / update_s1v1s2v2 (stvec_swm_mod.f90:38-80; grid_field_mod.f90:430-485):
+ * out = s1*v1 + s2*v2  /  out = out + s1*v1 + s2*v2, mesh points of all three fields */
+int swm_assign(swm_ctx *ctx, void *out, double s1, const void *v1, double s2, const void *v2);
+int swm_update(swm_ctx *ctx, void *out, double s1, const void *v1, double s2, const void *v2);
+
+/* timescheme_t%step for rk4 (time_schemes/rk4_mod.f90:20-42), nsteps times, in place,
+ * one fused kernel per RK stage (4 launches per step, captured in a HIP graph) */
+int swm_rk4_steps(swm_ctx *ctx, void *state, double dt, int nsteps);
+
+/* operator_t%get_diagnostics (operator_swm_mod.f90:115-260):
+ * out = {hmin, hmax, mass, TE, KE, PE, enstrophy}; integrals use the context's quadrature
+ * with compensated (Neumaier) FP64 summation */
+int swm_diagnostics(swm_ctx *ctx, const void *state, double out[7]);
+
+/* orography h_surf (operator_swm_factory_mod.f90:115-116); NULL removes it */
+int swm_set_orography(swm_ctx *ctx, const double *hs, int halo_width);
+
+/* Seeded spherical-harmonic initial state (the reference has no seeded generator; this is
+ * the one used for the benchmark configurations): h = h0 + h' with max|h'| = h_amp,
+ * V = r x grad(psi) with max|V| = v_max, degrees 1..lmax with N(0,1)*l^-2 coefficients;
+ * covariant u = V.a1, v = V.a2 as set_vector_test_field (test_fields/test_fields_mod.f90:397-433).
+ * Output in the host reference layout with halo width halo_width (halo left zero). */
+int swm_ic_spherical_harmonics(swm_ctx *ctx, unsigned long long seed, int lmax, double h0,
+                               double h_amp, double v_max, double *h, double *u, double *v,
+                               int halo_width);
+
+/* Measurement helper (no counterpart in the reference): `reps` back-to-back launches of one
+ * fused RK stage kernel (stage_id 1, 2 or 4) reading `state`, on the context stream; outputs go
+ * to internal work buffers. */
+int swm_bench_stage(swm_ctx *ctx, void *state, int stage_id, int reps);
+
+/* Diagnostics of the set-up (not part of the reference interface): device copies of the
+ * metric planes (which: 0 J, 1 Qi11, 2 Qi12, 3 Qi22, 4 f) in host layout, and the cube-edge
+ * neighbour table (6*4*n*2*4 ints, see geometry.h). */
+int swm_debug_geometry(swm_ctx *ctx, int which, double *out, int halo_width);
+int swm_debug_halo_table(swm_ctx *ctx, int *table);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
