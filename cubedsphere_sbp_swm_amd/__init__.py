@@ -112,6 +112,22 @@ class OperatorSWM:
                                                      float(v_max), _dp(h), _dp(u), _dp(v), int(halo_width)))
         return h, u, v
 
+    def set_tiling(self, tile_rows=8, metric_on_the_fly=False):
+        """Tile height (8 or 16 rows of 64 points) of the fused stage kernel; FP64 may evaluate
+        the metric terms on the fly instead of reading them (identical values)."""
+        check(_capi.lib().swm_ctx_set_tiling(self.ctx, int(tile_rows), int(bool(metric_on_the_fly))))
+
+    def get_tiling(self):
+        r, m = ctypes.c_int(), ctypes.c_int()
+        check(_capi.lib().swm_ctx_get_tiling(self.ctx, ctypes.byref(r), ctypes.byref(m)))
+        return {"tile_rows": r.value, "metric_on_the_fly": bool(m.value)}
+
+    def autotune(self, v, reps=50):
+        """Time every tile configuration on state `v` and keep the fastest (returns us/launch)."""
+        best = ctypes.c_double()
+        check(_capi.lib().swm_ctx_autotune(self.ctx, ctypes.c_void_p(v.ptr), int(reps), ctypes.byref(best)))
+        return best.value
+
     def debug_geometry(self, which, halo_width=0):
         out = np.zeros(self.host_shape(halo_width))
         check(_capi.lib().swm_debug_geometry(self.ctx, int(which), _dp(out), int(halo_width)))

@@ -35,6 +35,9 @@ SIGNATURES = {
     "swm_diagnostics": (_i, [_vp, _vp, _dp]),
     "swm_set_orography": (_i, [_vp, _dp, _i]),
     "swm_ic_spherical_harmonics": (_i, [_vp, ctypes.c_ulonglong, _i, _d, _d, _d, _dp, _dp, _dp, _i]),
+    "swm_ctx_set_tiling": (_i, [_vp, _i, _i]),
+    "swm_ctx_get_tiling": (_i, [_vp, _ip, _ip]),
+    "swm_ctx_autotune": (_i, [_vp, _vp, _i, _dp]),
     "swm_bench_stage": (_i, [_vp, _vp, _i, _i]),
     "swm_debug_geometry": (_i, [_vp, _i, _dp, _i]),
     "swm_debug_halo_table": (_i, [_vp, _ip]),

@@ -19,6 +19,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-function"]
+HIP_SOURCES = ["swm_capi.hip", "stage_o42.hip", "stage_o43.hip", "stage_o63.hip"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 
 
@@ -31,11 +32,23 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     geo_o = os.path.join(BUILD, "geometry.o")
-    capi_o = os.path.join(BUILD, "swm_capi.o")
-    _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o], verbose)
-    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "swm_capi.hip"), "-o", capi_o], verbose)
+    jobs = [["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o]]
+    objs = [geo_o]
+    for src in HIP_SOURCES:
+        o = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        jobs.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    # independent translation units: compile in parallel
+    procs = []
+    for cmd in jobs:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+    for cmd, pr in procs:
+        if pr.wait() != 0:
+            raise subprocess.CalledProcessError(pr.returncode, cmd)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, geo_o, capi_o, "-lpthread"], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

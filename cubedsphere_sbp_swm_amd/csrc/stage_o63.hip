@@ -1,0 +1,7 @@
+// Stage-kernel instantiations of the sbp63 family.
+#include "swm_launch.h"
+#include "swm_stage.hip"
+
+namespace swm {
+hipError_t launch_stage_o63(const StageLaunch &r) { return launch_stage_order<63>(r); }
+}  // namespace swm

@@ -8,11 +8,13 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/swm_gpu.h"
 #include "geometry.h"
-#include "swm_stage.hip"
+#include "swm_aux.hip"
+#include "swm_launch.h"
 
 namespace {
 
@@ -29,8 +31,7 @@ int fail(int code, const std::string &msg) {
         if (e_ != hipSuccess) return fail(SWM_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kTX = 64, kTY = 8, kRPT = 2;
-constexpr int NDEPTH = 2;
+constexpr int NDEPTH = 6;  // neighbour-table depths kept on the host (>= widest closure row)
 
 }  // namespace
 
@@ -45,6 +46,10 @@ struct swm_ctx {
     void *J = nullptr, *Qi11 = nullptr, *Qi12 = nullptr, *Qi22 = nullptr, *f = nullptr;
     void *q[4] = {nullptr, nullptr, nullptr, nullptr};
     int4 *nbr = nullptr;
+    int *einfo = nullptr;
+    double *tanA = nullptr;
+    bool otf = false;   // FP64: metric terms evaluated on the fly from tan(alpha)
+    int tile_ty = 8;    // tile height of the stage kernel (tile width 64)
     void *hs = nullptr;
     // FP64 copies used by the diagnostics
     double *J64 = nullptr, *Qi64[3] = {nullptr, nullptr, nullptr}, *f64 = nullptr, *A64 = nullptr;
@@ -77,10 +82,10 @@ swm::DevGeo<T> make_geo(const swm_ctx *c) {
     g.qt = (const T *)c->q[1];
     g.ql = (const T *)c->q[2];
     g.qr = (const T *)c->q[3];
-    g.nbr = c->nbr;
+    g.einfo = c->einfo;
+    g.tanA = c->tanA;
     g.n = c->n;
     g.pitch = c->pitch;
-    g.ndepth = NDEPTH;
     g.fs = c->fs;
     g.hxs = (T)(c->hx * c->scale);
     g.hx = (T)c->hx;
@@ -89,50 +94,30 @@ swm::DevGeo<T> make_geo(const swm_ctx *c) {
     return g;
 }
 
-template <int ORD, typename T, int MODE>
-int launch_stage_t(swm_ctx *c, swm::StageArgs<T> a) {
-    using C = swm::TileCfg<ORD, T, kTX, kTY, kRPT>;
-    const int ntx = (c->n + kTX - 1) / kTX, nty = (c->n + kTY - 1) / kTY;
-    const int nblocks = 6 * ntx * nty;
-    hipLaunchKernelGGL((swm::stage_kernel<ORD, T, MODE, kTX, kTY, kRPT>), dim3(nblocks), dim3(C::THREADS), 0, c->stream,
-                       make_geo<T>(c), a);
-    HIP_TRY(hipGetLastError());
-    return SWM_OK;
-}
-
-template <typename T, int MODE>
-int launch_stage(swm_ctx *c, swm::StageArgs<T> a) {
+int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, void *acc, double cy, double cacc) {
+    swm::StageLaunch r{};
+    r.prec = c->prec;
+    r.mode = mode;
+    r.ty = c->tile_ty;
+    r.otf = c->otf && c->prec == 64;
+    r.stream = c->stream;
+    if (c->prec == 64) {
+        r.gd = make_geo<double>(c);
+        r.ad = {(const double *)y, (double *)out, (const double *)v0, (double *)acc, (const double *)c->hs, cy, cacc};
+    } else {
+        r.gf = make_geo<float>(c);
+        r.af = {(const float *)y, (float *)out, (const float *)v0, (float *)acc, (const float *)c->hs, (float)cy,
+                (float)cacc};
+    }
+    hipError_t e;
     switch (c->grad_order) {
-    case 42: return launch_stage_t<42, T, MODE>(c, a);
-    case 43: return launch_stage_t<43, T, MODE>(c, a);
-    case 63: return launch_stage_t<63, T, MODE>(c, a);
+    case 42: e = swm::launch_stage_o42(r); break;
+    case 43: e = swm::launch_stage_o43(r); break;
+    case 63: e = swm::launch_stage_o63(r); break;
     default: return fail(SWM_ENAME, "operator set not available on the device path");
     }
-}
-
-template <typename T>
-int stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, void *acc, double cy, double cacc) {
-    swm::StageArgs<T> a;
-    a.y = (const T *)y;
-    a.out = (T *)out;
-    a.v0 = (const T *)v0;
-    a.acc = (T *)acc;
-    a.hs = (const T *)c->hs;
-    a.cy = (T)cy;
-    a.cacc = (T)cacc;
-    switch (mode) {
-    case swm::MODE_RHS: return launch_stage<T, swm::MODE_RHS>(c, a);
-    case swm::MODE_STAGE1: return launch_stage<T, swm::MODE_STAGE1>(c, a);
-    case swm::MODE_STAGE23: return launch_stage<T, swm::MODE_STAGE23>(c, a);
-    case swm::MODE_STAGE4: return launch_stage<T, swm::MODE_STAGE4>(c, a);
-    case swm::MODE_CURL: return launch_stage<T, swm::MODE_CURL>(c, a);
-    }
-    return fail(SWM_EARG, "bad stage mode");
-}
-
-int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, void *acc, double cy, double cacc) {
-    return c->prec == 64 ? stage<double>(c, mode, y, out, v0, acc, cy, cacc)
-                         : stage<float>(c, mode, y, out, v0, acc, cy, cacc);
+    if (e != hipSuccess) return fail(SWM_EHIP, std::string("stage launch: ") + hipGetErrorString(e));
+    return SWM_OK;
 }
 
 // one RK4 step = 4 fused stage launches (rk4_mod.f90:28-40)
@@ -220,7 +205,7 @@ int parse_names(swm_ctx *c, const char *div, const char *grad, const char *curl,
 void free_ctx(swm_ctx *c) {
     if (!c) return;
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    void *ptrs[] = {c->J, c->Qi11, c->Qi12, c->Qi22, c->f, c->q[0], c->q[1], c->q[2], c->q[3], c->nbr, c->hs,
+    void *ptrs[] = {c->J, c->Qi11, c->Qi12, c->Qi22, c->f, c->q[0], c->q[1], c->q[2], c->q[3], c->nbr, c->einfo, c->tanA, c->hs,
                     c->J64, c->Qi64[0], c->Qi64[1], c->Qi64[2], c->f64, c->A64, c->hs64, c->yA, c->yB, c->acc,
                     c->curl, c->red};
     for (void *p : ptrs)
@@ -238,7 +223,7 @@ extern "C" int swm_ctx_create(swm_ctx **out, int N, const char *div, const char 
                               const char *quad, double scale, double omega, double grav, int precision_bits) {
     if (!out) return fail(SWM_EARG, "ctx pointer is NULL");
     *out = nullptr;
-    if (N < 8 || N > 8192) return fail(SWM_EARG, "N must be in [8, 8192]");
+    if (N < 8 || N > 8192) return fail(SWM_EARG, "N must be in [8, 8192]");  // n = N+1 > tile height
     if (precision_bits != 64 && precision_bits != 32) return fail(SWM_EARG, "precision_bits must be 64 or 32");
     if (!(scale > 0) || !(grav > 0)) return fail(SWM_EARG, "scale and grav must be positive");
     swm_ctx *c = new swm_ctx();
@@ -289,6 +274,40 @@ extern "C" int swm_ctx_create(swm_ctx **out, int N, const char *div, const char 
     if ((rc = upload_f64(&c->J64, J)) || (rc = upload_f64(&c->Qi64[0], Qi11)) || (rc = upload_f64(&c->Qi64[1], Qi12)) ||
         (rc = upload_f64(&c->Qi64[2], Qi22)) || (rc = upload_f64(&c->f64, fc)) || (rc = upload_f64(&c->A64, A)))
         return bail(rc);
+    // per (panel, edge) orientation of the neighbour panel: q, its normal axis (x?), its index
+    // of the shared edge along that axis, and tangential index = a_base + s*k
+    std::vector<int> einfo(6 * 4 * 8, 0);
+    for (int p = 0; p < 6; ++p)
+        for (int e = 0; e < 4; ++e) {
+            auto at = [&](int k, int d) { return &c->host_nbr[((((size_t)p * 4 + e) * c->n + k) * NDEPTH + d) * 4]; };
+            const int *t0 = at(0, 0), *t1 = at(0, 1), *tk = at(1, 0);
+            const int nu_x = t0[1] != t1[1] ? 1 : 0;
+            const int tnu = nu_x ? t0[1] : t0[2];
+            const int abase = nu_x ? t0[2] : t0[1];
+            const int sgn = (nu_x ? tk[2] : tk[1]) - abase;
+            for (int k = 0; k < c->n; ++k)
+                for (int d = 0; d < NDEPTH; ++d) {
+                    const int *t = at(k, d);
+                    const int a_ = nu_x ? t[2] : t[1], nu_ = nu_x ? t[1] : t[2];
+                    const int nu_expect = tnu == 0 ? d : c->n - 1 - d;
+                    if (t[0] != t0[0] || a_ != abase + sgn * k || nu_ != nu_expect || (sgn != 1 && sgn != -1))
+                        return bail(fail(SWM_EARG, "neighbour table is not an edge strip"));
+                }
+            int *ei = &einfo[(p * 4 + e) * 8];
+            for (int k = 0; k < c->n; ++k)
+                for (int d = 0; d < NDEPTH; ++d)
+                    if (at(k, d)[3] != t0[3]) return bail(fail(SWM_EARG, "vector code varies along an edge"));
+            ei[0] = t0[0]; ei[1] = nu_x; ei[2] = tnu; ei[3] = abase; ei[4] = sgn; ei[5] = t0[3];
+        }
+    if (hipMalloc((void **)&c->einfo, einfo.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpy(c->einfo, einfo.data(), einfo.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SWM_EHIP, "edge info upload failed"));
+    {
+        // tan(alpha_i) of the grid coordinate alpha_i = -pi/4 + (i-1)*hx (mesh_mod.f90:83-92)
+        std::vector<double> ta(c->n);
+        for (int i = 0; i < c->n; ++i) ta[i] = std::tan(-0.25 * std::acos(-1.0) + ((double)i + 0.0) * c->hx);
+        if ((rc = upload_f64(&c->tanA, ta))) return bail(rc);
+    }
     if (hipMalloc((void **)&c->nbr, c->host_nbr.size() * sizeof(int)) != hipSuccess ||
         hipMemcpy(c->nbr, c->host_nbr.data(), c->host_nbr.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(SWM_EHIP, "neighbour table upload failed"));
@@ -432,6 +451,62 @@ extern "C" int swm_rk4_steps(swm_ctx *c, void *state, double dt, int nsteps) {
         c->graph_dt = dt;
     }
     for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_set_tiling(swm_ctx *c, int tile_rows, int metric_on_the_fly) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (tile_rows != 8 && tile_rows != 16) return fail(SWM_EARG, "tile_rows must be 8 or 16");
+    if (metric_on_the_fly && c->prec != 64) return fail(SWM_EARG, "metric on the fly is FP64 only");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    c->tile_ty = tile_rows;
+    c->otf = metric_on_the_fly != 0;
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_get_tiling(const swm_ctx *c, int *tile_rows, int *metric_on_the_fly) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (tile_rows) *tile_rows = c->tile_ty;
+    if (metric_on_the_fly) *metric_on_the_fly = c->otf ? 1 : 0;
+    return SWM_OK;
+}
+
+// Autotuner: times `reps` launches of the RK stage-2 kernel on `state` for every tile
+// configuration (tile height 8/16, and for FP64 metric read or evaluated on the fly) with HIP
+// events on the context stream, and keeps the fastest.
+extern "C" int swm_ctx_autotune(swm_ctx *c, void *state, int reps, double *best_us) {
+    if (!c || !state || reps < 1) return fail(SWM_EARG, "bad argument");
+    struct Cand { int ty; bool otf; };
+    std::vector<Cand> cands = {{8, false}, {16, false}};
+    if (c->prec == 64) { cands.push_back({8, true}); cands.push_back({16, true}); }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e300;
+    Cand pick = cands[0];
+    for (const Cand &cd : cands) {
+        c->tile_ty = cd.ty;
+        c->otf = cd.otf;
+        int rc = 0;
+        for (int r = 0; r < 3 && !rc; ++r) rc = run_stage(c, swm::MODE_STAGE23, state, c->yB, state, c->acc, 1.0, 0.0);
+        if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+        HIP_TRY(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps && !rc; ++r) rc = run_stage(c, swm::MODE_STAGE23, state, c->yB, state, c->acc, 1.0, 0.0);
+        HIP_TRY(hipEventRecord(e1, c->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / reps;
+        if (us < best) { best = us; pick = cd; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tile_ty = pick.ty;
+    c->otf = pick.otf;
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    if (best_us) *best_us = best;
     return SWM_OK;
 }
 

@@ -6,8 +6,9 @@
 //   * the five stencil inputs K = (u.ut+v.vt)/2 + g h + g hs, J*hu, J*hv, J*(v/J), J*(-u/J)
 //     are formed once per point in LDS from the state and the metric;
 //   * SBP derivative rows (interior stencil or boundary closure) are applied along x and y;
-//   * at cube edges the neighbour panel's one-sided derivative values at the shared point
-//     are recomputed from global memory and combined with the reference's scalar and
+//   * tiles on a cube edge also load the neighbour panel's points along the shared edge
+//     (depth 0 .. closure width) into LDS, evaluate the neighbour's one-sided derivative
+//     rows at the shared points there, and combine them with the reference's scalar and
 //     covariant-vector edge averaging (halo_Ah_scalar_sync_mod.f90:52-93,
 //     ecs_Ah_vec_sync_mod.f90:77-124) -- this replaces the MPI halo exchange;
 //   * the Coriolis term, tendencies and the RK stage combination finish in registers.
@@ -27,8 +28,9 @@ template <typename T>
 struct DevGeo {
     const T *J, *Qi11, *Qi12, *Qi22, *f;   // [panel][j][i], row pitch `pitch`
     const T *qb, *qt, *ql, *qr;            // covariant sync weights, n values each
-    const int4 *nbr;                       // neighbour table [6][4][n][ndepth]
-    int n, pitch, ndepth;
+    const int *einfo;                      // per (panel, edge): q, nu_is_x, tnu, a_base, s, code
+    const double *tanA;                    // tan(alpha_i), i = 0..n-1 (metric evaluated on the fly)
+    int n, pitch;
     long long fs;                          // elements per field (6*n*pitch)
     T hxs;                                 // hx*scale
     T hx, scale, grav;
@@ -61,7 +63,8 @@ struct Derived {
     T K, Gxd, Gyd, Gxc, Gyc;
 };
 
-template <typename T>
+// WHICH: 0 all five inputs, 2 only those of the y-derivative (K, J*hv, J*(-u/J))
+template <typename T, int WHICH = 0>
 __device__ __forceinline__ Derived<T> make_derived(T h, T u, T v, T J, T q11, T q12, T q22, T hsv, T grav) {
     Derived<T> d;
     const T ut = q11 * u + q12 * v;
@@ -71,9 +74,11 @@ __device__ __forceinline__ Derived<T> make_derived(T h, T u, T v, T J, T q11, T 
     T K = (u * ut + v * vt) / T(2);
     K = K + grav * h + grav * hsv;
     d.K = K;
-    d.Gxd = J * hu;
+    if constexpr (WHICH != 2) {
+        d.Gxd = J * hu;
+        d.Gxc = J * (v / J);
+    }
     d.Gyd = J * hv;
-    d.Gxc = J * (v / J);
     d.Gyc = J * (-u / J);
     return d;
 }
@@ -83,11 +88,39 @@ __device__ __forceinline__ long long pidx(const DevGeo<T> &g, int p, int i, int 
     return ((long long)p * g.n + j) * g.pitch + i;
 }
 
-template <typename T>
+// Metric terms of Ah point (i,j) from tan(alpha_i), tan(beta_j), with the operation order of
+// the host set-up (ecs_metric_mod.f90:413-463); FP64 only.
+struct Metric {
+    double J, Qi11, Qi12, Qi22;
+};
+__device__ __forceinline__ Metric metric_otf(const double *tanA, int i, int j) {
+    const double ta = tanA[i], tb = tanA[j];
+    const double sg = sqrt(1.0 + ta * ta + tb * tb);
+    Metric m;
+    m.J = (1.0 + ta * ta) * (1.0 + tb * tb) / (sg * sg * sg);
+    m.Qi11 = sg * sg / (1.0 + ta * ta);
+    m.Qi12 = sg * sg * ta * tb / ((1 + ta * ta) * (1 + tb * tb));
+    m.Qi22 = sg * sg / (1.0 + tb * tb);
+    return m;
+}
+
+template <bool OTF, typename T>
+__device__ __forceinline__ T metric_J(const DevGeo<T> &g, long long id, int i, int j) {
+    if constexpr (OTF) return metric_otf(g.tanA, i, j).J;
+    else return g.J[id];
+}
+
+template <bool OTF, typename T, int WHICH = 0>
 __device__ __forceinline__ Derived<T> derived_global(const DevGeo<T> &g, const T *y, const T *hs, int p, int i, int j) {
     const long long id = pidx(g, p, i, j);
-    return make_derived<T>(y[id], y[id + g.fs], y[id + 2 * g.fs], g.J[id], g.Qi11[id], g.Qi12[id], g.Qi22[id],
-                           hs ? hs[id] : T(0), g.grav);
+    if constexpr (OTF) {
+        const Metric m = metric_otf(g.tanA, i, j);
+        return make_derived<T, WHICH>(y[id], y[id + g.fs], y[id + 2 * g.fs], (T)m.J, (T)m.Qi11, (T)m.Qi12,
+                                      (T)m.Qi22, hs ? hs[id] : T(0), g.grav);
+    } else {
+        return make_derived<T, WHICH>(y[id], y[id + g.fs], y[id + 2 * g.fs], g.J[id], g.Qi11[id], g.Qi12[id],
+                                      g.Qi22[id], hs ? hs[id] : T(0), g.grav);
+    }
 }
 
 // ----------------------------------------------------------------------------------------
@@ -138,38 +171,6 @@ __device__ __forceinline__ T sbp_row(int t, int n, const G &get) {
     }
     if (t < S::NE) return sbp_left_dispatch<ORD, T>(t, get, std::make_integer_sequence<int, S::NE>{});
     return sbp_right_dispatch<ORD, T>(t - (n - S::NE), n, get, std::make_integer_sequence<int, S::NE>{});
-}
-
-// Raw (pre-sync) grad/div/curl at point (p,i,j) from global memory; used for the neighbour
-// panel's copy of a shared edge point.  Same arithmetic as the LDS path.
-template <int ORD, typename T>
-struct Raw4 {
-    T gx, gy, dv, cu;
-};
-
-template <int ORD, typename T>
-__device__ __noinline__ Raw4<ORD, T> raw_global(const DevGeo<T> &g, const T *y, const T *hs, int p, int i, int j) {
-    const int n = g.n;
-    auto gxK = [&](int a) { return derived_global(g, y, hs, p, a, j).K; };
-    auto gyK = [&](int a) { return derived_global(g, y, hs, p, i, a).K; };
-    auto gxD = [&](int a) { return derived_global(g, y, hs, p, a, j).Gxd; };
-    auto gyD = [&](int a) { return derived_global(g, y, hs, p, i, a).Gyd; };
-    auto gxC = [&](int a) { return derived_global(g, y, hs, p, a, j).Gxc; };
-    auto gyC = [&](int a) { return derived_global(g, y, hs, p, i, a).Gyc; };
-    const T dxK = sbp_row<ORD, T>(i, n, gxK);
-    const T dyK = sbp_row<ORD, T>(j, n, gyK);
-    const T dxD = sbp_row<ORD, T>(i, n, gxD);
-    const T dyD = sbp_row<ORD, T>(j, n, gyD);
-    const T dxC = sbp_row<ORD, T>(i, n, gxC);
-    const T dyC = sbp_row<ORD, T>(j, n, gyC);
-    const long long id = pidx(g, p, i, j);
-    const T Jd = g.J[id] * g.hx * g.scale;
-    Raw4<ORD, T> r;
-    r.gx = dxK / g.hxs;
-    r.gy = dyK / g.hxs;
-    r.dv = (dxD + dyD) / Jd;
-    r.cu = (dxC + dyC) / Jd;
-    return r;
 }
 
 // Vector halo convention: neighbour components (a,b) -> local (u,v)

@@ -6,14 +6,34 @@
 
 namespace swm {
 
-// Tile of TX x TY output points per workgroup of TX*(TY/RPT) threads; LDS holds the five
-// stencil inputs on the tile plus a halo of REACH points in x and y (cross-shaped region).
-template <int ORD, typename T, int TX, int TY, int RPT>
-struct TileCfg {
-    static constexpr int R = Sbp<ORD>::REACH;
+
+// Tile of TX x TY output points per workgroup of TX*TY threads (one point per thread).
+template <int ORD, typename T, int TX, int TY>
+struct Tile {
+    static constexpr int R = Sbp<ORD>::REACH;    // halo of the stencil inputs
+    static constexpr int D = Sbp<ORD>::last[0];  // points of the boundary row 0 (normal depth)
     static constexpr int LX = TX + 2 * R;
     static constexpr int LY = TY + 2 * R;
-    static constexpr int THREADS = TX * (TY / RPT);
+    static constexpr int NT = TX * TY;
+};
+
+// LDS of one tile.  K lives on the cross-shaped region; the x-derivative inputs only on the
+// tile rows, the y-derivative inputs only on the tile columns.  Edge tiles use the ghost
+// strips: the neighbour panel's stencil inputs along each shared edge (x-edges: slots 0/1
+// for i=1 / i=n, y-edge: j=1 or j=n), and the neighbour's raw values at the shared points.
+template <int ORD, typename T, int TX, int TY>
+struct Smem {
+    using C = Tile<ORD, T, TX, TY>;
+    static constexpr int R = C::R, D = C::D, LX = C::LX, LY = C::LY;
+    T K[LY][LX];
+    T Gxd[TY][LX], Gxc[TY][LX];
+    T Gyd[LY][TX], Gyc[LY][TX];
+    T gx0[2][TY + 2 * R][6];   // depth 0: K, Gxd, Gyd, Gxc, Gyc, J
+    T gxn[2][D - 1][TY][3];    // depth >= 1: K and the two inputs along the normal axis
+    T gy0[TX + 2 * R][6];
+    T gyn[D - 1][TX][3];
+    T nrx[2][TY][4];           // neighbour raw gx, gy, div, curl
+    T nry[TX][4];
 };
 
 // XCD-aware mapping: consecutive block ids go round-robin over the 8 XCDs; give each XCD a
@@ -24,197 +44,305 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-template <int ORD, typename T>
-__device__ __forceinline__ void edge_sync(const DevGeo<T> &g, const T *y, const T *hs, int p, int i, int j,
-                                          T &gx, T &gy, T &dv, T &cu) {
+// Raw values of the neighbour panel at the shared point k of edge e, from a ghost strip.
+// d0(kk, f) returns field f (0 K, 1 Gxd, 2 Gyd, 3 Gxc, 4 Gyc, 5 J) at tangential index kk,
+// depth 0; dn(d, f3) the normal-axis inputs (0 K, 1 Gnd, 2 Gnc) at depth d >= 1.
+template <int ORD, typename T, typename D0, typename DN>
+__device__ __forceinline__ void neighbour_raw(const DevGeo<T> &g, const int *ei, int k, const D0 &d0,
+                                              const DN &dn, T out[4]) {
     const int n = g.n;
-    // neighbour entry (depth 0) of edge e at position k
-    auto nb = [&](int e, int k) { return g.nbr[(((long long)p * 4 + e) * n + k) * g.ndepth]; };
-    auto halo = [&](int e, int k, T &hx_, T &hy_, T &hd, T &hc) {
-        const int4 t = nb(e, k);
-        const Raw4<ORD, T> r = raw_global<ORD, T>(g, y, hs, t.x, t.y, t.z);
-        map_vec(t.w, r.gx, r.gy, hx_, hy_);
-        hd = r.dv;
-        hc = r.cu;
+    const int nu_x = ei[1], tnu = ei[2], a0 = ei[3] + ei[4] * k, s = ei[4];
+    auto nu_get = [&](int fld3, int a) {  // along the normal axis of the neighbour
+        const int d = tnu == 0 ? a : n - 1 - a;
+        if (d == 0) return d0(k, fld3 == 0 ? 0 : (nu_x ? (fld3 == 1 ? 1 : 3) : (fld3 == 1 ? 2 : 4)));
+        return dn(d, fld3);
     };
-    // covariant components enter the sync routine as (U,V) = (gy,gx)
-    // (ecs_Ah_vec_sync_mod.f90:51-53)
-    const T U = gy, V = gx;
-    const bool L = i == 0, Rt = i == n - 1, B = j == 0, Tp = j == n - 1;
-    T Ux, Vx, Dx, Cx, Uy, Vy, Dy, Cy;  // halo values across the x-edge and the y-edge
-    if (L && B) {
-        halo(0, 0, Vx, Ux, Dx, Cx);
-        halo(2, 0, Vy, Uy, Dy, Cy);
-        dv = (dv + Dx + Dy) / T(3);
-        cu = (cu + Cx + Cy) / T(3);
-        gy = (U + Ux + Uy + g.qb[0] * Vy) / T(3);
-        gx = (V + Vx + Vy + g.ql[0] * Ux) / T(3);
-    } else if (Rt && B) {
-        halo(1, 0, Vx, Ux, Dx, Cx);
-        halo(2, n - 1, Vy, Uy, Dy, Cy);
-        dv = (dv + Dx + Dy) / T(3);
-        cu = (cu + Cx + Cy) / T(3);
-        gy = (U + Ux + Uy + g.qb[n - 1] * Vy) / T(3);
-        gx = (V + Vx + Vy + g.qr[0] * Ux) / T(3);
-    } else if (L && Tp) {
-        halo(0, n - 1, Vx, Ux, Dx, Cx);
-        halo(3, 0, Vy, Uy, Dy, Cy);
-        dv = (dv + Dx + Dy) / T(3);
-        cu = (cu + Cx + Cy) / T(3);
-        gy = (U + Ux + Uy + g.qt[0] * Vy) / T(3);
-        gx = (V + Vx + Vy + g.ql[n - 1] * Ux) / T(3);
-    } else if (Rt && Tp) {
-        halo(1, n - 1, Vx, Ux, Dx, Cx);
-        halo(3, n - 1, Vy, Uy, Dy, Cy);
-        dv = (dv + Dx + Dy) / T(3);
-        cu = (cu + Cx + Cy) / T(3);
-        gy = (U + Ux + Uy + g.qt[n - 1] * Vy) / T(3);
-        gx = (V + Vx + Vy + g.qr[n - 1] * Ux) / T(3);
-    } else if (L) {
-        halo(0, j, Vx, Ux, Dx, Cx);
-        dv = T(0.5) * (Dx + dv);
-        cu = T(0.5) * (Cx + cu);
-        gy = T(0.5) * (Ux + U);
-        gx = T(0.5) * (V + Vx + g.ql[j] * Ux);
-    } else if (Rt) {
-        halo(1, j, Vx, Ux, Dx, Cx);
-        dv = T(0.5) * (dv + Dx);
-        cu = T(0.5) * (cu + Cx);
-        gy = T(0.5) * (U + Ux);
-        gx = T(0.5) * (V + Vx + g.qr[j] * Ux);
-    } else if (B) {
-        halo(2, i, Vy, Uy, Dy, Cy);
-        dv = T(0.5) * (dv + Dy);
-        cu = T(0.5) * (cu + Cy);
-        gx = T(0.5) * (Vy + V);
-        gy = T(0.5) * (U + Uy + g.qb[i] * Vy);
-    } else if (Tp) {
-        halo(3, i, Vy, Uy, Dy, Cy);
-        dv = T(0.5) * (dv + Dy);
-        cu = T(0.5) * (cu + Cy);
-        gx = T(0.5) * (V + Vy);
-        gy = T(0.5) * (U + Uy + g.qt[i] * Vy);
-    }
+    auto tau_get = [&](int fld, int a) { return d0(k + s * (a - a0), fld); };
+    const T nK = sbp_row<ORD, T>(tnu, n, [&](int a) { return nu_get(0, a); });
+    const T nD = sbp_row<ORD, T>(tnu, n, [&](int a) { return nu_get(1, a); });
+    const T nC = sbp_row<ORD, T>(tnu, n, [&](int a) { return nu_get(2, a); });
+    const int fD = nu_x ? 2 : 1, fC = nu_x ? 4 : 3;  // inputs along the tangential axis
+    const T tK = sbp_row<ORD, T>(a0, n, [&](int a) { return tau_get(0, a); });
+    const T tD = sbp_row<ORD, T>(a0, n, [&](int a) { return tau_get(fD, a); });
+    const T tC = sbp_row<ORD, T>(a0, n, [&](int a) { return tau_get(fC, a); });
+    const T dxK = nu_x ? nK : tK, dyK = nu_x ? tK : nK;
+    const T dxD = nu_x ? nD : tD, dyD = nu_x ? tD : nD;
+    const T dxC = nu_x ? nC : tC, dyC = nu_x ? tC : nC;
+    const T Jd = d0(k, 5) * g.hx * g.scale;
+    out[0] = dxK / g.hxs;
+    out[1] = dyK / g.hxs;
+    out[2] = (dxD + dyD) / Jd;
+    out[3] = (dxC + dyC) / Jd;
 }
 
-template <int ORD, typename T, int MODE, int TX, int TY, int RPT>
-__global__ void __launch_bounds__(TX * (TY / RPT))
-stage_kernel(DevGeo<T> g, StageArgs<T> a) {
-    using C = TileCfg<ORD, T, TX, TY, RPT>;
-    constexpr int R = C::R, LX = C::LX, LY = C::LY, NT = C::THREADS;
-    __shared__ T sK[LY][LX], sGxd[LY][LX], sGyd[LY][LX], sGxc[LY][LX], sGyc[LY][LX];
+template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
+__global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T> a) {
+    using C = Tile<ORD, T, TX, TY>;
+    constexpr int R = C::R, D = C::D, LX = C::LX, LY = C::LY, NT = C::NT;
+    __shared__ Smem<ORD, T, TX, TY> sm;
 
     const int n = g.n;
     const int ntx = (n + TX - 1) / TX, nty = (n + TY - 1) / TY;
-    const int nblocks = 6 * ntx * nty;
-    const int tile = xcd_tile(blockIdx.x, nblocks);
+    const int tile = xcd_tile(blockIdx.x, 6 * ntx * nty);
     const int p = tile / (ntx * nty);
     const int rem = tile - p * ntx * nty;
     const int ty0 = (rem / ntx) * TY, tx0 = (rem % ntx) * TX;
     const int tid = threadIdx.x;
     const T *y = a.y;
 
-    // ---- load phase: stencil inputs on the cross-shaped region of the tile ----
-    for (int c = tid; c < LX * LY; c += NT) {
-        const int ly = c / LX, lx = c - ly * LX;
-        const bool cross = (ly >= R && ly < R + TY) || (lx >= R && lx < R + TX);
-        const int i = tx0 - R + lx, j = ty0 - R + ly;
-        if (!cross || i < 0 || i >= n || j < 0 || j >= n) continue;
-        const Derived<T> d = derived_global(g, y, a.hs, p, i, j);
-        sK[ly][lx] = d.K;
-        sGxd[ly][lx] = d.Gxd;
-        sGyd[ly][lx] = d.Gyd;
-        sGxc[ly][lx] = d.Gxc;
-        sGyc[ly][lx] = d.Gyc;
+    // ---- load phase: stencil inputs of the tile rows (incl. x-halo) and of the y-halo rows ----
+    for (int c = tid; c < TY * LX; c += NT) {
+        const int r = c / LX, lx = c - r * LX;
+        const int i = tx0 - R + lx, j = ty0 + r;
+        if (i < 0 || i >= n || j >= n) continue;
+        const Derived<T> d = derived_global<OTF>(g, y, a.hs, p, i, j);
+        sm.K[r + R][lx] = d.K;
+        sm.Gxd[r][lx] = d.Gxd;
+        sm.Gxc[r][lx] = d.Gxc;
+        if (lx >= R && lx < R + TX) {
+            sm.Gyd[r + R][lx - R] = d.Gyd;
+            sm.Gyc[r + R][lx - R] = d.Gyc;
+        }
+    }
+    for (int c = tid; c < 2 * R * TX; c += NT) {
+        const int r = c / TX, cx = c - r * TX;
+        const int ly = r < R ? r : r + TY;
+        const int i = tx0 + cx, j = ty0 - R + ly;
+        if (i >= n || j < 0 || j >= n) continue;
+        const Derived<T> d = derived_global<OTF, T, 2>(g, y, a.hs, p, i, j);
+        sm.K[ly][cx + R] = d.K;
+        sm.Gyd[ly][cx] = d.Gyd;
+        sm.Gyc[ly][cx] = d.Gyc;
+    }
+    // ---- ghost strips of edge tiles (neighbour panels' points, via the neighbour table) ----
+    const bool eL = tx0 == 0, eR = tx0 + TX >= n, eB = ty0 == 0, eT = ty0 + TY >= n;
+    const bool edge_tile = eL || eR || eB || eT;
+    const int ey = eB ? 2 : 3;  // the y-edge of this tile (n > TY: never both)
+    if (edge_tile) {
+        constexpr int NX0 = TY + 2 * R, NXN = (D - 1) * TY, NY0 = TX + 2 * R, NYN = (D - 1) * TX;
+        constexpr int NXS = NX0 + NXN, NYS = NY0 + NYN;
+        const int total = 2 * NXS + NYS;
+        for (int c = tid; c < total; c += NT) {
+            int slot, e, kk, d, idx;
+            if (c < 2 * NXS) {
+                slot = c / NXS;
+                const int r = c - slot * NXS;
+                if (!(slot == 0 ? eL : eR)) continue;
+                e = slot;
+                if (r < NX0) { d = 0; idx = r; kk = ty0 - R + r; }
+                else { d = 1 + (r - NX0) / TY; idx = (r - NX0) % TY; kk = ty0 + idx; }
+            } else {
+                slot = 2;
+                const int r = c - 2 * NXS;
+                if (!(eB || eT)) continue;
+                e = ey;
+                if (r < NY0) { d = 0; idx = r; kk = tx0 - R + r; }
+                else { d = 1 + (r - NY0) / TX; idx = (r - NY0) % TX; kk = tx0 + idx; }
+            }
+            if (kk < 0 || kk >= n) continue;
+            // neighbour point: normal index tnu -/+ d, tangential index a_base + s*kk
+            const int *ei = g.einfo + (p * 4 + e) * 8;
+            const int nidx = ei[2] == 0 ? d : n - 1 - d, tidx = ei[3] + ei[4] * kk;
+            const int qi = ei[1] ? nidx : tidx, qj = ei[1] ? tidx : nidx;
+            const Derived<T> dv = derived_global<OTF>(g, y, a.hs, ei[0], qi, qj);
+            if (d == 0) {
+                T *dst = slot < 2 ? sm.gx0[slot][idx] : sm.gy0[idx];
+                dst[0] = dv.K; dst[1] = dv.Gxd; dst[2] = dv.Gyd; dst[3] = dv.Gxc; dst[4] = dv.Gyc;
+                dst[5] = metric_J<OTF>(g, pidx(g, ei[0], qi, qj), qi, qj);
+            } else {
+                const bool nu_x = ei[1] != 0;
+                T *dst = slot < 2 ? sm.gxn[slot][d - 1][idx] : sm.gyn[d - 1][idx];
+                dst[0] = dv.K;
+                dst[1] = nu_x ? dv.Gxd : dv.Gyd;
+                dst[2] = nu_x ? dv.Gxc : dv.Gyc;
+            }
+        }
     }
     __syncthreads();
 
-    // ---- output phase ----
-    const int tx = tid % TX, tyb = tid / TX;
-#pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) {
-        const int ty = tyb + rr * (TY / RPT);
-        const int i = tx0 + tx, j = ty0 + ty;
-        if (i >= n || j >= n) continue;
-        const int lx = tx + R, ly = ty + R;
-        auto xK = [&](int ai) { return sK[ly][ai - tx0 + R]; };
-        auto yK = [&](int aj) { return sK[aj - ty0 + R][lx]; };
-        auto xD = [&](int ai) { return sGxd[ly][ai - tx0 + R]; };
-        auto yD = [&](int aj) { return sGyd[aj - ty0 + R][lx]; };
-        auto xC = [&](int ai) { return sGxc[ly][ai - tx0 + R]; };
-        auto yC = [&](int aj) { return sGyc[aj - ty0 + R][lx]; };
-        const T dxK = sbp_row<ORD, T>(i, n, xK);
-        const T dyK = sbp_row<ORD, T>(j, n, yK);
-        const T dxD = sbp_row<ORD, T>(i, n, xD);
-        const T dyD = sbp_row<ORD, T>(j, n, yD);
-        const T dxC = sbp_row<ORD, T>(i, n, xC);
-        const T dyC = sbp_row<ORD, T>(j, n, yC);
-        const long long id = pidx(g, p, i, j);
-        const T Jv = g.J[id];
-        const T Jd = Jv * g.hx * g.scale;
-        T gx = dxK / g.hxs;
-        T gy = dyK / g.hxs;
-        T dv = (dxD + dyD) / Jd;
-        T cu = (dxC + dyC) / Jd;
-        if (i == 0 || j == 0 || i == n - 1 || j == n - 1) edge_sync<ORD, T>(g, y, a.hs, p, i, j, gx, gy, dv, cu);
-
-        if constexpr (MODE == MODE_CURL) {
-            a.out[id] = cu;
-            continue;
+    // ---- neighbour raw values at the shared points of this tile (edge tiles only) ----
+    if (edge_tile) {
+        const int task = tid;
+        if (task < 2 * TY + TX) {
+            int slot, k, kloc;
+            if (task < 2 * TY) { slot = task / TY; kloc = task % TY; k = ty0 + kloc; }
+            else { slot = 2; kloc = task - 2 * TY; k = tx0 + kloc; }
+            const bool active = slot == 0 ? eL : slot == 1 ? eR : (eB || eT);
+            if (active && k < n) {
+                const int e = slot < 2 ? slot : ey;
+                const int *ei = g.einfo + (p * 4 + e) * 8;
+                T r[4];
+                if (slot < 2) {
+                    const int k0 = ty0 - R;
+                    neighbour_raw<ORD, T>(g, ei, k, [&](int kk, int f) { return sm.gx0[slot][kk - k0][f]; },
+                                          [&](int d, int f3) { return sm.gxn[slot][d - 1][k - ty0][f3]; }, r);
+                    for (int q = 0; q < 4; ++q) sm.nrx[slot][kloc][q] = r[q];
+                } else {
+                    const int k0 = tx0 - R;
+                    neighbour_raw<ORD, T>(g, ei, k, [&](int kk, int f) { return sm.gy0[kk - k0][f]; },
+                                          [&](int d, int f3) { return sm.gyn[d - 1][k - tx0][f3]; }, r);
+                    for (int q = 0; q < 4; ++q) sm.nry[kloc][q] = r[q];
+                }
+            }
         }
-        // Coriolis and tendencies (coriolis_colocated_mod.f90:97-98, operator_swm_mod.f90:95-101)
-        const T h = y[id], u = y[id + g.fs], v = y[id + 2 * g.fs];
-        const T q11 = g.Qi11[id], q12 = g.Qi12[id], q22 = g.Qi22[id];
-        const T ut = q11 * u + q12 * v;
-        const T vt = q12 * u + q22 * v;
-        const T hu = h * ut, hv = h * vt;
-        const T fz = g.f[id] + cu;
-        const T cor_u = fz * hv / h * Jv;
-        const T cor_v = -(fz * hu / h * Jv);
-        const T kh = -dv;
-        const T ku = -gx + cor_u;
-        const T kv = -gy + cor_v;
+        __syncthreads();
+    }
 
-        if constexpr (MODE == MODE_RHS) {
-            a.out[id] = kh;
-            a.out[id + g.fs] = ku;
-            a.out[id + 2 * g.fs] = kv;
-        } else if constexpr (MODE == MODE_STAGE1) {
-            a.out[id] = h + a.cy * kh;
-            a.out[id + g.fs] = u + a.cy * ku;
-            a.out[id + 2 * g.fs] = v + a.cy * kv;
-            a.acc[id] = h + a.cacc * kh;
-            a.acc[id + g.fs] = u + a.cacc * ku;
-            a.acc[id + 2 * g.fs] = v + a.cacc * kv;
-        } else if constexpr (MODE == MODE_STAGE23) {
-            a.out[id] = a.v0[id] + a.cy * kh;
-            a.out[id + g.fs] = a.v0[id + g.fs] + a.cy * ku;
-            a.out[id + 2 * g.fs] = a.v0[id + 2 * g.fs] + a.cy * kv;
-            a.acc[id] = a.acc[id] + a.cacc * kh;
-            a.acc[id + g.fs] = a.acc[id + g.fs] + a.cacc * ku;
-            a.acc[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cacc * kv;
-        } else if constexpr (MODE == MODE_STAGE4) {
-            a.out[id] = a.acc[id] + a.cy * kh;
-            a.out[id + g.fs] = a.acc[id + g.fs] + a.cy * ku;
-            a.out[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cy * kv;
+    // ---- output phase: one point per thread ----
+    const int tx = tid % TX, ty = tid / TX;
+    const int i = tx0 + tx, j = ty0 + ty;
+    if (i >= n || j >= n) return;
+    const int lx = tx + R, ly = ty + R;
+    const T dxK = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.K[ly][ai - tx0 + R]; });
+    const T dyK = sbp_row<ORD, T>(j, n, [&](int aj) { return sm.K[aj - ty0 + R][lx]; });
+    const T dxD = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.Gxd[ty][ai - tx0 + R]; });
+    const T dyD = sbp_row<ORD, T>(j, n, [&](int aj) { return sm.Gyd[aj - ty0 + R][tx]; });
+    const T dxC = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.Gxc[ty][ai - tx0 + R]; });
+    const T dyC = sbp_row<ORD, T>(j, n, [&](int aj) { return sm.Gyc[aj - ty0 + R][tx]; });
+    const long long id = pidx(g, p, i, j);
+    T Jv, q11, q12, q22;
+    if constexpr (OTF) {
+        const Metric m = metric_otf(g.tanA, i, j);
+        Jv = (T)m.J; q11 = (T)m.Qi11; q12 = (T)m.Qi12; q22 = (T)m.Qi22;
+    } else {
+        Jv = g.J[id]; q11 = g.Qi11[id]; q12 = g.Qi12[id]; q22 = g.Qi22[id];
+    }
+    const T Jd = Jv * g.hx * g.scale;
+    T gx = dxK / g.hxs;
+    T gy = dyK / g.hxs;
+    T dv = (dxD + dyD) / Jd;
+    T cu = (dxC + dyC) / Jd;
+
+    const bool L = i == 0, Rt = i == n - 1, B = j == 0, Tp = j == n - 1;
+    if (L || Rt || B || Tp) {
+        // halo values: neighbour raws mapped to this panel's components (vector halo convention)
+        auto hx_vals = [&](int slot, int k, T &V_, T &U_, T &D_, T &C_) {
+            const int code = g.einfo[(p * 4 + slot) * 8 + 5];
+            const T *r = sm.nrx[slot][k - ty0];
+            map_vec(code, r[0], r[1], V_, U_);
+            D_ = r[2];
+            C_ = r[3];
+        };
+        auto hy_vals = [&](int k, T &V_, T &U_, T &D_, T &C_) {
+            const int code = g.einfo[(p * 4 + ey) * 8 + 5];
+            const T *r = sm.nry[k - tx0];
+            map_vec(code, r[0], r[1], V_, U_);
+            D_ = r[2];
+            C_ = r[3];
+        };
+        // covariant components enter the sync routine as (U,V) = (gy,gx)
+        // (ecs_Ah_vec_sync_mod.f90:51-53); formulas of ecs_Ah_vec_sync_mod.f90:77-124 and
+        // halo_Ah_scalar_sync_mod.f90:52-93 in the reference's operand order
+        const T U = gy, V = gx;
+        T Ux, Vx, Dx, Cx, Uy, Vy, Dy, Cy;
+        if ((L || Rt) && (B || Tp)) {
+            hx_vals(L ? 0 : 1, j, Vx, Ux, Dx, Cx);
+            hy_vals(i, Vy, Uy, Dy, Cy);
+            const T qy = B ? g.qb[i] : g.qt[i];
+            const T qx = L ? g.ql[j] : g.qr[j];
+            dv = (dv + Dx + Dy) / T(3);
+            cu = (cu + Cx + Cy) / T(3);
+            gy = (U + Ux + Uy + qy * Vy) / T(3);
+            gx = (V + Vx + Vy + qx * Ux) / T(3);
+        } else if (L) {
+            hx_vals(0, j, Vx, Ux, Dx, Cx);
+            dv = T(0.5) * (Dx + dv);
+            cu = T(0.5) * (Cx + cu);
+            gy = T(0.5) * (Ux + U);
+            gx = T(0.5) * (V + Vx + g.ql[j] * Ux);
+        } else if (Rt) {
+            hx_vals(1, j, Vx, Ux, Dx, Cx);
+            dv = T(0.5) * (dv + Dx);
+            cu = T(0.5) * (cu + Cx);
+            gy = T(0.5) * (U + Ux);
+            gx = T(0.5) * (V + Vx + g.qr[j] * Ux);
+        } else if (B) {
+            hy_vals(i, Vy, Uy, Dy, Cy);
+            dv = T(0.5) * (dv + Dy);
+            cu = T(0.5) * (cu + Cy);
+            gx = T(0.5) * (Vy + V);
+            gy = T(0.5) * (U + Uy + g.qb[i] * Vy);
+        } else {
+            hy_vals(i, Vy, Uy, Dy, Cy);
+            dv = T(0.5) * (dv + Dy);
+            cu = T(0.5) * (cu + Cy);
+            gx = T(0.5) * (V + Vy);
+            gy = T(0.5) * (U + Uy + g.qt[i] * Vy);
         }
+    }
+
+    if constexpr (MODE == MODE_CURL) {
+        a.out[id] = cu;
+        return;
+    }
+    // Coriolis and tendencies (coriolis_colocated_mod.f90:97-98, operator_swm_mod.f90:95-101)
+    const T h = y[id], u = y[id + g.fs], v = y[id + 2 * g.fs];
+    const T ut = q11 * u + q12 * v;
+    const T vt = q12 * u + q22 * v;
+    const T hu = h * ut, hv = h * vt;
+    const T fz = g.f[id] + cu;
+    const T cor_u = fz * hv / h * Jv;
+    const T cor_v = -(fz * hu / h * Jv);
+    const T kh = -dv;
+    const T ku = -gx + cor_u;
+    const T kv = -gy + cor_v;
+
+    if constexpr (MODE == MODE_RHS) {
+        a.out[id] = kh;
+        a.out[id + g.fs] = ku;
+        a.out[id + 2 * g.fs] = kv;
+    } else if constexpr (MODE == MODE_STAGE1) {
+        a.out[id] = h + a.cy * kh;
+        a.out[id + g.fs] = u + a.cy * ku;
+        a.out[id + 2 * g.fs] = v + a.cy * kv;
+        a.acc[id] = h + a.cacc * kh;
+        a.acc[id + g.fs] = u + a.cacc * ku;
+        a.acc[id + 2 * g.fs] = v + a.cacc * kv;
+    } else if constexpr (MODE == MODE_STAGE23) {
+        a.out[id] = a.v0[id] + a.cy * kh;
+        a.out[id + g.fs] = a.v0[id + g.fs] + a.cy * ku;
+        a.out[id + 2 * g.fs] = a.v0[id + 2 * g.fs] + a.cy * kv;
+        a.acc[id] = a.acc[id] + a.cacc * kh;
+        a.acc[id + g.fs] = a.acc[id + g.fs] + a.cacc * ku;
+        a.acc[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cacc * kv;
+    } else if constexpr (MODE == MODE_STAGE4) {
+        a.out[id] = a.acc[id] + a.cy * kh;
+        a.out[id + g.fs] = a.acc[id + g.fs] + a.cy * ku;
+        a.out[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cy * kv;
     }
 }
 
-// out = s1*v1 + s2*v2 (assign) or out = out + s1*v1 + s2*v2 (update) over the mesh points of
-// all three fields (grid_field_mod.f90:430-485 via stvec_swm_mod.f90:38-106)
-template <typename T, bool UPDATE>
-__global__ void axpby_kernel(T *out, T s1, const T *v1, T s2, const T *v2, int n, int pitch, long long fs) {
-    const long long total = 3LL * 6 * n * (long long)n;
-    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total; k += (long long)gridDim.x * blockDim.x) {
-        const long long row = k / n;
-        const int i = (int)(k - row * n);
-        const long long id = row * pitch + i;
-        if (UPDATE)
-            out[id] = out[id] + s1 * v1[id] + s2 * v2[id];
-        else
-            out[id] = s1 * v1[id] + s2 * v2[id];
+}  // namespace swm
+
+namespace swm {
+
+template <int ORD, typename T, int TY, bool OTF>
+hipError_t launch_one(int mode, const DevGeo<T> &g, const StageArgs<T> &a, hipStream_t s) {
+    constexpr int TX = 64;
+    const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
+    const dim3 grid(6 * ntx * nty), block(TX * TY);
+    switch (mode) {
+    case MODE_RHS: hipLaunchKernelGGL((stage_kernel<ORD, T, MODE_RHS, TX, TY, OTF>), grid, block, 0, s, g, a); break;
+    case MODE_STAGE1: hipLaunchKernelGGL((stage_kernel<ORD, T, MODE_STAGE1, TX, TY, OTF>), grid, block, 0, s, g, a); break;
+    case MODE_STAGE23: hipLaunchKernelGGL((stage_kernel<ORD, T, MODE_STAGE23, TX, TY, OTF>), grid, block, 0, s, g, a); break;
+    case MODE_STAGE4: hipLaunchKernelGGL((stage_kernel<ORD, T, MODE_STAGE4, TX, TY, OTF>), grid, block, 0, s, g, a); break;
+    case MODE_CURL: hipLaunchKernelGGL((stage_kernel<ORD, T, MODE_CURL, TX, TY, OTF>), grid, block, 0, s, g, a); break;
+    default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+template <int ORD>
+hipError_t launch_stage_order(const StageLaunch &r) {
+    if (r.prec == 64) {
+        if (r.ty == 16)
+            return r.otf ? launch_one<ORD, double, 16, true>(r.mode, r.gd, r.ad, r.stream)
+                         : launch_one<ORD, double, 16, false>(r.mode, r.gd, r.ad, r.stream);
+        return r.otf ? launch_one<ORD, double, 8, true>(r.mode, r.gd, r.ad, r.stream)
+                     : launch_one<ORD, double, 8, false>(r.mode, r.gd, r.ad, r.stream);
+    }
+    if (r.ty == 16) return launch_one<ORD, float, 16, false>(r.mode, r.gf, r.af, r.stream);
+    return launch_one<ORD, float, 8, false>(r.mode, r.gf, r.af, r.stream);
 }
 
 }  // namespace swm

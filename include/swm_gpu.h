@@ -96,6 +96,15 @@ int swm_ic_spherical_harmonics(swm_ctx *ctx, unsigned long long seed, int lmax, 
                                double h_amp, double v_max, double *h, double *u, double *v,
                                int halo_width);
 
+/* Tile configuration of the fused stage kernel (no counterpart in the reference): tile height in
+ * rows of 64 points (8 or 16) and, for FP64, whether the metric terms J, Qi are read from memory
+ * (0) or evaluated on the fly from tan(alpha) (1; identical values).  swm_ctx_autotune times every
+ * configuration with `reps` launches of the RK stage-2 kernel on `state`, keeps the fastest and
+ * returns its time per launch. */
+int swm_ctx_set_tiling(swm_ctx *ctx, int tile_rows, int metric_on_the_fly);
+int swm_ctx_get_tiling(const swm_ctx *ctx, int *tile_rows, int *metric_on_the_fly);
+int swm_ctx_autotune(swm_ctx *ctx, void *state, int reps, double *best_us);
+
 /* Measurement helper (no counterpart in the reference): `reps` back-to-back launches of one
  * fused RK stage kernel (stage_id 1, 2 or 4) reading `state`, on the context stream; outputs go
  * to internal work buffers. */

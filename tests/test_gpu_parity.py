@@ -32,9 +32,11 @@ def _relerr(G, a, b):
     return float(np.max(np.abs(ai - bi)) / max(np.max(np.abs(bi)), 1e-300))
 
 
-@pytest.mark.parametrize("N,order", [(32, 42), (17, 42), (24, 43), (24, 63)])
-def test_rhs_matches_oracle(oracle_lib, N, order):
+@pytest.mark.parametrize("tiling", [(8, False), (16, True)])
+@pytest.mark.parametrize("N,order", [(32, 42), (17, 42), (8, 42), (24, 43), (24, 63), (70, 63)])
+def test_rhs_matches_oracle(oracle_lib, N, order, tiling):
     swm, op, G, h, u, v = _setup(oracle_lib, N, order)
+    op.set_tiling(*tiling)
     print(f"[rhs] N={N} order={order}", flush=True)
     x = swm.create_stvec_swm(op)
     k = x.create_similar()
@@ -48,10 +50,12 @@ def test_rhs_matches_oracle(oracle_lib, N, order):
     assert max(errs) < 1e-13
 
 
-@pytest.mark.parametrize("N,order,nsteps,dt", [(32, 42, 10, 900.0), (24, 63, 4, 900.0)])
-def test_rk4_matches_oracle(oracle_lib, N, order, nsteps, dt):
+@pytest.mark.parametrize("tiling", [(8, False), (16, False), (8, True), (16, True)])
+@pytest.mark.parametrize("N,order,nsteps,dt", [(32, 42, 10, 900.0), (24, 63, 4, 900.0), (40, 43, 3, 600.0)])
+def test_rk4_matches_oracle(oracle_lib, N, order, nsteps, dt, tiling):
     swm, op, G, h, u, v = _setup(oracle_lib, N, order, seed=3)
-    print(f"[rk4] N={N} order={order} steps={nsteps}", flush=True)
+    op.set_tiling(*tiling)
+    print(f"[rk4] N={N} order={order} steps={nsteps} tiling={tiling}", flush=True)
     x = swm.create_stvec_swm(op)
     x.upload(h, u, v, G.hw)
     swm.create_timescheme(op, "rk4").steps(x, dt, nsteps)
@@ -102,6 +106,20 @@ def test_device_geometry_is_host_geometry(oracle_lib):
     swm, op, G, h, u, v = _setup(oracle_lib, 16, 42)
     for which, name in enumerate(("J", "Qi11", "Qi12", "Qi22", "f")):
         assert np.array_equal(G.interior(op.debug_geometry(which, G.hw)), G.interior(G.geometry(name))), name
+
+
+def test_autotune_keeps_results_bitwise(oracle_lib):
+    swm, op, G, h, u, v = _setup(oracle_lib, 48, 42, seed=9)
+    x = swm.create_stvec_swm(op)
+    x.upload(h, u, v, G.hw)
+    us = op.autotune(x, reps=5)
+    print("  autotune", op.get_tiling(), us)
+    assert us > 0
+    k = x.create_similar()
+    op.apply(k, x)
+    ref = G.rhs(h.copy(), u.copy(), v.copy())
+    for a, b in zip(k.download(G.hw), ref):
+        assert np.array_equal(G.interior(a), G.interior(b))
 
 
 def test_bad_operator_name_is_an_error():
