@@ -458,6 +458,8 @@ extern "C" int swm_ctx_set_tiling(swm_ctx *c, int tile_rows, int metric_on_the_f
     if (!c) return fail(SWM_EARG, "ctx is NULL");
     if (tile_rows != 8 && tile_rows != 16) return fail(SWM_EARG, "tile_rows must be 8 or 16");
     if (metric_on_the_fly && c->prec != 64) return fail(SWM_EARG, "metric on the fly is FP64 only");
+    // a tile must not touch both the j=1 and the j=n edge of a panel
+    if (c->n <= tile_rows) return fail(SWM_EARG, "tile_rows must be smaller than N+1");
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     c->tile_ty = tile_rows;
@@ -486,6 +488,7 @@ extern "C" int swm_ctx_autotune(swm_ctx *c, void *state, int reps, double *best_
     double best = 1e300;
     Cand pick = cands[0];
     for (const Cand &cd : cands) {
+        if (c->n <= cd.ty) continue;
         c->tile_ty = cd.ty;
         c->otf = cd.otf;
         int rc = 0;
