@@ -19,7 +19,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-function"]
-HIP_SOURCES = ["swm_capi.hip", "stage_o42.hip", "stage_o43.hip", "stage_o63.hip"]
+HIP_SOURCES = ["swm_capi.hip", "stage_o21.hip", "stage_o42.hip", "stage_o43.hip", "stage_o63.hip"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 
 

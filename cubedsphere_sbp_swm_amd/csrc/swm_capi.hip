@@ -111,6 +111,7 @@ int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, vo
     }
     hipError_t e;
     switch (c->grad_order) {
+    case 21: e = swm::launch_stage_o21(r); break;
     case 42: e = swm::launch_stage_o42(r); break;
     case 43: e = swm::launch_stage_o43(r); break;
     case 63: e = swm::launch_stage_o63(r); break;
@@ -195,7 +196,6 @@ int parse_names(swm_ctx *c, const char *div, const char *grad, const char *curl,
     // the fused stage kernel pairs gradient and divergence of the same SBP family
     const bool ok = (dk == go) || (dk == 2 && go == 21);
     if (!ok) return fail(SWM_ENAME, "unsupported gradient/divergence pair: " + gr + " + " + d);
-    if (dk == 2) return fail(SWM_ENAME, "divergence_ah2 (sbp21 family) is not built on the device path yet");
     c->div_kind = dk;
     c->grad_order = go;
     c->quad_order = qo;

@@ -21,6 +21,7 @@ struct StageLaunch {
     StageArgs<float> af;
 };
 
+hipError_t launch_stage_o21(const StageLaunch &r);
 hipError_t launch_stage_o42(const StageLaunch &r);
 hipError_t launch_stage_o43(const StageLaunch &r);
 hipError_t launch_stage_o63(const StageLaunch &r);

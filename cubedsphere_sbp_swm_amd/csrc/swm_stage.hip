@@ -75,6 +75,52 @@ __device__ __forceinline__ void neighbour_raw(const DevGeo<T> &g, const int *ei,
     out[3] = (dxC + dyC) / Jd;
 }
 
+// Coriolis term, tendencies and RK stage combination of one point
+// (coriolis_colocated_mod.f90:97-98, operator_swm_mod.f90:95-101, rk4_mod.f90:28-40)
+template <typename T, int MODE>
+__device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs<T> &a, long long id, T Jv, T q11,
+                                             T q12, T q22, T gx, T gy, T dv, T cu) {
+    const T *y = a.y;
+    if constexpr (MODE == MODE_CURL) {
+        a.out[id] = cu;
+        return;
+    }
+    const T h = y[id], u = y[id + g.fs], v = y[id + 2 * g.fs];
+    const T ut = q11 * u + q12 * v;
+    const T vt = q12 * u + q22 * v;
+    const T hu = h * ut, hv = h * vt;
+    const T fz = g.f[id] + cu;
+    const T cor_u = fz * hv / h * Jv;
+    const T cor_v = -(fz * hu / h * Jv);
+    const T kh = -dv;
+    const T ku = -gx + cor_u;
+    const T kv = -gy + cor_v;
+
+    if constexpr (MODE == MODE_RHS) {
+        a.out[id] = kh;
+        a.out[id + g.fs] = ku;
+        a.out[id + 2 * g.fs] = kv;
+    } else if constexpr (MODE == MODE_STAGE1) {
+        a.out[id] = h + a.cy * kh;
+        a.out[id + g.fs] = u + a.cy * ku;
+        a.out[id + 2 * g.fs] = v + a.cy * kv;
+        a.acc[id] = h + a.cacc * kh;
+        a.acc[id + g.fs] = u + a.cacc * ku;
+        a.acc[id + 2 * g.fs] = v + a.cacc * kv;
+    } else if constexpr (MODE == MODE_STAGE23) {
+        a.out[id] = a.v0[id] + a.cy * kh;
+        a.out[id + g.fs] = a.v0[id + g.fs] + a.cy * ku;
+        a.out[id + 2 * g.fs] = a.v0[id + 2 * g.fs] + a.cy * kv;
+        a.acc[id] = a.acc[id] + a.cacc * kh;
+        a.acc[id + g.fs] = a.acc[id + g.fs] + a.cacc * ku;
+        a.acc[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cacc * kv;
+    } else if constexpr (MODE == MODE_STAGE4) {
+        a.out[id] = a.acc[id] + a.cy * kh;
+        a.out[id + g.fs] = a.acc[id + g.fs] + a.cy * ku;
+        a.out[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cy * kv;
+    }
+}
+
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
 __global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T> a) {
     using C = Tile<ORD, T, TX, TY>;
@@ -271,45 +317,7 @@ __global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T
         }
     }
 
-    if constexpr (MODE == MODE_CURL) {
-        a.out[id] = cu;
-        return;
-    }
-    // Coriolis and tendencies (coriolis_colocated_mod.f90:97-98, operator_swm_mod.f90:95-101)
-    const T h = y[id], u = y[id + g.fs], v = y[id + 2 * g.fs];
-    const T ut = q11 * u + q12 * v;
-    const T vt = q12 * u + q22 * v;
-    const T hu = h * ut, hv = h * vt;
-    const T fz = g.f[id] + cu;
-    const T cor_u = fz * hv / h * Jv;
-    const T cor_v = -(fz * hu / h * Jv);
-    const T kh = -dv;
-    const T ku = -gx + cor_u;
-    const T kv = -gy + cor_v;
-
-    if constexpr (MODE == MODE_RHS) {
-        a.out[id] = kh;
-        a.out[id + g.fs] = ku;
-        a.out[id + 2 * g.fs] = kv;
-    } else if constexpr (MODE == MODE_STAGE1) {
-        a.out[id] = h + a.cy * kh;
-        a.out[id + g.fs] = u + a.cy * ku;
-        a.out[id + 2 * g.fs] = v + a.cy * kv;
-        a.acc[id] = h + a.cacc * kh;
-        a.acc[id + g.fs] = u + a.cacc * ku;
-        a.acc[id + 2 * g.fs] = v + a.cacc * kv;
-    } else if constexpr (MODE == MODE_STAGE23) {
-        a.out[id] = a.v0[id] + a.cy * kh;
-        a.out[id + g.fs] = a.v0[id + g.fs] + a.cy * ku;
-        a.out[id + 2 * g.fs] = a.v0[id + 2 * g.fs] + a.cy * kv;
-        a.acc[id] = a.acc[id] + a.cacc * kh;
-        a.acc[id + g.fs] = a.acc[id + g.fs] + a.cacc * ku;
-        a.acc[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cacc * kv;
-    } else if constexpr (MODE == MODE_STAGE4) {
-        a.out[id] = a.acc[id] + a.cy * kh;
-        a.out[id + g.fs] = a.acc[id + g.fs] + a.cy * ku;
-        a.out[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cy * kv;
-    }
+    finish_point<T, MODE>(g, a, id, Jv, q11, q12, q22, gx, gy, dv, cu);
 }
 
 }  // namespace swm

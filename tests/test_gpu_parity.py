@@ -33,7 +33,8 @@ def _relerr(G, a, b):
 
 
 @pytest.mark.parametrize("tiling", [(8, False), (16, True)])
-@pytest.mark.parametrize("N,order", [(32, 42), (17, 42), (8, 42), (24, 43), (24, 63), (70, 63)])
+@pytest.mark.parametrize("N,order", [(32, 42), (17, 42), (8, 42), (24, 43), (24, 63), (70, 63), (32, 21), (9, 21),
+                                     (70, 21)])
 def test_rhs_matches_oracle(oracle_lib, N, order, tiling):
     if N + 1 <= tiling[0]:
         pytest.skip("tile taller than the panel")
@@ -53,7 +54,8 @@ def test_rhs_matches_oracle(oracle_lib, N, order, tiling):
 
 
 @pytest.mark.parametrize("tiling", [(8, False), (16, False), (8, True), (16, True)])
-@pytest.mark.parametrize("N,order,nsteps,dt", [(32, 42, 10, 900.0), (24, 63, 4, 900.0), (40, 43, 3, 600.0)])
+@pytest.mark.parametrize("N,order,nsteps,dt", [(32, 42, 10, 900.0), (24, 63, 4, 900.0), (40, 43, 3, 600.0),
+                                               (32, 21, 10, 900.0)])
 def test_rk4_matches_oracle(oracle_lib, N, order, nsteps, dt, tiling):
     swm, op, G, h, u, v = _setup(oracle_lib, N, order, seed=3)
     op.set_tiling(*tiling)
@@ -91,17 +93,31 @@ def test_fused_rk4_equals_operator_composition(oracle_lib):
         assert np.array_equal(G.interior(f1), G.interior(f2))
 
 
-@pytest.mark.parametrize("order", [42, 63])
+@pytest.mark.parametrize("order", [42, 63, 21])
 def test_diagnostics_match_oracle(oracle_lib, order):
     swm, op, G, h, u, v = _setup(oracle_lib, 24, order, seed=2)
     x = swm.create_stvec_swm(op)
     x.upload(h, u, v, G.hw)
     d = op.get_diagnostics(x)
-    ref = G.diagnostics(42 if order == 42 else 63, h.copy(), u.copy(), v.copy())
+    ref = G.diagnostics({42: 42, 63: 63, 21: 21}[order], h.copy(), u.copy(), v.copy())
     ours = [d[k] for k in ("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy")]
     print("  diag", ours, ref.tolist())
     for a, b in zip(ours, ref):
         assert abs(a - b) <= 1e-12 * abs(b)
+
+
+def test_c32_sbp21_100_steps(oracle_lib):
+    """Plain method on CPU configuration: C32 sbp21 FP64, RK4 dt=900 s, 100 steps, degree <= 8
+    seeded spherical-harmonic state; the device trajectory equals the oracle's."""
+    swm, op, G, h, u, v = _setup(oracle_lib, 32, 21, seed=0, lmax=8)
+    x = swm.create_stvec_swm(op)
+    x.upload(h, u, v, G.hw)
+    swm.create_timescheme(op, "rk4").steps(x, 900.0, 100)
+    gh, gu, gv = x.download(G.hw)
+    G.rk4(h, u, v, 900.0, 100)
+    errs = [_relerr(G, a, b) for a, b in ((gh, h), (gu, u), (gv, v))]
+    print(f"  C32 sbp21 100 steps rel err h,u,v = {errs}")
+    assert max(errs) < 1e-12
 
 
 def test_device_geometry_is_host_geometry(oracle_lib):
