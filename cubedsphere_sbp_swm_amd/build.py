@@ -1,8 +1,10 @@
-"""Build the device library libswm_gpu.so in-tree (gfx950).
+"""Build the device library libswm_gpu.so in-tree (gfx950) and the Fortran host driver.
 
 Host set-up code (geometry.cpp) is compiled by the host C++ compiler, the HIP sources by
 hipcc for gfx950 only; both with -ffp-contract=off so that FP64 results follow the
-reference's operation order without fused multiply-adds.
+reference's operation order without fused multiply-adds.  The Fortran binding module
+(fortran/swm_gpu_mod.f90) and the namelist driver (fortran/swm_gpu_driver.f90) are compiled
+with amdflang into _build/swm_gpu_driver, linked against libswm_gpu.so by rpath.
 """
 import os
 import shutil
@@ -21,6 +23,9 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch
              "-Wno-unused-result", "-Wno-unused-function"]
 HIP_SOURCES = ["swm_capi.hip", "stage_o21.hip", "stage_o42.hip", "stage_o43.hip", "stage_o63.hip"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(BUILD, "swm_gpu_driver")
 
 
 def _run(cmd, verbose):
@@ -50,7 +55,18 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the ISO_C_BINDING module and the namelist driver (needs libswm_gpu.so)."""
+    os.makedirs(BUILD, exist_ok=True)
+    mod_o = os.path.join(BUILD, "swm_gpu_mod.o")
+    _run([FC, "-O2", "-J", BUILD, "-c", os.path.join(FORTRAN, "swm_gpu_mod.f90"), "-o", mod_o], verbose)
+    _run([FC, "-O2", "-J", BUILD, os.path.join(FORTRAN, "swm_gpu_driver.f90"), mod_o, "-L" + HERE,
+          "-lswm_gpu", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER], verbose)
+    return DRIVER
 
 
 if __name__ == "__main__":
