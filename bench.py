@@ -3,7 +3,7 @@
 
 Workload (default): C384 (6 x 384 x 384 cells, 6 x 385^2 Ah points), sbp42 operators
 (divergence_ah42_sbp, gradient_ah42_sbp_ecs, curl_divergence_ah42_sbp), FP64, RK4 with
-dt = 150 s, Earth radius / gravity / rotation, seeded spherical-harmonic initial state
+dt = 75 s (gravity-wave Courant number 0.9; 150 s is beyond RK4's stability limit), Earth radius / gravity / rotation, seeded spherical-harmonic initial state
 (h = 1e4 m + perturbation of degree <= 32 with max 100 m, max |V| = 40 m/s).
 
 A "step" is one RK4 time step (4 fused stage kernels) over the whole sphere; the metric
@@ -44,7 +44,114 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp32",
+                    help="comma-separated extra forms to measure beside the FP64 headline: fp32")
+    # precision form: C768 sbp42, RK4 at gravity-wave Courant 0.9 (37.5 s), degree <= 64
+    ap.add_argument("--fp32-N", type=int, default=768)
+    ap.add_argument("--fp32-steps", type=int, default=1000)
+    ap.add_argument("--fp32-dt", type=float, default=37.5)
+    ap.add_argument("--fp32-lmax", type=int, default=64)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--tiling", default="auto",
+                    help="stage-kernel tiling: auto (autotuner) or ROWS[,otf] with ROWS 8 or 16")
     return ap.parse_args()
+
+
+def profiled_traffic(N, order, prec, tiling):
+    """HBM bytes per stage-2/3 launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-kernel
+    calibrated) for this workload and tiling, from the committed profile summary; None if absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"stage23_c{N}_sbp{order}_f{prec}.md")
+    if not os.path.exists(path):
+        return None
+    key = f"C{N} sbp{order} f{prec} rows={tiling['tile_rows']} otf={int(tiling['metric_on_the_fly'])}"
+    for line in open(path):
+        m = re.match(r"traffic_bytes_per_launch (.*): (\d+)\s*$", line)
+        if m and m.group(1) == key:
+            return int(m.group(2))
+    return None
+
+
+def _timed_steps(ts, x, dt, steps, stream):
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    ts.steps(x, dt, steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3
+
+
+def fp32_form(args, stream):
+    """FP32 stencils / halo / RK stages (metric precomputed in FP64, rounded once; FP64
+    compensated integrals) against the FP64 device path on the same initial state."""
+    import numpy as np
+    import cubedsphere_sbp_swm_amd as swm
+    N, dt, steps = args.fp32_N, args.fp32_dt, args.fp32_steps
+    ops, res = {}, {}  # ops keeps both contexts alive until the comparison is done
+    h0 = u0 = v0 = None
+    for prec in (64, 32):
+        op = swm.create_swm_operator(N, **swm.operator_names(42), scale=swm.EARTH["radius"],
+                                     omega=swm.EARTH["omega"], grav=swm.EARTH["grav"], precision=prec)
+        op.set_stream(stream.cuda_stream)
+        if h0 is None:
+            h0, u0, v0 = op.initial_state(seed=0, lmax=args.fp32_lmax)
+        x = swm.create_stvec_swm(op)
+        x.upload(h0, u0, v0)
+        op.autotune(x, reps=20)
+        x.upload(h0, u0, v0)
+        d0 = op.get_diagnostics(x)
+        ts = swm.create_timescheme(op, "rk4")
+        ts.steps(x, dt, 5)  # warm-up / graph capture
+        x.upload(h0, u0, v0)
+        secs = _timed_steps(ts, x, dt, steps, stream)
+        d1 = op.get_diagnostics(x)
+        res[prec] = {"secs": secs, "fields": x.download(), "d0": d0, "d1": d1, "tiling": op.get_tiling(),
+                     "eb": op.elem_bytes}
+        ops[prec] = (op, x)
+    cells, points = 6 * N * N, 6 * (N + 1) ** 2
+    errs = {}
+    for name, a, b in zip(("h", "u", "v"), res[32]["fields"], res[64]["fields"]):
+        d = a - b
+        errs[name] = {"rel_l2": float(np.sqrt(np.sum(d * d) / np.sum(b * b))),
+                      "rel_linf": float(np.max(np.abs(d)) / np.max(np.abs(b)))}
+    hp = res[64]["fields"][0] - 1.0e4
+    dh = res[32]["fields"][0] - res[64]["fields"][0]
+    errs["h_minus_1e4"] = {"rel_l2": float(np.sqrt(np.sum(dh * dh) / np.sum(hp * hp))),
+                           "rel_linf": float(np.max(np.abs(dh)) / np.max(np.abs(hp)))}
+    drift = {p: (res[p]["d1"]["mass"] - res[p]["d0"]["mass"]) / res[p]["d0"]["mass"] for p in (32, 64)}
+    rate = {p: cells * steps / res[p]["secs"] for p in (32, 64)}
+    step_gbs32 = STEP_VALUES * 4 * points / (res[32]["secs"] / steps) / 1e9
+    return {"workload": f"C{N} sbp42 RK4 dt={dt:g}s x {steps} steps, degree <= {args.fp32_lmax}",
+            "fp32_cell_steps_per_s": rate[32], "fp64_cell_steps_per_s": rate[64],
+            "speedup_vs_fp64": rate[32] / rate[64],
+            "fp32_ms_per_step": res[32]["secs"] * 1e3 / steps, "fp64_ms_per_step": res[64]["secs"] * 1e3 / steps,
+            "fp32_step_effective_GBs": step_gbs32, "fp32_step_frac": step_gbs32 / HBM_PEAK_GBS,
+            "error_vs_fp64_gpu": errs, "mass_drift": {"fp32": drift[32], "fp64": drift[64]},
+            "energy_drift": {f"fp{p}": (res[p]["d1"]["TE"] - res[p]["d0"]["TE"]) / res[p]["d0"]["TE"]
+                             for p in (32, 64)},
+            "tiling": {"fp32": res[32]["tiling"], "fp64": res[64]["tiling"]}}
+
+
+def copy_bandwidth(stream, mib):
+    """Measured device-to-device copy rate (read + write bytes / time) of torch's copy kernel."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.empty(n, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    dst = torch.empty_like(src)
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        reps = 10
+        for _ in range(reps):
+            dst.copy_(src)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    secs = e0.elapsed_time(e1) / 1e3 / reps
+    return {"bytes_per_copy": 2 * n * 8, "us_per_copy": secs * 1e6, "GBs": 2 * n * 8 / secs / 1e9}
 
 
 def cpu_baseline(N, order, dt, lmax, budget_s, op):
@@ -91,7 +198,12 @@ def main():
     h, u, v = op.initial_state(seed=0, lmax=args.lmax)
     x = swm.create_stvec_swm(op)
     x.upload(h, u, v)
-    tuned_us = op.autotune(x, reps=30)
+    if args.tiling == "auto":
+        tuned_us = op.autotune(x, reps=30)
+    else:
+        rows, _, otf = args.tiling.partition(",")
+        op.set_tiling(int(rows), otf == "otf")
+        tuned_us = None
     tiling = op.get_tiling()
     ts = swm.create_timescheme(op, "rk4")
     ts.steps(x, dt, args.warmup)
@@ -128,6 +240,12 @@ def main():
     step_bytes = STEP_VALUES * eb * points
     step_gbs = step_bytes / ((ms / 1e3) / args.steps) / 1e9
 
+    forms = {}
+    wanted = [f for f in args.forms.split(",") if f]
+    if "fp32" in wanted:
+        forms["fp32"] = fp32_form(args, stream)
+    copy = copy_bandwidth(stream, args.copy_mib) if args.copy_mib > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N, order, dt, args.lmax, args.cpu_seconds, op)
@@ -150,12 +268,15 @@ def main():
                    "parallelism": "single GPU, all six panels resident", "dt": dt,
                    "tiling": tiling, "autotune_stage_us": tuned_us},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(N, order, args.precision, tiling),
                      "kernel": "stage_kernel<42, double, MODE_STAGE23, 64, %d, %s> (RK stages 2 and 3)" % (
                          tiling["tile_rows"], str(tiling["metric_on_the_fly"]).lower()),
                      "kernel_avg_us": k_us, "algorithmic_bytes_per_launch": stage_bytes,
-                     "step_effective_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS},
+                     "step_effective_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
+                     "measured_copy": copy},
         "cpu_baseline": cpu,
+        "forms": forms,
         "diagnostics_after": {k: diag[k] for k in ("mass", "TE", "hmin", "hmax")},
     }
     print(json.dumps(out), flush=True)
