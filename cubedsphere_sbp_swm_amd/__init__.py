@@ -57,6 +57,7 @@ class OperatorSWM:
         self.ctx = h
         self.N = N
         self.precision = precision
+        self.quadrature_name = quadrature_name
         n, pitch, fe, eb = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
         check(L.swm_ctx_info(self.ctx, ctypes.byref(n), ctypes.byref(pitch), ctypes.byref(fe), ctypes.byref(eb)))
         self.n, self.pitch, self.field_elems, self.elem_bytes = n.value, pitch.value, fe.value, eb.value
@@ -128,6 +129,28 @@ class OperatorSWM:
         check(_capi.lib().swm_ctx_autotune(self.ctx, ctypes.c_void_p(v.ptr), int(reps), ctypes.byref(best)))
         return best.value
 
+    def quadrature_weights(self):
+        """(6, n, n) weights w = J hx^2 A_i A_j of the operator's SBP quadrature on the unit sphere
+        (sbp_quadrature_mod.f90:21-61): sum(w * f) is the quadrature of f."""
+        L = _capi.lib()
+        n, N = self.n, self.N
+        pitch = self.pitch
+        J = np.zeros(6 * n * pitch)
+        L.swmh_geometry(N, pitch, 1.0, (ctypes.c_double * 3)(0.0, 0.0, 1.0), _dp(J), None, None, None, None,
+                        None, None, None)
+        A = np.zeros(n)
+        order = {"SBP_Ah21_quadrature": 21, "SBP_Ah42_quadrature": 42, "SBP_Ah63_quadrature": 63}[self.quadrature_name]
+        check(0 if L.swmh_quad_weights(N, order, _dp(A)) == 0 else -1)
+        hx = (np.pi / 2) / N
+        J = J.reshape(6, n, pitch)[:, :, :n]
+        return J * hx * hx * A[None, :, None] * A[None, None, :]
+
+    # transposed linearised right-hand side (adjoint form)
+    def rhs_vjp(self, out, y, mu):
+        """out = (dF/dy)^T mu at state y (FP64, SBP families 42/43/63)."""
+        check(_capi.lib().swm_rhs_vjp(self.ctx, ctypes.c_void_p(y.ptr), ctypes.c_void_p(mu.ptr),
+                                      ctypes.c_void_p(out.ptr)))
+
     def debug_geometry(self, which, halo_width=0):
         out = np.zeros(self.host_shape(halo_width))
         check(_capi.lib().swm_debug_geometry(self.ctx, int(which), _dp(out), int(halo_width)))
@@ -170,6 +193,29 @@ class StvecSWM:
         check(_capi.lib().swm_state_download(self.op.ctx, ctypes.c_void_p(self.ptr), _dp(h), _dp(u), _dp(v),
                                              int(halo_width)))
         return h, u, v
+
+    def from_tensors(self, h, u, v):
+        """Copy three contiguous device tensors of shape (6, n, n) into this state (enqueued on
+        the operator's stream after the tensors' producers have finished)."""
+        import torch
+        ts = [h, u, v]
+        for t in ts:
+            if t.shape != (6, self.op.n, self.op.n) or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError("expected contiguous device tensors of shape (6, n, n)")
+        torch.cuda.current_stream().synchronize()
+        check(_capi.lib().swm_state_copy_from_device(self.op.ctx, ctypes.c_void_p(self.ptr),
+                                                     *(ctypes.c_void_p(t.data_ptr()) for t in ts)))
+
+    def to_tensors(self, h, u, v):
+        """Copy this state into three contiguous device tensors of shape (6, n, n); returns after
+        the copy has completed."""
+        ts = [h, u, v]
+        for t in ts:
+            if t.shape != (6, self.op.n, self.op.n) or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError("expected contiguous device tensors of shape (6, n, n)")
+        check(_capi.lib().swm_state_copy_to_device(self.op.ctx, ctypes.c_void_p(self.ptr),
+                                                   *(ctypes.c_void_p(t.data_ptr()) for t in ts)))
+        self.op.synchronize()
 
     # stvec_t%assign / update (stvec_swm_mod.f90:38-80)
     def assign(self, s1, v1, s2, v2):

@@ -41,6 +41,13 @@ SIGNATURES = {
     "swm_bench_stage": (_i, [_vp, _vp, _i, _i]),
     "swm_debug_geometry": (_i, [_vp, _i, _dp, _i]),
     "swm_debug_halo_table": (_i, [_vp, _ip]),
+    "swm_rhs_vjp": (_i, [_vp, _vp, _vp, _vp]),
+    "swm_adj_create": (_i, [_vp, _i, _d, _i, ctypes.POINTER(_vp)]),
+    "swm_adj_destroy": (_i, [_vp]),
+    "swm_adj_forward": (_i, [_vp, _vp, _vp]),
+    "swm_adj_backward": (_i, [_vp, _vp, _vp]),
+    "swm_state_copy_from_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "swm_state_copy_to_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
 }
 
 # host set-up routines (geometry.h), exported by the same library; they need no device
@@ -50,6 +57,7 @@ HOST_SIGNATURES = {
     "swmh_quad_weights": (_i, [_i, _i, _dp]),
     "swmh_halo_table": (None, [_i, _i, _ip]),
     "swmh_ic_sh": (_i, [_i, _i, ctypes.c_ulonglong, _i, _d, _d, _d, _dp, _dp, _dp, _i]),
+    "swmh_edge_sync_matrix": (_i, [_i, _i, _ip, _ip, _dp, _i]),
 }
 
 _lib = None

@@ -368,3 +368,135 @@ extern "C" int swmh_ic_sh(int N, int pitch, unsigned long long seed, int lmax, d
         }
     return 0;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Edge syncs as explicit sparse matrices over the panel-edge points (used by the adjoint).
+//
+// Slot numbering of the 4n-4 edge points of a panel (0-based i, j):
+//   j = 0: slot i;  j = n-1: slot n+i;  i = 0 (0<j<n-1): slot 2n+j-1;  i = n-1 (0<j<n-1): 3n+j-3.
+// Global slot = p*(4n-4) + slot.  Scalar entries: (row slot, column slot, weight).  Vector
+// entries index component c (0 = x, 1 = y) as c*6*(4n-4) + global slot.
+//
+// The scalar matrix is halo_Ah_scalar_sync_mod.f90:52-93 (halves at edges, thirds at cube
+// corners) and the vector matrix the covariant ecs_Ah_vec_sync_mod.f90:77-124, called with
+// (y, x) components as the gradient does; the neighbour's copy of a shared point and the
+// component mapping of the vector halo come from the neighbour table above.
+namespace {
+
+int edge_slot(int n, int i, int j) {
+    if (j == 0) return i;
+    if (j == n - 1) return n + i;
+    if (i == 0) return 2 * n + j - 1;
+    return 3 * n + j - 3;
+}
+
+struct EdgeBuilder {
+    int n, es;
+    std::vector<int> tab;
+    int *rows, *cols;
+    double *w;
+    int cap, nnz = 0;
+    EdgeBuilder(int N, int *r, int *c, double *ww, int cp) : n(N + 1), es(4 * N), rows(r), cols(c), w(ww), cap(cp) {
+        tab.assign((size_t)6 * 4 * n * 4, 0);
+        swmh_halo_table(N, 1, tab.data());
+    }
+    const int *nb(int p, int e, int k) const { return &tab[(((size_t)p * 4 + e) * n + k) * 4]; }
+    void add(int r, int c, double v) {
+        if (nnz < cap) { rows[nnz] = r; cols[nnz] = c; w[nnz] = v; }
+        ++nnz;
+    }
+    int gslot(int p, int i, int j) const { return p * es + edge_slot(n, i, j); }
+    // neighbour copy of the shared point k of edge e: global slot
+    int nslot(int p, int e, int k) const { const int *t = nb(p, e, k); return gslot(t[0], t[1], t[2]); }
+    // component `c` (0 x, 1 y) of the vector halo at (e, k): neighbour component and sign
+    void ncomp(int p, int e, int k, int c, int &col, double &sg) const {
+        const int *t = nb(p, e, k);
+        const int code = t[3];
+        const int src = c == 0 ? ((code & 1) ? 1 : 0) : ((code & 1) ? 0 : 1);
+        sg = c == 0 ? ((code & 2) ? -1.0 : 1.0) : ((code & 4) ? -1.0 : 1.0);
+        col = src * 6 * es + gslot(t[0], t[1], t[2]);
+    }
+};
+
+}  // namespace
+
+extern "C" int swmh_edge_sync_matrix(int N, int vector, int *rows, int *cols, double *w, int cap) {
+    EdgeBuilder b(N, rows, cols, w, cap);
+    const int n = b.n, e = n - 1, V = 6 * b.es;
+    if (!vector) {
+        for (int p = 0; p < 6; ++p) {
+            for (int m = 1; m < n - 1; ++m) {
+                const int side[4][3] = {{0, m, 0}, {e, m, 1}, {m, 0, 2}, {m, e, 3}};  // i, j, edge
+                for (auto &s : side) {
+                    const int r = b.gslot(p, s[0], s[1]);
+                    b.add(r, r, 0.5);
+                    b.add(r, b.nslot(p, s[2], m), 0.5);
+                }
+            }
+            const int corner[4][6] = {{0, 0, 0, 0, 2, 0}, {e, 0, 1, 0, 2, e}, {0, e, 0, e, 3, 0}, {e, e, 1, e, 3, e}};
+            for (auto &c : corner) {
+                const int r = b.gslot(p, c[0], c[1]);
+                b.add(r, r, 1.0 / 3.0);
+                b.add(r, b.nslot(p, c[2], c[3]), 1.0 / 3.0);
+                b.add(r, b.nslot(p, c[4], c[5]), 1.0 / 3.0);
+            }
+        }
+        return b.nnz;
+    }
+    std::vector<double> qb(n), qt(n), ql(n), qr(n);
+    swmh_sync_weights(N, qb.data(), qt.data(), ql.data(), qr.data());
+    auto X = [&](int s) { return s; };
+    auto Y = [&](int s) { return V + s; };
+    auto halo = [&](int row, int p, int ed, int k, int c, double wt) {
+        int col;
+        double sg;
+        b.ncomp(p, ed, k, c, col, sg);
+        b.add(row, col, sg * wt);
+    };
+    for (int p = 0; p < 6; ++p) {
+        // cube corners: (i, j, first edge, k1, second edge, k2, weight arrays)
+        {
+            const int s = b.gslot(p, 0, 0);
+            b.add(Y(s), Y(s), 1.0 / 3); halo(Y(s), p, 0, 0, 1, 1.0 / 3); halo(Y(s), p, 2, 0, 1, 1.0 / 3);
+            halo(Y(s), p, 2, 0, 0, qb[0] / 3);
+            b.add(X(s), X(s), 1.0 / 3); halo(X(s), p, 0, 0, 0, 1.0 / 3); halo(X(s), p, 2, 0, 0, 1.0 / 3);
+            halo(X(s), p, 0, 0, 1, ql[0] / 3);
+        }
+        {
+            const int s = b.gslot(p, e, 0);
+            b.add(Y(s), Y(s), 1.0 / 3); halo(Y(s), p, 1, 0, 1, 1.0 / 3); halo(Y(s), p, 2, e, 1, 1.0 / 3);
+            halo(Y(s), p, 2, e, 0, qb[e] / 3);
+            b.add(X(s), X(s), 1.0 / 3); halo(X(s), p, 1, 0, 0, 1.0 / 3); halo(X(s), p, 2, e, 0, 1.0 / 3);
+            halo(X(s), p, 1, 0, 1, qr[0] / 3);
+        }
+        {
+            const int s = b.gslot(p, 0, e);
+            b.add(Y(s), Y(s), 1.0 / 3); halo(Y(s), p, 0, e, 1, 1.0 / 3); halo(Y(s), p, 3, 0, 1, 1.0 / 3);
+            halo(Y(s), p, 3, 0, 0, qt[0] / 3);
+            b.add(X(s), X(s), 1.0 / 3); halo(X(s), p, 0, e, 0, 1.0 / 3); halo(X(s), p, 3, 0, 0, 1.0 / 3);
+            halo(X(s), p, 0, e, 1, ql[e] / 3);
+        }
+        {
+            const int s = b.gslot(p, e, e);
+            b.add(Y(s), Y(s), 1.0 / 3); halo(Y(s), p, 1, e, 1, 1.0 / 3); halo(Y(s), p, 3, e, 1, 1.0 / 3);
+            halo(Y(s), p, 3, e, 0, qt[e] / 3);
+            b.add(X(s), X(s), 1.0 / 3); halo(X(s), p, 1, e, 0, 1.0 / 3); halo(X(s), p, 3, e, 0, 1.0 / 3);
+            halo(X(s), p, 1, e, 1, qr[e] / 3);
+        }
+        for (int m = 1; m < n - 1; ++m) {
+            int s = b.gslot(p, 0, m);  // i = 0 edge
+            b.add(Y(s), Y(s), 0.5); halo(Y(s), p, 0, m, 1, 0.5);
+            b.add(X(s), X(s), 0.5); halo(X(s), p, 0, m, 0, 0.5); halo(X(s), p, 0, m, 1, 0.5 * ql[m]);
+            s = b.gslot(p, e, m);      // i = n-1 edge
+            b.add(Y(s), Y(s), 0.5); halo(Y(s), p, 1, m, 1, 0.5);
+            b.add(X(s), X(s), 0.5); halo(X(s), p, 1, m, 0, 0.5); halo(X(s), p, 1, m, 1, 0.5 * qr[m]);
+            s = b.gslot(p, m, 0);      // j = 0 edge
+            b.add(X(s), X(s), 0.5); halo(X(s), p, 2, m, 0, 0.5);
+            b.add(Y(s), Y(s), 0.5); halo(Y(s), p, 2, m, 1, 0.5); halo(Y(s), p, 2, m, 0, 0.5 * qb[m]);
+            s = b.gslot(p, m, e);      // j = n-1 edge
+            b.add(X(s), X(s), 0.5); halo(X(s), p, 3, m, 0, 0.5);
+            b.add(Y(s), Y(s), 0.5); halo(Y(s), p, 3, m, 1, 0.5); halo(Y(s), p, 3, m, 0, 0.5 * qt[m]);
+        }
+    }
+    return b.nnz;
+}

@@ -32,6 +32,14 @@ int swmh_quad_weights(int N, int order, double *A);
 // component negated, bit 2 = second component negated (vector halo convention).
 void swmh_halo_table(int N, int ndepth, int *tab);
 
+// Edge syncs as sparse matrices over the 6*(4n-4) panel-edge points (slot numbering: j = 0:
+// i; j = n-1: n+i; i = 0: 2n+j-1; i = n-1: 3n+j-3; global slot p*(4n-4)+slot).  vector = 0:
+// the scalar sync (halo_Ah_scalar_sync_mod.f90:52-93), entries over slots; vector = 1: the
+// covariant vector sync (ecs_Ah_vec_sync_mod.f90:77-124), entries over c*6*(4n-4)+slot with
+// component c = 0 (x) or 1 (y).  Writes at most `cap` (row, col, weight) triplets and
+// returns the number of entries.
+int swmh_edge_sync_matrix(int N, int vector, int *rows, int *cols, double *w, int cap);
+
 // Seeded initial state: h = h0 + h' (max|h'| = h_amp), V = r x grad(psi) (max|V| = v_max),
 // h', psi random real spherical-harmonic series of degree 1..lmax with N(0,1)*l^-2
 // coefficients drawn from a splitmix64/Box-Muller stream of `seed`; covariant components

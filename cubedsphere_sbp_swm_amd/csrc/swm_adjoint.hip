@@ -1,0 +1,545 @@
+// Hand-written adjoint of the Ah-grid SWM right-hand side and of the RK4 step, FP64, SBP
+// families 42 / 43 / 63 (included into swm_capi.hip).
+//
+// Forward map at a state y = (h, u, v) (operator_swm_mod.f90:61-113):
+//   ut, vt = Qi (u, v);  K = (u ut + v vt)/2 + g h (+ g hs)
+//   G  = Vsync( Dx K, Dy K ) / (hx a)                 gradient + covariant vector sync
+//   z  = Ssync( Dx(J v/J) + Dy(J (-u/J)) ) / (J hx a) curl through the divergence
+//   dh = -Ssync( Dx(J h ut) + Dy(J h vt) ) / (J hx a)
+//   du = -Gx + (f + z) vt J,   dv = -Gy - (f + z) ut J
+// The vector-Jacobian product g = (dF/dy)^T mu is evaluated as
+//   1. adj_pre:   pointwise seeds of the three linear operators
+//                 Ldiv = -mu_h, Lz = J (vt mu_u - ut mu_v), LG = -(mu_u, mu_v)/(hx a),
+//                 already divided by J hx a off the panel edges;
+//   2. adj_edge + adj_scatter: the transposed edge syncs on the panel-edge points (the scalar
+//                 sync is an averaging projector, hence symmetric; the transposed covariant
+//                 vector sync is a sparse matrix built on the host), then the division;
+//   3. adj_main:  transposed SBP derivatives D^T along x and y (boundary columns from the
+//                 closure blocks, interior columns -D), the pointwise chain rule back to
+//                 (h, u, v), and the RK4 reverse-stage combination.
+// RK4 reverse step for y1 = y0 + dt/2 k1, y2 = y0 + dt/2 k2, y3 = y0 + dt k3,
+// y+ = y0 + dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4 (rk4_mod.f90:28-40), given L = dJ/dy+:
+//   g3 = J3^T (dt/6 L);  MU = dt/3 L + dt g3;    ACC = L + g3
+//   g2 = J2^T MU;        MU = dt/3 L + dt/2 g2;  ACC += g2
+//   g1 = J1^T MU;        MU = dt/6 L + dt/2 g1;  ACC += g1
+//   g0 = J0^T MU;        L  = ACC + g0
+// Stage inputs y1..y3 and the vorticity of each are recomputed from the step's state with
+// the forward stage kernels; step states come from checkpoints every C steps and a
+// recomputed segment of C states.
+
+namespace swm {
+
+enum AdjMode : int { ADJ_VJP = 0, ADJ_S4 = 1, ADJ_S32 = 2, ADJ_S1 = 3 };
+
+struct V3 {
+    double a, b, c;
+};
+__device__ __forceinline__ V3 operator+(const V3 &x, const V3 &y) { return {x.a + y.a, x.b + y.b, x.c + y.c}; }
+__device__ __forceinline__ V3 operator*(double w, const V3 &x) { return {w * x.a, w * x.b, w * x.c}; }
+
+template <int ORD>
+struct SbpT {
+    using S = Sbp<ORD>;
+    static constexpr int R = -S::INSHIFT;
+    static constexpr int maxlast() {
+        int m = 0;
+        for (int k = 0; k < S::NE; ++k) m = S::last[k] > m ? S::last[k] : m;
+        return m;
+    }
+    static constexpr int W = (maxlast() - 1 > R) ? maxlast() - 1 : R;        // column bandwidth of D
+    static constexpr int B = (S::NE + R > maxlast()) ? S::NE + R : maxlast(); // boundary columns
+    static constexpr int MIN_N = B + W + S::NE;                               // left/right blocks disjoint
+    // D[r][c] of the left block and the interior (valid for r < n - NE)
+    static constexpr double left(int r, int c) {
+        if (r < S::NE) return (c >= 0 && c < S::last[r]) ? S::Ql[r][c] : 0.0;
+        const int k = c - r - S::INSHIFT;
+        return (k >= 0 && k < S::WIN) ? S::Qin[k] : 0.0;
+    }
+};
+
+template <int ORD, int C, typename G, int... Rs>
+__device__ __forceinline__ V3 dT_col(const G &get, std::integer_sequence<int, Rs...>) {
+    V3 s{0.0, 0.0, 0.0};
+    auto term = [&](auto rI) {
+        constexpr int r = decltype(rI)::value;
+        constexpr double w = SbpT<ORD>::left(r, C);
+        if constexpr (w != 0.0) s = s + w * get(r);
+    };
+    (term(std::integral_constant<int, Rs>{}), ...);
+    return s;
+}
+
+template <int ORD, typename G, int... Cs>
+__device__ __forceinline__ V3 dT_left(int c, const G &get, std::integer_sequence<int, Cs...>) {
+    V3 out{0.0, 0.0, 0.0};
+    constexpr int NR = SbpT<ORD>::B + SbpT<ORD>::W;
+    ((c == Cs ? (out = dT_col<ORD, Cs>(get, std::make_integer_sequence<int, NR>{}), 0) : 0), ...);
+    return out;
+}
+
+// (D^T y)_c for a line of n points; get(r) returns the three transported fields at row r
+template <int ORD, typename G>
+__device__ __forceinline__ V3 sbp_col_T(int c, int n, const G &get) {
+    using S = Sbp<ORD>;
+    using T = SbpT<ORD>;
+    if (c >= T::B && c < n - T::B) {
+        // interior columns: sum_k Qin[k] y[c + R - k]
+        V3 s{0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < S::WIN; ++k)
+            if (S::Qin[k] != 0.0) s = s + S::Qin[k] * get(c + T::R - k);
+        return s;
+    }
+    if (c < T::B) return dT_left<ORD>(c, get, std::make_integer_sequence<int, T::B>{});
+    // right block: D[n-1-r][n-1-c] = -D[r][c]
+    const V3 m = dT_left<ORD>(n - 1 - c, [&](int r) { return get(n - 1 - r); }, std::make_integer_sequence<int, T::B>{});
+    return {-m.a, -m.b, -m.c};
+}
+
+struct AdjGeo {
+    const double *J, *Qi11, *Qi12, *Qi22, *f;
+    int n, pitch, es;
+    long long fs;
+    double hxs, grav;
+};
+
+__device__ __forceinline__ void slot_point(int g, int n, int es, int &p, int &i, int &j) {
+    p = g / es;
+    const int s = g - p * es;
+    if (s < n) { i = s; j = 0; }
+    else if (s < 2 * n) { i = s - n; j = n - 1; }
+    else if (s < 3 * n - 2) { i = 0; j = s - 2 * n + 1; }
+    else { i = n - 1; j = s - (3 * n - 2) + 1; }
+}
+
+__global__ void adj_pre(AdjGeo g, const double *__restrict__ y, const double *__restrict__ M, double alpha,
+                        double *__restrict__ W) {
+    const int n = g.n;
+    const long long total = 6LL * n * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % n);
+        const long long pj = t / n;
+        const int j = (int)(pj % n);
+        const int p = (int)(pj / n);
+        const long long o = ((long long)p * n + j) * g.pitch + i;
+        const double u = y[g.fs + o], v = y[2 * g.fs + o];
+        const double Jv = g.J[o];
+        const double ut = g.Qi11[o] * u + g.Qi12[o] * v;
+        const double vt = g.Qi12[o] * u + g.Qi22[o] * v;
+        const double mh = alpha * M[o], mu = alpha * M[g.fs + o], mv = alpha * M[2 * g.fs + o];
+        const bool edge = i == 0 || j == 0 || i == n - 1 || j == n - 1;
+        const double z = Jv * (vt * mu - ut * mv);
+        const double dnm = Jv * g.hxs;
+        W[o] = edge ? -mh : -mh / dnm;
+        W[g.fs + o] = edge ? z : z / dnm;
+        W[2 * g.fs + o] = -mu / g.hxs;
+        W[3 * g.fs + o] = -mv / g.hxs;
+    }
+}
+
+// transposed edge syncs: E[k][slot] for k = 0 (S Ldiv), 1 (S Lz), 2/3 (V^T LG x/y)
+__global__ void adj_edge(AdjGeo g, const double *__restrict__ W, const int *__restrict__ s_ptr,
+                         const int *__restrict__ s_col, const double *__restrict__ s_w, const int *__restrict__ v_ptr,
+                         const int *__restrict__ v_col, const double *__restrict__ v_w, double *__restrict__ E) {
+    const int ns = 6 * g.es;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ns) return;
+    auto off = [&](int slot) {
+        int p, i, j;
+        slot_point(slot, g.n, g.es, p, i, j);
+        return ((long long)p * g.n + j) * g.pitch + i;
+    };
+    double a = 0.0, b = 0.0;
+    for (int k = s_ptr[r]; k < s_ptr[r + 1]; ++k) {
+        const long long o = off(s_col[k]);
+        a += s_w[k] * W[o];
+        b += s_w[k] * W[g.fs + o];
+    }
+    const long long o = off(r);
+    const double dnm = g.J[o] * g.hxs;
+    E[r] = a / dnm;
+    E[ns + r] = b / dnm;
+    for (int c = 0; c < 2; ++c) {
+        double s = 0.0;
+        const int row = c * ns + r;
+        for (int k = v_ptr[row]; k < v_ptr[row + 1]; ++k) {
+            const int col = v_col[k];
+            const int cc = col >= ns ? 1 : 0;
+            s += v_w[k] * W[(2 + cc) * g.fs + off(col - cc * ns)];
+        }
+        E[(2 + c) * ns + r] = s;
+    }
+}
+
+__global__ void adj_scatter(AdjGeo g, const double *__restrict__ E, double *__restrict__ W) {
+    const int ns = 6 * g.es;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ns) return;
+    int p, i, j;
+    slot_point(r, g.n, g.es, p, i, j);
+    const long long o = ((long long)p * g.n + j) * g.pitch + i;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) W[k * g.fs + o] = E[k * ns + r];
+}
+
+struct AdjArgs {
+    const double *y, *M, *zeta, *W;
+    double alpha;
+    double *out;         // ADJ_VJP
+    double *L, *MU, *ACC;
+    double a, b;         // MU = a L + b g
+};
+
+template <int ORD, int MODE>
+__global__ void __launch_bounds__(256) adj_main(AdjGeo g, AdjArgs A) {
+    const int n = g.n;
+    const long long total = 6LL * n * n;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int i = (int)(t % n);
+    const long long pj = t / n;
+    const int j = (int)(pj % n);
+    const int p = (int)(pj / n);
+    const long long row = ((long long)p * n + j) * g.pitch;
+    const long long o = row + i;
+    const double *W0 = A.W, *W1 = A.W + g.fs, *W2 = A.W + 2 * g.fs, *W3 = A.W + 3 * g.fs;
+    const V3 dx = sbp_col_T<ORD>(i, n, [&](int r) { return V3{W0[row + r], W1[row + r], W2[row + r]}; });
+    const long long col = (long long)p * n * g.pitch + i;
+    const V3 dy = sbp_col_T<ORD>(j, n, [&](int r) {
+        const long long q = col + (long long)r * g.pitch;
+        return V3{W0[q], W1[q], W3[q]};
+    });
+    const double Jv = g.J[o];
+    const double lFx = Jv * dx.a, lFy = Jv * dy.a, lAx = Jv * dx.b, lAy = Jv * dy.b, lK = dx.c + dy.c;
+    const double h = A.y[o], u = A.y[g.fs + o], v = A.y[2 * g.fs + o];
+    const double q11 = g.Qi11[o], q12 = g.Qi12[o], q22 = g.Qi22[o];
+    const double ut = q11 * u + q12 * v, vt = q12 * u + q22 * v;
+    const double qz = g.f[o] + A.zeta[o];
+    const double mu = A.alpha * A.M[g.fs + o], mv = A.alpha * A.M[2 * g.fs + o];
+    const double lut = h * lFx - qz * Jv * mv;
+    const double lvt = h * lFy + qz * Jv * mu;
+    const double gh = ut * lFx + vt * lFy + g.grav * lK;
+    const double gu = ut * lK - lAy / Jv + q11 * lut + q12 * lvt;
+    const double gv = vt * lK + lAx / Jv + q12 * lut + q22 * lvt;
+    const double gg[3] = {gh, gu, gv};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const long long ok = k * g.fs + o;
+        if constexpr (MODE == ADJ_VJP) {
+            A.out[ok] = gg[k];
+        } else if constexpr (MODE == ADJ_S4) {
+            const double l = A.L[ok];
+            A.MU[ok] = A.a * l + A.b * gg[k];
+            A.ACC[ok] = l + gg[k];
+        } else if constexpr (MODE == ADJ_S32) {
+            A.MU[ok] = A.a * A.L[ok] + A.b * gg[k];
+            A.ACC[ok] = A.ACC[ok] + gg[k];
+        } else {
+            A.L[ok] = A.ACC[ok] + gg[k];
+        }
+    }
+}
+
+}  // namespace swm
+
+// ------------------------------------------------------------------------------------------------
+// Host side
+namespace {
+
+struct AdjWork {
+    int es = 0, ns = 0;
+    int *s_ptr = nullptr, *s_col = nullptr, *v_ptr = nullptr, *v_col = nullptr;
+    double *s_w = nullptr, *v_w = nullptr;
+    double *W = nullptr, *E = nullptr, *Z = nullptr;  // 4 fields, 4*ns, 4 fields
+};
+
+void free_adj_work(AdjWork *w) {
+    if (!w) return;
+    void *ptrs[] = {w->s_ptr, w->s_col, w->v_ptr, w->v_col, w->s_w, w->v_w, w->W, w->E, w->Z};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete w;
+}
+
+// CSR of the scalar sync and of the TRANSPOSED covariant vector sync
+int build_csr(int N, int vector, bool transpose, int nrows, std::vector<int> &ptr, std::vector<int> &col,
+              std::vector<double> &w) {
+    int nnz = swmh_edge_sync_matrix(N, vector, nullptr, nullptr, nullptr, 0);
+    std::vector<int> r(nnz), c(nnz);
+    std::vector<double> v(nnz);
+    if (swmh_edge_sync_matrix(N, vector, r.data(), c.data(), v.data(), nnz) != nnz) return -1;
+    if (transpose) std::swap(r, c);
+    ptr.assign(nrows + 1, 0);
+    for (int k = 0; k < nnz; ++k) {
+        if (r[k] < 0 || r[k] >= nrows || c[k] < 0 || c[k] >= nrows) return -1;
+        ptr[r[k] + 1]++;
+    }
+    for (int k = 0; k < nrows; ++k) ptr[k + 1] += ptr[k];
+    col.assign(nnz, 0);
+    w.assign(nnz, 0.0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int k = 0; k < nnz; ++k) {
+        const int at = fill[r[k]]++;
+        col[at] = c[k];
+        w[at] = v[k];
+    }
+    return 0;
+}
+
+template <typename V>
+int upload_vec(V **dst, const std::vector<V> &src) {
+    HIP_TRY(hipMalloc((void **)dst, src.size() * sizeof(V)));
+    HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice));
+    return SWM_OK;
+}
+
+int adj_min_n(int order) {
+    switch (order) {
+    case 42: return swm::SbpT<42>::MIN_N;
+    case 43: return swm::SbpT<43>::MIN_N;
+    case 63: return swm::SbpT<63>::MIN_N;
+    default: return 1 << 30;
+    }
+}
+
+int ensure_adj(swm_ctx *c) {
+    if (c->adj) return SWM_OK;
+    if (c->prec != 64) return fail(SWM_EARG, "the adjoint runs in FP64 contexts only");
+    if (c->grad_order != 42 && c->grad_order != 43 && c->grad_order != 63)
+        return fail(SWM_EARG, "the adjoint covers the SBP families 42, 43 and 63");
+    if (c->n < adj_min_n(c->grad_order))
+        return fail(SWM_EARG, "the adjoint needs N+1 >= " + std::to_string(adj_min_n(c->grad_order)) + " for this operator");
+    AdjWork *w = new AdjWork();
+    w->es = 4 * c->N;
+    w->ns = 6 * w->es;
+    std::vector<int> sp, sc, vp, vc;
+    std::vector<double> sw, vw;
+    int rc = SWM_OK;
+    if (build_csr(c->N, 0, false, w->ns, sp, sc, sw) || build_csr(c->N, 1, true, 2 * w->ns, vp, vc, vw)) {
+        free_adj_work(w);
+        return fail(SWM_EARG, "edge sync matrix construction failed");
+    }
+    if ((rc = upload_vec(&w->s_ptr, sp)) || (rc = upload_vec(&w->s_col, sc)) || (rc = upload_vec(&w->s_w, sw)) ||
+        (rc = upload_vec(&w->v_ptr, vp)) || (rc = upload_vec(&w->v_col, vc)) || (rc = upload_vec(&w->v_w, vw))) {
+        free_adj_work(w);
+        return rc;
+    }
+    const size_t fb = c->field_bytes();
+    if (hipMalloc((void **)&w->W, 4 * fb) != hipSuccess || hipMalloc((void **)&w->E, 4 * sizeof(double) * w->ns) != hipSuccess ||
+        hipMalloc((void **)&w->Z, 4 * fb) != hipSuccess || hipMemset(w->W, 0, 4 * fb) != hipSuccess ||
+        hipMemset(w->Z, 0, 4 * fb) != hipSuccess) {
+        free_adj_work(w);
+        return fail(SWM_EHIP, "adjoint work allocation failed");
+    }
+    c->adj = w;
+    return SWM_OK;
+}
+
+swm::AdjGeo adj_geo(const swm_ctx *c) {
+    swm::AdjGeo g;
+    g.J = c->J64; g.Qi11 = c->Qi64[0]; g.Qi12 = c->Qi64[1]; g.Qi22 = c->Qi64[2]; g.f = c->f64;
+    g.n = c->n; g.pitch = c->pitch; g.es = 4 * c->N; g.fs = c->fs;
+    g.hxs = c->hx * c->scale; g.grav = c->grav;
+    return g;
+}
+
+template <int ORD>
+hipError_t launch_adj_main(int mode, const swm::AdjGeo &g, const swm::AdjArgs &a, hipStream_t s) {
+    const long long total = 6LL * g.n * g.n;
+    const dim3 blk(256), grd((unsigned)((total + 255) / 256));
+    switch (mode) {
+    case swm::ADJ_VJP: hipLaunchKernelGGL((swm::adj_main<ORD, swm::ADJ_VJP>), grd, blk, 0, s, g, a); break;
+    case swm::ADJ_S4: hipLaunchKernelGGL((swm::adj_main<ORD, swm::ADJ_S4>), grd, blk, 0, s, g, a); break;
+    case swm::ADJ_S32: hipLaunchKernelGGL((swm::adj_main<ORD, swm::ADJ_S32>), grd, blk, 0, s, g, a); break;
+    default: hipLaunchKernelGGL((swm::adj_main<ORD, swm::ADJ_S1>), grd, blk, 0, s, g, a); break;
+    }
+    return hipGetLastError();
+}
+
+// one transposed RHS evaluation at state y with seed alpha*M; zeta = synced vorticity of y
+int enqueue_vjp(swm_ctx *c, int mode, const void *y, const double *zeta, const double *M, double alpha,
+                swm::AdjArgs a) {
+    AdjWork *w = c->adj;
+    const swm::AdjGeo g = adj_geo(c);
+    hipLaunchKernelGGL(swm::adj_pre, dim3(2048), dim3(256), 0, c->stream, g, (const double *)y, M, alpha, w->W);
+    const dim3 eg((w->ns + 255) / 256);
+    hipLaunchKernelGGL(swm::adj_edge, eg, dim3(256), 0, c->stream, g, (const double *)w->W, w->s_ptr, w->s_col, w->s_w,
+                       w->v_ptr, w->v_col, w->v_w, w->E);
+    hipLaunchKernelGGL(swm::adj_scatter, eg, dim3(256), 0, c->stream, g, (const double *)w->E, w->W);
+    a.y = (const double *)y;
+    a.M = M;
+    a.alpha = alpha;
+    a.zeta = zeta;
+    a.W = w->W;
+    hipError_t e;
+    switch (c->grad_order) {
+    case 42: e = launch_adj_main<42>(mode, g, a, c->stream); break;
+    case 43: e = launch_adj_main<43>(mode, g, a, c->stream); break;
+    default: e = launch_adj_main<63>(mode, g, a, c->stream); break;
+    }
+    if (e != hipSuccess) return fail(SWM_EHIP, std::string("adjoint launch: ") + hipGetErrorString(e));
+    return SWM_OK;
+}
+
+}  // namespace
+
+struct swm_adj {
+    swm_ctx *c = nullptr;
+    int nsteps = 0, every = 0, nck = 0;
+    double dt = 0.0;
+    bool have_forward = false;
+    void *work = nullptr;           // forward state
+    std::vector<void *> ck;         // checkpoints: state at steps 0, every, 2*every, ...
+    std::vector<void *> seg;        // recomputed states of one segment
+    void *ys[3] = {nullptr, nullptr, nullptr};  // stage inputs y1, y2, y3
+    void *sacc = nullptr;           // accumulator of the recomputed stages
+    void *L = nullptr, *MU = nullptr, *ACC = nullptr;
+};
+
+namespace {
+
+void free_tape(swm_adj *t) {
+    if (!t) return;
+    std::vector<void *> all = {t->work, t->ys[0], t->ys[1], t->ys[2], t->sacc, t->L, t->MU, t->ACC};
+    all.insert(all.end(), t->ck.begin(), t->ck.end());
+    all.insert(all.end(), t->seg.begin(), t->seg.end());
+    for (void *p : all)
+        if (p) (void)hipFree(p);
+    delete t;
+}
+
+int copy_state(swm_ctx *c, void *dst, const void *src) {
+    HIP_TRY(hipMemcpyAsync(dst, src, 3 * c->field_bytes(), hipMemcpyDeviceToDevice, c->stream));
+    return SWM_OK;
+}
+
+// reverse of one RK4 step whose start state is y0; L holds dJ/dy(step end) on entry and
+// dJ/dy0 on exit
+int reverse_step(swm_adj *t, const void *y0) {
+    swm_ctx *c = t->c;
+    const double dt = t->dt, c1 = 0.5 * dt, c2 = dt / 6.0, c3 = dt / 3.0;
+    const size_t fe = (size_t)c->fs;
+    int rc;
+    if ((rc = run_stage(c, swm::MODE_STAGE1, y0, t->ys[0], nullptr, t->sacc, c1, c2))) return rc;
+    if ((rc = run_stage(c, swm::MODE_STAGE23, t->ys[0], t->ys[1], y0, t->sacc, c1, c3))) return rc;
+    if ((rc = run_stage(c, swm::MODE_STAGE23, t->ys[1], t->ys[2], y0, t->sacc, dt, c3))) return rc;
+    double *Z = c->adj->Z;
+    const void *yy[4] = {y0, t->ys[0], t->ys[1], t->ys[2]};
+    for (int k = 0; k < 4; ++k)
+        if ((rc = run_stage(c, swm::MODE_CURL, yy[k], Z + k * fe, nullptr, nullptr, 0.0, 0.0))) return rc;
+    swm::AdjArgs a{};
+    a.L = (double *)t->L;
+    a.MU = (double *)t->MU;
+    a.ACC = (double *)t->ACC;
+    a.a = c3; a.b = dt;
+    if ((rc = enqueue_vjp(c, swm::ADJ_S4, yy[3], Z + 3 * fe, (const double *)t->L, c2, a))) return rc;
+    a.a = c3; a.b = c1;
+    if ((rc = enqueue_vjp(c, swm::ADJ_S32, yy[2], Z + 2 * fe, (const double *)t->MU, 1.0, a))) return rc;
+    a.a = c2; a.b = c1;
+    if ((rc = enqueue_vjp(c, swm::ADJ_S32, yy[1], Z + 1 * fe, (const double *)t->MU, 1.0, a))) return rc;
+    return enqueue_vjp(c, swm::ADJ_S1, yy[0], Z, (const double *)t->MU, 1.0, a);
+}
+
+}  // namespace
+
+extern "C" int swm_rhs_vjp(swm_ctx *c, const void *y, const void *mu, void *out) {
+    if (!c || !y || !mu || !out) return fail(SWM_EARG, "NULL argument");
+    if (out == y || out == mu) return fail(SWM_EARG, "out must differ from y and mu");
+    int rc = ensure_adj(c);
+    if (rc) return rc;
+    double *Z = c->adj->Z;
+    if ((rc = run_stage(c, swm::MODE_CURL, y, Z, nullptr, nullptr, 0.0, 0.0))) return rc;
+    swm::AdjArgs a{};
+    a.out = (double *)out;
+    return enqueue_vjp(c, swm::ADJ_VJP, y, Z, (const double *)mu, 1.0, a);
+}
+
+extern "C" int swm_adj_create(swm_ctx *c, int nsteps, double dt, int checkpoint_every, swm_adj **out) {
+    if (!c || !out) return fail(SWM_EARG, "NULL argument");
+    *out = nullptr;
+    if (nsteps < 1 || checkpoint_every < 1) return fail(SWM_EARG, "nsteps and checkpoint_every must be >= 1");
+    int rc = ensure_adj(c);
+    if (rc) return rc;
+    swm_adj *t = new swm_adj();
+    t->c = c;
+    t->nsteps = nsteps;
+    t->dt = dt;
+    t->every = checkpoint_every < nsteps ? checkpoint_every : nsteps;
+    t->nck = (nsteps + t->every - 1) / t->every;
+    const size_t sb = 3 * c->field_bytes();
+    auto alloc = [&](void **p) { return hipMalloc(p, sb) == hipSuccess && hipMemset(*p, 0, sb) == hipSuccess; };
+    bool ok = alloc(&t->work) && alloc(&t->sacc) && alloc(&t->L) && alloc(&t->MU) && alloc(&t->ACC);
+    for (int k = 0; k < 3 && ok; ++k) ok = alloc(&t->ys[k]);
+    t->ck.assign(t->nck, nullptr);
+    for (int k = 0; k < t->nck && ok; ++k) ok = alloc(&t->ck[k]);
+    t->seg.assign(t->every, nullptr);
+    for (int k = 0; k < t->every && ok; ++k) ok = alloc(&t->seg[k]);
+    if (!ok) {
+        free_tape(t);
+        return fail(SWM_EHIP, "adjoint tape allocation failed");
+    }
+    *out = t;
+    return SWM_OK;
+}
+
+extern "C" int swm_adj_destroy(swm_adj *t) {
+    if (!t) return fail(SWM_EARG, "tape is NULL");
+    (void)hipStreamSynchronize(t->c->stream);
+    free_tape(t);
+    return SWM_OK;
+}
+
+extern "C" int swm_adj_forward(swm_adj *t, const void *y0, void *yT) {
+    if (!t || !y0 || !yT) return fail(SWM_EARG, "NULL argument");
+    swm_ctx *c = t->c;
+    int rc;
+    if ((rc = copy_state(c, t->work, y0))) return rc;
+    for (int k = 0; k < t->nck; ++k) {
+        if ((rc = copy_state(c, t->ck[k], t->work))) return rc;
+        const int len = std::min(t->every, t->nsteps - k * t->every);
+        if ((rc = swm_rk4_steps(c, t->work, t->dt, len))) return rc;
+    }
+    if ((rc = copy_state(c, yT, t->work))) return rc;
+    t->have_forward = true;
+    return SWM_OK;
+}
+
+extern "C" int swm_adj_backward(swm_adj *t, const void *lamT, void *lam0) {
+    if (!t || !lamT || !lam0) return fail(SWM_EARG, "NULL argument");
+    if (!t->have_forward) return fail(SWM_ESTATE, "swm_adj_backward needs swm_adj_forward first");
+    swm_ctx *c = t->c;
+    int rc;
+    if ((rc = copy_state(c, t->L, lamT))) return rc;
+    for (int k = t->nck - 1; k >= 0; --k) {
+        const int len = std::min(t->every, t->nsteps - k * t->every);
+        if ((rc = copy_state(c, t->seg[0], t->ck[k]))) return rc;
+        for (int s = 1; s < len; ++s) {
+            if ((rc = copy_state(c, t->seg[s], t->seg[s - 1]))) return rc;
+            if ((rc = enqueue_rk4_step(c, t->seg[s], t->dt))) return rc;
+        }
+        for (int s = len - 1; s >= 0; --s)
+            if ((rc = reverse_step(t, t->seg[s]))) return rc;
+    }
+    return copy_state(c, lam0, t->L);
+}
+
+// dense (6, n, n) device arrays of the context's element type <-> pitched state
+extern "C" int swm_state_copy_from_device(swm_ctx *c, void *state, const void *h, const void *u, const void *v) {
+    if (!c || !state || !h || !u || !v) return fail(SWM_EARG, "NULL argument");
+    const void *src[3] = {h, u, v};
+    const size_t e = c->esz();
+    for (int k = 0; k < 3; ++k)
+        HIP_TRY(hipMemcpy2DAsync((char *)state + k * c->field_bytes(), c->pitch * e, src[k], c->n * e, c->n * e,
+                                 (size_t)6 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return SWM_OK;
+}
+
+extern "C" int swm_state_copy_to_device(swm_ctx *c, const void *state, void *h, void *u, void *v) {
+    if (!c || !state || !h || !u || !v) return fail(SWM_EARG, "NULL argument");
+    void *dst[3] = {h, u, v};
+    const size_t e = c->esz();
+    for (int k = 0; k < 3; ++k)
+        HIP_TRY(hipMemcpy2DAsync(dst[k], c->n * e, (const char *)state + k * c->field_bytes(), c->pitch * e, c->n * e,
+                                 (size_t)6 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return SWM_OK;
+}

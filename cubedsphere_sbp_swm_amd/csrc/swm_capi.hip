@@ -9,6 +9,7 @@
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/swm_gpu.h"
@@ -32,6 +33,9 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 constexpr int NDEPTH = 6;  // neighbour-table depths kept on the host (>= widest closure row)
+
+struct AdjWork;  // adjoint work space (swm_adjoint.hip), created on first use
+void free_adj_work(AdjWork *w);
 
 }  // namespace
 
@@ -63,6 +67,7 @@ struct swm_ctx {
     void *graph_state = nullptr;
     double graph_dt = 0.0;
     std::vector<int> host_nbr;
+    AdjWork *adj = nullptr;
 
     size_t esz() const { return prec == 64 ? 8 : 4; }
     size_t field_bytes() const { return (size_t)fs * esz(); }
@@ -210,6 +215,7 @@ void free_ctx(swm_ctx *c) {
                     c->curl, c->red};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    free_adj_work(c->adj);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -581,3 +587,4 @@ extern "C" int swm_debug_halo_table(swm_ctx *c, int *table) {
 }
 
 #include "swm_diag.hip"
+#include "swm_adjoint.hip"

@@ -110,6 +110,32 @@ int swm_ctx_autotune(swm_ctx *ctx, void *state, int reps, double *best_us);
  * to internal work buffers. */
 int swm_bench_stage(swm_ctx *ctx, void *state, int stage_id, int reps);
 
+/* ---- Adjoint form (no counterpart in the reference, which has no adjoint) ----
+ * FP64 contexts with the SBP families 42/43/63, N+1 >= 15 (42, 43) or >= 23 (63).
+ *
+ * swm_rhs_vjp: out = (dF/dy)^T mu, the transpose of the linearised right-hand side of
+ * operator_swm_t%apply (operator_swm_mod.f90:61-113) at state y, applied to mu (all three are
+ * state vectors of `ctx`, out distinct from y and mu).
+ *
+ * swm_adj_*: reverse-mode differentiation of `nsteps` RK4 steps (rk4_mod.f90:20-42) of size dt.
+ * swm_adj_forward runs the steps from y0, keeps the state every `checkpoint_every` steps and
+ * writes the final state to yT; swm_adj_backward takes lamT = dJ/dy(T) and writes
+ * lam0 = dJ/dy(0), recomputing each segment of states from its checkpoint and each step's
+ * stage inputs.  The tape owns its buffers ((nsteps/checkpoint_every + checkpoint_every + 10)
+ * state vectors) until swm_adj_destroy. */
+typedef struct swm_adj swm_adj;
+int swm_rhs_vjp(swm_ctx *ctx, const void *y, const void *mu, void *out);
+int swm_adj_create(swm_ctx *ctx, int nsteps, double dt, int checkpoint_every, swm_adj **tape);
+int swm_adj_destroy(swm_adj *tape);
+int swm_adj_forward(swm_adj *tape, const void *y0, void *yT);
+int swm_adj_backward(swm_adj *tape, const void *lamT, void *lam0);
+
+/* Device-to-device copies between a state vector and three dense device arrays of shape
+ * (6, n, n) (panel, j, i) in the context precision (e.g. the storage of framework tensors); no
+ * counterpart in the reference. */
+int swm_state_copy_from_device(swm_ctx *ctx, void *state, const void *h, const void *u, const void *v);
+int swm_state_copy_to_device(swm_ctx *ctx, const void *state, void *h, void *u, void *v);
+
 /* Diagnostics of the set-up (not part of the reference interface): device copies of the
  * metric planes (which: 0 J, 1 Qi11, 2 Qi12, 3 Qi22, 4 f) in host layout, and the cube-edge
  * neighbour table (6*4*n*2*4 ints, see geometry.h). */

@@ -78,6 +78,7 @@ class Grid:
 
     def __init__(self, N, grad_order=42, div_kind=42, scale=1.0, omega=1.0, grav=1.0):
         self.N = N
+        self.scale, self.omega, self.grav = scale, omega, grav
         self.grad_order = grad_order
         self.div_kind = div_kind
         self.h = lib().ora_create(N, grad_order, div_kind, scale, omega, grav)
@@ -142,6 +143,14 @@ class Grid:
         out = self.zeros()
         lib().ora_sbp_apply(self.h, order, direction.encode(), _p(f), _p(out))
         return out
+
+    def scalar_sync(self, f):
+        """In place: halo_Ah_scalar_sync (after a width-1 exchange)."""
+        lib().ora_scalar_sync(self.h, _p(f))
+
+    def vec_sync_covariant(self, gx, gy):
+        """In place: covariant Ah vector sync of (gx, gy) (after a width-1 vector exchange)."""
+        lib().ora_vec_sync_covariant(self.h, _p(gx), _p(gy))
 
     def div(self, u, v):
         out = self.zeros()
