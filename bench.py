@@ -44,13 +44,19 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32",
-                    help="comma-separated extra forms to measure beside the FP64 headline: fp32")
+    ap.add_argument("--forms", default="fp32,adjoint",
+                    help="comma-separated extra forms to measure beside the FP64 headline: fp32, adjoint")
     # precision form: C768 sbp42, RK4 at gravity-wave Courant 0.9 (37.5 s), degree <= 64
     ap.add_argument("--fp32-N", type=int, default=768)
     ap.add_argument("--fp32-steps", type=int, default=1000)
     ap.add_argument("--fp32-dt", type=float, default=37.5)
     ap.add_argument("--fp32-lmax", type=int, default=64)
+    # adjoint form: C192 sbp42 FP64, 200 RK4 steps at Courant 0.9 (150 s), checkpoints every 20
+    ap.add_argument("--adj-N", type=int, default=192)
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--adj-dt", type=float, default=150.0)
+    ap.add_argument("--adj-every", type=int, default=20)
+    ap.add_argument("--adj-fd", type=int, default=4, help="finite-difference directions checked")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--tiling", default="auto",
                     help="stage-kernel tiling: auto (autotuner) or ROWS[,otf] with ROWS 8 or 16")
@@ -132,6 +138,85 @@ def fp32_form(args, stream):
             "energy_drift": {f"fp{p}": (res[p]["d1"]["TE"] - res[p]["d0"]["TE"]) / res[p]["d0"]["TE"]
                              for p in (32, 64)},
             "tiling": {"fp32": res[32]["tiling"], "fp64": res[64]["tiling"]}}
+
+
+def adjoint_form(args, stream):
+    """Gradient of 0.5*sum(w (h_T - h_target)^2) w.r.t. the initial state by the hand-written
+    adjoint (checkpointed forward + reverse sweep) vs the plain FP64 forward of the same steps."""
+    import ctypes
+    import numpy as np
+    import torch
+    import cubedsphere_sbp_swm_amd as swm
+    from cubedsphere_sbp_swm_amd import _capi
+    from cubedsphere_sbp_swm_amd.autograd import AdjointTape
+    N, steps, dt, every = args.adj_N, args.adj_steps, args.adj_dt, args.adj_every
+    op = swm.create_swm_operator(N, **swm.operator_names(42), scale=swm.EARTH["radius"],
+                                 omega=swm.EARTH["omega"], grav=swm.EARTH["grav"])
+    op.set_stream(stream.cuda_stream)
+    y0 = op.initial_state(seed=0, lmax=16)
+    x = swm.create_stvec_swm(op)
+    x.upload(*op.initial_state(seed=1, lmax=16))
+    op.autotune(x, reps=20)
+    ts = swm.create_timescheme(op, "rk4")
+    ts.steps(x, dt, steps)
+    tgt = x.download()[0]
+    w = op.quadrature_weights()
+    tape = AdjointTape(op, steps, dt, every)
+    L = _capi.lib()
+    cells = 6 * N * N
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps=3):
+        fn()  # warm-up (graph capture, lazy work buffers)
+        best = 1e30
+        for _ in range(reps):
+            e0, e1 = ev(), ev()
+            torch.cuda.synchronize()
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) / 1e3)
+        return best
+
+    x.upload(*y0)
+    xT, lam, g0 = (swm.create_stvec_swm(op) for _ in range(3))
+    t_fwd = timed(lambda: ts.steps(x, dt, steps))
+    _capi.check(L.swm_adj_forward(tape.tape, ctypes.c_void_p(tape.x0.ptr), ctypes.c_void_p(xT.ptr)))
+    tape.x0.upload(*y0)
+    t_tape_fwd = timed(lambda: _capi.check(L.swm_adj_forward(tape.tape, ctypes.c_void_p(tape.x0.ptr),
+                                                              ctypes.c_void_p(xT.ptr))))
+    hT = xT.download()[0]
+    loss = 0.5 * float(np.sum(w * (hT - tgt) ** 2))
+    z = np.zeros_like(hT)
+    lam.upload(w * (hT - tgt), z, z)
+    t_bwd = timed(lambda: _capi.check(L.swm_adj_backward(tape.tape, ctypes.c_void_p(lam.ptr), ctypes.c_void_p(g0.ptr))))
+    grads = g0.download()
+
+    def loss_at(y):
+        x.upload(*y)
+        ts.steps(x, dt, steps)
+        return 0.5 * float(np.sum(w * (x.download()[0] - tgt) ** 2))
+
+    rng = np.random.default_rng(42)
+    eps, ads, fds = 1e-4, [], []
+    for _ in range(args.adj_fd):
+        d = [rng.standard_normal(y0[0].shape) * sc for sc in (1.0, 0.1, 0.1)]
+        ads.append(sum(float(np.sum(g * dd)) for g, dd in zip(grads, d)))
+        fds.append((loss_at([a + eps * dd for a, dd in zip(y0, d)]) -
+                    loss_at([a - eps * dd for a, dd in zip(y0, d)])) / (2 * eps))
+    ads, fds = np.array(ads), np.array(fds)
+    t_grad = t_tape_fwd + t_bwd
+    return {"workload": f"C{N} sbp42 FP64 RK4 dt={dt:g}s x {steps} steps, checkpoints every {every}, "
+                        "loss 0.5*sum(w (h_T - h_target)^2)",
+            "adjoint_forward_time_ratio": t_grad / t_fwd,
+            "forward_ms": t_fwd * 1e3, "gradient_ms": t_grad * 1e3,
+            "checkpointed_forward_ms": t_tape_fwd * 1e3, "reverse_sweep_ms": t_bwd * 1e3,
+            "forward_cell_steps_per_s": cells * steps / t_fwd, "loss": loss,
+            "fd_check": {"directions": int(args.adj_fd), "step": eps,
+                         "max_rel_err": float(np.max(np.abs(ads - fds) / np.abs(fds))) if len(fds) else None}}
 
 
 def copy_bandwidth(stream, mib):
@@ -244,6 +329,8 @@ def main():
     wanted = [f for f in args.forms.split(",") if f]
     if "fp32" in wanted:
         forms["fp32"] = fp32_form(args, stream)
+    if "adjoint" in wanted:
+        forms["adjoint"] = adjoint_form(args, stream)
     copy = copy_bandwidth(stream, args.copy_mib) if args.copy_mib > 0 else None
 
     cpu = None
@@ -276,6 +363,7 @@ def main():
                      "step_effective_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
                      "measured_copy": copy},
         "cpu_baseline": cpu,
+        "adjoint_forward_time_ratio": forms.get("adjoint", {}).get("adjoint_forward_time_ratio"),
         "forms": forms,
         "diagnostics_after": {k: diag[k] for k in ("mass", "TE", "hmin", "hmax")},
     }

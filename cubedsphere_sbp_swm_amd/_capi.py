@@ -73,6 +73,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise SwmError(f"{LIB_PATH} is missing: build it with cubedsphere_sbp_swm_amd/build.py "
                            "(there is no CPU fallback)")
+        # PyTorch ships its own HIP runtime under the same soname (libamdhip64.so.7) as ROCm's.
+        # Whichever is loaded first serves the whole process; when the library loads ROCm's
+        # first, PyTorch initialised afterwards finds no device.  Loading PyTorch first makes
+        # both share its runtime (the Fortran host, without PyTorch, uses ROCm's).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in {**SIGNATURES, **HOST_SIGNATURES}.items():
             fn = getattr(L, name)

@@ -11,10 +11,11 @@
 //   1. adj_pre:   pointwise seeds of the three linear operators
 //                 Ldiv = -mu_h, Lz = J (vt mu_u - ut mu_v), LG = -(mu_u, mu_v)/(hx a),
 //                 already divided by J hx a off the panel edges;
-//   2. adj_edge + adj_scatter: the transposed edge syncs on the panel-edge points (the scalar
-//                 sync is an averaging projector, hence symmetric; the transposed covariant
-//                 vector sync is a sparse matrix built on the host), then the division;
-//   3. adj_main:  transposed SBP derivatives D^T along x and y (boundary columns from the
+//   2. adj_edge:  the transposed edge syncs on the panel-edge points into a compact buffer
+//                 (the scalar sync is an averaging projector, hence symmetric; the transposed
+//                 covariant vector sync is a sparse matrix built on the host), then the division;
+//   3. adj_main:  transposed SBP derivatives D^T along x and y, reading panel-edge points from
+//                 the compact buffer (boundary columns from the
 //                 closure blocks, interior columns -D), the pointwise chain rule back to
 //                 (h, u, v), and the RK4 reverse-stage combination.
 // RK4 reverse step for y1 = y0 + dt/2 k1, y2 = y0 + dt/2 k2, y3 = y0 + dt k3,
@@ -23,9 +24,9 @@
 //   g2 = J2^T MU;        MU = dt/3 L + dt/2 g2;  ACC += g2
 //   g1 = J1^T MU;        MU = dt/6 L + dt/2 g1;  ACC += g1
 //   g0 = J0^T MU;        L  = ACC + g0
-// Stage inputs y1..y3 and the vorticity of each are recomputed from the step's state with
-// the forward stage kernels; step states come from checkpoints every C steps and a
-// recomputed segment of C states.
+// Step states come from checkpoints every C steps: the reverse sweep recomputes one segment
+// of C steps from its checkpoint with the forward stage kernels, which also store each
+// step's stage inputs y1..y3 and the synced vorticity of y0..y3 for the adjoint kernels.
 
 namespace swm {
 
@@ -103,15 +104,6 @@ struct AdjGeo {
     double hxs, grav;
 };
 
-__device__ __forceinline__ void slot_point(int g, int n, int es, int &p, int &i, int &j) {
-    p = g / es;
-    const int s = g - p * es;
-    if (s < n) { i = s; j = 0; }
-    else if (s < 2 * n) { i = s - n; j = n - 1; }
-    else if (s < 3 * n - 2) { i = 0; j = s - 2 * n + 1; }
-    else { i = n - 1; j = s - (3 * n - 2) + 1; }
-}
-
 __global__ void adj_pre(AdjGeo g, const double *__restrict__ y, const double *__restrict__ M, double alpha,
                         double *__restrict__ W) {
     const int n = g.n;
@@ -137,53 +129,44 @@ __global__ void adj_pre(AdjGeo g, const double *__restrict__ y, const double *__
     }
 }
 
-// transposed edge syncs: E[k][slot] for k = 0 (S Ldiv), 1 (S Lz), 2/3 (V^T LG x/y)
+// transposed edge syncs into E = [S Ldiv | S Lz | (V^T LG)_x | (V^T LG)_y], ns slots each.
+// Task t < ns: scalar row t for Ldiv and Lz, divided by J hx a at the slot's point; task
+// t >= ns: row t - ns of V^T (x rows, then y rows).  Column offsets are precomputed memory
+// offsets into W (vector entries include the field base of LGx / LGy).
 __global__ void adj_edge(AdjGeo g, const double *__restrict__ W, const int *__restrict__ s_ptr,
-                         const int *__restrict__ s_col, const double *__restrict__ s_w, const int *__restrict__ v_ptr,
-                         const int *__restrict__ v_col, const double *__restrict__ v_w, double *__restrict__ E) {
+                         const long long *__restrict__ s_off, const double *__restrict__ s_w,
+                         const int *__restrict__ v_ptr, const long long *__restrict__ v_off,
+                         const double *__restrict__ v_w, const long long *__restrict__ r_off, double *__restrict__ E) {
     const int ns = 6 * g.es;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ns) return;
-    auto off = [&](int slot) {
-        int p, i, j;
-        slot_point(slot, g.n, g.es, p, i, j);
-        return ((long long)p * g.n + j) * g.pitch + i;
-    };
-    double a = 0.0, b = 0.0;
-    for (int k = s_ptr[r]; k < s_ptr[r + 1]; ++k) {
-        const long long o = off(s_col[k]);
-        a += s_w[k] * W[o];
-        b += s_w[k] * W[g.fs + o];
-    }
-    const long long o = off(r);
-    const double dnm = g.J[o] * g.hxs;
-    E[r] = a / dnm;
-    E[ns + r] = b / dnm;
-    for (int c = 0; c < 2; ++c) {
-        double s = 0.0;
-        const int row = c * ns + r;
-        for (int k = v_ptr[row]; k < v_ptr[row + 1]; ++k) {
-            const int col = v_col[k];
-            const int cc = col >= ns ? 1 : 0;
-            s += v_w[k] * W[(2 + cc) * g.fs + off(col - cc * ns)];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * ns) return;
+    if (t < ns) {
+        double a = 0.0, b = 0.0;
+        for (int k = s_ptr[t]; k < s_ptr[t + 1]; ++k) {
+            const long long o = s_off[k];
+            a += s_w[k] * W[o];
+            b += s_w[k] * W[g.fs + o];
         }
-        E[(2 + c) * ns + r] = s;
+        const double dnm = g.J[r_off[t]] * g.hxs;
+        E[t] = a / dnm;
+        E[ns + t] = b / dnm;
+        return;
     }
+    const int row = t - ns;
+    double s = 0.0;
+    for (int k = v_ptr[row]; k < v_ptr[row + 1]; ++k) s += v_w[k] * W[v_off[k]];
+    E[2 * ns + row] = s;
 }
 
-__global__ void adj_scatter(AdjGeo g, const double *__restrict__ E, double *__restrict__ W) {
-    const int ns = 6 * g.es;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ns) return;
-    int p, i, j;
-    slot_point(r, g.n, g.es, p, i, j);
-    const long long o = ((long long)p * g.n + j) * g.pitch + i;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) W[k * g.fs + o] = E[k * ns + r];
+__device__ __forceinline__ int edge_slot(int n, int i, int j) {
+    if (j == 0) return i;
+    if (j == n - 1) return n + i;
+    if (i == 0) return 2 * n + j - 1;
+    return 3 * n + j - 3;
 }
 
 struct AdjArgs {
-    const double *y, *M, *zeta, *W;
+    const double *y, *M, *zeta, *W, *E;
     double alpha;
     double *out;         // ADJ_VJP
     double *L, *MU, *ACC;
@@ -202,10 +185,24 @@ __global__ void __launch_bounds__(256) adj_main(AdjGeo g, AdjArgs A) {
     const int p = (int)(pj / n);
     const long long row = ((long long)p * n + j) * g.pitch;
     const long long o = row + i;
+    // off-edge points read W, panel-edge points the synced values in E
     const double *W0 = A.W, *W1 = A.W + g.fs, *W2 = A.W + 2 * g.fs, *W3 = A.W + 3 * g.fs;
-    const V3 dx = sbp_col_T<ORD>(i, n, [&](int r) { return V3{W0[row + r], W1[row + r], W2[row + r]}; });
+    const int ns = 6 * g.es;
+    const double *E0 = A.E + (long long)p * g.es, *E1 = E0 + ns, *E2 = E0 + 2 * ns, *E3 = E0 + 3 * ns;
+    const bool jedge = j == 0 || j == n - 1, iedge = i == 0 || i == n - 1;
+    const V3 dx = sbp_col_T<ORD>(i, n, [&](int r) {
+        if (jedge || r == 0 || r == n - 1) {
+            const int sl = edge_slot(n, r, j);
+            return V3{E0[sl], E1[sl], E2[sl]};
+        }
+        return V3{W0[row + r], W1[row + r], W2[row + r]};
+    });
     const long long col = (long long)p * n * g.pitch + i;
     const V3 dy = sbp_col_T<ORD>(j, n, [&](int r) {
+        if (iedge || r == 0 || r == n - 1) {
+            const int sl = edge_slot(n, i, r);
+            return V3{E0[sl], E1[sl], E3[sl]};
+        }
         const long long q = col + (long long)r * g.pitch;
         return V3{W0[q], W1[q], W3[q]};
     });
@@ -248,14 +245,15 @@ namespace {
 
 struct AdjWork {
     int es = 0, ns = 0;
-    int *s_ptr = nullptr, *s_col = nullptr, *v_ptr = nullptr, *v_col = nullptr;
+    int *s_ptr = nullptr, *v_ptr = nullptr;
+    long long *s_off = nullptr, *v_off = nullptr, *r_off = nullptr;
     double *s_w = nullptr, *v_w = nullptr;
     double *W = nullptr, *E = nullptr, *Z = nullptr;  // 4 fields, 4*ns, 4 fields
 };
 
 void free_adj_work(AdjWork *w) {
     if (!w) return;
-    void *ptrs[] = {w->s_ptr, w->s_col, w->v_ptr, w->v_col, w->s_w, w->v_w, w->W, w->E, w->Z};
+    void *ptrs[] = {w->s_ptr, w->v_ptr, w->s_off, w->v_off, w->r_off, w->s_w, w->v_w, w->W, w->E, w->Z};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete w;
@@ -319,8 +317,26 @@ int ensure_adj(swm_ctx *c) {
         free_adj_work(w);
         return fail(SWM_EARG, "edge sync matrix construction failed");
     }
-    if ((rc = upload_vec(&w->s_ptr, sp)) || (rc = upload_vec(&w->s_col, sc)) || (rc = upload_vec(&w->s_w, sw)) ||
-        (rc = upload_vec(&w->v_ptr, vp)) || (rc = upload_vec(&w->v_col, vc)) || (rc = upload_vec(&w->v_w, vw))) {
+    // memory offsets of the slot points (field-relative) and of the vector entries (into W)
+    auto point_off = [&](int slot) {
+        const int n = c->n, p = slot / w->es, sl = slot % w->es;
+        int i, j;
+        if (sl < n) { i = sl; j = 0; }
+        else if (sl < 2 * n) { i = sl - n; j = n - 1; }
+        else if (sl < 3 * n - 2) { i = 0; j = sl - 2 * n + 1; }
+        else { i = n - 1; j = sl - (3 * n - 2) + 1; }
+        return ((long long)p * n + j) * c->pitch + i;
+    };
+    std::vector<long long> so(sc.size()), vo(vc.size()), ro(w->ns);
+    for (size_t k = 0; k < sc.size(); ++k) so[k] = point_off(sc[k]);
+    for (size_t k = 0; k < vc.size(); ++k) {
+        const int comp = vc[k] >= w->ns ? 1 : 0;
+        vo[k] = (2 + comp) * c->fs + point_off(vc[k] - comp * w->ns);
+    }
+    for (int t = 0; t < w->ns; ++t) ro[t] = point_off(t);
+    if ((rc = upload_vec(&w->s_ptr, sp)) || (rc = upload_vec(&w->s_off, so)) || (rc = upload_vec(&w->s_w, sw)) ||
+        (rc = upload_vec(&w->v_ptr, vp)) || (rc = upload_vec(&w->v_off, vo)) || (rc = upload_vec(&w->v_w, vw)) ||
+        (rc = upload_vec(&w->r_off, ro))) {
         free_adj_work(w);
         return rc;
     }
@@ -362,11 +378,10 @@ int enqueue_vjp(swm_ctx *c, int mode, const void *y, const double *zeta, const d
     AdjWork *w = c->adj;
     const swm::AdjGeo g = adj_geo(c);
     hipLaunchKernelGGL(swm::adj_pre, dim3(2048), dim3(256), 0, c->stream, g, (const double *)y, M, alpha, w->W);
-    const dim3 eg((w->ns + 255) / 256);
-    hipLaunchKernelGGL(swm::adj_edge, eg, dim3(256), 0, c->stream, g, (const double *)w->W, w->s_ptr, w->s_col, w->s_w,
-                       w->v_ptr, w->v_col, w->v_w, w->E);
-    hipLaunchKernelGGL(swm::adj_scatter, eg, dim3(256), 0, c->stream, g, (const double *)w->E, w->W);
+    hipLaunchKernelGGL(swm::adj_edge, dim3((3 * w->ns + 127) / 128), dim3(128), 0, c->stream, g, (const double *)w->W,
+                       w->s_ptr, w->s_off, w->s_w, w->v_ptr, w->v_off, w->v_w, w->r_off, w->E);
     a.y = (const double *)y;
+    a.E = w->E;
     a.M = M;
     a.alpha = alpha;
     a.zeta = zeta;
@@ -390,9 +405,11 @@ struct swm_adj {
     bool have_forward = false;
     void *work = nullptr;           // forward state
     std::vector<void *> ck;         // checkpoints: state at steps 0, every, 2*every, ...
-    std::vector<void *> seg;        // recomputed states of one segment
-    void *ys[3] = {nullptr, nullptr, nullptr};  // stage inputs y1, y2, y3
+    // one recomputed segment: per step s the start state, the stage inputs y1, y2, y3 and the
+    // synced vorticity of y0..y3 (4 fields), all written by the forward stage kernels
+    std::vector<void *> seg, ys, zs;
     void *sacc = nullptr;           // accumulator of the recomputed stages
+    void *tail = nullptr;           // end state of a segment's last step (not kept)
     void *L = nullptr, *MU = nullptr, *ACC = nullptr;
 };
 
@@ -400,9 +417,8 @@ namespace {
 
 void free_tape(swm_adj *t) {
     if (!t) return;
-    std::vector<void *> all = {t->work, t->ys[0], t->ys[1], t->ys[2], t->sacc, t->L, t->MU, t->ACC};
-    all.insert(all.end(), t->ck.begin(), t->ck.end());
-    all.insert(all.end(), t->seg.begin(), t->seg.end());
+    std::vector<void *> all = {t->work, t->sacc, t->tail, t->L, t->MU, t->ACC};
+    for (auto *v : {&t->ck, &t->seg, &t->ys, &t->zs}) all.insert(all.end(), v->begin(), v->end());
     for (void *p : all)
         if (p) (void)hipFree(p);
     delete t;
@@ -413,20 +429,33 @@ int copy_state(swm_ctx *c, void *dst, const void *src) {
     return SWM_OK;
 }
 
-// reverse of one RK4 step whose start state is y0; L holds dJ/dy(step end) on entry and
-// dJ/dy0 on exit
-int reverse_step(swm_adj *t, const void *y0) {
+// recompute `len` steps from seg[0], keeping every step's stage inputs and vorticities
+int recompute_segment(swm_adj *t, int len) {
     swm_ctx *c = t->c;
     const double dt = t->dt, c1 = 0.5 * dt, c2 = dt / 6.0, c3 = dt / 3.0;
     const size_t fe = (size_t)c->fs;
     int rc;
-    if ((rc = run_stage(c, swm::MODE_STAGE1, y0, t->ys[0], nullptr, t->sacc, c1, c2))) return rc;
-    if ((rc = run_stage(c, swm::MODE_STAGE23, t->ys[0], t->ys[1], y0, t->sacc, c1, c3))) return rc;
-    if ((rc = run_stage(c, swm::MODE_STAGE23, t->ys[1], t->ys[2], y0, t->sacc, dt, c3))) return rc;
-    double *Z = c->adj->Z;
-    const void *yy[4] = {y0, t->ys[0], t->ys[1], t->ys[2]};
-    for (int k = 0; k < 4; ++k)
-        if ((rc = run_stage(c, swm::MODE_CURL, yy[k], Z + k * fe, nullptr, nullptr, 0.0, 0.0))) return rc;
+    for (int s = 0; s < len; ++s) {
+        void *y0 = t->seg[s], **y = &t->ys[3 * s];
+        double *Z = (double *)t->zs[s];
+        void *next = s + 1 < len ? t->seg[s + 1] : t->tail;
+        if ((rc = run_stage(c, swm::MODE_STAGE1, y0, y[0], nullptr, t->sacc, c1, c2, Z))) return rc;
+        if ((rc = run_stage(c, swm::MODE_STAGE23, y[0], y[1], y0, t->sacc, c1, c3, Z + fe))) return rc;
+        if ((rc = run_stage(c, swm::MODE_STAGE23, y[1], y[2], y0, t->sacc, dt, c3, Z + 2 * fe))) return rc;
+        if ((rc = run_stage(c, swm::MODE_STAGE4, y[2], next, nullptr, t->sacc, c2, 0.0, Z + 3 * fe))) return rc;
+    }
+    return SWM_OK;
+}
+
+// reverse of step s of the recomputed segment; L holds dJ/dy(step end) on entry and
+// dJ/dy(step start) on exit
+int reverse_step(swm_adj *t, int s) {
+    swm_ctx *c = t->c;
+    const double dt = t->dt, c1 = 0.5 * dt, c2 = dt / 6.0, c3 = dt / 3.0;
+    const size_t fe = (size_t)c->fs;
+    const double *Z = (const double *)t->zs[s];
+    const void *yy[4] = {t->seg[s], t->ys[3 * s], t->ys[3 * s + 1], t->ys[3 * s + 2]};
+    int rc;
     swm::AdjArgs a{};
     a.L = (double *)t->L;
     a.MU = (double *)t->MU;
@@ -466,14 +495,17 @@ extern "C" int swm_adj_create(swm_ctx *c, int nsteps, double dt, int checkpoint_
     t->dt = dt;
     t->every = checkpoint_every < nsteps ? checkpoint_every : nsteps;
     t->nck = (nsteps + t->every - 1) / t->every;
-    const size_t sb = 3 * c->field_bytes();
-    auto alloc = [&](void **p) { return hipMalloc(p, sb) == hipSuccess && hipMemset(*p, 0, sb) == hipSuccess; };
-    bool ok = alloc(&t->work) && alloc(&t->sacc) && alloc(&t->L) && alloc(&t->MU) && alloc(&t->ACC);
-    for (int k = 0; k < 3 && ok; ++k) ok = alloc(&t->ys[k]);
+    const size_t sb = 3 * c->field_bytes(), zb = 4 * c->field_bytes();
+    auto alloc = [&](void **p, size_t b) { return hipMalloc(p, b) == hipSuccess && hipMemset(*p, 0, b) == hipSuccess; };
+    bool ok = alloc(&t->work, sb) && alloc(&t->sacc, sb) && alloc(&t->tail, sb) && alloc(&t->L, sb) &&
+              alloc(&t->MU, sb) && alloc(&t->ACC, sb);
     t->ck.assign(t->nck, nullptr);
-    for (int k = 0; k < t->nck && ok; ++k) ok = alloc(&t->ck[k]);
+    for (int k = 0; k < t->nck && ok; ++k) ok = alloc(&t->ck[k], sb);
     t->seg.assign(t->every, nullptr);
-    for (int k = 0; k < t->every && ok; ++k) ok = alloc(&t->seg[k]);
+    t->ys.assign(3 * t->every, nullptr);
+    t->zs.assign(t->every, nullptr);
+    for (int k = 0; k < t->every && ok; ++k) ok = alloc(&t->seg[k], sb) && alloc(&t->zs[k], zb);
+    for (int k = 0; k < 3 * t->every && ok; ++k) ok = alloc(&t->ys[k], sb);
     if (!ok) {
         free_tape(t);
         return fail(SWM_EHIP, "adjoint tape allocation failed");
@@ -513,12 +545,9 @@ extern "C" int swm_adj_backward(swm_adj *t, const void *lamT, void *lam0) {
     for (int k = t->nck - 1; k >= 0; --k) {
         const int len = std::min(t->every, t->nsteps - k * t->every);
         if ((rc = copy_state(c, t->seg[0], t->ck[k]))) return rc;
-        for (int s = 1; s < len; ++s) {
-            if ((rc = copy_state(c, t->seg[s], t->seg[s - 1]))) return rc;
-            if ((rc = enqueue_rk4_step(c, t->seg[s], t->dt))) return rc;
-        }
+        if ((rc = recompute_segment(t, len))) return rc;
         for (int s = len - 1; s >= 0; --s)
-            if ((rc = reverse_step(t, t->seg[s]))) return rc;
+            if ((rc = reverse_step(t, s))) return rc;
     }
     return copy_state(c, lam0, t->L);
 }

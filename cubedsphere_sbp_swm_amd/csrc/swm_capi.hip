@@ -99,7 +99,8 @@ swm::DevGeo<T> make_geo(const swm_ctx *c) {
     return g;
 }
 
-int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, void *acc, double cy, double cacc) {
+int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, void *acc, double cy, double cacc,
+              void *zeta = nullptr) {
     swm::StageLaunch r{};
     r.prec = c->prec;
     r.mode = mode;
@@ -108,11 +109,12 @@ int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, vo
     r.stream = c->stream;
     if (c->prec == 64) {
         r.gd = make_geo<double>(c);
-        r.ad = {(const double *)y, (double *)out, (const double *)v0, (double *)acc, (const double *)c->hs, cy, cacc};
+        r.ad = {(const double *)y, (double *)out, (const double *)v0, (double *)acc, (const double *)c->hs, cy, cacc,
+                (double *)zeta};
     } else {
         r.gf = make_geo<float>(c);
         r.af = {(const float *)y, (float *)out, (const float *)v0, (float *)acc, (const float *)c->hs, (float)cy,
-                (float)cacc};
+                (float)cacc, (float *)zeta};
     }
     hipError_t e;
     switch (c->grad_order) {

@@ -52,6 +52,7 @@ struct StageArgs {
     T *acc;
     const T *hs;   // orography, may be null
     T cy, cacc;
+    T *zeta;       // optional: the synced relative vorticity of y is also written here
 };
 
 // ----------------------------------------------------------------------------------------

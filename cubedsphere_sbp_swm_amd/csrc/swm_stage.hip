@@ -85,6 +85,7 @@ __device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs
         a.out[id] = cu;
         return;
     }
+    if (a.zeta) a.zeta[id] = cu;
     const T h = y[id], u = y[id + g.fs], v = y[id + 2 * g.fs];
     const T ut = q11 * u + q12 * v;
     const T vt = q12 * u + q22 * v;

@@ -121,8 +121,8 @@ int swm_bench_stage(swm_ctx *ctx, void *state, int stage_id, int reps);
  * swm_adj_forward runs the steps from y0, keeps the state every `checkpoint_every` steps and
  * writes the final state to yT; swm_adj_backward takes lamT = dJ/dy(T) and writes
  * lam0 = dJ/dy(0), recomputing each segment of states from its checkpoint and each step's
- * stage inputs.  The tape owns its buffers ((nsteps/checkpoint_every + checkpoint_every + 10)
- * state vectors) until swm_adj_destroy. */
+ * stage inputs.  The tape owns its buffers until swm_adj_destroy: nsteps/checkpoint_every
+ * checkpoints plus, for one segment, checkpoint_every x (4 state vectors + 4 fields). */
 typedef struct swm_adj swm_adj;
 int swm_rhs_vjp(swm_ctx *ctx, const void *y, const void *mu, void *out);
 int swm_adj_create(swm_ctx *ctx, int nsteps, double dt, int checkpoint_every, swm_adj **tape);
