@@ -21,7 +21,7 @@ from . import _capi
 from ._capi import SwmError, check
 
 __all__ = ["SwmError", "operator_names", "create_swm_operator", "create_stvec_swm", "create_timescheme",
-           "OperatorSWM", "StvecSWM", "RK4", "EARTH"]
+           "OperatorSWM", "StvecSWM", "RK4", "Ensemble", "EARTH"]
 
 # const_mod.f90:6-9
 EARTH = {"radius": 0.6371229e7, "grav": 9.80616, "omega": 2.0 * np.pi / (23 * 60 * 60 + 56 * 60 + 4)}
@@ -238,6 +238,58 @@ class RK4:
 
     def steps(self, v0, dt, nsteps):
         check(_capi.lib().swm_rk4_steps(self.op.ctx, ctypes.c_void_p(v0.ptr), float(dt), int(nsteps)))
+
+
+class Ensemble:
+    """Cubes of different resolutions (one operator and one state each) advanced together:
+    one persistent launch per RK stage over all members, one captured hipGraph per step."""
+
+    def __init__(self, ops, states):
+        if len(ops) != len(states) or not ops:
+            raise ValueError("one state per operator")
+        self.ops, self.states = list(ops), list(states)
+        m = len(ops)
+        ctxs = (ctypes.c_void_p * m)(*[o.ctx.value for o in ops])
+        sts = (ctypes.c_void_p * m)(*[s.ptr for s in states])
+        h = ctypes.c_void_p()
+        check(_capi.lib().swm_ens_create(ctxs, sts, m, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _capi.lib().swm_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def steps(self, dts, nsteps):
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        if d.shape != (len(self.ops),):
+            raise ValueError("one time step per member")
+        check(_capi.lib().swm_ens_rk4_steps(self.h, _dp(d), int(nsteps)))
+
+    def set_tiling(self, tile_rows=8, metric_on_the_fly=False, blocks_per_slot=1):
+        check(_capi.lib().swm_ens_set_tiling(self.h, int(tile_rows), int(bool(metric_on_the_fly)), int(blocks_per_slot)))
+
+    def get_tiling(self):
+        r, m, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_capi.lib().swm_ens_get_tiling(self.h, ctypes.byref(r), ctypes.byref(m), ctypes.byref(b)))
+        return {"tile_rows": r.value, "metric_on_the_fly": bool(m.value), "blocks_per_slot": b.value}
+
+    def autotune(self, reps=20):
+        best = ctypes.c_double()
+        check(_capi.lib().swm_ens_autotune(self.h, int(reps), ctypes.byref(best)))
+        return best.value
+
+    def bench_stage(self, reps):
+        check(_capi.lib().swm_ens_bench_stage(self.h, int(reps)))
+
+    def synchronize(self):
+        self.ops[0].synchronize()
 
 
 def create_swm_operator(N, div_op_name, grad_op_name, curl_op_name, quadrature_name, scale=1.0, omega=1.0,

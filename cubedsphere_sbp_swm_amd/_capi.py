@@ -48,6 +48,13 @@ SIGNATURES = {
     "swm_adj_backward": (_i, [_vp, _vp, _vp]),
     "swm_state_copy_from_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "swm_state_copy_to_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "swm_ens_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
+    "swm_ens_destroy": (_i, [_vp]),
+    "swm_ens_rk4_steps": (_i, [_vp, _dp, _i]),
+    "swm_ens_set_tiling": (_i, [_vp, _i, _i, _i]),
+    "swm_ens_get_tiling": (_i, [_vp, _ip, _ip, _ip]),
+    "swm_ens_autotune": (_i, [_vp, _i, _dp]),
+    "swm_ens_bench_stage": (_i, [_vp, _i]),
 }
 
 # host set-up routines (geometry.h), exported by the same library; they need no device

@@ -4,4 +4,5 @@
 
 namespace swm {
 hipError_t launch_stage_o42(const StageLaunch &r) { return launch_stage_order<42>(r); }
+hipError_t launch_ens_o42(const EnsLaunch &r) { return launch_ens_order<42>(r); }
 }  // namespace swm

@@ -4,4 +4,5 @@
 
 namespace swm {
 hipError_t launch_stage_o43(const StageLaunch &r) { return launch_stage_order<43>(r); }
+hipError_t launch_ens_o43(const EnsLaunch &r) { return launch_ens_order<43>(r); }
 }  // namespace swm

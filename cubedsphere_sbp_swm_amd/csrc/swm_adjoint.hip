@@ -347,6 +347,11 @@ int ensure_adj(swm_ctx *c) {
         free_adj_work(w);
         return fail(SWM_EHIP, "adjoint work allocation failed");
     }
+    // null-stream copies and clears complete before the context stream uses the buffers
+    if (hipDeviceSynchronize() != hipSuccess) {
+        free_adj_work(w);
+        return fail(SWM_EHIP, "adjoint set-up synchronisation failed");
+    }
     c->adj = w;
     return SWM_OK;
 }
@@ -506,7 +511,7 @@ extern "C" int swm_adj_create(swm_ctx *c, int nsteps, double dt, int checkpoint_
     t->zs.assign(t->every, nullptr);
     for (int k = 0; k < t->every && ok; ++k) ok = alloc(&t->seg[k], sb) && alloc(&t->zs[k], zb);
     for (int k = 0; k < 3 * t->every && ok; ++k) ok = alloc(&t->ys[k], sb);
-    if (!ok) {
+    if (!ok || hipDeviceSynchronize() != hipSuccess) {  // null-stream clears complete first
         free_tape(t);
         return fail(SWM_EHIP, "adjoint tape allocation failed");
     }

@@ -330,6 +330,9 @@ extern "C" int swm_ctx_create(swm_ctx **out, int N, const char *div, const char 
     c->red_blocks = 1024;
     if (hipMalloc((void **)&c->red, sizeof(double) * 16 * (c->red_blocks + 1)) != hipSuccess)
         return bail(fail(SWM_EHIP, "reduction scratch allocation failed"));
+    // the copies and clears above run on the null stream, which the context's non-blocking
+    // stream does not wait for: finish them before any kernel of the context can start
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(SWM_EHIP, "set-up synchronisation failed"));
     *out = c;
     return SWM_OK;
 }
@@ -374,7 +377,9 @@ extern "C" int swm_state_alloc(swm_ctx *c, void **state) {
     if (!c || !state) return fail(SWM_EARG, "NULL argument");
     const size_t sb = 3 * c->field_bytes();
     HIP_TRY(hipMalloc(state, sb));
-    HIP_TRY(hipMemset(*state, 0, sb));
+    // cleared on the context stream: a null-stream clear could land after a later upload
+    HIP_TRY(hipMemsetAsync(*state, 0, sb, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return SWM_OK;
 }
 
@@ -590,3 +595,4 @@ extern "C" int swm_debug_halo_table(swm_ctx *c, int *table) {
 
 #include "swm_diag.hip"
 #include "swm_adjoint.hip"
+#include "swm_ensemble.hip"

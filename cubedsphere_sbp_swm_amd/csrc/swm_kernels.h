@@ -55,6 +55,13 @@ struct StageArgs {
     T *zeta;       // optional: the synced relative vorticity of y is also written here
 };
 
+// one cube of an ensemble: its geometry and the arguments of the current RK stage
+template <typename T>
+struct EnsMember {
+    DevGeo<T> g;
+    StageArgs<T> a;
+};
+
 // ----------------------------------------------------------------------------------------
 // Pointwise stencil inputs of one point (co2contra_colocated_mod.f90:55-56,
 // massflux_colocated_mod.f90:26-27, KE_colocated_mod.f90:44, operator_swm_mod.f90:85,

@@ -21,6 +21,21 @@ struct StageLaunch {
     StageArgs<float> af;
 };
 
+// ensemble stage launch: member table (EnsMember<T>[nmembers]) and tile prefix sums on the
+// device; the persistent grid is num_cu x resident blocks per CU x blocks_per_slot
+struct EnsLaunch {
+    int prec, mode, ty;
+    bool otf;
+    hipStream_t stream;
+    const void *members;
+    const int *tile_start;
+    int nmembers, ntiles, num_cu, blocks_per_slot;
+};
+
+hipError_t launch_ens_o42(const EnsLaunch &r);
+hipError_t launch_ens_o43(const EnsLaunch &r);
+hipError_t launch_ens_o63(const EnsLaunch &r);
+
 hipError_t launch_stage_o21(const StageLaunch &r);
 hipError_t launch_stage_o42(const StageLaunch &r);
 hipError_t launch_stage_o43(const StageLaunch &r);

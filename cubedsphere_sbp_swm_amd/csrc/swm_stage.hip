@@ -122,19 +122,18 @@ __device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs
     }
 }
 
-template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
-__global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T> a) {
+// One 64 x TY tile of panel p starting at (tx0, ty0): the whole RHS + RK stage update.
+template <int ORD, typename T, int MODE, int TX, int TY, bool OTF, bool PERSIST = false>
+__device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T> &a, int p, int tx0, int ty0,
+                                           Smem<ORD, T, TX, TY> &sm) {
     using C = Tile<ORD, T, TX, TY>;
     constexpr int R = C::R, D = C::D, LX = C::LX, LY = C::LY, NT = C::NT;
-    __shared__ Smem<ORD, T, TX, TY> sm;
-
     const int n = g.n;
-    const int ntx = (n + TX - 1) / TX, nty = (n + TY - 1) / TY;
-    const int tile = xcd_tile(blockIdx.x, 6 * ntx * nty);
-    const int p = tile / (ntx * nty);
-    const int rem = tile - p * ntx * nty;
-    const int ty0 = (rem / ntx) * TY, tx0 = (rem % ntx) * TX;
-    const int tid = threadIdx.x;
+    // in the persistent ensemble loop the thread index is made opaque per tile, so that the
+    // thread-index arithmetic is recomputed per tile instead of being hoisted out of the loop
+    // and held in registers
+    int tid = threadIdx.x;
+    if constexpr (PERSIST) asm volatile("" : "+v"(tid));
     const T *y = a.y;
 
     // ---- load phase: stencil inputs of the tile rows (incl. x-halo) and of the y-halo rows ----
@@ -321,6 +320,59 @@ __global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T
     finish_point<T, MODE>(g, a, id, Jv, q11, q12, q22, gx, gy, dv, cu);
 }
 
+template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
+__global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T> a) {
+    __shared__ Smem<ORD, T, TX, TY> sm;
+    const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
+    const int tile = xcd_tile(blockIdx.x, 6 * ntx * nty);
+    const int p = tile / (ntx * nty);
+    const int rem = tile - p * ntx * nty;
+    stage_tile<ORD, T, MODE, TX, TY, OTF>(g, a, p, (rem % ntx) * TX, (rem / ntx) * TY, sm);
+}
+
+// Copy of a block-uniform struct into scalar registers (word by word through readfirstlane):
+// the stores of the stage body keep the compiler from proving the table unchanged, which it
+// would need to use scalar loads by itself.
+template <typename S>
+__device__ __forceinline__ S uniform_copy(const S *src) {
+    static_assert(sizeof(S) % 4 == 0, "word-sized struct");
+    S out;
+    const unsigned *w = reinterpret_cast<const unsigned *>(src);
+    unsigned *o = reinterpret_cast<unsigned *>(&out);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(S) / 4); ++k) o[k] = __builtin_amdgcn_readfirstlane(w[k]);
+    return out;
+}
+
+// Ensemble of cubes of different sizes in one persistent launch: the blocks walk the
+// concatenated tile list of all members (tile_start[m] = first tile of member m, prefix sums
+// over the members' 6*ntx*nty tiles), XCD-contiguous, and take each tile's geometry and
+// stage arguments from the member table.
+template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
+__global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *__restrict__ members,
+                                                            const int *__restrict__ tile_start, int nmembers,
+                                                            int ntiles) {
+    __shared__ Smem<ORD, T, TX, TY> sm;
+    for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
+        const int tile = xcd_tile(it, ntiles);
+        int lo = 0, hi = nmembers - 1;  // last member with tile_start <= tile
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_start[mid] <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);  // block-uniform member index
+        const DevGeo<T> g = uniform_copy(&members[lo].g);
+        const StageArgs<T> a = uniform_copy(&members[lo].a);
+        const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
+        const int lt = tile - tile_start[lo];
+        const int p = lt / (ntx * nty);
+        const int rem = lt - p * ntx * nty;
+        __syncthreads();  // the previous tile's LDS reads are done
+        stage_tile<ORD, T, MODE, TX, TY, OTF, true>(g, a, p, (rem % ntx) * TX, (rem / ntx) * TY, sm);
+    }
+}
+
 }  // namespace swm
 
 namespace swm {
@@ -339,6 +391,49 @@ hipError_t launch_one(int mode, const DevGeo<T> &g, const StageArgs<T> &a, hipSt
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int ORD, typename T, int TY, bool OTF, int MODE>
+hipError_t launch_ens_mode(const EnsLaunch &r) {
+    constexpr int TX = 64;
+    auto kern = ens_stage_kernel<ORD, T, MODE, TX, TY, OTF>;
+    static int per_cu = 0;  // resident blocks per CU of this instantiation
+    if (per_cu == 0) {
+        int nb = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, TX * TY, 0);
+        if (e != hipSuccess) return e;
+        per_cu = nb > 0 ? nb : 1;
+    }
+    // resident blocks x the requested waves of blocks; a multiple of 8 keeps a block's tiles
+    // on its XCD
+    long long grid = (long long)r.num_cu * per_cu * (r.blocks_per_slot > 0 ? r.blocks_per_slot : 1);
+    grid = (grid + 7) / 8 * 8;
+    if (grid > r.ntiles) grid = (r.ntiles + 7) / 8 * 8;
+    if (grid < 8) grid = 8;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(TX * TY), 0, r.stream, (const EnsMember<T> *)r.members,
+                       r.tile_start, r.nmembers, r.ntiles);
+    return hipGetLastError();
+}
+
+template <int ORD, typename T, int TY, bool OTF>
+hipError_t launch_ens_one(const EnsLaunch &r) {
+    switch (r.mode) {
+    case MODE_STAGE1: return launch_ens_mode<ORD, T, TY, OTF, MODE_STAGE1>(r);
+    case MODE_STAGE23: return launch_ens_mode<ORD, T, TY, OTF, MODE_STAGE23>(r);
+    case MODE_STAGE4: return launch_ens_mode<ORD, T, TY, OTF, MODE_STAGE4>(r);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int ORD>
+hipError_t launch_ens_order(const EnsLaunch &r) {
+    if (r.prec == 64) {
+        if (r.ty == 16)
+            return r.otf ? launch_ens_one<ORD, double, 16, true>(r) : launch_ens_one<ORD, double, 16, false>(r);
+        return r.otf ? launch_ens_one<ORD, double, 8, true>(r) : launch_ens_one<ORD, double, 8, false>(r);
+    }
+    if (r.ty == 16) return launch_ens_one<ORD, float, 16, false>(r);
+    return launch_ens_one<ORD, float, 8, false>(r);
 }
 
 template <int ORD>

@@ -130,6 +130,24 @@ int swm_adj_destroy(swm_adj *tape);
 int swm_adj_forward(swm_adj *tape, const void *y0, void *yT);
 int swm_adj_backward(swm_adj *tape, const void *lamT, void *lam0);
 
+/* ---- Batched ensemble form (no counterpart in the reference) ----
+ * Cubes of different resolutions, one context and one state vector each (same precision and
+ * SBP family 42/43/63, N >= 16), advanced together: every RK4 stage of the whole ensemble is
+ * one persistent kernel launch over all members' tiles, and one ensemble step (4 launches) is
+ * captured once in a hipGraph.  dt holds one time step per member.  The ensemble runs on the
+ * first member's stream; the states stay owned by the caller.  swm_ens_autotune times the
+ * stage-2 launch over tile heights (8, 16), the metric source (FP64: read or on the fly) and
+ * the persistent grid size (1, 2, 4 blocks per resident slot) and keeps the fastest;
+ * swm_ens_bench_stage runs `reps` stage-2 launches (measurement helper). */
+typedef struct swm_ens swm_ens;
+int swm_ens_create(swm_ctx *const *members, void *const *states, int nmembers, swm_ens **ens);
+int swm_ens_destroy(swm_ens *ens);
+int swm_ens_rk4_steps(swm_ens *ens, const double *dt, int nsteps);
+int swm_ens_set_tiling(swm_ens *ens, int tile_rows, int metric_on_the_fly, int blocks_per_slot);
+int swm_ens_get_tiling(const swm_ens *ens, int *tile_rows, int *metric_on_the_fly, int *blocks_per_slot);
+int swm_ens_autotune(swm_ens *ens, int reps, double *best_us);
+int swm_ens_bench_stage(swm_ens *ens, int reps);
+
 /* Device-to-device copies between a state vector and three dense device arrays of shape
  * (6, n, n) (panel, j, i) in the context precision (e.g. the storage of framework tensors); no
  * counterpart in the reference. */
