@@ -44,8 +44,8 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,adjoint",
-                    help="comma-separated extra forms to measure beside the FP64 headline: fp32, adjoint")
+    ap.add_argument("--forms", default="fp32,adjoint,batched",
+                    help="comma-separated extra forms to measure beside the FP64 headline: fp32, adjoint, batched")
     # precision form: C768 sbp42, RK4 at gravity-wave Courant 0.9 (37.5 s), degree <= 64
     ap.add_argument("--fp32-N", type=int, default=768)
     ap.add_argument("--fp32-steps", type=int, default=1000)
@@ -57,6 +57,11 @@ def parse():
     ap.add_argument("--adj-dt", type=float, default=150.0)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-fd", type=int, default=4, help="finite-difference directions checked")
+    # batched form: 16 members each at C48, C96, C144, C192, sbp42 FP64, 200 RK4 steps at
+    # Courant 0.5 per member, degree <= 16, seed = member id
+    ap.add_argument("--ens-sizes", default="48,96,144,192")
+    ap.add_argument("--ens-per-size", type=int, default=16)
+    ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--tiling", default="auto",
                     help="stage-kernel tiling: auto (autotuner) or ROWS[,otf] with ROWS 8 or 16")
@@ -219,6 +224,91 @@ def adjoint_form(args, stream):
                          "max_rel_err": float(np.max(np.abs(ads - fds) / np.abs(fds))) if len(fds) else None}}
 
 
+def batched_form(args, stream):
+    """One persistent launch per RK stage over all members (hipGraph per step) vs the plain
+    single-cube path run member after member; same states bit for bit."""
+    import numpy as np
+    import torch
+    import cubedsphere_sbp_swm_amd as swm
+    a, g = swm.EARTH["radius"], swm.EARTH["grav"]
+    sizes = [int(v) for v in args.ens_sizes.split(",")]
+    Ns = [N for N in sizes for _ in range(args.ens_per_size)]
+    steps = args.ens_steps
+    dts = [0.5 * a * (np.pi / 2) / N / (np.sqrt(g * 1.0e4) + 40.0) for N in Ns]
+    ops, xs, ics = [], [], []
+    for k, N in enumerate(Ns):
+        op = swm.create_swm_operator(N, **swm.operator_names(42), scale=a, omega=swm.EARTH["omega"], grav=g)
+        ic = op.initial_state(seed=k, lmax=16)
+        x = swm.create_stvec_swm(op)
+        x.upload(*ic)
+        ops.append(op)
+        xs.append(x)
+        ics.append(ic)
+    ops[0].set_stream(stream.cuda_stream)
+    ens = swm.Ensemble(ops, xs)
+    tuned_us = ens.autotune(reps=10)
+    for x, ic in zip(xs, ics):
+        x.upload(*ic)
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    ens.steps(dts, 2)  # tables, graph capture
+    for x, ic in zip(xs, ics):
+        x.upload(*ic)
+    torch.cuda.synchronize()
+    e0, e1 = ev(), ev()
+    e0.record(stream)
+    ens.steps(dts, steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_ens = e0.elapsed_time(e1) / 1e3
+    finals = [x.download() for x in xs]
+    # dominant kernel: the ensemble stage-2/3 launch
+    ens.bench_stage(3)
+    torch.cuda.synchronize()
+    k0, k1 = ev(), ev()
+    k0.record(stream)
+    reps = 20
+    ens.bench_stage(reps)
+    k1.record(stream)
+    torch.cuda.synchronize()
+    k_us = k0.elapsed_time(k1) * 1e3 / reps
+    points = sum(6 * (N + 1) ** 2 for N in Ns)
+    cells = sum(6 * N * N for N in Ns)
+    stage_bytes = STAGE23_VALUES * 8 * points
+    # plain path: the same members one after another (each its own autotuned tiling, graph)
+    for op, x, ic in zip(ops, xs, ics):
+        op.set_stream(stream.cuda_stream)
+        x.upload(*ic)
+        op.autotune(x, reps=5)
+        x.upload(*ic)
+    for op, x, ic, dt in zip(ops, xs, ics, dts):
+        swm.create_timescheme(op, "rk4").steps(x, dt, 1)  # graph capture for this state and dt
+        x.upload(*ic)
+    torch.cuda.synchronize()
+    p0, p1 = ev(), ev()
+    p0.record(stream)
+    for op, x, dt in zip(ops, xs, dts):
+        swm.create_timescheme(op, "rk4").steps(x, dt, steps)
+    p1.record(stream)
+    torch.cuda.synchronize()
+    t_plain = p0.elapsed_time(p1) / 1e3
+    same = all(np.array_equal(f, q) for fin, x in zip(finals, xs) for f, q in zip(fin, x.download()))
+    tiling = ens.get_tiling()
+    ens.close()
+    return {"workload": f"{len(Ns)} members ({args.ens_per_size} each at C{', C'.join(map(str, sizes))}), sbp42 FP64, "
+                        f"RK4 x {steps} steps, dt per member at Courant 0.5, degree <= 16, seed = member id",
+            "cell_steps_per_s": cells * steps / t_ens, "ms_per_step": t_ens * 1e3 / steps,
+            "plain_cell_steps_per_s": cells * steps / t_plain, "speedup_vs_plain_fp64": t_plain / t_ens,
+            "same_states_as_plain_path": bool(same), "tiling": tiling,
+            "autotune_stage_us": tuned_us,
+            "roofline": {"kernel": "ens_stage_kernel<42, double, MODE_STAGE23> (all members)",
+                         "kernel_avg_us": k_us, "algorithmic_bytes_per_launch": stage_bytes,
+                         "achieved_GBs": stage_bytes / (k_us * 1e-6) / 1e9,
+                         "frac": stage_bytes / (k_us * 1e-6) / 1e9 / HBM_PEAK_GBS}}
+
+
 def copy_bandwidth(stream, mib):
     """Measured device-to-device copy rate (read + write bytes / time) of torch's copy kernel."""
     import torch
@@ -331,6 +421,8 @@ def main():
         forms["fp32"] = fp32_form(args, stream)
     if "adjoint" in wanted:
         forms["adjoint"] = adjoint_form(args, stream)
+    if "batched" in wanted:
+        forms["batched"] = batched_form(args, stream)
     copy = copy_bandwidth(stream, args.copy_mib) if args.copy_mib > 0 else None
 
     cpu = None
