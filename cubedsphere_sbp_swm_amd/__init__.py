@@ -145,6 +145,15 @@ class OperatorSWM:
         J = J.reshape(6, n, pitch)[:, :, :n]
         return J * hx * hx * A[None, :, None] * A[None, None, :]
 
+    def ecs_halo_xy(self, x, component, halo_width):
+        """ECS cubic halo (ecs_halo_xy_t%get_halo_scalar) of field `component` (0 h, 1 u, 2 v)
+        of state `x`: host array (6, n+2hw, n+2hw), [panel, j, i] with i, j from 1-hw to n+hw."""
+        L = self.n + 2 * halo_width
+        out = np.zeros((6, L, L))
+        field = ctypes.c_void_p(x.ptr + int(component) * self.field_elems * self.elem_bytes)
+        check(_capi.lib().swm_ecs_halo_xy_host(self.ctx, field, int(halo_width), _dp(out)))
+        return out
+
     # transposed linearised right-hand side (adjoint form)
     def rhs_vjp(self, out, y, mu):
         """out = (dF/dy)^T mu at state y (FP64, SBP families 42/43/63)."""

@@ -55,6 +55,8 @@ SIGNATURES = {
     "swm_ens_get_tiling": (_i, [_vp, _ip, _ip, _ip]),
     "swm_ens_autotune": (_i, [_vp, _i, _dp]),
     "swm_ens_bench_stage": (_i, [_vp, _i]),
+    "swm_ecs_halo_xy": (_i, [_vp, _vp, _i, _vp]),
+    "swm_ecs_halo_xy_host": (_i, [_vp, _vp, _i, _dp]),
 }
 
 # host set-up routines (geometry.h), exported by the same library; they need no device
@@ -65,6 +67,7 @@ HOST_SIGNATURES = {
     "swmh_halo_table": (None, [_i, _i, _ip]),
     "swmh_ic_sh": (_i, [_i, _i, ctypes.c_ulonglong, _i, _d, _d, _d, _dp, _dp, _dp, _i]),
     "swmh_edge_sync_matrix": (_i, [_i, _i, _ip, _ip, _dp, _i]),
+    "swmh_ecs_xy_weights": (_i, [_i, _i, _dp, _ip]),
 }
 
 _lib = None

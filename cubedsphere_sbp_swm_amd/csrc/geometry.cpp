@@ -500,3 +500,37 @@ extern "C" int swmh_edge_sync_matrix(int N, int vector, int *rows, int *cols, do
     }
     return b.nnz;
 }
+
+// ---------------------------------------------------------------------------------------------
+// ECS cubic halo on xy points: interpolation weights of init_halo_xy_interp
+// (equiang_cs/ecs_halo_factory_mod.f90:228-269) for target indices i = 1-hw .. n+hw and halo
+// rows j = 1..hw: w[((j-1)*(n+2hw) + i-1+hw)*4 + s+1], s = -1..2, base index ind (1-based).
+extern "C" int swmh_ecs_xy_weights(int N, int hw, double *w, int *ind) {
+    const int n = N + 1, L = n + 2 * hw;
+    if (hw < 1 || n < 4) return -1;
+    const double pi = pi_();
+    const double dxa = grid_hx(N);
+    for (int j = 1; j <= hw; ++j) {
+        const double zb = 0.25 * pi + j * dxa;
+        for (int i = 1 - hw; i <= n + hw; ++i) {
+            double za = -0.25 * pi + (i - 1.0) * dxa;
+            // target point on the prototype face, projected through the sphere onto the
+            // source face: alpha' = atan(tan(alpha) / tan(beta))
+            double zx = std::tan(za);
+            const double zz = std::tan(zb);
+            zx = zx / zz;
+            za = std::atan(zx);
+            int id = (int)std::floor((za + 0.25 * pi) / dxa) + 1;
+            id = std::min(std::max(2, id), n - 2);
+            const double t = (za + 0.25 * pi) / dxa - (id - 1.0);
+            const int o = (j - 1) * L + (i - 1 + hw);
+            ind[o] = id;
+            // cubic Lagrange weights on the nodes -1, 0, 1, 2
+            w[4 * o + 0] = -t * (t - 1.0) * (t - 2.0) / 6.0;
+            w[4 * o + 1] = (t + 1.0) * (t - 1.0) * (t - 2.0) / 2.0;
+            w[4 * o + 2] = -(t + 1.0) * t * (t - 2.0) / 2.0;
+            w[4 * o + 3] = (t + 1.0) * t * (t - 1.0) / 6.0;
+        }
+    }
+    return 0;
+}

@@ -40,6 +40,11 @@ void swmh_halo_table(int N, int ndepth, int *tab);
 // returns the number of entries.
 int swmh_edge_sync_matrix(int N, int vector, int *rows, int *cols, double *w, int cap);
 
+// ECS cubic halo on xy points: weights w[((j-1)*(n+2hw) + i-1+hw)*4 + s+1] (s = -1..2) and
+// 1-based stencil base indices ind[(j-1)*(n+2hw) + i-1+hw] for i = 1-hw .. n+hw, halo row
+// j = 1..hw (init_halo_xy_interp, ecs_halo_factory_mod.f90:228-269).  Returns 0 on success.
+int swmh_ecs_xy_weights(int N, int hw, double *w, int *ind);
+
 // Seeded initial state: h = h0 + h' (max|h'| = h_amp), V = r x grad(psi) (max|V| = v_max),
 // h', psi random real spherical-harmonic series of degree 1..lmax with N(0,1)*l^-2
 // coefficients drawn from a splitmix64/Box-Muller stream of `seed`; covariant components

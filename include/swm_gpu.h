@@ -148,6 +148,17 @@ int swm_ens_get_tiling(const swm_ens *ens, int *tile_rows, int *metric_on_the_fl
 int swm_ens_autotune(swm_ens *ens, int reps, double *best_us);
 int swm_ens_bench_stage(swm_ens *ens, int reps);
 
+/* ECS cubic halo on Ah points: ecs_halo_xy_t%get_halo_scalar (equiang_cs/ecs_halo_xy_mod.f90:68-219)
+ * with the weights of init_halo_xy_interp (equiang_cs/ecs_halo_factory_mod.f90:228-269).  `field`
+ * is one field of the context (e.g. a state vector's h, u or v plane: state + k*field_elems
+ * elements); the panel extended by halo_width (1..8, N >= halo_width + 3) rows of virtual points
+ * beyond every edge is written to `out` as 6 x (n+2hw) x (n+2hw) elements of the context
+ * precision, [panel][j][i] with 1-based i, j from 1-hw to n+hw, including the averaged
+ * duplicated edge lines and cube corners.  swm_ecs_halo_xy writes device memory,
+ * swm_ecs_halo_xy_host FP64 host memory. */
+int swm_ecs_halo_xy(swm_ctx *ctx, const void *field, int halo_width, void *out);
+int swm_ecs_halo_xy_host(swm_ctx *ctx, const void *field, int halo_width, double *out);
+
 /* Device-to-device copies between a state vector and three dense device arrays of shape
  * (6, n, n) (panel, j, i) in the context precision (e.g. the storage of framework tensors); no
  * counterpart in the reference. */

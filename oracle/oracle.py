@@ -45,6 +45,9 @@ def lib():
         L.ora_curl.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.ora_scalar_sync.argtypes = [ctypes.c_void_p, _dp]
         L.ora_vec_sync_covariant.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ora_ecs_xy_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip]
+        L.ora_ecs_halo_xy.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        L.ora_ecs_halo_xy.restype = ctypes.c_int
         L.ora_rhs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.ora_rk4.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int]
         L.ora_diagnostics.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
@@ -151,6 +154,18 @@ class Grid:
     def vec_sync_covariant(self, gx, gy):
         """In place: covariant Ah vector sync of (gx, gy) (after a width-1 vector exchange)."""
         lib().ora_vec_sync_covariant(self.h, _p(gx), _p(gy))
+
+    def ecs_halo_xy(self, f, hw):
+        """In place: ECS cubic halo of width hw on the stored field f."""
+        if lib().ora_ecs_halo_xy(self.h, _p(f), int(hw)) != 0:
+            raise ValueError("halo width out of range")
+
+    def ecs_xy_weights(self, hw):
+        L = self.n + 2 * hw
+        w = np.zeros((hw, L, 4))
+        ind = np.zeros((hw, L), dtype=np.int32)
+        lib().ora_ecs_xy_weights(self.h, int(hw), _p(w), ind.ctypes.data_as(_ip))
+        return w, ind
 
     def div(self, u, v):
         out = self.zeros()

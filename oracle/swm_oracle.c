@@ -19,7 +19,7 @@
 #include <string.h>
 #include <complex.h>
 
-#define HW 3 /* stored halo width: the Ah path reads at most depth 2 */
+#define HW 4 /* stored halo width: the Ah path reads at most depth 2, the ECS xy halo depth 3 */
 
 /* ---- cube topology tables: topology/cubed_sphere_topology_mod.f90:29-55 ---- */
 static const int EX[6][3] = {{-1, 0, 0}, {0, 0, -1}, {1, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 0}};
@@ -667,6 +667,115 @@ void ora_curl(const ora_grid *g, double *u, double *v, double *curl) {
     ora_div(g, ut, vt, curl);
 }
 
+/* ---- ECS cubic halo on xy (Ah) points ----
+ * Weights: init_halo_xy_interp (equiang_cs/ecs_halo_factory_mod.f90:228-269) for the whole
+ * panel tile (indices 1-hw .. n+hw), the same for x- and y-edges.  Procedure:
+ * get_ecs_halo / ext_ecs_tile_halo (equiang_cs/ecs_halo_xy_mod.f90:68-219): exchange of depth
+ * 0..hw, corner diagonals, edge averaging, then per halo row j the 1-D cubic interpolation
+ * ecs_ext_halo_1e (ecs_halo_xy_mod.f90:221-235) of the neighbour's row at depth j. */
+void ora_ecs_xy_weights(const ora_grid *g, int hw, double *w, int *ind) {
+    /* w[((j-1)*(n+2hw) + (i-1+hw))*4 + s+1], ind[(j-1)*(n+2hw) + (i-1+hw)] */
+    const double pi = pi_();
+    const int n = g->n, L = n + 2 * hw;
+    const double dxa = g->hx;
+    for (int j = 1; j <= hw; j++) {
+        double zb = 0.25 * pi + j * dxa;
+        for (int i = 1 - hw; i <= n + hw; i++) {
+            double za = -0.25 * pi + (i - 1.0) * dxa;
+            double zx = tan(za);
+            double zz = tan(zb);
+            zx = zx / zz;
+            za = atan(zx);
+            int id = (int)floor((za + 0.25 * pi) / dxa) + 1;
+            if (id < 2) id = 2;
+            if (id > n - 2) id = n - 2;
+            double zxi = (za + 0.25 * pi) / dxa - (id - 1.0);
+            int o = (j - 1) * L + (i - 1 + hw);
+            ind[o] = id;
+            w[4 * o + 0] = -zxi * (zxi - 1.0) * (zxi - 2.0) / 6.0;
+            w[4 * o + 1] = (zxi + 1.0) * (zxi - 1.0) * (zxi - 2.0) / 2.0;
+            w[4 * o + 2] = -(zxi + 1.0) * zxi * (zxi - 2.0) / 2.0;
+            w[4 * o + 3] = (zxi + 1.0) * zxi * (zxi - 1.0) / 6.0;
+        }
+    }
+}
+
+int ora_ecs_halo_xy(const ora_grid *g, double *f, int hw) {
+    const int n = g->n, L = n + 2 * hw;
+    if (hw < 1 || hw + 1 > g->hw) return -1;
+    double *w = malloc(sizeof(double) * 4 * hw * L);
+    int *ind = malloc(sizeof(int) * hw * L);
+    double *zbuf = malloc(sizeof(double) * (n + 2 * g->hw));
+    double *zh = malloc(sizeof(double) * (n + 2 * g->hw));
+    ora_ecs_xy_weights(g, hw, w, ind);
+    ora_exchange(g, f, hw + 1);
+#define P(i, j) f[IDX(g, p, (i), (j))]
+#define W(s, i, j) w[4 * ((j - 1) * L + ((i) - 1 + hw)) + (s) + 1]
+#define IND(i, j) ind[(j - 1) * L + ((i) - 1 + hw)]
+    for (int p = 0; p < 6; p++) {
+        /* corner diagonals (ecs_halo_xy_mod.f90:113-156) */
+        P(1, 1) = (P(1, 1) + P(1, 0) + P(0, 1)) / 3.0;
+        for (int j = 1; j <= hw; j++) {
+            P(1 - j, 1 - j) = 0.5 * (P(-j, 1) + P(1, -j));
+            P(-j, 1) = P(1 - j, 1 - j);
+            P(1, -j) = P(1 - j, 1 - j);
+        }
+        P(n, 1) = (P(n, 1) + P(n, 0) + P(n + 1, 1)) / 3.0;
+        for (int j = 1; j <= hw; j++) {
+            P(n + j, 1 - j) = 0.5 * (P(n + j + 1, 1) + P(n, -j));
+            P(n + j + 1, 1) = P(n + j, 1 - j);
+            P(n, -j) = P(n + j, 1 - j);
+        }
+        P(n, n) = (P(n, n) + P(n, n + 1) + P(n + 1, n)) / 3.0;
+        for (int j = 1; j <= hw; j++) {
+            P(n + j, n + j) = 0.5 * (P(n + j + 1, n) + P(n, n + j + 1));
+            P(n + j + 1, n) = P(n + j, n + j);
+            P(n, n + j + 1) = P(n + j, n + j);
+        }
+        P(1, n) = (P(1, n) + P(0, n) + P(1, n + 1)) / 3.0;
+        for (int j = 1; j <= hw; j++) {
+            P(1 - j, n + j) = 0.5 * (P(1, n + j + 1) + P(-j, n));
+            P(1, n + j + 1) = P(1 - j, n + j);
+            P(-j, n) = P(1 - j, n + j);
+        }
+        /* edges (ecs_halo_xy_mod.f90:158-219): averaging of the shared line, then the rows */
+        for (int e = 0; e < 4; e++) {
+            for (int k = 2; k <= n - 1; k++) {
+                if (e == 0) P(k, 1) = 0.5 * (P(k, 1) + P(k, 0));
+                if (e == 1) P(k, n) = 0.5 * (P(k, n) + P(k, n + 1));
+                if (e == 2) P(1, k) = 0.5 * (P(1, k) + P(0, k));
+                if (e == 3) P(n, k) = 0.5 * (P(n, k) + P(n + 1, k));
+            }
+            for (int j = 1; j <= hw; j++) {
+                for (int k = 1; k <= n; k++) {
+                    /* the neighbour's row at depth j, indexed along the edge */
+                    zbuf[k] = e == 0 ? P(k, -j) : e == 1 ? P(k, n + j + 1) : e == 2 ? P(-j, k) : P(n + j + 1, k);
+                }
+                int i1 = 2 - j, i2 = n + j - 1;
+                for (int i = i1; i <= i2; i++) {
+                    int ii = IND(i, j);
+                    double s = W(-1, i, j) * zbuf[ii - 1];
+                    s = s + W(0, i, j) * zbuf[ii];
+                    s = s + W(1, i, j) * zbuf[ii + 1];
+                    s = s + W(2, i, j) * zbuf[ii + 2];
+                    zh[i - i1] = s;
+                }
+                for (int i = i1; i <= i2; i++) {
+                    if (e == 0) P(i, 1 - j) = zh[i - i1];
+                    if (e == 1) P(i, n + j) = zh[i - i1];
+                    if (e == 2) P(1 - j, i) = zh[i - i1];
+                    if (e == 3) P(n + j, i) = zh[i - i1];
+                }
+            }
+        }
+    }
+#undef P
+#undef W
+#undef IND
+    free(w); free(ind); free(zbuf); free(zh);
+    return 0;
+}
+
 /* ---- operator_swm_t%apply: models/shallow_water/operator/operator_swm_mod.f90:61-113 ---- */
 void ora_rhs(const ora_grid *g, double *h, double *u, double *v, const double *hs,
              double *dh, double *du, double *dv) {
@@ -921,7 +1030,7 @@ ora_grid *ora_create(int N, int grad_order, int div_kind, double scale, double o
         for (int c = 0; c < 3; c++) g->geo[base + c] = blk + c * g->flen;
     }
     build_sync_weights(g);
-    build_halo_map(g, 2);
+    build_halo_map(g, HW);
     for (int w = 0; w < 16; w++) g->work[w] = alloc_field(g);
     return g;
 }

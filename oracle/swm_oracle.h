@@ -77,6 +77,12 @@ double ora_maxabs(const ora_grid *g, const double *f);
 void ora_project_vector(const ora_grid *g, int kind, const double *vx,
                         const double *vy, const double *vz, double *u, double *v);
 
+/* ECS cubic halo on xy points (ecs_halo_xy_mod.f90, ecs_halo_factory_mod.f90:228-269):
+ * in place on a stored field, halo rows 1..hw (hw + 1 <= stored halo width); weights
+ * w[((j-1)*(n+2hw) + i-1+hw)*4 + s+1] and base indices for i = 1-hw..n+hw, j = 1..hw */
+void ora_ecs_xy_weights(const ora_grid *g, int hw, double *w, int *ind);
+int ora_ecs_halo_xy(const ora_grid *g, double *f, int hw);
+
 /* generic classical RK4 on a diagonal complex linear ODE df/dt = w f (the
  * reference's iomega mock model); returns f after one step */
 void ora_rk4_iomega(int n, const double *w_re, const double *w_im,

@@ -65,7 +65,7 @@ def test_geometry_bitwise_vs_oracle(capi, oracle_lib, N):
 def test_neighbour_table_matches_reference_exchange(capi, oracle_lib, N):
     L = capi.lib()
     n = N + 1
-    nd = 2
+    nd = 4  # depths 0..3: the edge-sync stencils and the ECS halo rows
     tab = np.zeros((6, 4, n, nd, 4), dtype=np.int32)
     L.swmh_halo_table(N, nd, tab.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     G = oracle_lib.Grid(N)

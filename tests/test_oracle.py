@@ -203,7 +203,7 @@ def test_sbp_polynomial_exactness(oracle_lib, order, p_in, p_bd):
 
 def test_halo_map_connectivity(oracle_lib):
     """Depth-0 halo points hold the neighbour's copy of the shared edge point (same position);
-    depth-1 halo points hold the neighbour's first interior line (one grid step away)."""
+    depth-d halo points hold the neighbour's d-th interior line (d grid steps away)."""
     G = oracle_lib.Grid(16)
     src, vec = G.halo_map()
     r = np.stack(G.geometry("r"))  # (3, 6, ld, ld)
@@ -226,8 +226,8 @@ def test_halo_map_connectivity(oracle_lib):
                 if depth == 0:
                     assert d < 1e-15
                 else:
-                    assert 0.3 * hx < d < 1.2 * hx
-    assert count == 6 * 4 * n * 2
+                    assert 0.3 * depth * hx < d < 1.2 * depth * hx
+    assert count == 6 * 4 * n * hw
 
 
 def test_scalar_sync_agreement(oracle_lib):
