@@ -250,7 +250,8 @@ __global__ void __launch_bounds__(TX * TY) stage21_kernel(DevGeo<T> g, StageArgs
     } else {
         q11 = g.Qi11[id]; q12 = g.Qi12[id]; q22 = g.Qi22[id];
     }
-    finish_point<T, MODE>(g, a, id, sm.J[ly][lx], q11, q12, q22, gx, gy, dv, cu);
+    finish_point<T, MODE>(g, a, id, a.y[id], a.y[id + g.fs], a.y[id + 2 * g.fs], sm.J[ly][lx], q11, q12, q22, gx, gy, dv,
+                          cu, prefetch_point<T, MODE>(g, a, id));
 }
 
 template <typename T, int TY, bool OTF>

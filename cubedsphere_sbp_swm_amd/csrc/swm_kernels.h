@@ -131,6 +131,35 @@ __device__ __forceinline__ Derived<T> derived_global(const DevGeo<T> &g, const T
     }
 }
 
+// Raw inputs of one point (state, metric, orography) and the derived stencil inputs formed
+// from them; split so that the loads of several points can be issued before any is used.
+template <typename T>
+struct RawPoint {
+    T h, u, v, J, q11, q12, q22, hs;
+};
+
+template <bool OTF, typename T>
+__device__ __forceinline__ RawPoint<T> load_point(const DevGeo<T> &g, const T *y, const T *hs, int p, int i, int j) {
+    const long long id = pidx(g, p, i, j);
+    RawPoint<T> r;
+    r.h = y[id];
+    r.u = y[id + g.fs];
+    r.v = y[id + 2 * g.fs];
+    if constexpr (OTF) {
+        const Metric m = metric_otf(g.tanA, i, j);
+        r.J = (T)m.J; r.q11 = (T)m.Qi11; r.q12 = (T)m.Qi12; r.q22 = (T)m.Qi22;
+    } else {
+        r.J = g.J[id]; r.q11 = g.Qi11[id]; r.q12 = g.Qi12[id]; r.q22 = g.Qi22[id];
+    }
+    r.hs = hs ? hs[id] : T(0);
+    return r;
+}
+
+template <typename T, int WHICH = 0>
+__device__ __forceinline__ Derived<T> derived_of(const DevGeo<T> &g, const RawPoint<T> &r) {
+    return make_derived<T, WHICH>(r.h, r.u, r.v, r.J, r.q11, r.q12, r.q22, r.hs, g.grav);
+}
+
 // ----------------------------------------------------------------------------------------
 // SBP rows (sbp_operator_mod.f90:187-223).  `get(a)` returns the input at 0-based index a
 // along the line; t is the 0-based output index, n the line length.

@@ -75,22 +75,41 @@ __device__ __forceinline__ void neighbour_raw(const DevGeo<T> &g, const int *ei,
     out[3] = (dxC + dyC) / Jd;
 }
 
+// Values of the output point that only the final combination reads (Coriolis parameter, RK
+// stage vectors); loaded at the start of the tile so that their latency overlaps the stencil
+// phases.
+template <typename T>
+struct OutPre {
+    T f, v0[3], acc[3];
+};
+
+template <typename T, int MODE>
+__device__ __forceinline__ OutPre<T> prefetch_point(const DevGeo<T> &g, const StageArgs<T> &a, long long id) {
+    OutPre<T> o{};
+    if constexpr (MODE == MODE_CURL) return o;
+    o.f = g.f[id];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if constexpr (MODE == MODE_STAGE23) o.v0[k] = a.v0[id + k * g.fs];
+        if constexpr (MODE == MODE_STAGE23 || MODE == MODE_STAGE4) o.acc[k] = a.acc[id + k * g.fs];
+    }
+    return o;
+}
+
 // Coriolis term, tendencies and RK stage combination of one point
 // (coriolis_colocated_mod.f90:97-98, operator_swm_mod.f90:95-101, rk4_mod.f90:28-40)
 template <typename T, int MODE>
-__device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs<T> &a, long long id, T Jv, T q11,
-                                             T q12, T q22, T gx, T gy, T dv, T cu) {
-    const T *y = a.y;
+__device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs<T> &a, long long id, T h, T u, T v,
+                                             T Jv, T q11, T q12, T q22, T gx, T gy, T dv, T cu, const OutPre<T> &pre) {
     if constexpr (MODE == MODE_CURL) {
         a.out[id] = cu;
         return;
     }
     if (a.zeta) a.zeta[id] = cu;
-    const T h = y[id], u = y[id + g.fs], v = y[id + 2 * g.fs];
     const T ut = q11 * u + q12 * v;
     const T vt = q12 * u + q22 * v;
     const T hu = h * ut, hv = h * vt;
-    const T fz = g.f[id] + cu;
+    const T fz = pre.f + cu;
     const T cor_u = fz * hv / h * Jv;
     const T cor_v = -(fz * hu / h * Jv);
     const T kh = -dv;
@@ -109,16 +128,16 @@ __device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs
         a.acc[id + g.fs] = u + a.cacc * ku;
         a.acc[id + 2 * g.fs] = v + a.cacc * kv;
     } else if constexpr (MODE == MODE_STAGE23) {
-        a.out[id] = a.v0[id] + a.cy * kh;
-        a.out[id + g.fs] = a.v0[id + g.fs] + a.cy * ku;
-        a.out[id + 2 * g.fs] = a.v0[id + 2 * g.fs] + a.cy * kv;
-        a.acc[id] = a.acc[id] + a.cacc * kh;
-        a.acc[id + g.fs] = a.acc[id + g.fs] + a.cacc * ku;
-        a.acc[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cacc * kv;
+        a.out[id] = pre.v0[0] + a.cy * kh;
+        a.out[id + g.fs] = pre.v0[1] + a.cy * ku;
+        a.out[id + 2 * g.fs] = pre.v0[2] + a.cy * kv;
+        a.acc[id] = pre.acc[0] + a.cacc * kh;
+        a.acc[id + g.fs] = pre.acc[1] + a.cacc * ku;
+        a.acc[id + 2 * g.fs] = pre.acc[2] + a.cacc * kv;
     } else if constexpr (MODE == MODE_STAGE4) {
-        a.out[id] = a.acc[id] + a.cy * kh;
-        a.out[id + g.fs] = a.acc[id + g.fs] + a.cy * ku;
-        a.out[id + 2 * g.fs] = a.acc[id + 2 * g.fs] + a.cy * kv;
+        a.out[id] = pre.acc[0] + a.cy * kh;
+        a.out[id + g.fs] = pre.acc[1] + a.cy * ku;
+        a.out[id + 2 * g.fs] = pre.acc[2] + a.cy * kv;
     }
 }
 
@@ -135,13 +154,41 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     int tid = threadIdx.x;
     if constexpr (PERSIST) asm volatile("" : "+v"(tid));
     const T *y = a.y;
+    OutPre<T> pre{};
+    {
+        const int oi = tx0 + tid % TX, oj = ty0 + tid / TX;
+        if (oi < n && oj < n) pre = prefetch_point<T, MODE>(g, a, pidx(g, p, oi, oj));
+    }
 
-    // ---- load phase: stencil inputs of the tile rows (incl. x-halo) and of the y-halo rows ----
-    for (int c = tid; c < TY * LX; c += NT) {
+    // ---- load phase: stencil inputs of the tile rows (incl. x-halo) and of the y-halo rows.
+    // A thread owns at most NXR row points and NYH y-halo points; all their global loads are
+    // issued before any derived value is formed, so that they are in flight together. ----
+    constexpr int NXR = (TY * LX + NT - 1) / NT, NYH = (2 * R * TX + NT - 1) / NT;
+    RawPoint<T> rx[NXR], ry[NYH];
+    bool okx[NXR], oky[NYH];
+#pragma unroll
+    for (int k = 0; k < NXR; ++k) {
+        const int c = tid + k * NT;
         const int r = c / LX, lx = c - r * LX;
         const int i = tx0 - R + lx, j = ty0 + r;
-        if (i < 0 || i >= n || j >= n) continue;
-        const Derived<T> d = derived_global<OTF>(g, y, a.hs, p, i, j);
+        okx[k] = c < TY * LX && i >= 0 && i < n && j < n;
+        if (okx[k]) rx[k] = load_point<OTF>(g, y, a.hs, p, i, j);
+    }
+#pragma unroll
+    for (int k = 0; k < NYH; ++k) {
+        const int c = tid + k * NT;
+        const int r = c / TX, cx = c - r * TX;
+        const int ly = r < R ? r : r + TY;
+        const int i = tx0 + cx, j = ty0 - R + ly;
+        oky[k] = c < 2 * R * TX && i < n && j >= 0 && j < n;
+        if (oky[k]) ry[k] = load_point<OTF>(g, y, a.hs, p, i, j);
+    }
+#pragma unroll
+    for (int k = 0; k < NXR; ++k) {
+        if (!okx[k]) continue;
+        const int c = tid + k * NT;
+        const int r = c / LX, lx = c - r * LX;
+        const Derived<T> d = derived_of(g, rx[k]);
         sm.K[r + R][lx] = d.K;
         sm.Gxd[r][lx] = d.Gxd;
         sm.Gxc[r][lx] = d.Gxc;
@@ -150,12 +197,13 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
             sm.Gyc[r + R][lx - R] = d.Gyc;
         }
     }
-    for (int c = tid; c < 2 * R * TX; c += NT) {
+#pragma unroll
+    for (int k = 0; k < NYH; ++k) {
+        if (!oky[k]) continue;
+        const int c = tid + k * NT;
         const int r = c / TX, cx = c - r * TX;
         const int ly = r < R ? r : r + TY;
-        const int i = tx0 + cx, j = ty0 - R + ly;
-        if (i >= n || j < 0 || j >= n) continue;
-        const Derived<T> d = derived_global<OTF, T, 2>(g, y, a.hs, p, i, j);
+        const Derived<T> d = derived_of<T, 2>(g, ry[k]);
         sm.K[ly][cx + R] = d.K;
         sm.Gyd[ly][cx] = d.Gyd;
         sm.Gyc[ly][cx] = d.Gyc;
@@ -317,7 +365,7 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         }
     }
 
-    finish_point<T, MODE>(g, a, id, Jv, q11, q12, q22, gx, gy, dv, cu);
+    finish_point<T, MODE>(g, a, id, y[id], y[id + g.fs], y[id + 2 * g.fs], Jv, q11, q12, q22, gx, gy, dv, cu, pre);
 }
 
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
