@@ -114,8 +114,9 @@ class OperatorSWM:
         return h, u, v
 
     def set_tiling(self, tile_rows=8, metric_on_the_fly=False):
-        """Tile height (8 or 16 rows of 64 points) of the fused stage kernel; FP64 may evaluate
-        the metric terms on the fly instead of reading them (identical values)."""
+        """Tile shape of the fused stage kernel by its height: 8, 16 or 32 rows (64 x 8, 32 x 16,
+        16 x 32 points); FP64 may evaluate the metric terms on the fly instead of reading them
+        (identical values)."""
         check(_capi.lib().swm_ctx_set_tiling(self.ctx, int(tile_rows), int(bool(metric_on_the_fly))))
 
     def get_tiling(self):

@@ -472,7 +472,8 @@ extern "C" int swm_rk4_steps(swm_ctx *c, void *state, double dt, int nsteps) {
 
 extern "C" int swm_ctx_set_tiling(swm_ctx *c, int tile_rows, int metric_on_the_fly) {
     if (!c) return fail(SWM_EARG, "ctx is NULL");
-    if (tile_rows != 8 && tile_rows != 16) return fail(SWM_EARG, "tile_rows must be 8 or 16");
+    if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32)
+        return fail(SWM_EARG, "tile_rows must be 8, 16 or 32 (tiles of 64 x 8, 32 x 16, 16 x 32 points)");
     if (metric_on_the_fly && c->prec != 64) return fail(SWM_EARG, "metric on the fly is FP64 only");
     // a tile must not touch both the j=1 and the j=n edge of a panel
     if (c->n <= tile_rows) return fail(SWM_EARG, "tile_rows must be smaller than N+1");
@@ -496,8 +497,8 @@ extern "C" int swm_ctx_get_tiling(const swm_ctx *c, int *tile_rows, int *metric_
 extern "C" int swm_ctx_autotune(swm_ctx *c, void *state, int reps, double *best_us) {
     if (!c || !state || reps < 1) return fail(SWM_EARG, "bad argument");
     struct Cand { int ty; bool otf; };
-    std::vector<Cand> cands = {{8, false}, {16, false}};
-    if (c->prec == 64) { cands.push_back({8, true}); cands.push_back({16, true}); }
+    std::vector<Cand> cands = {{8, false}, {16, false}, {32, false}};
+    if (c->prec == 64) { cands.push_back({8, true}); cands.push_back({16, true}); cands.push_back({32, true}); }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

@@ -81,7 +81,8 @@ int ens_build_tables(swm_ens *e) {
     std::vector<int> ts(M + 1, 0);
     for (size_t k = 0; k < M; ++k) {
         const int n = e->m[k]->n;
-        ts[k + 1] = ts[k] + 6 * ((n + 63) / 64) * ((n + e->ty - 1) / e->ty);
+        const int tx = swm::tile_width(e->ty);
+        ts[k + 1] = ts[k] + 6 * ((n + tx - 1) / tx) * ((n + e->ty - 1) / e->ty);
     }
     if (!e->tile_start) HIP_TRY(hipMalloc((void **)&e->tile_start, (M + 1) * sizeof(int)));
     HIP_TRY(hipMemcpy(e->tile_start, ts.data(), (M + 1) * sizeof(int), hipMemcpyHostToDevice));
@@ -184,9 +185,11 @@ extern "C" int swm_ens_destroy(swm_ens *e) {
 
 extern "C" int swm_ens_set_tiling(swm_ens *e, int tile_rows, int metric_on_the_fly, int blocks_per_slot) {
     if (!e) return fail(SWM_EARG, "ensemble is NULL");
-    if (tile_rows != 8 && tile_rows != 16) return fail(SWM_EARG, "tile_rows must be 8 or 16");
+    if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32) return fail(SWM_EARG, "tile_rows must be 8, 16 or 32");
     if (metric_on_the_fly && e->prec != 64) return fail(SWM_EARG, "metric on the fly is FP64 only");
     if (blocks_per_slot < 1 || blocks_per_slot > 64) return fail(SWM_EARG, "blocks_per_slot must be in [1, 64]");
+    for (swm_ctx *c : e->m)
+        if (c->n <= tile_rows) return fail(SWM_EARG, "tile_rows must be smaller than every member's N+1");
     HIP_TRY(hipStreamSynchronize(e->stream));
     ens_drop_graph(e);
     e->ty = tile_rows;
@@ -241,7 +244,8 @@ extern "C" int swm_ens_rk4_steps(swm_ens *e, const double *dt, int nsteps) {
     return SWM_OK;
 }
 
-// Autotuner: times `reps` stage-2 launches of the whole ensemble for every tile height (8, 16),
+// Autotuner: times `reps` stage-2 launches of the whole ensemble for every tile shape (64 x 8,
+// 32 x 16, 16 x 32),
 // metric source (FP64: read or on the fly) and persistent-grid size (1, 2 or 4 blocks per
 // resident slot); keeps the fastest.  The stage-2 launch reads the work buffers and writes
 // yB / acc of every member, so the states are unchanged.
@@ -253,9 +257,13 @@ extern "C" int swm_ens_autotune(swm_ens *e, int reps, double *best_us) {
     }
     struct Cand { int ty; bool otf; int bps; };
     std::vector<Cand> cands;
-    for (int ty : {8, 16})
+    int min_n = 1 << 30;
+    for (swm_ctx *c : e->m) min_n = std::min(min_n, c->n);
+    for (int ty : {8, 16, 32}) {
+        if (min_n <= ty) continue;  // a tile may not span both y-edges of a panel
         for (int otf = 0; otf < (e->prec == 64 ? 2 : 1); ++otf)
             for (int bps : {1, 2, 4}) cands.push_back({ty, otf != 0, bps});
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

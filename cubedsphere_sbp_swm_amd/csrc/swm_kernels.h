@@ -56,6 +56,9 @@ struct StageArgs {
 };
 
 // one cube of an ensemble: its geometry and the arguments of the current RK stage
+// tile width of a tile of `ty` rows (512 points per tile)
+SWM_HD constexpr int tile_width(int ty) { return 512 / ty; }
+
 template <typename T>
 struct EnsMember {
     DevGeo<T> g;

@@ -7,7 +7,8 @@
 namespace swm {
 
 
-// Tile of TX x TY output points per workgroup of TX*TY threads (one point per thread).
+// Tile of TX x TY output points per workgroup of TX*TY threads (one point per thread).  Every
+// tiling has 512 threads: 64 x 8, 32 x 16 or 16 x 32 points.
 template <int ORD, typename T, int TX, int TY>
 struct Tile {
     static constexpr int R = Sbp<ORD>::REACH;    // halo of the stencil inputs
@@ -427,7 +428,7 @@ namespace swm {
 
 template <int ORD, typename T, int TY, bool OTF>
 hipError_t launch_one(int mode, const DevGeo<T> &g, const StageArgs<T> &a, hipStream_t s) {
-    constexpr int TX = 64;
+    constexpr int TX = tile_width(TY);
     const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
     const dim3 grid(6 * ntx * nty), block(TX * TY);
     switch (mode) {
@@ -443,7 +444,7 @@ hipError_t launch_one(int mode, const DevGeo<T> &g, const StageArgs<T> &a, hipSt
 
 template <int ORD, typename T, int TY, bool OTF, int MODE>
 hipError_t launch_ens_mode(const EnsLaunch &r) {
-    constexpr int TX = 64;
+    constexpr int TX = tile_width(TY);
     auto kern = ens_stage_kernel<ORD, T, MODE, TX, TY, OTF>;
     static int per_cu = 0;  // resident blocks per CU of this instantiation
     if (per_cu == 0) {
@@ -476,10 +477,13 @@ hipError_t launch_ens_one(const EnsLaunch &r) {
 template <int ORD>
 hipError_t launch_ens_order(const EnsLaunch &r) {
     if (r.prec == 64) {
+        if (r.ty == 32)
+            return r.otf ? launch_ens_one<ORD, double, 32, true>(r) : launch_ens_one<ORD, double, 32, false>(r);
         if (r.ty == 16)
             return r.otf ? launch_ens_one<ORD, double, 16, true>(r) : launch_ens_one<ORD, double, 16, false>(r);
         return r.otf ? launch_ens_one<ORD, double, 8, true>(r) : launch_ens_one<ORD, double, 8, false>(r);
     }
+    if (r.ty == 32) return launch_ens_one<ORD, float, 32, false>(r);
     if (r.ty == 16) return launch_ens_one<ORD, float, 16, false>(r);
     return launch_ens_one<ORD, float, 8, false>(r);
 }
@@ -487,12 +491,16 @@ hipError_t launch_ens_order(const EnsLaunch &r) {
 template <int ORD>
 hipError_t launch_stage_order(const StageLaunch &r) {
     if (r.prec == 64) {
+        if (r.ty == 32)
+            return r.otf ? launch_one<ORD, double, 32, true>(r.mode, r.gd, r.ad, r.stream)
+                         : launch_one<ORD, double, 32, false>(r.mode, r.gd, r.ad, r.stream);
         if (r.ty == 16)
             return r.otf ? launch_one<ORD, double, 16, true>(r.mode, r.gd, r.ad, r.stream)
                          : launch_one<ORD, double, 16, false>(r.mode, r.gd, r.ad, r.stream);
         return r.otf ? launch_one<ORD, double, 8, true>(r.mode, r.gd, r.ad, r.stream)
                      : launch_one<ORD, double, 8, false>(r.mode, r.gd, r.ad, r.stream);
     }
+    if (r.ty == 32) return launch_one<ORD, float, 32, false>(r.mode, r.gf, r.af, r.stream);
     if (r.ty == 16) return launch_one<ORD, float, 16, false>(r.mode, r.gf, r.af, r.stream);
     return launch_one<ORD, float, 8, false>(r.mode, r.gf, r.af, r.stream);
 }

@@ -96,9 +96,10 @@ int swm_ic_spherical_harmonics(swm_ctx *ctx, unsigned long long seed, int lmax, 
                                double h_amp, double v_max, double *h, double *u, double *v,
                                int halo_width);
 
-/* Tile configuration of the fused stage kernel (no counterpart in the reference): tile height in
- * rows of 64 points (8 or 16) and, for FP64, whether the metric terms J, Qi are read from memory
- * (0) or evaluated on the fly from tan(alpha) (1; identical values).  swm_ctx_autotune times every
+/* Tile configuration of the fused stage kernel (no counterpart in the reference): tile height
+ * 8, 16 or 32 rows (tiles of 64 x 8, 32 x 16 or 16 x 32 points, 512 threads; the height must be
+ * below N+1) and, for FP64, whether the metric terms J, Qi are read from memory (0) or evaluated
+ * on the fly from tan(alpha) (1; identical values).  swm_ctx_autotune times every
  * configuration with `reps` launches of the RK stage-2 kernel on `state`, keeps the fastest and
  * returns its time per launch. */
 int swm_ctx_set_tiling(swm_ctx *ctx, int tile_rows, int metric_on_the_fly);
@@ -136,7 +137,7 @@ int swm_adj_backward(swm_adj *tape, const void *lamT, void *lam0);
  * one persistent kernel launch over all members' tiles, and one ensemble step (4 launches) is
  * captured once in a hipGraph.  dt holds one time step per member.  The ensemble runs on the
  * first member's stream; the states stay owned by the caller.  swm_ens_autotune times the
- * stage-2 launch over tile heights (8, 16), the metric source (FP64: read or on the fly) and
+ * stage-2 launch over tile shapes (64 x 8, 32 x 16, 16 x 32), the metric source (FP64: read or on the fly) and
  * the persistent grid size (1, 2, 4 blocks per resident slot) and keeps the fastest;
  * swm_ens_bench_stage runs `reps` stage-2 launches (measurement helper). */
 typedef struct swm_ens swm_ens;
