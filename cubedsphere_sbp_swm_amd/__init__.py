@@ -161,6 +161,10 @@ class OperatorSWM:
         check(_capi.lib().swm_rhs_vjp(self.ctx, ctypes.c_void_p(y.ptr), ctypes.c_void_p(mu.ptr),
                                       ctypes.c_void_p(out.ptr)))
 
+    def set_adjoint_tiling(self, tile_x):
+        """Transposed RHS as one fused launch on tile_x x 16 tiles (16 or 32), or as two launches (0)."""
+        check(_capi.lib().swm_adj_set_tiling(self.ctx, int(tile_x)))
+
     def debug_geometry(self, which, halo_width=0):
         out = np.zeros(self.host_shape(halo_width))
         check(_capi.lib().swm_debug_geometry(self.ctx, int(which), _dp(out), int(halo_width)))

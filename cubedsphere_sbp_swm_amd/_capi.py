@@ -46,6 +46,7 @@ SIGNATURES = {
     "swm_adj_destroy": (_i, [_vp]),
     "swm_adj_forward": (_i, [_vp, _vp, _vp]),
     "swm_adj_backward": (_i, [_vp, _vp, _vp]),
+    "swm_adj_set_tiling": (_i, [_vp, _i]),
     "swm_state_copy_from_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "swm_state_copy_to_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "swm_ens_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),

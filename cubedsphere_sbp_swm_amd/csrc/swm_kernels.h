@@ -222,4 +222,12 @@ __device__ __forceinline__ void map_vec(int code, T a, T b, T &u, T &v) {
     v = (code & 4) ? -z : z;
 }
 
+// XCD-aware mapping: consecutive block ids go round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so that tiles sharing halo rows share an L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
+    const int q = nblocks / 8, r = nblocks % 8;
+    const int xcd = bid % 8, local = bid / 8;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
 }  // namespace swm

@@ -37,14 +37,6 @@ struct Smem {
     T nry[TX][4];
 };
 
-// XCD-aware mapping: consecutive block ids go round-robin over the 8 XCDs; give each XCD a
-// contiguous range of tiles so that tiles sharing halo rows share an L2.
-__device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
-    const int q = nblocks / 8, r = nblocks % 8;
-    const int xcd = bid % 8, local = bid / 8;
-    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
-}
-
 // Raw values of the neighbour panel at the shared point k of edge e, from a ghost strip.
 // d0(kk, f) returns field f (0 K, 1 Gxd, 2 Gyd, 3 Gxc, 4 Gyc, 5 J) at tangential index kk,
 // depth 0; dn(d, f3) the normal-axis inputs (0 K, 1 Gnd, 2 Gnc) at depth d >= 1.

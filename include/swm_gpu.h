@@ -122,7 +122,9 @@ int swm_bench_stage(swm_ctx *ctx, void *state, int stage_id, int reps);
  * swm_adj_forward runs the steps from y0, keeps the state every `checkpoint_every` steps and
  * writes the final state to yT; swm_adj_backward takes lamT = dJ/dy(T) and writes
  * lam0 = dJ/dy(0), recomputing each segment of states from its checkpoint and each step's
- * stage inputs.  The tape owns its buffers until swm_adj_destroy: nsteps/checkpoint_every
+ * stage inputs.  The forward records the last segment's stage inputs as it runs them, so the
+ * first backward after it recomputes one segment fewer; backward may be called again with
+ * another lamT.  The tape owns its buffers until swm_adj_destroy: nsteps/checkpoint_every
  * checkpoints plus, for one segment, checkpoint_every x (4 state vectors + 4 fields). */
 typedef struct swm_adj swm_adj;
 int swm_rhs_vjp(swm_ctx *ctx, const void *y, const void *mu, void *out);
@@ -130,6 +132,11 @@ int swm_adj_create(swm_ctx *ctx, int nsteps, double dt, int checkpoint_every, sw
 int swm_adj_destroy(swm_adj *tape);
 int swm_adj_forward(swm_adj *tape, const void *y0, void *yT);
 int swm_adj_backward(swm_adj *tape, const void *lamT, void *lam0);
+/* Launch configuration of the transposed RHS (no counterpart in the reference): tile_x = 16
+ * or 32 runs it as one fused launch per evaluation on tiles of tile_x x 16 points (16 is the
+ * default); 0 runs it as two launches (seeds and edge syncs, then D^T).  All give the same
+ * bits. */
+int swm_adj_set_tiling(swm_ctx *ctx, int tile_x);
 
 /* ---- Batched ensemble form (no counterpart in the reference) ----
  * Cubes of different resolutions, one context and one state vector each (same precision and
