@@ -348,8 +348,8 @@ def copy_bandwidth(stream, mib):
 
 
 def cpu_baseline(N, order, dt, lmax, budget_s, op):
-    """The oracle (single-threaded FP64 C restatement) on the host, bounded sample."""
-    from oracle.oracle import Grid
+    """The oracle (FP64 C restatement, point loops under OpenMP) on the host, bounded sample."""
+    from oracle.oracle import Grid, lib as oracle_lib
     import cubedsphere_sbp_swm_amd as swm
     G = Grid(N, order, order, swm.EARTH["radius"], swm.EARTH["omega"], swm.EARTH["grav"])
     h, u, v = op.initial_state(seed=0, lmax=lmax, halo_width=G.hw)
@@ -366,9 +366,10 @@ def cpu_baseline(N, order, dt, lmax, budget_s, op):
     else:
         total = one
     cells = 6 * N * N
-    return {"value": cells * steps / total, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+    threads = int(oracle_lib().ora_num_threads())
+    return {"value": cells * steps / total, "unit": "cell-steps/s", "cores": threads, "kind": "port",
             "sample": f"C{N} sbp{order} FP64 RK4 dt={dt:g}s, {steps} steps ({total:.1f} s) of the oracle "
-                      f"(single-threaded C restatement, gcc -O2) on the GPU box host"}
+                      f"(C restatement, gcc -O2, OpenMP over {threads} threads) on the GPU box host"}
 
 
 def main():

@@ -31,6 +31,8 @@ def lib():
         for f in ("ora_n", "ora_hw", "ora_ld"):
             getattr(L, f).restype = ctypes.c_int
             getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.ora_num_threads.restype = ctypes.c_int
+        L.ora_num_threads.argtypes = []
         L.ora_field_len.restype = ctypes.c_long
         L.ora_field_len.argtypes = [ctypes.c_void_p]
         L.ora_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]

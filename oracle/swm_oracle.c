@@ -5,6 +5,9 @@
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the
  * timed CPU baseline.  Nothing in cubedsphere_sbp_swm_amd links or calls it.
  *
+ * Loops over points that write distinct points run under OpenMP (every value is the
+ * same for any thread count); reductions are serial.
+ *
  * Compiled with -ffp-contract=off so that every a*b+c is rounded twice, as in
  * the reference's debug build (-fp-model strict).  Fortran expressions are
  * evaluated left to right as written; SUM() over an array section is taken as
@@ -18,6 +21,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <complex.h>
+#include <omp.h>
 
 #define HW 4 /* stored halo width: the Ah path reads at most depth 2, the ECS xy halo depth 3 */
 
@@ -150,6 +154,7 @@ struct ora_grid {
 #define IDX(g, p, i, j) (((long)(p) * (g)->ld + ((j) - 1 + (g)->hw)) * (g)->ld + ((i) - 1 + (g)->hw))
 
 int ora_n(const ora_grid *g) { return g->n; }
+int ora_num_threads(void) { return omp_get_max_threads(); }
 int ora_hw(const ora_grid *g) { return g->hw; }
 int ora_ld(const ora_grid *g) { return g->ld; }
 long ora_field_len(const ora_grid *g) { return g->flen; }
@@ -463,6 +468,7 @@ void ora_exchange_vec(const ora_grid *g, double *u, double *v, int width) {
 /* ---- SBP apply: sbp_operator_mod.f90:152-228 (sbp_apply) over the whole panel ---- */
 static void sbp_apply_panel(const ora_grid *g, const sbp_op *o, char dir, const double *f, double *df) {
     int n = g->n;
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -561,6 +567,7 @@ static void div_ah_sbp(const ora_grid *g, const double *u, const double *v, doub
     int n = g->n;
     double *Gu = g->work[0], *Gv = g->work[1], *Dx = g->work[2], *Dy = g->work[3];
     const double *J = g->geo[0];
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -570,6 +577,7 @@ static void div_ah_sbp(const ora_grid *g, const double *u, const double *v, doub
             }
     sbp_apply_panel(g, &g->div_sbp, 'x', Gu, Dx);
     sbp_apply_panel(g, &g->div_sbp, 'y', Gv, Dy);
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -642,6 +650,7 @@ void ora_grad(const ora_grid *g, double *f, double *gx, double *gy) {
     double *Dx = g->work[2], *Dy = g->work[3];
     sbp_apply_panel(g, &g->grad_sbp, 'x', f, Dx);
     sbp_apply_panel(g, &g->grad_sbp, 'y', f, Dy);
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -657,6 +666,7 @@ void ora_curl(const ora_grid *g, double *u, double *v, double *curl) {
     int n = g->n;
     double *ut = g->work[4], *vt = g->work[5];
     const double *J = g->geo[0];
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -785,6 +795,7 @@ void ora_rhs(const ora_grid *g, double *h, double *u, double *v, const double *h
     double *div = g->work[14];
     const double *J = g->geo[0], *Qi1 = g->geo[1], *Qi2 = g->geo[2], *Qi3 = g->geo[3], *fc = g->geo[4];
     const double grav = g->grav;
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -802,6 +813,7 @@ void ora_rhs(const ora_grid *g, double *h, double *u, double *v, const double *h
             }
     ora_grad(g, KE, gx, gy);
     ora_curl(g, u, v, curl);
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -814,6 +826,7 @@ void ora_rhs(const ora_grid *g, double *h, double *u, double *v, const double *h
                 dv[id] = -1.0 * gy[id] + 1.0 * cor_v;
             }
     ora_div(g, hu, hv, div);
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -834,6 +847,7 @@ void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *h
     const double c1 = 0.5 * dt, c2 = dt / 6.0, c3 = dt / 3.0;
     for (int step = 0; step < nsteps; step++) {
         ora_rhs(g, v0[0], v0[1], v0[2], hs, k[0][0], k[0][1], k[0][2]);
+        #pragma omp parallel for collapse(3) schedule(static)
         for (int c = 0; c < 3; c++)
             for (int p = 0; p < 6; p++)
                 for (int j = 1; j <= n; j++)
@@ -842,6 +856,7 @@ void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *h
                         y[c][id] = 1.0 * v0[c][id] + c1 * k[0][c][id];
                     }
         ora_rhs(g, y[0], y[1], y[2], hs, k[1][0], k[1][1], k[1][2]);
+        #pragma omp parallel for collapse(3) schedule(static)
         for (int c = 0; c < 3; c++)
             for (int p = 0; p < 6; p++)
                 for (int j = 1; j <= n; j++)
@@ -850,6 +865,7 @@ void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *h
                         y[c][id] = 1.0 * v0[c][id] + c1 * k[1][c][id];
                     }
         ora_rhs(g, y[0], y[1], y[2], hs, k[2][0], k[2][1], k[2][2]);
+        #pragma omp parallel for collapse(3) schedule(static)
         for (int c = 0; c < 3; c++)
             for (int p = 0; p < 6; p++)
                 for (int j = 1; j <= n; j++)
@@ -858,6 +874,7 @@ void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *h
                         y[c][id] = 1.0 * v0[c][id] + dt * k[2][c][id];
                     }
         ora_rhs(g, y[0], y[1], y[2], hs, k[3][0], k[3][1], k[3][2]);
+        #pragma omp parallel for collapse(3) schedule(static)
         for (int c = 0; c < 3; c++)
             for (int p = 0; p < 6; p++)
                 for (int j = 1; j <= n; j++)
@@ -865,6 +882,7 @@ void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *h
                         long id = IDX(g, p, i, j);
                         v0[c][id] = v0[c][id] + c2 * k[0][c][id] + c3 * k[1][c][id];
                     }
+        #pragma omp parallel for collapse(3) schedule(static)
         for (int c = 0; c < 3; c++)
             for (int p = 0; p < 6; p++)
                 for (int j = 1; j <= n; j++)

@@ -29,6 +29,7 @@ ora_grid *ora_create(int N, int grad_order, int div_kind, double scale,
 void ora_destroy(ora_grid *g);
 
 int ora_n(const ora_grid *g);      /* points per panel side (N+1) */
+int ora_num_threads(void);         /* OpenMP threads the point loops use */
 int ora_hw(const ora_grid *g);     /* halo width of stored fields */
 int ora_ld(const ora_grid *g);     /* leading dimension n+2*hw */
 long ora_field_len(const ora_grid *g); /* 6*ld*ld */
