@@ -468,7 +468,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(N, order, args.precision, tiling),
-                     "kernel": "stage_kernel<42, double, MODE_STAGE23, 64, %d, %s> (RK stages 2 and 3)" % (
+                     "kernel": "stage_kernel<%d, %s, MODE_STAGE23, %d, %d, %s> (RK stages 2 and 3)" % (
+                         order, "double" if args.precision == 64 else "float", 512 // tiling["tile_rows"],
                          tiling["tile_rows"], str(tiling["metric_on_the_fly"]).lower()),
                      "kernel_avg_us": k_us, "algorithmic_bytes_per_launch": stage_bytes,
                      "step_effective_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,

@@ -161,6 +161,26 @@ class OperatorSWM:
         check(_capi.lib().swm_rhs_vjp(self.ctx, ctypes.c_void_p(y.ptr), ctypes.c_void_p(mu.ptr),
                                       ctypes.c_void_p(out.ptr)))
 
+    def laplace(self, out, x):
+        """h plane of `out` = narrow-stencil SBP Laplacian (sbp42) of the h plane of `x`
+        (laplace_Ah_sbp42_narrow_t%calc_laplace); FP64, N+1 >= 12."""
+        check(_capi.lib().swm_laplace(self.ctx, ctypes.c_void_p(x.ptr), ctypes.c_void_p(out.ptr)))
+
+    def diff_apply(self, vout, vin, uv_diff_coeff, h_diff_coeff,
+                   hordiff_uv_name="hordiff_vec_xyz_Ah_sbp_42_narrow",
+                   hordiff_h_name="hordiff_scalar_Ah_sbp_42_narrow"):
+        """vout = hyperdiffusion tendency of vin (operator_swm_diff_t%apply)."""
+        check(_capi.lib().swm_diff_apply(self.ctx, ctypes.c_void_p(vin.ptr), ctypes.c_void_p(vout.ptr),
+                                         hordiff_uv_name.encode(), float(uv_diff_coeff),
+                                         hordiff_h_name.encode(), float(h_diff_coeff)))
+
+    def diff_step(self, x, dt, uv_diff_coeff, h_diff_coeff,
+                  hordiff_uv_name="hordiff_vec_xyz_Ah_sbp_42_narrow",
+                  hordiff_h_name="hordiff_scalar_Ah_sbp_42_narrow"):
+        """x += dt * hyperdiffusion tendency of x (explicit_Eul1 on operator_swm_diff_t)."""
+        check(_capi.lib().swm_diff_step(self.ctx, ctypes.c_void_p(x.ptr), float(dt), hordiff_uv_name.encode(),
+                                        float(uv_diff_coeff), hordiff_h_name.encode(), float(h_diff_coeff)))
+
     def set_adjoint_tiling(self, tile_x):
         """Transposed RHS as one fused launch on tile_x x 16 tiles (16 or 32), or as two launches (0)."""
         check(_capi.lib().swm_adj_set_tiling(self.ctx, int(tile_x)))

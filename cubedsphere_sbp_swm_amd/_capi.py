@@ -47,6 +47,9 @@ SIGNATURES = {
     "swm_adj_forward": (_i, [_vp, _vp, _vp]),
     "swm_adj_backward": (_i, [_vp, _vp, _vp]),
     "swm_adj_set_tiling": (_i, [_vp, _i]),
+    "swm_laplace": (_i, [_vp, _vp, _vp]),
+    "swm_diff_apply": (_i, [_vp, _vp, _vp, ctypes.c_char_p, _d, ctypes.c_char_p, _d]),
+    "swm_diff_step": (_i, [_vp, _vp, _d, ctypes.c_char_p, _d, ctypes.c_char_p, _d]),
     "swm_state_copy_from_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "swm_state_copy_to_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "swm_ens_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
@@ -69,6 +72,7 @@ HOST_SIGNATURES = {
     "swmh_ic_sh": (_i, [_i, _i, ctypes.c_ulonglong, _i, _d, _d, _d, _dp, _dp, _dp, _i]),
     "swmh_edge_sync_matrix": (_i, [_i, _i, _ip, _ip, _dp, _i]),
     "swmh_ecs_xy_weights": (_i, [_i, _i, _dp, _ip]),
+    "swmh_basis_b": (None, [_i, _i, _dp, _dp]),
 }
 
 _lib = None

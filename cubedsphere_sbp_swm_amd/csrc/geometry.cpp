@@ -125,6 +125,26 @@ extern "C" void swmh_geometry(int N, int pitch, double omega, const double axis[
         }
 }
 
+extern "C" void swmh_basis_b(int N, int pitch, double *b1, double *b2) {
+    const int n = N + 1;
+    const long plane = 6L * n * pitch;
+    for (int p = 0; p < 6; ++p)
+        for (int j = 1; j <= n; ++j) {
+            const double beta = grid_coord(N, j);
+            for (int i = 1; i <= n; ++i) {
+                const double alpha = grid_coord(N, i);
+                const long id = ((long)p * n + (j - 1)) * pitch + (i - 1);
+                double v[3], w[3];
+                b1_proto(alpha, beta, v);
+                to_panel(p, v, w);
+                for (int k = 0; k < 3; ++k) b1[id + k * plane] = w[k];
+                b2_proto(alpha, beta, v);
+                to_panel(p, v, w);
+                for (int k = 0; k < 3; ++k) b2[id + k * plane] = w[k];
+            }
+        }
+}
+
 extern "C" void swmh_sync_weights(int N, double *qb, double *qt, double *ql, double *qr) {
     const int n = N + 1;
     const double pi = pi_();

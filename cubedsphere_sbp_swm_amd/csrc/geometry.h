@@ -17,6 +17,10 @@ void swmh_geometry(int N, int pitch, double omega, const double axis[3],
                    double *J, double *Qi11, double *Qi12, double *Qi22, double *f,
                    double *r /*3 planes*/, double *a1 /*3 planes*/, double *a2 /*3 planes*/);
 
+// Contravariant basis vectors b1, b2 (3 planes each) of every Ah point, device layout
+// (ecs_metric_mod.f90:367-395 on the prototype face, rotated to panel p).
+void swmh_basis_b(int N, int pitch, double *b1, double *b2);
+
 // Covariant Ah vector-sync weights qb, qt, ql, qr (n values each)
 // (ecs_Ah_vec_sync_factory_mod.f90:101-149).
 void swmh_sync_weights(int N, double *qb, double *qt, double *ql, double *qr);

@@ -36,6 +36,8 @@ constexpr int NDEPTH = 6;  // neighbour-table depths kept on the host (>= widest
 
 struct AdjWork;  // adjoint work space (swm_adjoint.hip), created on first use
 void free_adj_work(AdjWork *w);
+struct HdWork;   // hyperdiffusion work space (swm_hordiff.hip), created on first use
+void free_hd_work(HdWork *w);
 
 }  // namespace
 
@@ -68,6 +70,7 @@ struct swm_ctx {
     double graph_dt = 0.0;
     std::vector<int> host_nbr;
     AdjWork *adj = nullptr;
+    HdWork *hd = nullptr;
     // ECS cubic halo: weights / base indices of the last halo width used, neighbour rows
     int ecs_hw = 0;
     void *ecs_w = nullptr, *ecs_ind = nullptr, *ecs_src = nullptr;
@@ -221,6 +224,7 @@ void free_ctx(swm_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_adj_work(c->adj);
+    free_hd_work(c->hd);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -601,3 +605,4 @@ extern "C" int swm_debug_halo_table(swm_ctx *c, int *table) {
 #include "swm_adjoint.hip"
 #include "swm_ensemble.hip"
 #include "swm_ecs.hip"
+#include "swm_hordiff.hip"

@@ -60,6 +60,11 @@ def lib():
         L.ora_maxabs.restype = ctypes.c_double
         L.ora_maxabs.argtypes = [ctypes.c_void_p, _dp]
         L.ora_project_vector.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.ora_laplace42.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ora_hordiff_scalar42.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp]
+        L.ora_hordiff_vec42.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.ora_diff_step42.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double]
         L.ora_rk4_iomega.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double]
         _lib = L
     return _lib
@@ -191,6 +196,25 @@ class Grid:
 
     def rk4(self, h, u, v, dt, nsteps, hs=None):
         lib().ora_rk4(self.h, _p(h), _p(u), _p(v), _pn(hs), dt, nsteps)
+
+    # narrow-stencil SBP Laplacian (sbp42) and hyperdiffusion; N+1 >= 12
+    def laplace42(self, f):
+        out = self.zeros()
+        lib().ora_laplace42(self.h, _p(f), _p(out))
+        return out
+
+    def hordiff_scalar42(self, coeff, f):
+        out = self.zeros()
+        lib().ora_hordiff_scalar42(self.h, float(coeff), _p(f), _p(out))
+        return out
+
+    def hordiff_vec42(self, coeff, u, v):
+        ut, vt = self.zeros(), self.zeros()
+        lib().ora_hordiff_vec42(self.h, float(coeff), _p(u), _p(v), _p(ut), _p(vt))
+        return ut, vt
+
+    def diff_step42(self, h, u, v, dt, uv_coeff, h_coeff):
+        lib().ora_diff_step42(self.h, _p(h), _p(u), _p(v), float(dt), float(uv_coeff), float(h_coeff))
 
     def diagnostics(self, quad_order, h, u, v, hs=None):
         out = np.zeros(7)

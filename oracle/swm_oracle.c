@@ -16,6 +16,7 @@
  * Citations are to files under the reference's src/ directory.
  */
 #include "swm_oracle.h"
+#include "../cubedsphere_sbp_swm_amd/csrc/laplace_sbp42_narrow.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -895,6 +896,204 @@ void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *h
     for (int s = 0; s < 4; s++)
         for (int c = 0; c < 3; c++) free(k[s][c]);
     for (int c = 0; c < 3; c++) free(y[c]);
+}
+
+/* ---- narrow-stencil SBP Laplacian, laplace_Ah_sbp42_narrow_t (differential_operators/
+ * horizontal/laplace/laplace_Ah_sbp42_narrow_mod.f90): calc_laplace :23-58, calc_tile_d1x_d1y
+ * :60-93, calc_laplace_tile :94-763.  The closure and interior coefficients come from the
+ * generated table laplace_sbp42_narrow.h (terms in the reference's source order). ---- */
+
+/* closure row r: b[k], fv[c] = values k, c points from the panel edge */
+static double ln42_row(int r, const double *b, const double *fv) {
+    double s = 0.0;
+    for (int c = 0; c < LN42_NCOLS[r]; c++) {
+        double m = 0.0;
+        for (int t = LN42_MSTART[r][c]; t < LN42_MSTART[r][c + 1]; t++) {
+            double term = (LN42_TNUM[t] / LN42_TDEN[t]) * b[LN42_TB[t]];
+            if (t == LN42_MSTART[r][c]) m = LN42_TSIGN[t] < 0 ? -term : term;
+            else m = LN42_TSIGN[t] < 0 ? m - term : m + term;
+        }
+        double pr = m * fv[c];
+        s = c == 0 ? pr : s + pr;
+    }
+    /* -1.0_8/h_rr*(...) */
+    return -((1.0 / (LN42_HNUM[r] / LN42_HDEN[r])) * s);
+}
+/* interior row: b[o], fv[o] at offsets o-2, o = 0..4 */
+static double ln42_interior(const double *b, const double *fv) {
+    double s = 0.0;
+    for (int o = 0; o < 5; o++) {
+        double cf = 0.0;
+        for (int t = LN42_ISTART[o]; t < LN42_ISTART[o + 1]; t++) {
+            double term = (LN42_INUM[t] / LN42_IDEN[t]) * b[LN42_IB[t] + 2];
+            if (t == LN42_ISTART[o]) cf = LN42_ISIGN[t] < 0 ? -term : term;
+            else cf = LN42_ISIGN[t] < 0 ? cf - term : cf + term;
+        }
+        double pr = cf * fv[o];
+        s = o == 0 ? pr : s + pr;
+    }
+    return s;
+}
+
+/* D2(b) f along x ('x') or y at the point (i, j) of panel p, b = Qi11 J (x) or Qi22 J (y) */
+static double ln42_d2(const ora_grid *g, int p, int i, int j, char dir, const double *f) {
+    const int n = g->n;
+    const double *J = g->geo[0], *Q = dir == 'x' ? g->geo[1] : g->geo[3];
+    const int t = dir == 'x' ? i : j;
+#define AT(a) (dir == 'x' ? IDX(g, p, (a), j) : IDX(g, p, i, (a)))
+    double b[LN42_NCOL], fv[LN42_NCOL];
+    if (t <= LN42_NB || t >= n + 1 - LN42_NB) {
+        const int left = t <= LN42_NB;
+        const int r = left ? t - 1 : n - t;
+        for (int k = 0; k < LN42_NCOL; k++) {
+            long id = AT(left ? 1 + k : n - k);
+            b[k] = Q[id] * J[id];
+            fv[k] = f[id];
+        }
+        return ln42_row(r, b, fv);
+    }
+    for (int o = 0; o < 5; o++) {
+        long id = AT(t - 2 + o);
+        b[o] = Q[id] * J[id];
+        fv[o] = f[id];
+    }
+#undef AT
+    return ln42_interior(b, fv);
+}
+
+/* out = L f (in place allowed: f is read completely before out is written) */
+void ora_laplace42(const ora_grid *g, const double *f, double *out) {
+    const int n = g->n;
+    sbp_op d1;
+    sbp_make(&d1, 42);
+    const double *J = g->geo[0], *Q12 = g->geo[2];
+    double *d1x = alloc_field(g), *d1y = alloc_field(g), *d2x = alloc_field(g), *d2y = alloc_field(g);
+    double *d2xy = alloc_field(g), *d2yx = alloc_field(g);
+    /* calc_tile_d1x_d1y */
+    sbp_apply_panel(g, &d1, 'x', f, d1x);
+    sbp_apply_panel(g, &d1, 'y', f, d1y);
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                d1x[id] = d1x[id] * J[id] * Q12[id];
+                d1y[id] = d1y[id] * J[id] * Q12[id];
+            }
+    /* second derivatives: closure rows and interior (:127-714) */
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                d2x[id] = ln42_d2(g, p, i, j, 'x', f);
+                d2y[id] = ln42_d2(g, p, i, j, 'y', f);
+            }
+    /* boundary penalties of the cross terms, all into d2f_x (:716-742) */
+    const double pc = LN42_PNUM / LN42_PDEN;
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= n; j++) {
+            long id = IDX(g, p, 1, j);
+            double pen = pc * (-2.0 * d1y[id]);
+            d2x[id] = d2x[id] - 0.5 * pen;
+        }
+        for (int j = 1; j <= n; j++) {
+            long id = IDX(g, p, n, j);
+            double pen = pc * (2.0 * d1y[id]);
+            d2x[id] = d2x[id] - 0.5 * pen;
+        }
+        for (int i = 1; i <= n; i++) {
+            long id = IDX(g, p, i, 1);
+            double pen = pc * (-2.0 * d1x[id]);
+            d2x[id] = d2x[id] - 0.5 * pen;
+        }
+        for (int i = 1; i <= n; i++) {
+            long id = IDX(g, p, i, n);
+            double pen = pc * (2.0 * d1x[id]);
+            d2x[id] = d2x[id] - 0.5 * pen;
+        }
+    }
+    sbp_apply_panel(g, &d1, 'y', d1x, d2yx);
+    sbp_apply_panel(g, &d1, 'x', d1y, d2xy);
+    const double hs = g->hx * g->scale;
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                double v = 1.0 * d2x[id] + 1.0 * d2y[id];   /* fout%assign */
+                v = v + 1.0 * d2xy[id] + 1.0 * d2yx[id];     /* fout%update */
+                v = v / (hs * hs);                           /* /(hx*scale)**2 */
+                out[id] = v / J[id];                         /* divide_by_J_self */
+            }
+    ora_scalar_sync(g, out);
+    free(d1x); free(d1y); free(d2x); free(d2y); free(d2xy); free(d2yx);
+}
+
+/* hordiff_scalar_t%calc_diff (hordiff/hordiff_scalar_mod.f90:34-59) with diff_order 2 and
+ * diff_coeff = hordiff_coeff*scale*hx (hordiff_factory_mod.f90:158-188):
+ * tend = (-1)**3 * diff_coeff**4 * L(L(f)); the integer power as (c*c)*(c*c) */
+void ora_hordiff_scalar42(const ora_grid *g, double hordiff_coeff, const double *f, double *tend) {
+    const int n = g->n;
+    double *w = alloc_field(g);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) w[IDX(g, p, i, j)] = 1.0 * f[IDX(g, p, i, j)];
+    ora_laplace42(g, w, w);
+    ora_laplace42(g, w, w);
+    const double dc = hordiff_coeff * g->scale * g->hx;
+    const double coeff = -1.0 * ((dc * dc) * (dc * dc));
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) tend[IDX(g, p, i, j)] = coeff * w[IDX(g, p, i, j)];
+    free(w);
+}
+
+/* hordiff_colocated_xyz_t%calc_diff_vec (hordiff/hordiff_colocated_mod.f90:39-58): covariant
+ * (u, v) to Cartesian with b1, b2 (vec_math_mod.f90:107-126), the scalar hyperdiffusion of each
+ * component, back with b1, b2 (vec_math_mod.f90:144-165), covariant edge sync */
+void ora_hordiff_vec42(const ora_grid *g, double hordiff_coeff, const double *u, const double *v,
+                       double *ut, double *vt) {
+    const int n = g->n;
+    const long fl = g->flen;
+    const double *b1 = g->geo[14], *b2 = g->geo[17];
+    double *vx[3], *tx[3];
+    for (int c = 0; c < 3; c++) { vx[c] = alloc_field(g); tx[c] = alloc_field(g); }
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                for (int c = 0; c < 3; c++) vx[c][id] = b1[id + c * fl] * u[id] + b2[id + c * fl] * v[id];
+            }
+    for (int c = 0; c < 3; c++) ora_hordiff_scalar42(g, hordiff_coeff, vx[c], tx[c]);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                ut[id] = tx[0][id] * b1[id] + tx[1][id] * b1[id + fl] + tx[2][id] * b1[id + 2 * fl];
+                vt[id] = tx[0][id] * b2[id] + tx[1][id] * b2[id + fl] + tx[2][id] * b2[id + 2 * fl];
+            }
+    ora_vec_sync_covariant(g, ut, vt);
+    for (int c = 0; c < 3; c++) { free(vx[c]); free(tx[c]); }
+}
+
+/* operator_swm_diff_t%apply (operator_swm_diff_mod.f90:25-46) followed by explicit_Eul1
+ * (time_schemes/explicit_Eul1_mod.f90:20-30): y = y + dt * diff(y) */
+void ora_diff_step42(const ora_grid *g, double *h, double *u, double *v, double dt,
+                     double uv_diff_coeff, double h_diff_coeff) {
+    const int n = g->n;
+    double *ht = alloc_field(g), *ut = alloc_field(g), *vt = alloc_field(g);
+    ora_hordiff_vec42(g, uv_diff_coeff, u, v, ut, vt);
+    ora_hordiff_scalar42(g, h_diff_coeff, h, ht);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                long id = IDX(g, p, i, j);
+                h[id] = h[id] + dt * ht[id];
+                u[id] = u[id] + dt * ut[id];
+                v[id] = v[id] + dt * vt[id];
+            }
+    free(ht); free(ut); free(vt);
 }
 
 /* ---- quadrature: quadrature/quadrature_factory_mod.f90:39-86,158-177, sbp_quadrature_mod.f90:21-40 ---- */

@@ -58,6 +58,13 @@ void ora_grad(const ora_grid *g, double *f, double *gx, double *gy);
 void ora_curl(const ora_grid *g, double *u, double *v, double *curl);
 void ora_scalar_sync(const ora_grid *g, double *f);
 void ora_vec_sync_covariant(const ora_grid *g, double *gx, double *gy);
+/* narrow-stencil SBP Laplacian (sbp42) and the hyperdiffusion built on it (N+1 >= 12) */
+void ora_laplace42(const ora_grid *g, const double *f, double *out);
+void ora_hordiff_scalar42(const ora_grid *g, double hordiff_coeff, const double *f, double *tend);
+void ora_hordiff_vec42(const ora_grid *g, double hordiff_coeff, const double *u, const double *v,
+                       double *ut, double *vt);
+void ora_diff_step42(const ora_grid *g, double *h, double *u, double *v, double dt,
+                     double uv_diff_coeff, double h_diff_coeff);
 
 /* full RHS: out = RHS(in); in/out are [h,u,v] each a stored field; hs may be NULL */
 void ora_rhs(const ora_grid *g, double *in_h, double *in_u, double *in_v,
