@@ -44,8 +44,10 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,adjoint,batched",
-                    help="comma-separated extra forms to measure beside the FP64 headline: fp32, adjoint, batched")
+    ap.add_argument("--forms", default="fp32,adjoint,batched,hyperdiffusion",
+                    help="comma-separated extra measurements beside the FP64 headline: fp32, adjoint, batched, "
+                         "hyperdiffusion")
+    ap.add_argument("--diff-steps", type=int, default=100, help="hyperdiffusion steps timed")
     # precision form: C768 sbp42, RK4 at gravity-wave Courant 0.9 (37.5 s), degree <= 64
     ap.add_argument("--fp32-N", type=int, default=768)
     ap.add_argument("--fp32-steps", type=int, default=1000)
@@ -240,6 +242,34 @@ def adjoint_form(args, stream):
             "forward_cell_steps_per_s": cells * steps / t_fwd, "loss": loss,
             "fd_check": {"directions": int(args.adj_fd), "step": eps,
                          "max_rel_err": float(np.max(np.abs(ads - fds) / np.abs(fds))) if len(fds) else None}}
+
+
+# algorithmic FP64 values per point of one explicit hyperdiffusion step (4 fields: h and the
+# Cartesian velocity): to Cartesian 9 read / 4 written; per Laplacian application d1 6 / 8,
+# D2 + cross terms + combination 15 / 4; tendencies and Euler step 13 / 3 (edge syncs O(N))
+DIFF_VALUES = (9 + 4) + 2 * ((6 + 8) + (15 + 4)) + (13 + 3)
+
+
+def hyperdiffusion_form(args, stream, op, x):
+    """One explicit_Eul1 hyperdiffusion step (sbp42 narrow Laplacian, coefficients 0.03 / 0.01)
+    on the headline state, timed over --diff-steps steps, against the RK4 step."""
+    import torch
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    op.diff_step(x, args.dt, 0.03, 0.01)
+    torch.cuda.synchronize()
+    e0, e1 = ev(), ev()
+    e0.record(stream)
+    for _ in range(args.diff_steps):
+        op.diff_step(x, args.dt, 0.03, 0.01)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / args.diff_steps
+    points = 6 * (args.N + 1) ** 2
+    gbs = DIFF_VALUES * 8 * points / (us * 1e-6) / 1e9
+    return {"workload": f"C{args.N} sbp42 FP64 explicit_Eul1 hyperdiffusion step, hordiff_vec_xyz_Ah_sbp_42_narrow "
+                        "0.03 + hordiff_scalar_Ah_sbp_42_narrow 0.01 (9 launches)",
+            "us_per_step": us, "algorithmic_bytes_per_step": DIFF_VALUES * 8 * points,
+            "effective_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
 
 
 def batched_form(args, stream):
@@ -442,6 +472,9 @@ def main():
         forms["adjoint"] = adjoint_form(args, stream)
     if "batched" in wanted:
         forms["batched"] = batched_form(args, stream)
+    if "hyperdiffusion" in wanted and args.precision == 64:
+        forms["hyperdiffusion"] = hyperdiffusion_form(args, stream, op, y)
+        forms["hyperdiffusion"]["rk4_step_us"] = ms * 1e3 / args.steps
     copy = copy_bandwidth(stream, args.copy_mib) if args.copy_mib > 0 else None
 
     cpu = None

@@ -14,8 +14,8 @@
 // Step, operator_swm_diff_t%apply + explicit_Eul1 (operator_swm_diff_mod.f90:25-46,
 // time_schemes/explicit_Eul1_mod.f90:20-30): y += dt * tend.
 //
-// Kernels work on NF fields at once (blockIdx.y = field): h and the three Cartesian velocity
-// components share every Laplacian launch.  Panel-edge results are written to a compact edge
+// The Laplacian kernels take one field per block row (blockIdx.y): h and the three Cartesian
+// velocity components share every launch.  Panel-edge results are written to a compact edge
 // buffer first and synchronised from there, so that no kernel reads a value another block
 // of the same launch overwrites.
 
@@ -49,56 +49,98 @@ __device__ __forceinline__ int nbr_slot(const HdGeo &g, int p, int e, int k) {
     return ei[0] * g.es + edge_slot(n, qi, qj);
 }
 
-// closure row R of the narrow D2: b[k], fv[c] taken k, c points from the panel edge
+// Closure row R of the narrow D2 with b[k], fv[c] taken k, c points from the panel edge, and
+// the interior row, expanded at compile time (every index and weight a constant); each weight
+// p/q is the correctly rounded quotient, as the run-time division of the oracle.
+template <int T>
+__device__ __forceinline__ double ln42_term(const double *b) {
+    constexpr double w = LN42_TNUM[T] / LN42_TDEN[T];
+    return w * b[LN42_TB[T]];
+}
+template <int T0, int... Is>
+__device__ __forceinline__ double ln42_sum(const double *b, std::integer_sequence<int, Is...>) {
+    double m = 0.0;
+    ((m = Is == 0 ? (LN42_TSIGN[T0 + Is] < 0 ? -ln42_term<T0 + Is>(b) : ln42_term<T0 + Is>(b))
+                  : (LN42_TSIGN[T0 + Is] < 0 ? m - ln42_term<T0 + Is>(b) : m + ln42_term<T0 + Is>(b))),
+     ...);
+    return m;
+}
+template <int R, int C>
+__device__ __forceinline__ double ln42_m(const double *b) {
+    constexpr int t0 = LN42_MSTART[R][C], cnt = LN42_MSTART[R][C + 1] - LN42_MSTART[R][C];
+    return ln42_sum<t0>(b, std::make_integer_sequence<int, cnt>{});
+}
+template <int R, int... Cs>
+__device__ __forceinline__ double ln42_row_cols(const double *b, const double *fv, std::integer_sequence<int, Cs...>) {
+    double s = 0.0;
+    ((s = Cs == 0 ? ln42_m<R, Cs>(b) * fv[Cs] : s + ln42_m<R, Cs>(b) * fv[Cs]), ...);
+    constexpr double h = LN42_HNUM[R] / LN42_HDEN[R];
+    return -((1.0 / h) * s);
+}
 template <int R>
 __device__ __forceinline__ double ln42_row(const double *b, const double *fv) {
-    double s = 0.0;
-#pragma unroll
-    for (int c = 0; c < LN42_NCOLS[R]; ++c) {
-        double m = 0.0;
-#pragma unroll
-        for (int t = LN42_MSTART[R][c]; t < LN42_MSTART[R][c + 1]; ++t) {
-            const double term = (LN42_TNUM[t] / LN42_TDEN[t]) * b[LN42_TB[t]];
-            if (t == LN42_MSTART[R][c]) m = LN42_TSIGN[t] < 0 ? -term : term;
-            else m = LN42_TSIGN[t] < 0 ? m - term : m + term;
-        }
-        const double pr = m * fv[c];
-        s = c == 0 ? pr : s + pr;
-    }
-    return -((1.0 / (LN42_HNUM[R] / LN42_HDEN[R])) * s);
+    return ln42_row_cols<R>(b, fv, std::make_integer_sequence<int, LN42_NCOLS[R]>{});
 }
 
-__device__ __forceinline__ double ln42_interior(const double *b, const double *fv) {
+template <int T>
+__device__ __forceinline__ double ln42_iterm(const double *b) {
+    constexpr double w = LN42_INUM[T] / LN42_IDEN[T];
+    return w * b[LN42_IB[T] + 2];
+}
+template <int T0, int... Is>
+__device__ __forceinline__ double ln42_isum(const double *b, std::integer_sequence<int, Is...>) {
+    double m = 0.0;
+    ((m = Is == 0 ? (LN42_ISIGN[T0 + Is] < 0 ? -ln42_iterm<T0 + Is>(b) : ln42_iterm<T0 + Is>(b))
+                  : (LN42_ISIGN[T0 + Is] < 0 ? m - ln42_iterm<T0 + Is>(b) : m + ln42_iterm<T0 + Is>(b))),
+     ...);
+    return m;
+}
+template <int O>
+__device__ __forceinline__ double ln42_icoef(const double *b) {
+    return ln42_isum<LN42_ISTART[O]>(b, std::make_integer_sequence<int, LN42_ISTART[O + 1] - LN42_ISTART[O]>{});
+}
+template <int... Os>
+__device__ __forceinline__ double ln42_interior_seq(const double *b, const double *fv, std::integer_sequence<int, Os...>) {
     double s = 0.0;
-#pragma unroll
-    for (int o = 0; o < 5; ++o) {
-        double cf = 0.0;
-#pragma unroll
-        for (int t = LN42_ISTART[o]; t < LN42_ISTART[o + 1]; ++t) {
-            const double term = (LN42_INUM[t] / LN42_IDEN[t]) * b[LN42_IB[t] + 2];
-            if (t == LN42_ISTART[o]) cf = LN42_ISIGN[t] < 0 ? -term : term;
-            else cf = LN42_ISIGN[t] < 0 ? cf - term : cf + term;
-        }
-        const double pr = cf * fv[o];
-        s = o == 0 ? pr : s + pr;
-    }
+    ((s = Os == 0 ? ln42_icoef<Os>(b) * fv[Os] : s + ln42_icoef<Os>(b) * fv[Os]), ...);
     return s;
 }
+__device__ __forceinline__ double ln42_interior(const double *b, const double *fv) {
+    return ln42_interior_seq(b, fv, std::make_integer_sequence<int, 5>{});
+}
 
-// D2(b) f at index t of a line of n points; at(a) = memory offset of point a, Q = Qi11 or Qi22
+// D2(b) f at index t of a line of n points, in two parts so that the coefficients b are
+// loaded once for several fields: ln42_b gives b (Q = Qi11 or Qi22 times J at the stencil
+// points, at(a) = memory offset of point a) and the closure row r (-1 in the interior),
+// ln42_apply the row applied to one field.
 template <typename A>
-__device__ __forceinline__ double ln42_d2(int t, int n, const double *__restrict__ Q, const double *__restrict__ J,
-                                          const double *__restrict__ f, const A &at) {
-    double b[LN42_NCOL], fv[LN42_NCOL];
+__device__ __forceinline__ int ln42_b(int t, int n, const double *__restrict__ Q, const double *__restrict__ J,
+                                      const A &at, double *b) {
     if (t < LN42_NB || t >= n - LN42_NB) {
         const bool left = t < LN42_NB;
-        const int r = left ? t : n - 1 - t;
 #pragma unroll
         for (int k = 0; k < LN42_NCOL; ++k) {
             const long long o = at(left ? k : n - 1 - k);
             b[k] = Q[o] * J[o];
-            fv[k] = f[o];
         }
+        return left ? t : n - 1 - t;
+    }
+#pragma unroll
+    for (int o = 0; o < 5; ++o) {
+        const long long q = at(t - 2 + o);
+        b[o] = Q[q] * J[q];
+    }
+    return -1;
+}
+
+template <typename A>
+__device__ __forceinline__ double ln42_apply(int t, int n, int r, const double *b, const double *__restrict__ f,
+                                             const A &at) {
+    double fv[LN42_NCOL];
+    if (r >= 0) {
+        const bool left = t < LN42_NB;
+#pragma unroll
+        for (int k = 0; k < LN42_NCOL; ++k) fv[k] = f[at(left ? k : n - 1 - k)];
         switch (r) {
         case 0: return ln42_row<0>(b, fv);
         case 1: return ln42_row<1>(b, fv);
@@ -109,11 +151,7 @@ __device__ __forceinline__ double ln42_d2(int t, int n, const double *__restrict
         }
     }
 #pragma unroll
-    for (int o = 0; o < 5; ++o) {
-        const long long q = at(t - 2 + o);
-        b[o] = Q[q] * J[q];
-        fv[o] = f[q];
-    }
+    for (int o = 0; o < 5; ++o) fv[o] = f[at(t - 2 + o)];
     return ln42_interior(b, fv);
 }
 
@@ -132,7 +170,7 @@ __global__ void hd_to_cart(HdGeo g, const double *__restrict__ y, double *__rest
     for (int c = 0; c < 3; ++c) W[(1 + c) * g.fs + o] = g.b1[c * g.fs + o] * u + g.b2[c * g.fs + o] * v;
 }
 
-// d1x = Dx(f) J Qi12, d1y = Dy(f) J Qi12 for field blockIdx.y: D[2k], D[2k+1]
+// d1x = Dx(f) J Qi12, d1y = Dy(f) J Qi12 of field k = blockIdx.y: D[2k], D[2k+1]
 __global__ void hd_d1(HdGeo g, const double *__restrict__ W, double *__restrict__ D) {
     const int n = g.n;
     const long long total = 6LL * n * n;
@@ -151,7 +189,8 @@ __global__ void hd_d1(HdGeo g, const double *__restrict__ W, double *__restrict_
     D[(2 * k + 1) * g.fs + o] = dy * g.J[o] * g.Qi12[o];
 }
 
-// L f before the edge sync: off-edge points to Wout, panel-edge points to E (ns slots per field)
+// L f of field k = blockIdx.y before the edge sync: off-edge points to Wout, panel-edge
+// points to E (ns slots per field)
 __global__ void hd_lap(HdGeo g, const double *__restrict__ W, const double *__restrict__ D, double *__restrict__ Wout,
                        double *__restrict__ E) {
     const int n = g.n;
@@ -164,10 +203,21 @@ __global__ void hd_lap(HdGeo g, const double *__restrict__ W, const double *__re
     const int j = (int)(pj % n);
     const int p = (int)(pj / n);
     const long long row = pj * g.pitch, o = row + i, col = (long long)p * n * g.pitch + i;
+    auto atx = [&](int a) { return row + a; };
+    auto aty = [&](int a) { return col + (long long)a * g.pitch; };
     const double *f = W + k * g.fs;
     const double *d1x = D + 2 * k * g.fs, *d1y = D + (2 * k + 1) * g.fs;
-    double d2x = ln42_d2(i, n, g.Qi11, g.J, f, [&](int a) { return row + a; });
-    const double d2y = ln42_d2(j, n, g.Qi22, g.J, f, [&](int a) { return col + (long long)a * g.pitch; });
+    double d2x, d2y;
+    {
+        double b[LN42_NCOL];
+        const int r = ln42_b(i, n, g.Qi11, g.J, atx, b);
+        d2x = ln42_apply(i, n, r, b, f, atx);
+    }
+    {
+        double b[LN42_NCOL];
+        const int r = ln42_b(j, n, g.Qi22, g.J, aty, b);
+        d2y = ln42_apply(j, n, r, b, f, aty);
+    }
     // penalties of the cross terms (laplace_Ah_sbp42_narrow_mod.f90:716-742), x edges first
     const double pc = LN42_PNUM / LN42_PDEN;
     if (i == 0) d2x = d2x - 0.5 * (pc * (-2.0 * d1y[o]));

@@ -8,9 +8,11 @@
 ! variable names, plus &swm_gpu_run for what the reference takes from its test-case modules: the
 ! seeded spherical-harmonic initial state, gravity, arithmetic precision and the output file.
 ! Settings the device path does not implement (another staggering, topology, metric, rotated
-! frame, operator family or a nonzero diffusion coefficient) stop the run with a message.
+! frame, operator family, or diffusion other than the sbp42 narrow-stencil hyperdiffusion with
+! explicit_Eul1) stop the run with a message.
 !
-! Loop as in run_ts2 (ts2_mod.f90:141-156): rk4 steps, key/value diagnostics every tau_diagnostics.
+! Loop as in run_ts2 (ts2_mod.f90:141-156): an rk4 step, then, when a diffusion coefficient is
+! nonzero, the explicit hyperdiffusion step (:145-146); key/value diagnostics every tau_diagnostics.
 ! The final state is written to output_file as raw FP64 h, u, v, each real(8) f(n, n, 6).
 program swm_gpu_driver
 
@@ -58,11 +60,14 @@ namelist /swm_gpu_run/ seed, lmax, h0, h_amp, v_max, grav, precision, autotune_r
 type(operator_swm_gpu_t) :: operator
 type(stvec_swm_gpu_t)    :: state
 type(rk4_gpu_t)          :: timescheme
+type(operator_swm_diff_gpu_t) :: operator_diff
+type(explicit_eul1_gpu_t)     :: timescheme_diff
+logical                       :: use_diff
 type(key_value_r8_gpu_t) :: diagnostics
 character(len=1024) :: fname
 real(kind=8), allocatable :: h(:,:,:), u(:,:,:), v(:,:,:)
 real(kind=8) :: simulation_time, best_us
-integer(kind=4) :: nsteps, nstep_diagnostics, it, chunk, unit_nml, ios, n_pts
+integer(kind=4) :: nsteps, nstep_diagnostics, it, chunk, unit_nml, ios, n_pts, k
 
 fname = "namelist_swm"
 if (command_argument_count() >= 1) call get_command_argument(1, fname)
@@ -84,8 +89,13 @@ if (trim(swm_op_type) /= "vector_invariant" .or. trim(v_components_type) /= "cov
     trim(coriolis_op_name) /= "coriolis_colocated" .or. trim(KE_op_name) /= "KE_colocated" .or. &
     trim(massflux_op_name) /= "massflux_colocated" .or. trim(co2contra_op_name) /= "co2contra_colocated") &
     call fail("device path implements the covariant vector-invariant colocated operator only")
-if (uv_diff_coeff /= 0.0_8 .or. h_diff_coeff /= 0.0_8) &
-    call fail("horizontal diffusion is not part of the device path (set uv_diff_coeff = h_diff_coeff = 0)")
+use_diff = uv_diff_coeff /= 0.0_8 .or. h_diff_coeff /= 0.0_8
+if (use_diff .and. trim(diff_time_scheme) /= "explicit_Eul1") &
+    call fail("device path implements diff_time_scheme = explicit_Eul1 only")
+if (use_diff .and. (trim(hordiff_uv_name) /= "hordiff_vec_xyz_Ah_sbp_42_narrow" .or. &
+                    trim(hordiff_h_name) /= "hordiff_scalar_Ah_sbp_42_narrow")) &
+    call fail("device path implements the hyperdiffusion pair hordiff_vec_xyz_Ah_sbp_42_narrow / " // &
+              "hordiff_scalar_Ah_sbp_42_narrow only")
 
 simulation_time = simulation_time_days*86400.0_8 + simulation_time_hours*3600.0_8 + &
                   simulation_time_min*60.0_8 + simulation_time_sec
@@ -105,6 +115,11 @@ call operator%initial_state(seed, lmax, h0, h_amp, v_max, h, u, v, 0)
 call create_stvec_swm_gpu(state, operator)
 call state%upload(h, u, v, 0)
 call create_rk4_gpu(timescheme)
+if (use_diff) then
+    call create_swm_diff_gpu_operator(operator_diff, operator, hordiff_uv_name, uv_diff_coeff, &
+                                      hordiff_h_name, h_diff_coeff)
+    call create_explicit_eul1_gpu(timescheme_diff)
+end if
 if (autotune_reps > 0) then
     best_us = operator%autotune(state, autotune_reps)
     print '(A,F10.2,A)', "autotuned stage kernel: ", best_us, " us"
@@ -116,7 +131,14 @@ call diagnostics%print()
 it = 0
 do while (it < nsteps)
     chunk = min(nstep_diagnostics - mod(it, nstep_diagnostics), nsteps - it)
-    call timescheme%steps(state, operator, dt, chunk)
+    if (use_diff) then
+        do k = 1, chunk
+            call timescheme%step(state, operator, dt)
+            call timescheme_diff%step(state, operator_diff, dt)
+        end do
+    else
+        call timescheme%steps(state, operator, dt, chunk)
+    end if
     it = it + chunk
     if (mod(it, nstep_diagnostics) == 0 .or. it == nsteps) then
         diagnostics = operator%get_diagnostics(state)

@@ -16,6 +16,7 @@ implicit none
 private
 public :: operator_swm_gpu_t, stvec_swm_gpu_t, rk4_gpu_t, key_value_r8_gpu_t
 public :: create_swm_gpu_operator, create_stvec_swm_gpu, create_rk4_gpu, swm_gpu_check
+public :: operator_swm_diff_gpu_t, explicit_eul1_gpu_t, create_swm_diff_gpu_operator, create_explicit_eul1_gpu
 
 interface
     function c_swm_last_error() bind(C, name="swm_last_error") result(p)
@@ -105,6 +106,21 @@ interface
         real(c_double), value :: h0, hamp, vmax
         real(c_double), intent(out) :: h(*), u(*), v(*)
     end function
+    integer(c_int) function c_swm_diff_apply(ctx, vin, vout, uv_name, uv_coeff, h_name, h_coeff) &
+                                              bind(C, name="swm_diff_apply")
+        import :: c_ptr, c_int, c_char, c_double
+        type(c_ptr), value :: ctx, vin, vout
+        character(kind=c_char), intent(in) :: uv_name(*), h_name(*)
+        real(c_double), value :: uv_coeff, h_coeff
+    end function
+    integer(c_int) function c_swm_diff_step(ctx, state, dt, uv_name, uv_coeff, h_name, h_coeff) &
+                                             bind(C, name="swm_diff_step")
+        import :: c_ptr, c_int, c_char, c_double
+        type(c_ptr), value :: ctx, state
+        real(c_double), value :: dt
+        character(kind=c_char), intent(in) :: uv_name(*), h_name(*)
+        real(c_double), value :: uv_coeff, h_coeff
+    end function
     function c_strlen(s) bind(C, name="strlen") result(n)
         import :: c_ptr, c_size_t
         type(c_ptr), value :: s
@@ -142,6 +158,21 @@ contains
     procedure, public :: download => download_gpu
     procedure, public :: free => free_gpu
 end type stvec_swm_gpu_t
+
+! operator_swm_diff_t (operator_swm_diff_mod.f90:14-21): hyperdiffusion tendency operator
+type operator_swm_diff_gpu_t
+    type(c_ptr)         :: ctx = c_null_ptr
+    character(len=255)  :: hordiff_uv_name = "", hordiff_h_name = ""
+    real(kind=8)        :: uv_diff_coeff = 0.0_8, h_diff_coeff = 0.0_8
+contains
+    procedure, public :: apply => apply_swm_diff_gpu
+end type operator_swm_diff_gpu_t
+
+! explicit_Eul1_t (time_schemes/explicit_Eul1_mod.f90:12-15)
+type explicit_eul1_gpu_t
+contains
+    procedure, public :: step => step_explicit_eul1_gpu
+end type explicit_eul1_gpu_t
 
 type rk4_gpu_t
 contains
@@ -305,6 +336,45 @@ subroutine steps_rk4_gpu(this, v0, operator, dt, nsteps)
     integer(kind=4),           intent(in)    :: nsteps
     call swm_gpu_check(c_swm_rk4_steps(operator%ctx, v0%state, dt, int(nsteps, c_int)), "rk4%steps")
 end subroutine steps_rk4_gpu
+
+! create_swm_diff_operator (operator_swm_diff_factory_mod.f90:11-34) for the names of the
+! &shallow_water_model namelist; the device implements the sbp42 narrow-stencil pair
+! ("hordiff_vec_xyz_Ah_sbp_42_narrow", "hordiff_scalar_Ah_sbp_42_narrow")
+subroutine create_swm_diff_gpu_operator(op_diff, op, hordiff_uv_name, uv_diff_coeff, hordiff_h_name, h_diff_coeff)
+    type(operator_swm_diff_gpu_t), intent(out) :: op_diff
+    type(operator_swm_gpu_t),      intent(in)  :: op
+    character(len=*),              intent(in)  :: hordiff_uv_name, hordiff_h_name
+    real(kind=8),                  intent(in)  :: uv_diff_coeff, h_diff_coeff
+    op_diff%ctx = op%ctx
+    op_diff%hordiff_uv_name = hordiff_uv_name
+    op_diff%hordiff_h_name = hordiff_h_name
+    op_diff%uv_diff_coeff = uv_diff_coeff
+    op_diff%h_diff_coeff = h_diff_coeff
+end subroutine create_swm_diff_gpu_operator
+
+! operator_swm_diff_t%apply (operator_swm_diff_mod.f90:25-46): vout = diffusion tendency of vin
+subroutine apply_swm_diff_gpu(this, vout, vin)
+    class(operator_swm_diff_gpu_t), intent(inout) :: this
+    class(stvec_swm_gpu_t),         intent(inout) :: vout, vin
+    call swm_gpu_check(c_swm_diff_apply(this%ctx, vin%state, vout%state, trim(this%hordiff_uv_name)//c_null_char, &
+                                        this%uv_diff_coeff, trim(this%hordiff_h_name)//c_null_char, &
+                                        this%h_diff_coeff), "operator_diff%apply")
+end subroutine apply_swm_diff_gpu
+
+subroutine create_explicit_eul1_gpu(ts)
+    type(explicit_eul1_gpu_t), intent(out) :: ts
+end subroutine create_explicit_eul1_gpu
+
+! explicit_Eul1_t%step (explicit_Eul1_mod.f90:20-30): v0 = v0 + dt * operator(v0)
+subroutine step_explicit_eul1_gpu(this, v0, operator, dt)
+    class(explicit_eul1_gpu_t),     intent(inout) :: this
+    class(stvec_swm_gpu_t),         intent(inout) :: v0
+    class(operator_swm_diff_gpu_t), intent(inout) :: operator
+    real(kind=8),                   intent(in)    :: dt
+    call swm_gpu_check(c_swm_diff_step(operator%ctx, v0%state, dt, trim(operator%hordiff_uv_name)//c_null_char, &
+                                       operator%uv_diff_coeff, trim(operator%hordiff_h_name)//c_null_char, &
+                                       operator%h_diff_coeff), "timescheme_diff%step")
+end subroutine step_explicit_eul1_gpu
 
 subroutine print_key_value(this)
     class(key_value_r8_gpu_t), intent(in) :: this

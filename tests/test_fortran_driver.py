@@ -36,8 +36,10 @@ NAMELIST = """
     massflux_op_name  = 'massflux_colocated',
     quadrature_name   = '{quad}',
     diff_time_scheme  = 'explicit_Eul1'
-    uv_diff_coeff     =  0.0,
-    h_diff_coeff      =  0.0,
+    uv_diff_coeff     =  {uvc},
+    hordiff_uv_name   = '{uvn}',
+    h_diff_coeff      =  {hc},
+    hordiff_h_name    = '{hn}',
     dt={dt},
     tau_write = 86400.0,
     tau_diagnostics = {tau}
@@ -57,13 +59,15 @@ def _need_driver():
         pytest.fail("Fortran driver not built (run __graft_entry__.build())")
 
 
-def _write_nml(tmp_path, N, order, dt, nsteps, seed, stag="Ah"):
+def _write_nml(tmp_path, N, order, dt, nsteps, seed, stag="Ah", uvc=0.0, hc=0.0,
+               uvn="hordiff_empty", hn="hordiff_empty"):
     import cubedsphere_sbp_swm_amd as swm
     names = swm.operator_names(order)
     out = tmp_path / "state.bin"
     text = NAMELIST.format(N=N, stag=stag, omega=swm.EARTH["omega"], div=names["div_op_name"],
                            grad=names["grad_op_name"], curl=names["curl_op_name"], quad=names["quadrature_name"],
-                           dt=dt, tau=dt * 3, tsec=dt * nsteps, seed=seed, out=out)
+                           dt=dt, tau=dt * 3, tsec=dt * nsteps, seed=seed, out=out, uvc=uvc, hc=hc, uvn=uvn,
+                           hn=hn)
     p = tmp_path / "namelist_swm"
     p.write_text(text)
     return p, out
@@ -110,3 +114,41 @@ def test_driver_equals_python_path(tmp_path, N, order, nsteps):
     vals = dict((k, float(val)) for k, val in re.findall(r"(\w+) = +([-+0-9.Ee]+)", last))
     for k in ("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"):
         assert vals[k] == pytest.approx(d[k], rel=1e-15, abs=0.0), k
+
+
+@pytest.mark.gpu
+def test_driver_with_hyperdiffusion_equals_python_path(tmp_path):
+    """&shallow_water_model with the sbp42 narrow-stencil hyperdiffusion: the driver's RK4 +
+    explicit_Eul1 loop equals the Python path step for step, bit for bit."""
+    _need_driver()
+    import cubedsphere_sbp_swm_amd as swm
+    N, order, nsteps, dt, seed = 24, 42, 5, 600.0, 3
+    uvn, hn = "hordiff_vec_xyz_Ah_sbp_42_narrow", "hordiff_scalar_Ah_sbp_42_narrow"
+    nml, out = _write_nml(tmp_path, N, order, dt, nsteps, seed, uvc=0.03, hc=0.01, uvn=uvn, hn=hn)
+    print(f"[fortran driver + diffusion] N={N} steps={nsteps}", flush=True)
+    r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    n = N + 1
+    raw = np.fromfile(out, dtype=np.float64)
+    fh, fu, fv = (a.reshape(6, n, n) for a in np.split(raw, 3))
+    op = swm.create_swm_operator(N, **swm.operator_names(order), scale=swm.EARTH["radius"],
+                                 omega=swm.EARTH["omega"], grav=swm.EARTH["grav"])
+    h, u, v = op.initial_state(seed=seed, lmax=8)
+    x = swm.create_stvec_swm(op)
+    x.upload(h, u, v)
+    ts = swm.create_timescheme(op, "rk4")
+    for _ in range(nsteps):
+        ts.steps(x, dt, 1)
+        op.diff_step(x, dt, 0.03, 0.01, uvn, hn)
+    for a, b in zip((fh, fu, fv), x.download()):
+        assert np.array_equal(a, b)
+
+
+def test_driver_rejects_other_hyperdiffusion(tmp_path):
+    _need_driver()
+    nml, _ = _write_nml(tmp_path, 16, 42, 600.0, 2, 1, uvc=0.03, hc=0.01,
+                        uvn="hordiff_vec_xyz_Ah_sbp_63_narrow", hn="hordiff_scalar_Ah_sbp_63_narrow")
+    r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode != 0
+    assert "hyperdiffusion" in r.stdout
