@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(TX * TY) stage21_kernel(DevGeo<T> g, StageArgs
                     return slot < 2 ? sm.gx[slot][d][k - k0][0] : sm.gy[d][k - k0][0];
                 };
                 T r[4];
-                neighbour_raw<21, T>(g, ei, k, d0, dn, r);
+                neighbour_raw<21, T>(g, edge_desc(ei), k, d0, dn, r);
                 if (slot < 2) { sm.nrx[slot][kloc][0] = r[0]; sm.nrx[slot][kloc][1] = r[1]; }
                 else { sm.nry[kloc][0] = r[0]; sm.nry[kloc][1] = r[1]; }
             }

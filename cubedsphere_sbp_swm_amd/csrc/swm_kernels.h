@@ -6,11 +6,13 @@
 //   * the five stencil inputs K = (u.ut+v.vt)/2 + g h + g hs, J*hu, J*hv, J*(v/J), J*(-u/J)
 //     are formed once per point in LDS from the state and the metric;
 //   * SBP derivative rows (interior stencil or boundary closure) are applied along x and y;
-//   * tiles on a cube edge also load the neighbour panel's points along the shared edge
-//     (depth 0 .. closure width) into LDS, evaluate the neighbour's one-sided derivative
-//     rows at the shared points there, and combine them with the reference's scalar and
-//     covariant-vector edge averaging (halo_Ah_scalar_sync_mod.f90:52-93,
-//     ecs_Ah_vec_sync_mod.f90:77-124) -- this replaces the MPI halo exchange;
+//     the tiles compute every point off the panel edges;
+//   * blocks of their own in the same launch compute the panel-edge points: they load both
+//     panels' points along a segment of a shared edge (depth 0 .. closure width) into LDS,
+//     evaluate both one-sided derivative rows at the shared points, and combine them with
+//     the reference's scalar and covariant-vector edge averaging
+//     (halo_Ah_scalar_sync_mod.f90:52-93, ecs_Ah_vec_sync_mod.f90:77-124) -- this replaces
+//     the MPI halo exchange;
 //   * the Coriolis term, tendencies and the RK stage combination finish in registers.
 // FP64 builds use -ffp-contract=off and the reference's operation order, so results agree
 // bit for bit with an FP64 CPU evaluation of the same algorithm.
@@ -58,6 +60,14 @@ struct StageArgs {
 // one cube of an ensemble: its geometry and the arguments of the current RK stage
 // tile width of a tile of `ty` rows (512 points per tile)
 SWM_HD constexpr int tile_width(int ty) { return 512 / ty; }
+
+// The panel-edge points are computed by edge segments of EDGE_SEG consecutive points: per
+// panel the x-edges (i = 0 and i = n-1, positions j = 0..n-1, cube corners included) and the
+// y-edges (j = 0 and j = n-1, positions i = 1..n-2).
+constexpr int EDGE_SEG = 64;
+SWM_HD constexpr int edge_segs_x(int n) { return (n + EDGE_SEG - 1) / EDGE_SEG; }
+SWM_HD constexpr int edge_segs_y(int n) { return (n - 2 + EDGE_SEG - 1) / EDGE_SEG; }
+SWM_HD constexpr int edge_segs(int n) { return 6 * 2 * (edge_segs_x(n) + edge_segs_y(n)); }
 
 template <typename T>
 struct EnsMember {

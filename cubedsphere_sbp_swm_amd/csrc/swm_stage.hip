@@ -19,32 +19,52 @@ struct Tile {
 };
 
 // LDS of one tile.  K lives on the cross-shaped region; the x-derivative inputs only on the
-// tile rows, the y-derivative inputs only on the tile columns.  Edge tiles use the ghost
-// strips: the neighbour panel's stencil inputs along each shared edge (x-edges: slots 0/1
-// for i=1 / i=n, y-edge: j=1 or j=n), and the neighbour's raw values at the shared points.
+// tile rows, the y-derivative inputs only on the tile columns.
 template <int ORD, typename T, int TX, int TY>
-struct Smem {
+struct TileSmem {
     using C = Tile<ORD, T, TX, TY>;
-    static constexpr int R = C::R, D = C::D, LX = C::LX, LY = C::LY;
+    static constexpr int R = C::R, LX = C::LX, LY = C::LY;
     T K[LY][LX];
     T Gxd[TY][LX], Gxc[TY][LX];
     T Gyd[LY][TX], Gyc[LY][TX];
-    T gx0[2][TY + 2 * R][6];   // depth 0: K, Gxd, Gyd, Gxc, Gyc, J
-    T gxn[2][D - 1][TY][3];    // depth >= 1: K and the two inputs along the normal axis
-    T gy0[TX + 2 * R][6];
-    T gyn[D - 1][TX][3];
-    T nrx[2][TY][4];           // neighbour raw gx, gy, div, curl
-    T nry[TX][4];
 };
 
-// Raw values of the neighbour panel at the shared point k of edge e, from a ghost strip.
+// LDS of one edge segment: the stencil inputs along the edge of the own panel (side 0) and of
+// the neighbour across the edge (side 1), and at the cube corners the neighbour across the
+// other edge (slot 0: position 0, slot 1: position n-1).
+template <int ORD, typename T>
+struct EdgeSmem {
+    static constexpr int R = Sbp<ORD>::REACH, D = Sbp<ORD>::last[0];
+    T e0[2][EDGE_SEG + 2 * R][6];   // depth 0: K, Gxd, Gyd, Gxc, Gyc, J
+    T en[2][D - 1][EDGE_SEG][3];    // depth >= 1: K and the two inputs along the normal axis
+    T c0[2][2 * R + 1][6];
+    T cn[2][D - 1][3];
+};
+
+template <int ORD, typename T, int TX, int TY>
+union Smem {
+    TileSmem<ORD, T, TX, TY> t;
+    EdgeSmem<ORD, T> e;
+};
+
+// Raw values (gx, gy, div, curl before the edge syncs) of a panel at the point k of one of
+// its edges, from strips of stencil inputs along that edge.  ei describes the edge in the
+// neighbour-table form (panel, normal axis is x, normal index, tangential base and step); for
+// a neighbour panel it maps the local position k to the neighbour's tangential index.
 // d0(kk, f) returns field f (0 K, 1 Gxd, 2 Gyd, 3 Gxc, 4 Gyc, 5 J) at tangential index kk,
 // depth 0; dn(d, f3) the normal-axis inputs (0 K, 1 Gnd, 2 Gnc) at depth d >= 1.
+// One panel edge in neighbour-table form: panel, normal axis is x, normal index of the edge
+// line, tangential index = a_base + s * (local position), component code of the vector halo.
+struct EdgeDesc {
+    int q, nu_x, tnu, a_base, s, code;
+};
+__device__ __forceinline__ EdgeDesc edge_desc(const int *ei) { return {ei[0], ei[1], ei[2], ei[3], ei[4], ei[5]}; }
+
 template <int ORD, typename T, typename D0, typename DN>
-__device__ __forceinline__ void neighbour_raw(const DevGeo<T> &g, const int *ei, int k, const D0 &d0,
+__device__ __forceinline__ void neighbour_raw(const DevGeo<T> &g, const EdgeDesc &ed, int k, const D0 &d0,
                                               const DN &dn, T out[4]) {
     const int n = g.n;
-    const int nu_x = ei[1], tnu = ei[2], a0 = ei[3] + ei[4] * k, s = ei[4];
+    const int nu_x = ed.nu_x, tnu = ed.tnu, a0 = ed.a_base + ed.s * k, s = ed.s;
     auto nu_get = [&](int fld3, int a) {  // along the normal axis of the neighbour
         const int d = tnu == 0 ? a : n - 1 - a;
         if (d == 0) return d0(k, fld3 == 0 ? 0 : (nu_x ? (fld3 == 1 ? 1 : 3) : (fld3 == 1 ? 2 : 4)));
@@ -134,10 +154,11 @@ __device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs
     }
 }
 
-// One 64 x TY tile of panel p starting at (tx0, ty0): the whole RHS + RK stage update.
+// One TX x TY tile of panel p starting at (tx0, ty0): the whole RHS + RK stage update of its
+// points off the panel edges (the panel-edge points belong to the edge segments).
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF, bool PERSIST = false>
 __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T> &a, int p, int tx0, int ty0,
-                                           Smem<ORD, T, TX, TY> &sm) {
+                                           TileSmem<ORD, T, TX, TY> &sm) {
     using C = Tile<ORD, T, TX, TY>;
     constexpr int R = C::R, D = C::D, LX = C::LX, LY = C::LY, NT = C::NT;
     const int n = g.n;
@@ -150,7 +171,7 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     OutPre<T> pre{};
     {
         const int oi = tx0 + tid % TX, oj = ty0 + tid / TX;
-        if (oi < n && oj < n) pre = prefetch_point<T, MODE>(g, a, pidx(g, p, oi, oj));
+        if (oi > 0 && oj > 0 && oi < n - 1 && oj < n - 1) pre = prefetch_point<T, MODE>(g, a, pidx(g, p, oi, oj));
     }
 
     // ---- load phase: stencil inputs of the tile rows (incl. x-halo) and of the y-halo rows.
@@ -201,84 +222,12 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         sm.Gyd[ly][cx] = d.Gyd;
         sm.Gyc[ly][cx] = d.Gyc;
     }
-    // ---- ghost strips of edge tiles (neighbour panels' points, via the neighbour table) ----
-    const bool eL = tx0 == 0, eR = tx0 + TX >= n, eB = ty0 == 0, eT = ty0 + TY >= n;
-    const bool edge_tile = eL || eR || eB || eT;
-    const int ey = eB ? 2 : 3;  // the y-edge of this tile (n > TY: never both)
-    if (edge_tile) {
-        constexpr int NX0 = TY + 2 * R, NXN = (D - 1) * TY, NY0 = TX + 2 * R, NYN = (D - 1) * TX;
-        constexpr int NXS = NX0 + NXN, NYS = NY0 + NYN;
-        const int total = 2 * NXS + NYS;
-        for (int c = tid; c < total; c += NT) {
-            int slot, e, kk, d, idx;
-            if (c < 2 * NXS) {
-                slot = c / NXS;
-                const int r = c - slot * NXS;
-                if (!(slot == 0 ? eL : eR)) continue;
-                e = slot;
-                if (r < NX0) { d = 0; idx = r; kk = ty0 - R + r; }
-                else { d = 1 + (r - NX0) / TY; idx = (r - NX0) % TY; kk = ty0 + idx; }
-            } else {
-                slot = 2;
-                const int r = c - 2 * NXS;
-                if (!(eB || eT)) continue;
-                e = ey;
-                if (r < NY0) { d = 0; idx = r; kk = tx0 - R + r; }
-                else { d = 1 + (r - NY0) / TX; idx = (r - NY0) % TX; kk = tx0 + idx; }
-            }
-            if (kk < 0 || kk >= n) continue;
-            // neighbour point: normal index tnu -/+ d, tangential index a_base + s*kk
-            const int *ei = g.einfo + (p * 4 + e) * 8;
-            const int nidx = ei[2] == 0 ? d : n - 1 - d, tidx = ei[3] + ei[4] * kk;
-            const int qi = ei[1] ? nidx : tidx, qj = ei[1] ? tidx : nidx;
-            const Derived<T> dv = derived_global<OTF>(g, y, a.hs, ei[0], qi, qj);
-            if (d == 0) {
-                T *dst = slot < 2 ? sm.gx0[slot][idx] : sm.gy0[idx];
-                dst[0] = dv.K; dst[1] = dv.Gxd; dst[2] = dv.Gyd; dst[3] = dv.Gxc; dst[4] = dv.Gyc;
-                dst[5] = metric_J<OTF>(g, pidx(g, ei[0], qi, qj), qi, qj);
-            } else {
-                const bool nu_x = ei[1] != 0;
-                T *dst = slot < 2 ? sm.gxn[slot][d - 1][idx] : sm.gyn[d - 1][idx];
-                dst[0] = dv.K;
-                dst[1] = nu_x ? dv.Gxd : dv.Gyd;
-                dst[2] = nu_x ? dv.Gxc : dv.Gyc;
-            }
-        }
-    }
     __syncthreads();
-
-    // ---- neighbour raw values at the shared points of this tile (edge tiles only) ----
-    if (edge_tile) {
-        const int task = tid;
-        if (task < 2 * TY + TX) {
-            int slot, k, kloc;
-            if (task < 2 * TY) { slot = task / TY; kloc = task % TY; k = ty0 + kloc; }
-            else { slot = 2; kloc = task - 2 * TY; k = tx0 + kloc; }
-            const bool active = slot == 0 ? eL : slot == 1 ? eR : (eB || eT);
-            if (active && k < n) {
-                const int e = slot < 2 ? slot : ey;
-                const int *ei = g.einfo + (p * 4 + e) * 8;
-                T r[4];
-                if (slot < 2) {
-                    const int k0 = ty0 - R;
-                    neighbour_raw<ORD, T>(g, ei, k, [&](int kk, int f) { return sm.gx0[slot][kk - k0][f]; },
-                                          [&](int d, int f3) { return sm.gxn[slot][d - 1][k - ty0][f3]; }, r);
-                    for (int q = 0; q < 4; ++q) sm.nrx[slot][kloc][q] = r[q];
-                } else {
-                    const int k0 = tx0 - R;
-                    neighbour_raw<ORD, T>(g, ei, k, [&](int kk, int f) { return sm.gy0[kk - k0][f]; },
-                                          [&](int d, int f3) { return sm.gyn[d - 1][k - tx0][f3]; }, r);
-                    for (int q = 0; q < 4; ++q) sm.nry[kloc][q] = r[q];
-                }
-            }
-        }
-        __syncthreads();
-    }
 
     // ---- output phase: one point per thread ----
     const int tx = tid % TX, ty = tid / TX;
     const int i = tx0 + tx, j = ty0 + ty;
-    if (i >= n || j >= n) return;
+    if (i <= 0 || j <= 0 || i >= n - 1 || j >= n - 1) return;
     const int lx = tx + R, ly = ty + R;
     const T dxK = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.K[ly][ai - tx0 + R]; });
     const T dyK = sbp_row<ORD, T>(j, n, [&](int aj) { return sm.K[aj - ty0 + R][lx]; });
@@ -295,80 +244,165 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         Jv = g.J[id]; q11 = g.Qi11[id]; q12 = g.Qi12[id]; q22 = g.Qi22[id];
     }
     const T Jd = Jv * g.hx * g.scale;
-    T gx = dxK / g.hxs;
-    T gy = dyK / g.hxs;
-    T dv = (dxD + dyD) / Jd;
-    T cu = (dxC + dyC) / Jd;
-
-    const bool L = i == 0, Rt = i == n - 1, B = j == 0, Tp = j == n - 1;
-    if (L || Rt || B || Tp) {
-        // halo values: neighbour raws mapped to this panel's components (vector halo convention)
-        auto hx_vals = [&](int slot, int k, T &V_, T &U_, T &D_, T &C_) {
-            const int code = g.einfo[(p * 4 + slot) * 8 + 5];
-            const T *r = sm.nrx[slot][k - ty0];
-            map_vec(code, r[0], r[1], V_, U_);
-            D_ = r[2];
-            C_ = r[3];
-        };
-        auto hy_vals = [&](int k, T &V_, T &U_, T &D_, T &C_) {
-            const int code = g.einfo[(p * 4 + ey) * 8 + 5];
-            const T *r = sm.nry[k - tx0];
-            map_vec(code, r[0], r[1], V_, U_);
-            D_ = r[2];
-            C_ = r[3];
-        };
-        // covariant components enter the sync routine as (U,V) = (gy,gx)
-        // (ecs_Ah_vec_sync_mod.f90:51-53); formulas of ecs_Ah_vec_sync_mod.f90:77-124 and
-        // halo_Ah_scalar_sync_mod.f90:52-93 in the reference's operand order
-        const T U = gy, V = gx;
-        T Ux, Vx, Dx, Cx, Uy, Vy, Dy, Cy;
-        if ((L || Rt) && (B || Tp)) {
-            hx_vals(L ? 0 : 1, j, Vx, Ux, Dx, Cx);
-            hy_vals(i, Vy, Uy, Dy, Cy);
-            const T qy = B ? g.qb[i] : g.qt[i];
-            const T qx = L ? g.ql[j] : g.qr[j];
-            dv = (dv + Dx + Dy) / T(3);
-            cu = (cu + Cx + Cy) / T(3);
-            gy = (U + Ux + Uy + qy * Vy) / T(3);
-            gx = (V + Vx + Vy + qx * Ux) / T(3);
-        } else if (L) {
-            hx_vals(0, j, Vx, Ux, Dx, Cx);
-            dv = T(0.5) * (Dx + dv);
-            cu = T(0.5) * (Cx + cu);
-            gy = T(0.5) * (Ux + U);
-            gx = T(0.5) * (V + Vx + g.ql[j] * Ux);
-        } else if (Rt) {
-            hx_vals(1, j, Vx, Ux, Dx, Cx);
-            dv = T(0.5) * (dv + Dx);
-            cu = T(0.5) * (cu + Cx);
-            gy = T(0.5) * (U + Ux);
-            gx = T(0.5) * (V + Vx + g.qr[j] * Ux);
-        } else if (B) {
-            hy_vals(i, Vy, Uy, Dy, Cy);
-            dv = T(0.5) * (dv + Dy);
-            cu = T(0.5) * (cu + Cy);
-            gx = T(0.5) * (Vy + V);
-            gy = T(0.5) * (U + Uy + g.qb[i] * Vy);
-        } else {
-            hy_vals(i, Vy, Uy, Dy, Cy);
-            dv = T(0.5) * (dv + Dy);
-            cu = T(0.5) * (cu + Cy);
-            gx = T(0.5) * (V + Vy);
-            gy = T(0.5) * (U + Uy + g.qt[i] * Vy);
-        }
-    }
+    const T gx = dxK / g.hxs;
+    const T gy = dyK / g.hxs;
+    const T dv = (dxD + dyD) / Jd;
+    const T cu = (dxC + dyC) / Jd;
 
     finish_point<T, MODE>(g, a, id, y[id], y[id + g.fs], y[id + 2 * g.fs], Jv, q11, q12, q22, gx, gy, dv, cu, pre);
 }
 
+// Edge segment `seg` of the cube (edge_segs(n) per cube): positions k0..k1-1 of one panel
+// edge.  The own panel's and the neighbour's stencil inputs along the edge go to LDS; each
+// point then forms both panels' raw values, applies the scalar and covariant vector edge syncs
+// (ecs_Ah_vec_sync_mod.f90:77-124 with (U, V) = (gy, gx), halo_Ah_scalar_sync_mod.f90:52-93,
+// in the reference's operand order) and the RK stage update.
+template <int ORD, typename T, int MODE, bool OTF, bool PERSIST = false>
+__device__ __forceinline__ void stage_edge(const DevGeo<T> &g, const StageArgs<T> &a, int seg, EdgeSmem<ORD, T> &sm,
+                                           int nthreads) {
+    constexpr int R = Sbp<ORD>::REACH, D = Sbp<ORD>::last[0], S = EDGE_SEG;
+    const int n = g.n;
+    int tid = threadIdx.x;
+    if constexpr (PERSIST) asm volatile("" : "+v"(tid));
+    const int nsx = edge_segs_x(n), nsy = edge_segs_y(n), per = 2 * (nsx + nsy);
+    const int p = seg / per, r = seg - p * per;
+    int e, si;
+    if (r < 2 * nsx) { e = r / nsx; si = r - e * nsx; }
+    else { e = 2 + (r - 2 * nsx) / nsy; si = r - 2 * nsx - (e - 2) * nsy; }
+    const int k0 = e < 2 ? si * S : 1 + si * S;
+    const int k1 = e < 2 ? min(n, k0 + S) : min(n - 1, k0 + S);
+    // side 0: the own panel in neighbour-table form (identity orientation); side 1: across e
+    const EdgeDesc own{p, e < 2 ? 1 : 0, (e == 0 || e == 2) ? 0 : n - 1, 0, 1, 0};
+    const EdgeDesc nbr = edge_desc(g.einfo + (p * 4 + e) * 8);
+    // cube corners of the x-edges: the neighbour across the y-edge at i = 0 / n-1
+    const bool c0 = e < 2 && k0 == 0, c1 = e < 2 && k1 == n;
+    const int ci = e == 0 ? 0 : n - 1;
+    const T *y = a.y;
+
+    const int k = k0 + tid;
+    const bool act = tid < k1 - k0;
+
+    // ---- strips: per side S + 2R positions at depth 0, S positions at depths 1..D-1; at a
+    // corner 2R+1 positions at depth 0 and the corner position at depths 1..D-1 ----
+    constexpr int N0 = S + 2 * R, NS = N0 + (D - 1) * S, NC = 2 * R + 1 + (D - 1);
+    const int total = 2 * NS + 2 * NC;
+    for (int c = tid; c < total; c += nthreads) {
+        EdgeDesc ei;
+        int d, kk, slot, side, idx;
+        if (c < 2 * NS) {
+            side = c / NS;
+            const int q = c - side * NS;
+            ei = side == 0 ? own : nbr;
+            slot = -1;
+            if (q < N0) { d = 0; idx = q; kk = k0 - R + q; }
+            else { d = 1 + (q - N0) / S; idx = (q - N0) % S; kk = k0 + idx; if (kk >= k1) continue; }
+        } else {
+            const int q = c - 2 * NS;
+            slot = q / NC;
+            if (!(slot == 0 ? c0 : c1)) continue;
+            const int w = q - slot * NC;
+            ei = edge_desc(g.einfo + (p * 4 + (slot == 0 ? 2 : 3)) * 8);
+            side = 0;
+            if (w < 2 * R + 1) { d = 0; idx = w; kk = ci - R + w; }
+            else { d = 1 + (w - (2 * R + 1)); idx = 0; kk = ci; }
+        }
+        if (kk < 0 || kk >= n) continue;
+        const int nidx = ei.tnu == 0 ? d : n - 1 - d, tidx = ei.a_base + ei.s * kk;
+        const int qi = ei.nu_x ? nidx : tidx, qj = ei.nu_x ? tidx : nidx;
+        const Derived<T> dv = derived_global<OTF>(g, y, a.hs, ei.q, qi, qj);
+        if (d == 0) {
+            T *dst = slot < 0 ? sm.e0[side][idx] : sm.c0[slot][idx];
+            dst[0] = dv.K; dst[1] = dv.Gxd; dst[2] = dv.Gyd; dst[3] = dv.Gxc; dst[4] = dv.Gyc;
+            dst[5] = metric_J<OTF>(g, pidx(g, ei.q, qi, qj), qi, qj);
+        } else {
+            const bool nu_x = ei.nu_x != 0;
+            T *dst = slot < 0 ? sm.en[side][d - 1][idx] : sm.cn[slot][d - 1];
+            dst[0] = dv.K;
+            dst[1] = nu_x ? dv.Gxd : dv.Gyd;
+            dst[2] = nu_x ? dv.Gxc : dv.Gyc;
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+    int i, j;
+    if (e < 2) { i = e == 0 ? 0 : n - 1; j = k; }
+    else { i = k; j = e == 2 ? 0 : n - 1; }
+    const long long id = pidx(g, p, i, j);
+    const OutPre<T> pre = prefetch_point<T, MODE>(g, a, id);
+
+    T ro[4], rn[4];
+    neighbour_raw<ORD, T>(g, own, k, [&](int kk, int f) { return sm.e0[0][kk - (k0 - R)][f]; },
+                          [&](int d, int f3) { return sm.en[0][d - 1][k - k0][f3]; }, ro);
+    neighbour_raw<ORD, T>(g, nbr, k, [&](int kk, int f) { return sm.e0[1][kk - (k0 - R)][f]; },
+                          [&](int d, int f3) { return sm.en[1][d - 1][k - k0][f3]; }, rn);
+    T gx = ro[0], gy = ro[1], dv = ro[2], cu = ro[3];
+    const T U = gy, V = gx;
+    // the neighbour's raw values mapped to this panel's components (vector halo convention)
+    T Vn, Un;
+    map_vec(nbr.code, rn[0], rn[1], Vn, Un);
+    const T Dn = rn[2], Cn = rn[3];
+    const bool corner = e < 2 && (k == 0 || k == n - 1);
+    if (corner) {
+        const int slot = k == 0 ? 0 : 1;
+        const EdgeDesc ec = edge_desc(g.einfo + (p * 4 + (slot == 0 ? 2 : 3)) * 8);
+        T rc[4];
+        neighbour_raw<ORD, T>(g, ec, ci, [&](int kk, int f) { return sm.c0[slot][kk - (ci - R)][f]; },
+                              [&](int d, int f3) { return sm.cn[slot][d - 1][f3]; }, rc);
+        T Vy, Uy;
+        map_vec(ec.code, rc[0], rc[1], Vy, Uy);
+        const T qy = slot == 0 ? g.qb[i] : g.qt[i];
+        const T qx = e == 0 ? g.ql[j] : g.qr[j];
+        dv = (dv + Dn + rc[2]) / T(3);
+        cu = (cu + Cn + rc[3]) / T(3);
+        gy = (U + Un + Uy + qy * Vy) / T(3);
+        gx = (V + Vn + Vy + qx * Un) / T(3);
+    } else if (e == 0) {
+        dv = T(0.5) * (Dn + dv);
+        cu = T(0.5) * (Cn + cu);
+        gy = T(0.5) * (Un + U);
+        gx = T(0.5) * (V + Vn + g.ql[j] * Un);
+    } else if (e == 1) {
+        dv = T(0.5) * (dv + Dn);
+        cu = T(0.5) * (cu + Cn);
+        gy = T(0.5) * (U + Un);
+        gx = T(0.5) * (V + Vn + g.qr[j] * Un);
+    } else if (e == 2) {
+        dv = T(0.5) * (dv + Dn);
+        cu = T(0.5) * (cu + Cn);
+        gx = T(0.5) * (Vn + V);
+        gy = T(0.5) * (U + Un + g.qb[i] * Vn);
+    } else {
+        dv = T(0.5) * (dv + Dn);
+        cu = T(0.5) * (cu + Cn);
+        gx = T(0.5) * (V + Vn);
+        gy = T(0.5) * (U + Un + g.qt[i] * Vn);
+    }
+    T Jv, q11, q12, q22;
+    if constexpr (OTF) {
+        const Metric m = metric_otf(g.tanA, i, j);
+        Jv = (T)m.J; q11 = (T)m.Qi11; q12 = (T)m.Qi12; q22 = (T)m.Qi22;
+    } else {
+        Jv = g.J[id]; q11 = g.Qi11[id]; q12 = g.Qi12[id]; q22 = g.Qi22[id];
+    }
+    finish_point<T, MODE>(g, a, id, y[id], y[id + g.fs], y[id + 2 * g.fs], Jv, q11, q12, q22, gx, gy, dv, cu, pre);
+}
+
+// edge blocks first (a multiple of 8, so that the tiles keep their XCD-contiguous mapping)
+SWM_HD constexpr int edge_blocks(int n) { return (edge_segs(n) + 7) / 8 * 8; }
+
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
 __global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T> a) {
     __shared__ Smem<ORD, T, TX, TY> sm;
+    const int ne = edge_blocks(g.n);
+    if ((int)blockIdx.x < ne) {
+        if ((int)blockIdx.x < edge_segs(g.n)) stage_edge<ORD, T, MODE, OTF>(g, a, blockIdx.x, sm.e, TX * TY);
+        return;
+    }
     const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
-    const int tile = xcd_tile(blockIdx.x, 6 * ntx * nty);
+    const int tile = xcd_tile(blockIdx.x - ne, 6 * ntx * nty);
     const int p = tile / (ntx * nty);
     const int rem = tile - p * ntx * nty;
-    stage_tile<ORD, T, MODE, TX, TY, OTF>(g, a, p, (rem % ntx) * TX, (rem / ntx) * TY, sm);
+    stage_tile<ORD, T, MODE, TX, TY, OTF>(g, a, p, (rem % ntx) * TX, (rem / ntx) * TY, sm.t);
 }
 
 // Copy of a block-uniform struct into scalar registers (word by word through readfirstlane):
@@ -386,9 +420,9 @@ __device__ __forceinline__ S uniform_copy(const S *src) {
 }
 
 // Ensemble of cubes of different sizes in one persistent launch: the blocks walk the
-// concatenated tile list of all members (tile_start[m] = first tile of member m, prefix sums
-// over the members' 6*ntx*nty tiles), XCD-contiguous, and take each tile's geometry and
-// stage arguments from the member table.
+// concatenated task list of all members (tile_start[m] = first task of member m, prefix sums
+// over the members' 6*ntx*nty tiles followed by their edge_segs(n) edge segments),
+// XCD-contiguous, and take each task's geometry and stage arguments from the member table.
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
 __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *__restrict__ members,
                                                             const int *__restrict__ tile_start, int nmembers,
@@ -407,10 +441,14 @@ __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *
         const StageArgs<T> a = uniform_copy(&members[lo].a);
         const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
         const int lt = tile - tile_start[lo];
+        __syncthreads();  // the previous task's LDS reads are done
+        if (lt >= 6 * ntx * nty) {
+            stage_edge<ORD, T, MODE, OTF, true>(g, a, lt - 6 * ntx * nty, sm.e, TX * TY);
+            continue;
+        }
         const int p = lt / (ntx * nty);
         const int rem = lt - p * ntx * nty;
-        __syncthreads();  // the previous tile's LDS reads are done
-        stage_tile<ORD, T, MODE, TX, TY, OTF, true>(g, a, p, (rem % ntx) * TX, (rem / ntx) * TY, sm);
+        stage_tile<ORD, T, MODE, TX, TY, OTF, true>(g, a, p, (rem % ntx) * TX, (rem / ntx) * TY, sm.t);
     }
 }
 
@@ -422,7 +460,7 @@ template <int ORD, typename T, int TY, bool OTF>
 hipError_t launch_one(int mode, const DevGeo<T> &g, const StageArgs<T> &a, hipStream_t s) {
     constexpr int TX = tile_width(TY);
     const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
-    const dim3 grid(6 * ntx * nty), block(TX * TY);
+    const dim3 grid(edge_blocks(g.n) + 6 * ntx * nty), block(TX * TY);
     switch (mode) {
     case MODE_RHS: hipLaunchKernelGGL((stage_kernel<ORD, T, MODE_RHS, TX, TY, OTF>), grid, block, 0, s, g, a); break;
     case MODE_STAGE1: hipLaunchKernelGGL((stage_kernel<ORD, T, MODE_STAGE1, TX, TY, OTF>), grid, block, 0, s, g, a); break;

@@ -120,8 +120,8 @@ int swm_diff_step(swm_ctx *ctx, void *state, double dt, const char *hordiff_uv_n
  * 8, 16 or 32 rows (tiles of 64 x 8, 32 x 16 or 16 x 32 points, 512 threads; the height must be
  * below N+1) and, for FP64, whether the metric terms J, Qi are read from memory (0) or evaluated
  * on the fly from tan(alpha) (1; identical values).  swm_ctx_autotune times every
- * configuration with `reps` launches of the RK stage-2 kernel on `state`, keeps the fastest and
- * returns its time per launch. */
+ * configuration with `reps` launches of the RK stage-2 kernel on `state`, in two interleaved
+ * rounds, keeps the fastest (minimum over the rounds) and returns its time per launch. */
 int swm_ctx_set_tiling(swm_ctx *ctx, int tile_rows, int metric_on_the_fly);
 int swm_ctx_get_tiling(const swm_ctx *ctx, int *tile_rows, int *metric_on_the_fly);
 int swm_ctx_autotune(swm_ctx *ctx, void *state, int reps, double *best_us);
