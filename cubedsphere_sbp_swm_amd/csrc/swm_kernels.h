@@ -61,13 +61,22 @@ struct StageArgs {
 // tile width of a tile of `ty` rows (512 points per tile)
 SWM_HD constexpr int tile_width(int ty) { return 512 / ty; }
 
-// The panel-edge points are computed by edge segments of EDGE_SEG consecutive points: per
-// panel the x-edges (i = 0 and i = n-1, positions j = 0..n-1, cube corners included) and the
-// y-edges (j = 0 and j = n-1, positions i = 1..n-2).
-constexpr int EDGE_SEG = 64;
-SWM_HD constexpr int edge_segs_x(int n) { return (n + EDGE_SEG - 1) / EDGE_SEG; }
-SWM_HD constexpr int edge_segs_y(int n) { return (n - 2 + EDGE_SEG - 1) / EDGE_SEG; }
-SWM_HD constexpr int edge_segs(int n) { return 6 * 2 * (edge_segs_x(n) + edge_segs_y(n)); }
+// The panel-edge points are computed by edge segments of edge_seg_len(n) consecutive points:
+// per panel the x-edges (i = 0 and i = n-1, positions j = 0..n-1, cube corners included) and
+// the y-edges (j = 0 and j = n-1, positions i = 1..n-2).  A single cube with small panels
+// takes shorter segments, i.e. more blocks for the same edge work; the persistent ensemble
+// loop, where every task costs a fixed overhead, keeps the longest.
+constexpr int EDGE_SEG = 64;  // longest segment (LDS sizing)
+SWM_HD constexpr int edge_seg_len(int n, bool persist = false) { return (persist || n > 256) ? EDGE_SEG : 48; }
+SWM_HD constexpr int edge_segs_x(int n, bool persist = false) {
+    return (n + edge_seg_len(n, persist) - 1) / edge_seg_len(n, persist);
+}
+SWM_HD constexpr int edge_segs_y(int n, bool persist = false) {
+    return (n - 2 + edge_seg_len(n, persist) - 1) / edge_seg_len(n, persist);
+}
+SWM_HD constexpr int edge_segs(int n, bool persist = false) {
+    return 6 * 2 * (edge_segs_x(n, persist) + edge_segs_y(n, persist));
+}
 
 template <typename T>
 struct EnsMember {

@@ -252,19 +252,19 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     finish_point<T, MODE>(g, a, id, y[id], y[id + g.fs], y[id + 2 * g.fs], Jv, q11, q12, q22, gx, gy, dv, cu, pre);
 }
 
-// Edge segment `seg` of the cube (edge_segs(n) per cube): positions k0..k1-1 of one panel
-// edge.  The own panel's and the neighbour's stencil inputs along the edge go to LDS; each
+// Edge segment `seg` of the cube (edge_segs(n, PERSIST) per cube): positions k0..k1-1 of one
+// panel edge, at most edge_seg_len(n, PERSIST) points.  The own panel's and the neighbour's stencil inputs along the edge go to LDS; each
 // point then forms both panels' raw values, applies the scalar and covariant vector edge syncs
 // (ecs_Ah_vec_sync_mod.f90:77-124 with (U, V) = (gy, gx), halo_Ah_scalar_sync_mod.f90:52-93,
 // in the reference's operand order) and the RK stage update.
 template <int ORD, typename T, int MODE, bool OTF, bool PERSIST = false>
 __device__ __forceinline__ void stage_edge(const DevGeo<T> &g, const StageArgs<T> &a, int seg, EdgeSmem<ORD, T> &sm,
                                            int nthreads) {
-    constexpr int R = Sbp<ORD>::REACH, D = Sbp<ORD>::last[0], S = EDGE_SEG;
-    const int n = g.n;
+    constexpr int R = Sbp<ORD>::REACH, D = Sbp<ORD>::last[0];
+    const int n = g.n, S = edge_seg_len(n, PERSIST);
     int tid = threadIdx.x;
     if constexpr (PERSIST) asm volatile("" : "+v"(tid));
-    const int nsx = edge_segs_x(n), nsy = edge_segs_y(n), per = 2 * (nsx + nsy);
+    const int nsx = edge_segs_x(n, PERSIST), nsy = edge_segs_y(n, PERSIST), per = 2 * (nsx + nsy);
     const int p = seg / per, r = seg - p * per;
     int e, si;
     if (r < 2 * nsx) { e = r / nsx; si = r - e * nsx; }
@@ -284,7 +284,8 @@ __device__ __forceinline__ void stage_edge(const DevGeo<T> &g, const StageArgs<T
 
     // ---- strips: per side S + 2R positions at depth 0, S positions at depths 1..D-1; at a
     // corner 2R+1 positions at depth 0 and the corner position at depths 1..D-1 ----
-    constexpr int N0 = S + 2 * R, NS = N0 + (D - 1) * S, NC = 2 * R + 1 + (D - 1);
+    const int N0 = S + 2 * R, NS = N0 + (D - 1) * S;
+    constexpr int NC = 2 * R + 1 + (D - 1);
     const int total = 2 * NS + 2 * NC;
     for (int c = tid; c < total; c += nthreads) {
         EdgeDesc ei;
@@ -421,7 +422,7 @@ __device__ __forceinline__ S uniform_copy(const S *src) {
 
 // Ensemble of cubes of different sizes in one persistent launch: the blocks walk the
 // concatenated task list of all members (tile_start[m] = first task of member m, prefix sums
-// over the members' 6*ntx*nty tiles followed by their edge_segs(n) edge segments),
+// over the members' 6*ntx*nty tiles followed by their edge_segs(n, true) edge segments),
 // XCD-contiguous, and take each task's geometry and stage arguments from the member table.
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
 __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *__restrict__ members,
