@@ -82,7 +82,7 @@ int ens_build_tables(swm_ens *e) {
     for (size_t k = 0; k < M; ++k) {
         const int n = e->m[k]->n;
         const int tx = swm::tile_width(e->ty);
-        ts[k + 1] = ts[k] + 6 * ((n + tx - 1) / tx) * ((n + e->ty - 1) / e->ty) + swm::edge_segs(n, true);
+        ts[k + 1] = ts[k] + 6 * swm::tiles_along(n, tx) * swm::tiles_along(n, e->ty) + swm::edge_segs(n, true);
     }
     if (!e->tile_start) HIP_TRY(hipMalloc((void **)&e->tile_start, (M + 1) * sizeof(int)));
     HIP_TRY(hipMemcpy(e->tile_start, ts.data(), (M + 1) * sizeof(int), hipMemcpyHostToDevice));

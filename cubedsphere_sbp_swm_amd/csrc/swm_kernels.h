@@ -60,6 +60,9 @@ struct StageArgs {
 // one cube of an ensemble: its geometry and the arguments of the current RK stage
 // tile width of a tile of `ty` rows (512 points per tile)
 SWM_HD constexpr int tile_width(int ty) { return 512 / ty; }
+// The tiles cover the points 1 .. n-2 of a panel line (the edge segments own 0 and n-1):
+// tiles_along(n, t) tiles of t points, the b-th starting at point 1 + b*t.
+SWM_HD constexpr int tiles_along(int n, int t) { return (n - 2 + t - 1) / t; }
 
 // The panel-edge points are computed by edge segments of edge_seg_len(n) consecutive points:
 // per panel the x-edges (i = 0 and i = n-1, positions j = 0..n-1, cube corners included) and

@@ -154,8 +154,8 @@ __device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs
     }
 }
 
-// One TX x TY tile of panel p starting at (tx0, ty0): the whole RHS + RK stage update of its
-// points off the panel edges (the panel-edge points belong to the edge segments).
+// One TX x TY tile of panel p starting at (tx0, ty0) >= (1, 1): the whole RHS + RK stage update
+// of its points off the panel edges (the panel-edge points belong to the edge segments).
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF, bool PERSIST = false>
 __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T> &a, int p, int tx0, int ty0,
                                            TileSmem<ORD, T, TX, TY> &sm) {
@@ -399,11 +399,11 @@ __global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T
         if ((int)blockIdx.x < edge_segs(g.n)) stage_edge<ORD, T, MODE, OTF>(g, a, blockIdx.x, sm.e, TX * TY);
         return;
     }
-    const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
+    const int ntx = tiles_along(g.n, TX), nty = tiles_along(g.n, TY);
     const int tile = xcd_tile(blockIdx.x - ne, 6 * ntx * nty);
     const int p = tile / (ntx * nty);
     const int rem = tile - p * ntx * nty;
-    stage_tile<ORD, T, MODE, TX, TY, OTF>(g, a, p, (rem % ntx) * TX, (rem / ntx) * TY, sm.t);
+    stage_tile<ORD, T, MODE, TX, TY, OTF>(g, a, p, 1 + (rem % ntx) * TX, 1 + (rem / ntx) * TY, sm.t);
 }
 
 // Copy of a block-uniform struct into scalar registers (word by word through readfirstlane):
@@ -422,7 +422,8 @@ __device__ __forceinline__ S uniform_copy(const S *src) {
 
 // Ensemble of cubes of different sizes in one persistent launch: the blocks walk the
 // concatenated task list of all members (tile_start[m] = first task of member m, prefix sums
-// over the members' 6*ntx*nty tiles followed by their edge_segs(n, true) edge segments),
+// over the members' 6*ntx*nty tiles (tiles_along) followed by their edge_segs(n, true) edge
+// segments),
 // XCD-contiguous, and take each task's geometry and stage arguments from the member table.
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
 __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *__restrict__ members,
@@ -440,7 +441,7 @@ __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *
         lo = __builtin_amdgcn_readfirstlane(lo);  // block-uniform member index
         const DevGeo<T> g = uniform_copy(&members[lo].g);
         const StageArgs<T> a = uniform_copy(&members[lo].a);
-        const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
+        const int ntx = tiles_along(g.n, TX), nty = tiles_along(g.n, TY);
         const int lt = tile - tile_start[lo];
         __syncthreads();  // the previous task's LDS reads are done
         if (lt >= 6 * ntx * nty) {
@@ -449,7 +450,7 @@ __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *
         }
         const int p = lt / (ntx * nty);
         const int rem = lt - p * ntx * nty;
-        stage_tile<ORD, T, MODE, TX, TY, OTF, true>(g, a, p, (rem % ntx) * TX, (rem / ntx) * TY, sm.t);
+        stage_tile<ORD, T, MODE, TX, TY, OTF, true>(g, a, p, 1 + (rem % ntx) * TX, 1 + (rem / ntx) * TY, sm.t);
     }
 }
 
@@ -460,7 +461,7 @@ namespace swm {
 template <int ORD, typename T, int TY, bool OTF>
 hipError_t launch_one(int mode, const DevGeo<T> &g, const StageArgs<T> &a, hipStream_t s) {
     constexpr int TX = tile_width(TY);
-    const int ntx = (g.n + TX - 1) / TX, nty = (g.n + TY - 1) / TY;
+    const int ntx = tiles_along(g.n, TX), nty = tiles_along(g.n, TY);
     const dim3 grid(edge_blocks(g.n) + 6 * ntx * nty), block(TX * TY);
     switch (mode) {
     case MODE_RHS: hipLaunchKernelGGL((stage_kernel<ORD, T, MODE_RHS, TX, TY, OTF>), grid, block, 0, s, g, a); break;
