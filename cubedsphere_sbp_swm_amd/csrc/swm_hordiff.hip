@@ -14,10 +14,10 @@
 // Step, operator_swm_diff_t%apply + explicit_Eul1 (operator_swm_diff_mod.f90:25-46,
 // time_schemes/explicit_Eul1_mod.f90:20-30): y += dt * tend.
 //
-// The Laplacian kernels take one field per block row (blockIdx.y): h and the three Cartesian
-// velocity components share every launch.  Panel-edge results are written to a compact edge
-// buffer first and synchronised from there, so that no kernel reads a value another block
-// of the same launch overwrites.
+// One Laplacian is two launches, hd_lap_tile and hd_sync, over several fields (blockIdx.y):
+// h and the three Cartesian velocity components share every launch.  Panel-edge results are
+// written to a compact edge buffer first and synchronised from there, so that no kernel reads
+// a value another block of the same launch overwrites.
 
 #include "laplace_sbp42_narrow.h"
 
@@ -109,52 +109,6 @@ __device__ __forceinline__ double ln42_interior(const double *b, const double *f
     return ln42_interior_seq(b, fv, std::make_integer_sequence<int, 5>{});
 }
 
-// D2(b) f at index t of a line of n points, in two parts so that the coefficients b are
-// loaded once for several fields: ln42_b gives b (Q = Qi11 or Qi22 times J at the stencil
-// points, at(a) = memory offset of point a) and the closure row r (-1 in the interior),
-// ln42_apply the row applied to one field.
-template <typename A>
-__device__ __forceinline__ int ln42_b(int t, int n, const double *__restrict__ Q, const double *__restrict__ J,
-                                      const A &at, double *b) {
-    if (t < LN42_NB || t >= n - LN42_NB) {
-        const bool left = t < LN42_NB;
-#pragma unroll
-        for (int k = 0; k < LN42_NCOL; ++k) {
-            const long long o = at(left ? k : n - 1 - k);
-            b[k] = Q[o] * J[o];
-        }
-        return left ? t : n - 1 - t;
-    }
-#pragma unroll
-    for (int o = 0; o < 5; ++o) {
-        const long long q = at(t - 2 + o);
-        b[o] = Q[q] * J[q];
-    }
-    return -1;
-}
-
-template <typename A>
-__device__ __forceinline__ double ln42_apply(int t, int n, int r, const double *b, const double *__restrict__ f,
-                                             const A &at) {
-    double fv[LN42_NCOL];
-    if (r >= 0) {
-        const bool left = t < LN42_NB;
-#pragma unroll
-        for (int k = 0; k < LN42_NCOL; ++k) fv[k] = f[at(left ? k : n - 1 - k)];
-        switch (r) {
-        case 0: return ln42_row<0>(b, fv);
-        case 1: return ln42_row<1>(b, fv);
-        case 2: return ln42_row<2>(b, fv);
-        case 3: return ln42_row<3>(b, fv);
-        case 4: return ln42_row<4>(b, fv);
-        default: return ln42_row<5>(b, fv);
-        }
-    }
-#pragma unroll
-    for (int o = 0; o < 5; ++o) fv[o] = f[at(t - 2 + o)];
-    return ln42_interior(b, fv);
-}
-
 // V = b1 u + b2 v (vec_math_mod.f90:107-126); W[0] = h, W[1..3] = Vx, Vy, Vz
 __global__ void hd_to_cart(HdGeo g, const double *__restrict__ y, double *__restrict__ W) {
     const int n = g.n;
@@ -170,62 +124,167 @@ __global__ void hd_to_cart(HdGeo g, const double *__restrict__ y, double *__rest
     for (int c = 0; c < 3; ++c) W[(1 + c) * g.fs + o] = g.b1[c * g.fs + o] * u + g.b2[c * g.fs + o] * v;
 }
 
-// d1x = Dx(f) J Qi12, d1y = Dy(f) J Qi12 of field k = blockIdx.y: D[2k], D[2k+1]
-__global__ void hd_d1(HdGeo g, const double *__restrict__ W, double *__restrict__ D) {
-    const int n = g.n;
-    const long long total = 6LL * n * n;
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int k = blockIdx.y;
-    const int i = (int)(t % n);
-    const long long pj = t / n;
-    const int j = (int)(pj % n);
-    const int p = (int)(pj / n);
-    const long long row = pj * g.pitch, o = row + i, col = (long long)p * n * g.pitch + i;
-    const double *f = W + k * g.fs;
-    const double dx = sbp_row<42, double>(i, n, [&](int a) { return f[row + a]; });
-    const double dy = sbp_row<42, double>(j, n, [&](int a) { return f[col + (long long)a * g.pitch]; });
-    D[2 * k * g.fs + o] = dx * g.J[o] * g.Qi12[o];
-    D[(2 * k + 1) * g.fs + o] = dy * g.J[o] * g.Qi12[o];
+// The Laplacian of field k = blockIdx.y on LDS tiles of TX x TY points, before the edge sync:
+// off-edge points to Wout, panel-edge points to E (ns slots per field).  A tile loads f on the
+// box of lines its rows read (+-2 points in the interior, points 0..7 / n-8..n-1 where it holds
+// closure rows: the narrow D2 closure reads 8 points, the first-derivative closure 6), and
+// b = Qi11 J on the tile rows, b = Qi22 J on the tile columns.  It then forms the first
+// derivatives d1y = Dy(f) J Qi12 on the tile rows and d1x = Dx(f) J Qi12 on the tile columns
+// (each at the positions whose J and Qi12 the same thread loaded), and finally the D2 rows,
+// the cross-term penalties and the cross derivatives at its points.
+template <int TX, int TY>
+struct HdTile {
+    static constexpr int R = 2, B = 6, NR = 8;
+    static constexpr int span(int t) { return t + R + B - 1 > NR ? t + R + B - 1 : NR; }
+    static constexpr int LW = span(TX), LH = span(TY), NT = TX * TY;
+    static constexpr int KF = (LH * LW + NT - 1) / NT;              // box positions per thread
+    static constexpr int KX = (TY * LW + LH * TX + NT - 1) / NT;    // cross positions per thread
+};
+
+__device__ __forceinline__ void hd_rows(int a0, int ae, int n, int &lo, int &hi) {
+    constexpr int R = 2, B = 6, NR = 8;
+    lo = a0 - R > 0 ? a0 - R : 0;
+    hi = ae + R < n ? ae + R : n;
+    if (a0 < B) {
+        lo = 0;
+        const int m = NR < n ? NR : n;
+        hi = hi > m ? hi : m;
+    }
+    if (ae > n - B) {
+        hi = n;
+        const int m = n - NR > 0 ? n - NR : 0;
+        lo = lo < m ? lo : m;
+    }
 }
 
-// L f of field k = blockIdx.y before the edge sync: off-edge points to Wout, panel-edge
-// points to E (ns slots per field)
-__global__ void hd_lap(HdGeo g, const double *__restrict__ W, const double *__restrict__ D, double *__restrict__ Wout,
-                       double *__restrict__ E) {
-    const int n = g.n;
-    const long long total = 6LL * n * n;
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int k = blockIdx.y;
-    const int i = (int)(t % n);
-    const long long pj = t / n;
-    const int j = (int)(pj % n);
-    const int p = (int)(pj / n);
-    const long long row = pj * g.pitch, o = row + i, col = (long long)p * n * g.pitch + i;
-    auto atx = [&](int a) { return row + a; };
-    auto aty = [&](int a) { return col + (long long)a * g.pitch; };
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__restrict__ W,
+                                                       double *__restrict__ Wout, double *__restrict__ E) {
+    using C = HdTile<TX, TY>;
+    __shared__ double sf[C::LH][C::LW];                  // f on the box
+    __shared__ double sbx[TY][C::LW], sdy[TY][C::LW];    // tile rows: Qi11 J, d1y
+    __shared__ double sby[C::LH][TX], sdx[C::LH][TX];    // tile columns: Qi22 J, d1x
+    const int n = g.n, k = blockIdx.y;
+    const int ntx = (n + TX - 1) / TX, nty = (n + TY - 1) / TY;
+    const int tile = xcd_tile(blockIdx.x, 6 * ntx * nty);
+    const int p = tile / (ntx * nty), rem = tile - p * ntx * nty;
+    const int x0 = (rem % ntx) * TX, y0 = (rem / ntx) * TY;
+    const int xe = x0 + TX < n ? x0 + TX : n, ye = y0 + TY < n ? y0 + TY : n;
+    int lx, hx, ly, hy;
+    hd_rows(x0, xe, n, lx, hx);
+    hd_rows(y0, ye, n, ly, hy);
     const double *f = W + k * g.fs;
-    const double *d1x = D + 2 * k * g.fs, *d1y = D + (2 * k + 1) * g.fs;
-    double d2x, d2y;
-    {
-        double b[LN42_NCOL];
-        const int r = ln42_b(i, n, g.Qi11, g.J, atx, b);
-        d2x = ln42_apply(i, n, r, b, f, atx);
+    const long long pb = (long long)p * n;
+    const int wa = hx - lx, nbox = (hy - ly) * wa;
+    const int nA = (ye - y0) * wa, nc = xe - x0, nB = (hy - ly) * nc;
+    const int tid = threadIdx.x;
+
+    // ---- loads: f on the box; Qi, J, Qi12 at the cross positions (tile rows x line range:
+    // kind 1; line range x tile columns: kind 2) ----
+    double fv[C::KF];
+#pragma unroll
+    for (int u = 0; u < C::KF; ++u) {
+        const int q = tid + u * C::NT;
+        if (q < nbox) {
+            const int r = q / wa;
+            fv[u] = f[(pb + ly + r) * g.pitch + lx + (q - r * wa)];
+        }
     }
-    {
-        double b[LN42_NCOL];
-        const int r = ln42_b(j, n, g.Qi22, g.J, aty, b);
-        d2y = ln42_apply(j, n, r, b, f, aty);
+    double qv[C::KX], jv[C::KX], q12[C::KX];
+    int kind[C::KX], px[C::KX], py[C::KX];
+#pragma unroll
+    for (int u = 0; u < C::KX; ++u) {
+        const int q = tid + u * C::NT;
+        kind[u] = 0;
+        px[u] = 0;
+        py[u] = 0;
+        if (q < nA) {
+            const int r = q / wa;
+            py[u] = y0 + r;
+            px[u] = lx + (q - r * wa);
+            kind[u] = 1;
+        } else if (q < nA + nB) {
+            const int q2 = q - nA, r = q2 / nc;
+            py[u] = ly + r;
+            px[u] = x0 + (q2 - r * nc);
+            kind[u] = 2;
+        }
+        if (kind[u]) {
+            const long long o = (pb + py[u]) * g.pitch + px[u];
+            qv[u] = kind[u] == 1 ? g.Qi11[o] : g.Qi22[o];
+            jv[u] = g.J[o];
+            q12[u] = g.Qi12[o];
+        }
     }
+#pragma unroll
+    for (int u = 0; u < C::KF; ++u) {
+        const int q = tid + u * C::NT;
+        if (q < nbox) {
+            const int r = q / wa;
+            sf[r][q - r * wa] = fv[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < C::KX; ++u) {
+        if (kind[u] == 1) sbx[py[u] - y0][px[u] - lx] = qv[u] * jv[u];
+        else if (kind[u] == 2) sby[py[u] - ly][px[u] - x0] = qv[u] * jv[u];
+    }
+    __syncthreads();
+
+    // ---- first derivatives at the cross positions (calc_tile_d1x_d1y, :60-93) ----
+#pragma unroll
+    for (int u = 0; u < C::KX; ++u) {
+        if (kind[u] == 1) {
+            const int c = px[u] - lx;
+            const double d = sbp_row<42, double>(py[u], n, [&](int a) { return sf[a - ly][c]; });
+            sdy[py[u] - y0][c] = d * jv[u] * q12[u];
+        } else if (kind[u] == 2) {
+            const int r = py[u] - ly;
+            const double d = sbp_row<42, double>(px[u], n, [&](int a) { return sf[r][a - lx]; });
+            sdx[r][px[u] - x0] = d * jv[u] * q12[u];
+        }
+    }
+    __syncthreads();
+
+    const int tx = tid % TX, ty = tid / TX, i = x0 + tx, j = y0 + ty;
+    if (i >= n || j >= n) return;
+    const long long o = (pb + j) * g.pitch + i;
+    auto d2 = [&](int t, const auto &bget, const auto &fget) {
+        double bb[LN42_NCOL], ff[LN42_NCOL];
+        if (t < LN42_NB || t >= n - LN42_NB) {
+            const bool left = t < LN42_NB;
+#pragma unroll
+            for (int q = 0; q < LN42_NCOL; ++q) {
+                const int a = left ? q : n - 1 - q;
+                bb[q] = bget(a);
+                ff[q] = fget(a);
+            }
+            switch (left ? t : n - 1 - t) {
+            case 0: return ln42_row<0>(bb, ff);
+            case 1: return ln42_row<1>(bb, ff);
+            case 2: return ln42_row<2>(bb, ff);
+            case 3: return ln42_row<3>(bb, ff);
+            case 4: return ln42_row<4>(bb, ff);
+            default: return ln42_row<5>(bb, ff);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            bb[q] = bget(t - 2 + q);
+            ff[q] = fget(t - 2 + q);
+        }
+        return ln42_interior(bb, ff);
+    };
+    double d2x = d2(i, [&](int a) { return sbx[ty][a - lx]; }, [&](int a) { return sf[j - ly][a - lx]; });
+    const double d2y = d2(j, [&](int a) { return sby[a - ly][tx]; }, [&](int a) { return sf[a - ly][i - lx]; });
     // penalties of the cross terms (laplace_Ah_sbp42_narrow_mod.f90:716-742), x edges first
     const double pc = LN42_PNUM / LN42_PDEN;
-    if (i == 0) d2x = d2x - 0.5 * (pc * (-2.0 * d1y[o]));
-    if (i == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * d1y[o]));
-    if (j == 0) d2x = d2x - 0.5 * (pc * (-2.0 * d1x[o]));
-    if (j == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * d1x[o]));
-    const double d2yx = sbp_row<42, double>(j, n, [&](int a) { return d1x[col + (long long)a * g.pitch]; });
-    const double d2xy = sbp_row<42, double>(i, n, [&](int a) { return d1y[row + a]; });
+    if (i == 0) d2x = d2x - 0.5 * (pc * (-2.0 * sdy[ty][i - lx]));
+    if (i == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * sdy[ty][i - lx]));
+    if (j == 0) d2x = d2x - 0.5 * (pc * (-2.0 * sdx[j - ly][tx]));
+    if (j == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * sdx[j - ly][tx]));
+    const double d2yx = sbp_row<42, double>(j, n, [&](int a) { return sdx[a - ly][tx]; });
+    const double d2xy = sbp_row<42, double>(i, n, [&](int a) { return sdy[ty][a - lx]; });
     double v = 1.0 * d2x + 1.0 * d2y;
     v = v + 1.0 * d2xy + 1.0 * d2yx;
     v = v / (g.hs * g.hs);
@@ -346,7 +405,6 @@ namespace {
 struct HdWork {
     double *b1 = nullptr, *b2 = nullptr;          // 3 planes each
     double *WA = nullptr, *WB = nullptr;          // 4 fields each (h, Vx, Vy, Vz)
-    double *D = nullptr;                          // 8 fields (d1x, d1y per field)
     double *E = nullptr, *E2 = nullptr;           // 4 x ns, 2 x ns edge values
 };
 
@@ -365,7 +423,6 @@ int ensure_hd(swm_ctx *c) {
     HIP_TRY(hipMalloc((void **)&w->b2, 3 * fb));
     HIP_TRY(hipMalloc((void **)&w->WA, 4 * fb));
     HIP_TRY(hipMalloc((void **)&w->WB, 4 * fb));
-    HIP_TRY(hipMalloc((void **)&w->D, 8 * fb));
     HIP_TRY(hipMalloc((void **)&w->E, 4 * ns * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&w->E2, 2 * ns * sizeof(double)));
     HIP_TRY(hipMemcpy(w->b1, b1.data(), 3 * fb, hipMemcpyHostToDevice));
@@ -392,11 +449,12 @@ swm::HdGeo hd_geo(const swm_ctx *c) {
 int enqueue_laplace(swm_ctx *c, int nf, const double *in, double *out) {
     HdWork *w = c->hd;
     const swm::HdGeo g = hd_geo(c);
-    const long long total = 6LL * c->n * c->n;
-    const dim3 blk(256), grd((unsigned)((total + 255) / 256), nf);
+    const dim3 blk(256);
     const int ns = 24 * c->N;
-    hipLaunchKernelGGL(swm::hd_d1, grd, blk, 0, c->stream, g, in, w->D);
-    hipLaunchKernelGGL(swm::hd_lap, grd, blk, 0, c->stream, g, in, (const double *)w->D, out, w->E);
+    constexpr int LTX = 32, LTY = 8;
+    const int ltiles = 6 * ((c->n + LTX - 1) / LTX) * ((c->n + LTY - 1) / LTY);
+    hipLaunchKernelGGL((swm::hd_lap_tile<LTX, LTY>), dim3(ltiles, nf), dim3(LTX * LTY), 0, c->stream, g, in, out,
+                       w->E);
     hipLaunchKernelGGL(swm::hd_sync, dim3((ns + 255) / 256, nf), blk, 0, c->stream, g, (const double *)w->E, out);
     HIP_TRY(hipGetLastError());
     return SWM_OK;
@@ -434,7 +492,7 @@ int enqueue_diff(swm_ctx *c, const double *yin, const double *y0, double *yo, do
 
 void free_hd_work(HdWork *w) {
     if (!w) return;
-    void *ptrs[] = {w->b1, w->b2, w->WA, w->WB, w->D, w->E, w->E2};
+    void *ptrs[] = {w->b1, w->b2, w->WA, w->WB, w->E, w->E2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete w;
