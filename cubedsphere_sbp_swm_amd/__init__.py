@@ -161,10 +161,13 @@ class OperatorSWM:
         check(_capi.lib().swm_rhs_vjp(self.ctx, ctypes.c_void_p(y.ptr), ctypes.c_void_p(mu.ptr),
                                       ctypes.c_void_p(out.ptr)))
 
-    def laplace(self, out, x):
-        """h plane of `out` = narrow-stencil SBP Laplacian (sbp42) of the h plane of `x`
-        (laplace_Ah_sbp42_narrow_t%calc_laplace); FP64, N+1 >= 12."""
-        check(_capi.lib().swm_laplace(self.ctx, ctypes.c_void_p(x.ptr), ctypes.c_void_p(out.ptr)))
+    def laplace(self, out, x, laplace_name="laplace_ah_sbp42_narrow"):
+        """h plane of `out` = narrow-stencil SBP Laplacian of the h plane of `x`
+        (laplace_Ah_sbp42_narrow_t / laplace_Ah_sbp63_narrow_t %calc_laplace, picked by the
+        reference's names laplace_ah_sbp42_narrow / laplace_ah_sbp63_narrow); FP64,
+        N+1 >= 12 (sbp42) or 18 (sbp63)."""
+        check(_capi.lib().swm_laplace(self.ctx, laplace_name.encode(), ctypes.c_void_p(x.ptr),
+                                      ctypes.c_void_p(out.ptr)))
 
     def diff_apply(self, vout, vin, uv_diff_coeff, h_diff_coeff,
                    hordiff_uv_name="hordiff_vec_xyz_Ah_sbp_42_narrow",

@@ -47,7 +47,7 @@ SIGNATURES = {
     "swm_adj_forward": (_i, [_vp, _vp, _vp]),
     "swm_adj_backward": (_i, [_vp, _vp, _vp]),
     "swm_adj_set_tiling": (_i, [_vp, _i]),
-    "swm_laplace": (_i, [_vp, _vp, _vp]),
+    "swm_laplace": (_i, [_vp, ctypes.c_char_p, _vp, _vp]),
     "swm_diff_apply": (_i, [_vp, _vp, _vp, ctypes.c_char_p, _d, ctypes.c_char_p, _d]),
     "swm_diff_step": (_i, [_vp, _vp, _d, ctypes.c_char_p, _d, ctypes.c_char_p, _d]),
     "swm_state_copy_from_device": (_i, [_vp, _vp, _vp, _vp, _vp]),

@@ -1,29 +1,38 @@
-"""Generate laplace_sbp42_narrow.h, the coefficient table of the narrow-stencil SBP second
-derivative with a variable coefficient, from the reference's
-differential_operators/horizontal/laplace/laplace_Ah_sbp42_narrow_mod.f90 (calc_laplace_tile,
-:94-763).
+"""Generate the coefficient tables of the narrow-stencil SBP second derivative with a variable
+coefficient, laplace_sbp42_narrow.h and laplace_sbp63_narrow.h, from the reference's
+differential_operators/horizontal/laplace/laplace_Ah_sbp{42,63}_narrow_mod.f90
+(calc_laplace_tile).
 
 The reference writes each closure row r of D2(b) f as
-    d2(r) = -1/h_rr * sum_c m_rc f(c),   m_rc = sum_k (+-) (p_k / q_k) * b(k)
-with b(k) = Qi11 J (x) or Qi22 J (y) at the k-th point from the panel edge, and the interior
-row as five coefficients c(-2..2), each a signed sum of rationals times b(i-2..i+2).  This
-script reads those expressions, checks that the four closures (x/y, left/right mirror) carry
-identical ones, and writes the terms in source order so that C and HIP code evaluate them with
-the same operations: each product (p/q)*b, the sums left to right.
+    d2(r) = scale_r * sum_c m_rc f(c),   m_rc = sum_k (+-) coef_k * b(k)
+with b(k) = Qi11 J (x) or Qi22 J (y) at the k-th point from the panel edge; scale_r is
+-1/h_rr (sbp42) or 1 (sbp63), coef_k a rational p/q written in place (sbp42) or a named
+PARAMETER constant (sbp63).  The interior row has 2R+1 coefficients c(-R..R), each a signed
+sum of coefficients times b(i-R..i+R).  This script reads those expressions, checks that the
+four closures (x/y, left/right mirror) carry identical ones, and writes every coefficient as a
+C constant expression in the reference's operation order, so that C and HIP code fold it to
+the same double as the Fortran compiler and then evaluate each product coef*b and the sums
+left to right.
 
 Run once, with the reference checked out:  python3 gen_laplace_tables.py /path/to/reference
-The generated header is committed; nothing reads the reference at build or run time.
+The generated headers are committed; nothing reads the reference at build or run time.
 """
 import os
 import re
 import sys
 
-SRC = "src/differential_operators/horizontal/laplace/laplace_Ah_sbp42_narrow_mod.f90"
 HERE = os.path.dirname(os.path.abspath(__file__))
-OUT = os.path.join(HERE, "laplace_sbp42_narrow.h")
-
+SRC = "src/differential_operators/horizontal/laplace/laplace_Ah_sbp{o}_narrow_mod.f90"
 NUM = r"(\d+(?:\.\d*)?)(?:_8)?"
-TERM = re.compile(r"([+-])?\s*" + NUM + r"\s*/\s*" + NUM + r"\s*\*\s*(b\w+)")
+
+
+def fortran_to_c(expr):
+    """A Fortran constant expression of real(8) literals as a C expression (same operations)."""
+    e = re.sub(r"(\d+\.\d*)_8", r"\1", expr.strip())
+    e = re.sub(r"(\d+)_8", r"\1", e)
+    if not re.fullmatch(r"[0-9.+\-*/() ]+", e):
+        raise ValueError(expr)
+    return "(" + e + ")"
 
 
 def joined_lines(text):
@@ -38,35 +47,34 @@ def joined_lines(text):
     return out
 
 
-def terms(expr):
-    """Signed rational * b-name terms of an expression, in source order."""
+def terms(expr, params):
+    """Signed (coefficient C expression, b name) terms of a coefficient expression."""
+    pat = re.compile(r"([+-])?\s*(?:" + NUM + r"\s*/\s*" + NUM + r"|([A-Za-z]\w*))\s*\*\s*(b\w+)")
     expr = expr.strip()
     res, pos = [], 0
-    for m in TERM.finditer(expr):
+    for m in pat.finditer(expr):
         gap = expr[pos:m.start()].strip()
         if gap:
             raise ValueError(f"unparsed text {gap!r} in {expr!r}")
         sign = -1 if m.group(1) == "-" else 1
-        res.append((sign, m.group(2), m.group(3), m.group(4)))
+        if m.group(4):
+            coef = params[m.group(4)]
+        else:
+            coef = fortran_to_c(f"{m.group(2)}/{m.group(3)}")
+        res.append((sign, coef, m.group(5)))
         pos = m.end()
     if expr[pos:].strip():
         raise ValueError(f"unparsed tail in {expr!r}")
     return res
 
 
-def rational(expr):
-    m = re.fullmatch(r"\s*" + NUM + r"(?:\s*/\s*" + NUM + r")?\s*", expr)
-    if not m:
-        raise ValueError(expr)
-    return m.group(1), m.group(2) or "1"
-
-
-def lit(s):
-    return s if "." in s else s + ".0"
-
-
 def parse(path):
     text = open(path).read()
+    params = {}
+    head = text[:text.index("contains")]
+    for ln in joined_lines(head):
+        for m in re.finditer(r"\b(d\d{6}|di_\w+)\s*=\s*([^,]+?(?:\([^)]*\)[^,]*)?)\s*(?:,|$)", ln):
+            params[m.group(1)] = fortran_to_c(m.group(2))
     body = text[text.index("subroutine calc_laplace_tile"):text.index("end subroutine calc_laplace_tile")]
     lines = joined_lines(body)
     blocks, cur = [], None
@@ -83,34 +91,34 @@ def parse(path):
             continue
         mh = re.fullmatch(r"h(\d)(\d)\s*=\s*(.+)", s)
         if mh:
-            cur["h"] = rational(mh.group(3))
+            cur["h"] = fortran_to_c(mh.group(3))
             continue
         mm = re.fullmatch(r"m(\d)(\d+)\s*=\s*(.+)", s)
         if mm:
-            cur["m"][int(mm.group(2))] = terms(mm.group(3))
+            cur["m"][int(mm.group(2))] = terms(mm.group(3), params)
             cur["row"] = int(mm.group(1))
             continue
-        md = re.fullmatch(r"d2f_([xy])%p\(([^=]+)\)\s*=\s*-1\.0_8/h(\d)(\d)\*\((.+)\)", s)
+        md = re.fullmatch(r"d2f_([xy])%p\(([^=]+)\)\s*=\s*(-1\.0_8/h\d\d|1\.0_8)\*\((.+)\)", s)
         if md and cur.get("m"):
-            d = md.group(1)
-            cols = re.findall(r"m\d(\d+)\*f%p\(([^)]*)\)", md.group(5))
-            cur["dir"], cur["target"], cur["cols"] = d, md.group(2), cols
+            cur["dir"], cur["target"] = md.group(1), md.group(2)
+            cur["scale"] = "neg_inv_h" if md.group(3).startswith("-") else "one"
+            cur["cols"] = re.findall(r"m\d(\d+)\*f%p\(([^)]*)\)", md.group(4))
+            assert [int(c) for c, _ in cur["cols"]] == list(range(1, len(cur["cols"]) + 1))
             blocks.append(cur)
             cur = None
-    return blocks, lines
+    return params, blocks, lines
 
 
-def main():
-    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
-    blocks, lines = parse(os.path.join(ref, SRC))
-    assert len(blocks) == 24, len(blocks)  # 6 rows x (x, y) x (left, right)
+def generate(order, ref):
+    params, blocks, lines = parse(os.path.join(ref, SRC.format(o=order)))
+    nb = {42: 6, 63: 9}[order]
+    assert len(blocks) == 4 * nb, len(blocks)
     canon = {}
     for b in blocks:
         d = b["dir"]
         idx = b["target"].split(",")[0 if d == "x" else 1].strip()
         right = "n" in idx
         row = b["row"]
-        # the k-th b is taken k-1 points from the edge, the c-th column of m likewise
         qi = {"x": 1, "y": 3}[d]
         for name, (q, i1, i2) in b["b"].items():
             k = int(name[1:])
@@ -122,111 +130,115 @@ def main():
             c = int(c)
             want = (f"mesh_xy%n{d}" + ("" if c == 1 else f"-{c - 1}")) if right else str(c)
             assert pos == want, (d, right, row, c, pos)
-        key = (b["h"], tuple((c, tuple(b["m"][c])) for c in sorted(b["m"])))
+        key = (b["scale"], b["h"], tuple((c, tuple(b["m"][c])) for c in sorted(b["m"])))
         if row in canon:
             assert canon[row] == key, f"closure row {row} differs between x/y or left/right"
         canon[row] = key
     rows = sorted(canon)
-    assert rows == list(range(1, 7))
+    assert rows == list(range(1, nb + 1))
+    scales = {canon[r][0] for r in rows}
+    assert len(scales) == 1
+    scale = scales.pop()
 
-    # interior coefficients c(-2..2) in terms of b(i-2..i+2)
-    names = {"bmm": -2, "bm": -1, "bc": 0, "bp": 1, "bpp": 2}
+    R = {42: 2, 63: 3}[order]
+    names = {"bmmm": -3, "bmm": -2, "bm": -1, "bc": 0, "bp": 1, "bpp": 2, "bppp": 3}
+    cnames = {"cmmm": -3, "cmm": -2, "cm": -1, "cc": 0, "cp": 1, "cpp": 2, "cppp": 3}
     inter = {}
     for ln in lines:
-        m = re.fullmatch(r"\s*(cmm|cm|cc|cp|cpp)\s*=\s*(.+)", ln)
+        m = re.fullmatch(r"\s*(cmmm|cmm|cm|cc|cp|cpp|cppp)\s*=\s*(.+)", ln)
         if m:
-            t = [(sg, p, q, names[bn]) for sg, p, q, bn in terms(m.group(2))]
-            off = {"cmm": -2, "cm": -1, "cc": 0, "cp": 1, "cpp": 2}[m.group(1)]
-            assert inter.setdefault(off, t) == t
-    assert sorted(inter) == [-2, -1, 0, 1, 2]
+            t = [(sg, cf, names[bn]) for sg, cf, bn in terms(m.group(2), params)]
+            assert inter.setdefault(cnames[m.group(1)], t) == t
+    assert sorted(inter) == list(range(-R, R + 1)), sorted(inter)
     pen = set(re.findall(r"penalty = " + NUM + r"/" + NUM + r"\*", "\n".join(lines)))
     assert len(pen) == 1
     pnum, pden = pen.pop()
 
-    tn, tq, ts, tb = [], [], [], []
-    start = []
-    ncols = []
+    ncol = max(max(c for c, _ in canon[r][2]) for r in rows)
+    tc, ts, tb, start, ncols = [], [], [], [], []
     for r in rows:
-        h, ms = canon[r]
+        _, _, ms = canon[r]
         ncols.append(max(c for c, _ in ms))
         row_start = []
-        for c in range(1, 9):
-            row_start.append(len(tn))
+        for c in range(1, ncol + 1):
+            row_start.append(len(tc))
             for cc, tl in ms:
                 if cc != c:
                     continue
-                for sg, p, q, bn in tl:
-                    tn.append(lit(p)); tq.append(lit(q)); ts.append(str(sg)); tb.append(str(int(bn[1:]) - 1))
-        row_start.append(len(tn))
+                for sg, cf, bn in tl:
+                    tc.append(cf); ts.append(str(sg)); tb.append(str(int(bn[1:]) - 1))
+        row_start.append(len(tc))
         start.append(row_start)
-    it_n, it_q, it_s, it_b, it_start = [], [], [], [], []
-    for off in (-2, -1, 0, 1, 2):
-        it_start.append(len(it_n))
-        for sg, p, q, bo in inter[off]:
-            it_n.append(lit(p)); it_q.append(lit(q)); it_s.append(str(sg)); it_b.append(str(bo))
-    it_start.append(len(it_n))
+    ic, isg, ib, istart = [], [], [], []
+    for off in range(-R, R + 1):
+        istart.append(len(ic))
+        for sg, cf, bo in inter[off]:
+            ic.append(cf); isg.append(str(sg)); ib.append(str(bo))
+    istart.append(len(ic))
+    hs = [canon[r][1] or "(1.0)" for r in rows]
 
-    def arr(vals, per=6):
-        out = []
-        for i in range(0, len(vals), per):
-            out.append("    " + ", ".join(vals[i:i + per]) + ",")
-        return "\n".join(out)
+    P = f"LN{order}_"
 
-    hn = [lit(canon[r][0][0]) for r in rows]
-    hd = [lit(canon[r][0][1]) for r in rows]
-    with open(OUT, "w") as f:
-        f.write(f"""/* laplace_sbp42_narrow.h -- GENERATED by gen_laplace_tables.py; do not edit.
+    def arr(vals, per):
+        return "\n".join("    " + ", ".join(vals[i:i + per]) + "," for i in range(0, len(vals), per))
+
+    out = os.path.join(HERE, f"laplace_sbp{order}_narrow.h")
+    rowval = ("-((1.0 / h) * (m[r][0] f[0] + m[r][1] f[1] + ...))" if scale == "neg_inv_h"
+              else "1.0 * (m[r][0] f[0] + m[r][1] f[1] + ...)")
+    with open(out, "w") as f:
+        f.write(f"""/* laplace_sbp{order}_narrow.h -- GENERATED by gen_laplace_tables.py; do not edit.
  *
- * Narrow-stencil SBP second derivative D2(b) of 4th order interior accuracy, as the reference's
- * laplace_Ah_sbp42_narrow_t (differential_operators/horizontal/laplace/
- * laplace_Ah_sbp42_narrow_mod.f90:94-763).  Closure row r = 0..5 (r-th point from a panel edge,
- * mirrored at the far edge):
- *     d2[r] = (-1 / (LN42_HNUM[r] / LN42_HDEN[r])) * (m[r][0] f[0] + m[r][1] f[1] + ...)
- *     m[r][c] = sum over t in [LN42_MSTART[r][c], LN42_MSTART[r][c+1]) of
- *               LN42_TSIGN[t] * ((LN42_TNUM[t] / LN42_TDEN[t]) * b[LN42_TB[t]]),
+ * Narrow-stencil SBP second derivative D2(b) with a variable coefficient, as the reference's
+ * laplace_Ah_sbp{order}_narrow_t (differential_operators/horizontal/laplace/
+ * laplace_Ah_sbp{order}_narrow_mod.f90, calc_laplace_tile).  Closure row r = 0..{nb - 1} (r-th point
+ * from a panel edge, mirrored at the far edge), with h = {P}H[r]:
+ *     d2[r] = {rowval}   ({P}ROW_NEG_INV_H = {1 if scale == "neg_inv_h" else 0})
+ *     m[r][c] = sum over t in [{P}MSTART[r][c], {P}MSTART[r][c+1]) of
+ *               {P}TSIGN[t] * ({P}TCOEF[t] * b[{P}TB[t]]),
  * summed left to right (the first term's sign applied by negation).  b[k], f[c]: the k-th /
- * c-th point from the edge.  Interior row i: d2 = c(-2) f[i-2] + ... + c(2) f[i+2] with c(o)
- * the terms [LN42_ISTART[o+2], LN42_ISTART[o+3]) over b[i + LN42_IB[t]].  The boundary
- * penalty of the cross terms uses LN42_PNUM / LN42_PDEN.
+ * c-th point from the edge.  Interior row i: d2 = c(-{R}) f[i-{R}] + ... + c({R}) f[i+{R}] with c(o)
+ * the terms [{P}ISTART[o+{R}], {P}ISTART[o+{R + 1}]) over b[i + {P}IB[t]].  The boundary penalty of
+ * the cross terms uses {P}PC.  Every coefficient is a constant expression in the reference's
+ * operation order.
  */
-#ifndef LAPLACE_SBP42_NARROW_H
-#define LAPLACE_SBP42_NARROW_H
+#ifndef LAPLACE_SBP{order}_NARROW_H
+#define LAPLACE_SBP{order}_NARROW_H
 #ifdef __cplusplus
-#define LN42_CONST static constexpr
+#define {P}CONST static constexpr
 #else
-#define LN42_CONST static const
+#define {P}CONST static const
 #endif
-#define LN42_NB 6
-#define LN42_NCOL 8
-LN42_CONST double LN42_HNUM[6] = {{{", ".join(hn)}}};
-LN42_CONST double LN42_HDEN[6] = {{{", ".join(hd)}}};
-LN42_CONST int LN42_NCOLS[6] = {{{", ".join(map(str, ncols))}}};
-LN42_CONST int LN42_MSTART[6][9] = {{
+#define {P}NB {nb}
+#define {P}NCOL {ncol}
+#define {P}R {R}
+#define {P}ROW_NEG_INV_H {1 if scale == "neg_inv_h" else 0}
+{P}CONST double {P}H[{nb}] = {{{", ".join(hs)}}};
+{P}CONST int {P}NCOLS[{nb}] = {{{", ".join(map(str, ncols))}}};
+{P}CONST int {P}MSTART[{nb}][{ncol + 1}] = {{
 {chr(10).join("    {" + ", ".join(map(str, s)) + "}," for s in start)}
 }};
-LN42_CONST double LN42_TNUM[{len(tn)}] = {{
-{arr(tn, 4)}
+{P}CONST double {P}TCOEF[{len(tc)}] = {{
+{arr(tc, 1)}
 }};
-LN42_CONST double LN42_TDEN[{len(tq)}] = {{
-{arr(tq, 4)}
-}};
-LN42_CONST int LN42_TSIGN[{len(ts)}] = {{
+{P}CONST int {P}TSIGN[{len(ts)}] = {{
 {arr(ts, 16)}
 }};
-LN42_CONST int LN42_TB[{len(tb)}] = {{
+{P}CONST int {P}TB[{len(tb)}] = {{
 {arr(tb, 16)}
 }};
-LN42_CONST int LN42_ISTART[6] = {{{", ".join(map(str, it_start))}}};
-LN42_CONST double LN42_INUM[{len(it_n)}] = {{{", ".join(it_n)}}};
-LN42_CONST double LN42_IDEN[{len(it_q)}] = {{{", ".join(it_q)}}};
-LN42_CONST int LN42_ISIGN[{len(it_s)}] = {{{", ".join(it_s)}}};
-LN42_CONST int LN42_IB[{len(it_b)}] = {{{", ".join(it_b)}}};
-LN42_CONST double LN42_PNUM = {lit(pnum)};
-LN42_CONST double LN42_PDEN = {lit(pden)};
+{P}CONST int {P}ISTART[{2 * R + 2}] = {{{", ".join(map(str, istart))}}};
+{P}CONST double {P}ICOEF[{len(ic)}] = {{
+{arr(ic, 1)}
+}};
+{P}CONST int {P}ISIGN[{len(isg)}] = {{{", ".join(isg)}}};
+{P}CONST int {P}IB[{len(ib)}] = {{{", ".join(ib)}}};
+{P}CONST double {P}PC = {fortran_to_c(f"{pnum}/{pden}")};
 #endif
 """)
-    print(f"wrote {OUT}: {len(tn)} closure terms, {len(it_n)} interior terms")
+    print(f"wrote {out}: {len(tc)} closure terms, {len(ic)} interior terms, scale {scale}")
 
 
 if __name__ == "__main__":
-    main()
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    for order in (42, 63):
+        generate(order, ref)

@@ -1,11 +1,13 @@
-// Hyperdiffusion of the Ah state with the narrow-stencil SBP Laplacian (sbp42), FP64
+// Hyperdiffusion of the Ah state with the narrow-stencil SBP Laplacians sbp42 and sbp63, FP64
 // (included into swm_capi.hip).
 //
 // Laplacian, laplace_Ah_sbp42_narrow_t%calc_laplace (differential_operators/horizontal/
-// laplace/laplace_Ah_sbp42_narrow_mod.f90:23-763):
-//   d1x = Dx(f) J Qi12,  d1y = Dy(f) J Qi12                      (first-derivative SBP d42)
+// laplace/laplace_Ah_sbp42_narrow_mod.f90:23-763) and laplace_Ah_sbp63_narrow_t%calc_laplace
+// (laplace_Ah_sbp63_narrow_mod.f90:494-1717), the same steps with their own tables:
+//   d1x = Dx(f) J Qi12,  d1y = Dy(f) J Qi12                      (first-derivative SBP d42 / d63)
 //   d2x = D2x(Qi11 J) f, d2y = D2y(Qi22 J) f                     (narrow variable-coefficient D2)
-//   d2x -= 0.5 * (48/17) * (-+2 d1y) on the x edges, then (-+2 d1x) on the y edges
+//   d2x -= 0.5 * pc * (-+2 d1y) on the x edges, then (-+2 d1x) on the y edges
+//        (pc = 48/17 for sbp42, 43200/13649 for sbp63)
 //   L f = ((d2x + d2y) + Dx(d1y) + Dy(d1x)) / (hx a)^2 / J, then the scalar edge sync.
 // Hyperdiffusion of one field, hordiff_scalar_t%calc_diff (hordiff/hordiff_scalar_mod.f90:
 // 34-59): tend = -(c a hx)^4 L(L f).  Velocity, hordiff_colocated_xyz_t%calc_diff_vec
@@ -20,8 +22,42 @@
 // a value another block of the same launch overwrites.
 
 #include "laplace_sbp42_narrow.h"
+#include "laplace_sbp63_narrow.h"
 
 namespace swm {
+
+// The generated coefficient tables of one narrow Laplacian family, bound by name.
+template <int ORD> struct Ln;
+template <> struct Ln<42> {
+    static constexpr int NB = LN42_NB, NCOL = LN42_NCOL, R = LN42_R;
+    static constexpr bool NEG_INV_H = LN42_ROW_NEG_INV_H != 0;
+    static constexpr auto &H = LN42_H;
+    static constexpr auto &NCOLS = LN42_NCOLS;
+    static constexpr auto &MSTART = LN42_MSTART;
+    static constexpr auto &TCOEF = LN42_TCOEF;
+    static constexpr auto &TSIGN = LN42_TSIGN;
+    static constexpr auto &TB = LN42_TB;
+    static constexpr auto &ISTART = LN42_ISTART;
+    static constexpr auto &ICOEF = LN42_ICOEF;
+    static constexpr auto &ISIGN = LN42_ISIGN;
+    static constexpr auto &IB = LN42_IB;
+    static constexpr double PC = LN42_PC;
+};
+template <> struct Ln<63> {
+    static constexpr int NB = LN63_NB, NCOL = LN63_NCOL, R = LN63_R;
+    static constexpr bool NEG_INV_H = LN63_ROW_NEG_INV_H != 0;
+    static constexpr auto &H = LN63_H;
+    static constexpr auto &NCOLS = LN63_NCOLS;
+    static constexpr auto &MSTART = LN63_MSTART;
+    static constexpr auto &TCOEF = LN63_TCOEF;
+    static constexpr auto &TSIGN = LN63_TSIGN;
+    static constexpr auto &TB = LN63_TB;
+    static constexpr auto &ISTART = LN63_ISTART;
+    static constexpr auto &ICOEF = LN63_ICOEF;
+    static constexpr auto &ISIGN = LN63_ISIGN;
+    static constexpr auto &IB = LN63_IB;
+    static constexpr double PC = LN63_PC;
+};
 
 struct HdGeo {
     const double *J, *Qi11, *Qi12, *Qi22;
@@ -49,64 +85,80 @@ __device__ __forceinline__ int nbr_slot(const HdGeo &g, int p, int e, int k) {
     return ei[0] * g.es + edge_slot(n, qi, qj);
 }
 
-// Closure row R of the narrow D2 with b[k], fv[c] taken k, c points from the panel edge, and
-// the interior row, expanded at compile time (every index and weight a constant); each weight
-// p/q is the correctly rounded quotient, as the run-time division of the oracle.
-template <int T>
-__device__ __forceinline__ double ln42_term(const double *b) {
-    constexpr double w = LN42_TNUM[T] / LN42_TDEN[T];
-    return w * b[LN42_TB[T]];
+// Closure row RW of the narrow D2 with b[k], fv[c] taken k, c points from the panel edge, and
+// the interior row, expanded at compile time (every index and weight a constant): each weight
+// is the table's constant expression folded to a double, products and sums in table order.
+template <int ORD, int T>
+__device__ __forceinline__ double ln_term(const double *b) {
+    constexpr double w = Ln<ORD>::TCOEF[T];
+    return w * b[Ln<ORD>::TB[T]];
 }
-template <int T0, int... Is>
-__device__ __forceinline__ double ln42_sum(const double *b, std::integer_sequence<int, Is...>) {
+template <int ORD, int T0, int... Is>
+__device__ __forceinline__ double ln_sum(const double *b, std::integer_sequence<int, Is...>) {
+    using L = Ln<ORD>;
     double m = 0.0;
-    ((m = Is == 0 ? (LN42_TSIGN[T0 + Is] < 0 ? -ln42_term<T0 + Is>(b) : ln42_term<T0 + Is>(b))
-                  : (LN42_TSIGN[T0 + Is] < 0 ? m - ln42_term<T0 + Is>(b) : m + ln42_term<T0 + Is>(b))),
+    ((m = Is == 0 ? (L::TSIGN[T0 + Is] < 0 ? -ln_term<ORD, T0 + Is>(b) : ln_term<ORD, T0 + Is>(b))
+                  : (L::TSIGN[T0 + Is] < 0 ? m - ln_term<ORD, T0 + Is>(b) : m + ln_term<ORD, T0 + Is>(b))),
      ...);
     return m;
 }
-template <int R, int C>
-__device__ __forceinline__ double ln42_m(const double *b) {
-    constexpr int t0 = LN42_MSTART[R][C], cnt = LN42_MSTART[R][C + 1] - LN42_MSTART[R][C];
-    return ln42_sum<t0>(b, std::make_integer_sequence<int, cnt>{});
+template <int ORD, int RW, int C>
+__device__ __forceinline__ double ln_m(const double *b) {
+    using L = Ln<ORD>;
+    constexpr int t0 = L::MSTART[RW][C], cnt = L::MSTART[RW][C + 1] - L::MSTART[RW][C];
+    return ln_sum<ORD, t0>(b, std::make_integer_sequence<int, cnt>{});
 }
-template <int R, int... Cs>
-__device__ __forceinline__ double ln42_row_cols(const double *b, const double *fv, std::integer_sequence<int, Cs...>) {
+template <int ORD, int RW, int... Cs>
+__device__ __forceinline__ double ln_row_cols(const double *b, const double *fv, std::integer_sequence<int, Cs...>) {
     double s = 0.0;
-    ((s = Cs == 0 ? ln42_m<R, Cs>(b) * fv[Cs] : s + ln42_m<R, Cs>(b) * fv[Cs]), ...);
-    constexpr double h = LN42_HNUM[R] / LN42_HDEN[R];
-    return -((1.0 / h) * s);
+    ((s = Cs == 0 ? ln_m<ORD, RW, Cs>(b) * fv[Cs] : s + ln_m<ORD, RW, Cs>(b) * fv[Cs]), ...);
+    if constexpr (Ln<ORD>::NEG_INV_H) {
+        constexpr double h = Ln<ORD>::H[RW];
+        return -((1.0 / h) * s);   // -1/h_rr * (...)
+    } else {
+        return s;                  // 1.0 * (...), exact
+    }
 }
-template <int R>
-__device__ __forceinline__ double ln42_row(const double *b, const double *fv) {
-    return ln42_row_cols<R>(b, fv, std::make_integer_sequence<int, LN42_NCOLS[R]>{});
+template <int ORD, int RW>
+__device__ __forceinline__ double ln_row(const double *b, const double *fv) {
+    return ln_row_cols<ORD, RW>(b, fv, std::make_integer_sequence<int, Ln<ORD>::NCOLS[RW]>{});
+}
+template <int ORD, int... RWs>
+__device__ __forceinline__ double ln_row_any(int r, const double *b, const double *fv,
+                                             std::integer_sequence<int, RWs...>) {
+    double out = 0.0;
+    ((r == RWs ? (void)(out = ln_row<ORD, RWs>(b, fv)) : (void)0), ...);
+    return out;
 }
 
-template <int T>
-__device__ __forceinline__ double ln42_iterm(const double *b) {
-    constexpr double w = LN42_INUM[T] / LN42_IDEN[T];
-    return w * b[LN42_IB[T] + 2];
+template <int ORD, int T>
+__device__ __forceinline__ double ln_iterm(const double *b) {
+    constexpr double w = Ln<ORD>::ICOEF[T];
+    return w * b[Ln<ORD>::IB[T] + Ln<ORD>::R];
 }
-template <int T0, int... Is>
-__device__ __forceinline__ double ln42_isum(const double *b, std::integer_sequence<int, Is...>) {
+template <int ORD, int T0, int... Is>
+__device__ __forceinline__ double ln_isum(const double *b, std::integer_sequence<int, Is...>) {
+    using L = Ln<ORD>;
     double m = 0.0;
-    ((m = Is == 0 ? (LN42_ISIGN[T0 + Is] < 0 ? -ln42_iterm<T0 + Is>(b) : ln42_iterm<T0 + Is>(b))
-                  : (LN42_ISIGN[T0 + Is] < 0 ? m - ln42_iterm<T0 + Is>(b) : m + ln42_iterm<T0 + Is>(b))),
+    ((m = Is == 0 ? (L::ISIGN[T0 + Is] < 0 ? -ln_iterm<ORD, T0 + Is>(b) : ln_iterm<ORD, T0 + Is>(b))
+                  : (L::ISIGN[T0 + Is] < 0 ? m - ln_iterm<ORD, T0 + Is>(b) : m + ln_iterm<ORD, T0 + Is>(b))),
      ...);
     return m;
 }
-template <int O>
-__device__ __forceinline__ double ln42_icoef(const double *b) {
-    return ln42_isum<LN42_ISTART[O]>(b, std::make_integer_sequence<int, LN42_ISTART[O + 1] - LN42_ISTART[O]>{});
+template <int ORD, int O>
+__device__ __forceinline__ double ln_icoef(const double *b) {
+    using L = Ln<ORD>;
+    return ln_isum<ORD, L::ISTART[O]>(b, std::make_integer_sequence<int, L::ISTART[O + 1] - L::ISTART[O]>{});
 }
-template <int... Os>
-__device__ __forceinline__ double ln42_interior_seq(const double *b, const double *fv, std::integer_sequence<int, Os...>) {
+template <int ORD, int... Os>
+__device__ __forceinline__ double ln_interior_seq(const double *b, const double *fv, std::integer_sequence<int, Os...>) {
     double s = 0.0;
-    ((s = Os == 0 ? ln42_icoef<Os>(b) * fv[Os] : s + ln42_icoef<Os>(b) * fv[Os]), ...);
+    ((s = Os == 0 ? ln_icoef<ORD, Os>(b) * fv[Os] : s + ln_icoef<ORD, Os>(b) * fv[Os]), ...);
     return s;
 }
-__device__ __forceinline__ double ln42_interior(const double *b, const double *fv) {
-    return ln42_interior_seq(b, fv, std::make_integer_sequence<int, 5>{});
+template <int ORD>
+__device__ __forceinline__ double ln_interior(const double *b, const double *fv) {
+    return ln_interior_seq<ORD>(b, fv, std::make_integer_sequence<int, 2 * Ln<ORD>::R + 1>{});
 }
 
 // V = b1 u + b2 v (vec_math_mod.f90:107-126); W[0] = h, W[1..3] = Vx, Vy, Vz
@@ -126,23 +178,33 @@ __global__ void hd_to_cart(HdGeo g, const double *__restrict__ y, double *__rest
 
 // The Laplacian of field k = blockIdx.y on LDS tiles of TX x TY points, before the edge sync:
 // off-edge points to Wout, panel-edge points to E (ns slots per field).  A tile loads f on the
-// box of lines its rows read (+-2 points in the interior, points 0..7 / n-8..n-1 where it holds
-// closure rows: the narrow D2 closure reads 8 points, the first-derivative closure 6), and
-// b = Qi11 J on the tile rows, b = Qi22 J on the tile columns.  It then forms the first
-// derivatives d1y = Dy(f) J Qi12 on the tile rows and d1x = Dx(f) J Qi12 on the tile columns
-// (each at the positions whose J and Qi12 the same thread loaded), and finally the D2 rows,
-// the cross-term penalties and the cross derivatives at its points.
-template <int TX, int TY>
+// box of lines its rows read (+-R points in the interior, points 0..NR-1 / n-NR..n-1 where it
+// holds closure rows: the narrow D2 closure reads NR = 8 (sbp42) / 12 (sbp63) points, the
+// first-derivative closure 6 / 9), and b = Qi11 J on the tile rows, b = Qi22 J on the tile
+// columns.  It then forms the first derivatives d1y = Dy(f) J Qi12 on the tile rows and
+// d1x = Dx(f) J Qi12 on the tile columns (each at the positions whose J and Qi12 the same
+// thread loaded), and finally the D2 rows, the cross-term penalties and the cross derivatives at
+// its points.
+template <int ORD, int TX, int TY>
 struct HdTile {
-    static constexpr int R = 2, B = 6, NR = 8;
-    static constexpr int span(int t) { return t + R + B - 1 > NR ? t + R + B - 1 : NR; }
+    static constexpr int R = Ln<ORD>::R, B = Ln<ORD>::NB, NR = Ln<ORD>::NCOL;
+    // the first-derivative operator of the family reads within the same box
+    static_assert(-Sbp<ORD>::INSHIFT <= R && Sbp<ORD>::WIN + Sbp<ORD>::INSHIFT - 1 <= R, "d1 reach");
+    static_assert(Sbp<ORD>::NE <= B && Sbp<ORD>::NW <= NR, "d1 closure");
+    // box extent along a tile side of t points: closure rows at one edge give at most
+    // t + R + B - 1 lines (or NR); a tile holding closure rows of both edges (a0 < B, a0 a
+    // multiple of t, and a0 + t > n - B) covers the whole line, n <= a0 + t + B - 1
+    static constexpr int both(int t) { return ((B - 1) / t) * t + t + B - 1; }
+    static constexpr int one(int t) { return t + R + B - 1 > NR ? t + R + B - 1 : NR; }
+    static constexpr int span(int t) { return one(t) > both(t) ? one(t) : both(t); }
     static constexpr int LW = span(TX), LH = span(TY), NT = TX * TY;
     static constexpr int KF = (LH * LW + NT - 1) / NT;              // box positions per thread
     static constexpr int KX = (TY * LW + LH * TX + NT - 1) / NT;    // cross positions per thread
 };
 
+template <int ORD>
 __device__ __forceinline__ void hd_rows(int a0, int ae, int n, int &lo, int &hi) {
-    constexpr int R = 2, B = 6, NR = 8;
+    constexpr int R = Ln<ORD>::R, B = Ln<ORD>::NB, NR = Ln<ORD>::NCOL;
     lo = a0 - R > 0 ? a0 - R : 0;
     hi = ae + R < n ? ae + R : n;
     if (a0 < B) {
@@ -157,10 +219,11 @@ __device__ __forceinline__ void hd_rows(int a0, int ae, int n, int &lo, int &hi)
     }
 }
 
-template <int TX, int TY>
+template <int ORD, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__restrict__ W,
                                                        double *__restrict__ Wout, double *__restrict__ E) {
-    using C = HdTile<TX, TY>;
+    using C = HdTile<ORD, TX, TY>;
+    using L = Ln<ORD>;
     __shared__ double sf[C::LH][C::LW];                  // f on the box
     __shared__ double sbx[TY][C::LW], sdy[TY][C::LW];    // tile rows: Qi11 J, d1y
     __shared__ double sby[C::LH][TX], sdx[C::LH][TX];    // tile columns: Qi22 J, d1x
@@ -171,8 +234,8 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
     const int x0 = (rem % ntx) * TX, y0 = (rem / ntx) * TY;
     const int xe = x0 + TX < n ? x0 + TX : n, ye = y0 + TY < n ? y0 + TY : n;
     int lx, hx, ly, hy;
-    hd_rows(x0, xe, n, lx, hx);
-    hd_rows(y0, ye, n, ly, hy);
+    hd_rows<ORD>(x0, xe, n, lx, hx);
+    hd_rows<ORD>(y0, ye, n, ly, hy);
     const double *f = W + k * g.fs;
     const long long pb = (long long)p * n;
     const int wa = hx - lx, nbox = (hy - ly) * wa;
@@ -231,16 +294,17 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
     }
     __syncthreads();
 
-    // ---- first derivatives at the cross positions (calc_tile_d1x_d1y, :60-93) ----
+    // ---- first derivatives at the cross positions (calc_tile_d1x_d1y: sbp42 :60-93,
+    // sbp63 :531-564) ----
 #pragma unroll
     for (int u = 0; u < C::KX; ++u) {
         if (kind[u] == 1) {
             const int c = px[u] - lx;
-            const double d = sbp_row<42, double>(py[u], n, [&](int a) { return sf[a - ly][c]; });
+            const double d = sbp_row<ORD, double>(py[u], n, [&](int a) { return sf[a - ly][c]; });
             sdy[py[u] - y0][c] = d * jv[u] * q12[u];
         } else if (kind[u] == 2) {
             const int r = py[u] - ly;
-            const double d = sbp_row<42, double>(px[u], n, [&](int a) { return sf[r][a - lx]; });
+            const double d = sbp_row<ORD, double>(px[u], n, [&](int a) { return sf[r][a - lx]; });
             sdx[r][px[u] - x0] = d * jv[u] * q12[u];
         }
     }
@@ -250,41 +314,34 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
     if (i >= n || j >= n) return;
     const long long o = (pb + j) * g.pitch + i;
     auto d2 = [&](int t, const auto &bget, const auto &fget) {
-        double bb[LN42_NCOL], ff[LN42_NCOL];
-        if (t < LN42_NB || t >= n - LN42_NB) {
-            const bool left = t < LN42_NB;
+        double bb[L::NCOL], ff[L::NCOL];
+        if (t < L::NB || t >= n - L::NB) {
+            const bool left = t < L::NB;
 #pragma unroll
-            for (int q = 0; q < LN42_NCOL; ++q) {
+            for (int q = 0; q < L::NCOL; ++q) {
                 const int a = left ? q : n - 1 - q;
                 bb[q] = bget(a);
                 ff[q] = fget(a);
             }
-            switch (left ? t : n - 1 - t) {
-            case 0: return ln42_row<0>(bb, ff);
-            case 1: return ln42_row<1>(bb, ff);
-            case 2: return ln42_row<2>(bb, ff);
-            case 3: return ln42_row<3>(bb, ff);
-            case 4: return ln42_row<4>(bb, ff);
-            default: return ln42_row<5>(bb, ff);
-            }
+            return ln_row_any<ORD>(left ? t : n - 1 - t, bb, ff, std::make_integer_sequence<int, L::NB>{});
         }
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            bb[q] = bget(t - 2 + q);
-            ff[q] = fget(t - 2 + q);
+        for (int q = 0; q < 2 * L::R + 1; ++q) {
+            bb[q] = bget(t - L::R + q);
+            ff[q] = fget(t - L::R + q);
         }
-        return ln42_interior(bb, ff);
+        return ln_interior<ORD>(bb, ff);
     };
     double d2x = d2(i, [&](int a) { return sbx[ty][a - lx]; }, [&](int a) { return sf[j - ly][a - lx]; });
     const double d2y = d2(j, [&](int a) { return sby[a - ly][tx]; }, [&](int a) { return sf[a - ly][i - lx]; });
-    // penalties of the cross terms (laplace_Ah_sbp42_narrow_mod.f90:716-742), x edges first
-    const double pc = LN42_PNUM / LN42_PDEN;
+    // penalties of the cross terms (sbp42 :716-742, sbp63 :1671-1697), x edges first
+    constexpr double pc = L::PC;
     if (i == 0) d2x = d2x - 0.5 * (pc * (-2.0 * sdy[ty][i - lx]));
     if (i == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * sdy[ty][i - lx]));
     if (j == 0) d2x = d2x - 0.5 * (pc * (-2.0 * sdx[j - ly][tx]));
     if (j == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * sdx[j - ly][tx]));
-    const double d2yx = sbp_row<42, double>(j, n, [&](int a) { return sdx[a - ly][tx]; });
-    const double d2xy = sbp_row<42, double>(i, n, [&](int a) { return sdy[ty][a - lx]; });
+    const double d2yx = sbp_row<ORD, double>(j, n, [&](int a) { return sdx[a - ly][tx]; });
+    const double d2xy = sbp_row<ORD, double>(i, n, [&](int a) { return sdy[ty][a - lx]; });
     double v = 1.0 * d2x + 1.0 * d2y;
     v = v + 1.0 * d2xy + 1.0 * d2yx;
     v = v / (g.hs * g.hs);
@@ -408,12 +465,15 @@ struct HdWork {
     double *E = nullptr, *E2 = nullptr;           // 4 x ns, 2 x ns edge values
 };
 
-constexpr int HD_MIN_N = 2 * LN42_NB;  // closure rows of both edges disjoint
+// closure rows of both panel edges disjoint: N+1 >= 12 (sbp42), 18 (sbp63)
+int hd_min_n(int order) { return 2 * (order == 63 ? swm::Ln<63>::NB : swm::Ln<42>::NB); }
 
-int ensure_hd(swm_ctx *c) {
-    if (c->hd) return SWM_OK;
+int ensure_hd(swm_ctx *c, int order) {
     if (c->prec != 64) return fail(SWM_EARG, "the hyperdiffusion runs in FP64 contexts only");
-    if (c->n < HD_MIN_N) return fail(SWM_EARG, "the sbp42 narrow Laplacian needs N+1 >= " + std::to_string(HD_MIN_N));
+    if (c->n < hd_min_n(order))
+        return fail(SWM_EARG, "the sbp" + std::to_string(order) + " narrow Laplacian needs N+1 >= " +
+                                  std::to_string(hd_min_n(order)));
+    if (c->hd) return SWM_OK;
     HdWork *w = new HdWork();
     c->hd = w;
     const size_t fb = c->field_bytes(), ns = (size_t)24 * c->N;
@@ -445,33 +505,61 @@ swm::HdGeo hd_geo(const swm_ctx *c) {
     return g;
 }
 
-// nf fields of `in` (4-field work layout) -> L of each in `out`, synced
-int enqueue_laplace(swm_ctx *c, int nf, const double *in, double *out) {
+// fields k0 .. k0+nf-1 of `in` (4-field work layout) -> L of each in the same fields of `out`,
+// synced; order 42 or 63 picks the narrow Laplacian
+int enqueue_laplace(swm_ctx *c, int order, int k0, int nf, const double *in, double *out) {
     HdWork *w = c->hd;
     const swm::HdGeo g = hd_geo(c);
     const dim3 blk(256);
     const int ns = 24 * c->N;
     constexpr int LTX = 32, LTY = 8;
     const int ltiles = 6 * ((c->n + LTX - 1) / LTX) * ((c->n + LTY - 1) / LTY);
-    hipLaunchKernelGGL((swm::hd_lap_tile<LTX, LTY>), dim3(ltiles, nf), dim3(LTX * LTY), 0, c->stream, g, in, out,
-                       w->E);
-    hipLaunchKernelGGL(swm::hd_sync, dim3((ns + 255) / 256, nf), blk, 0, c->stream, g, (const double *)w->E, out);
+    const double *ik = in + (long long)k0 * c->fs;
+    double *ok = out + (long long)k0 * c->fs, *ek = w->E + (long long)k0 * ns;
+    if (order == 63)
+        hipLaunchKernelGGL((swm::hd_lap_tile<63, LTX, LTY>), dim3(ltiles, nf), dim3(LTX * LTY), 0, c->stream, g, ik,
+                           ok, ek);
+    else
+        hipLaunchKernelGGL((swm::hd_lap_tile<42, LTX, LTY>), dim3(ltiles, nf), dim3(LTX * LTY), 0, c->stream, g, ik,
+                           ok, ek);
+    hipLaunchKernelGGL(swm::hd_sync, dim3((ns + 255) / 256, nf), blk, 0, c->stream, g, (const double *)ek, ok);
     HIP_TRY(hipGetLastError());
     return SWM_OK;
 }
 
-int parse_hordiff(const char *uv_name, const char *h_name) {
+// hordiff_factory_mod.f90:25-50: the narrow-Laplacian hyperdiffusion names and their families
+int hordiff_order(const char *name, const char *prefix, int &order) {
+    const std::string s(name);
+    for (int o : {42, 63})
+        if (s == prefix + std::to_string(o) + "_narrow") {
+            order = o;
+            return SWM_OK;
+        }
+    return fail(SWM_ENAME, std::string("unsupported hyperdiffusion: ") + name +
+                               " (supported: " + prefix + "42_narrow, " + prefix + "63_narrow)");
+}
+
+int parse_hordiff(const char *uv_name, const char *h_name, int &uv_order, int &h_order) {
     if (!uv_name || !h_name) return fail(SWM_EARG, "NULL hyperdiffusion name");
-    if (std::string(uv_name) != "hordiff_vec_xyz_Ah_sbp_42_narrow")
-        return fail(SWM_ENAME, std::string("unsupported velocity hyperdiffusion: ") + uv_name);
-    if (std::string(h_name) != "hordiff_scalar_Ah_sbp_42_narrow")
-        return fail(SWM_ENAME, std::string("unsupported height hyperdiffusion: ") + h_name);
+    int rc = hordiff_order(uv_name, "hordiff_vec_xyz_Ah_sbp_", uv_order);
+    if (rc) return rc;
+    return hordiff_order(h_name, "hordiff_scalar_Ah_sbp_", h_order);
+}
+
+// laplace_factory_mod.f90:20-30: the narrow-Laplacian names
+int parse_laplace(const char *name, int &order) {
+    if (!name) return fail(SWM_EARG, "NULL Laplacian name");
+    const std::string s(name);
+    if (s == "laplace_ah_sbp42_narrow") order = 42;
+    else if (s == "laplace_ah_sbp63_narrow") order = 63;
+    else return fail(SWM_ENAME, "unsupported Laplacian: " + s +
+                                    " (supported: laplace_ah_sbp42_narrow, laplace_ah_sbp63_narrow)");
     return SWM_OK;
 }
 
 // yo = y0 + dt * diff(yin) (y0 = yin = yo for the Euler step), or yo = diff(yin) with y0 = NULL
-int enqueue_diff(swm_ctx *c, const double *yin, const double *y0, double *yo, double dt, double uv_coeff,
-                 double h_coeff) {
+int enqueue_diff(swm_ctx *c, int uv_order, int h_order, const double *yin, const double *y0, double *yo,
+                 double dt, double uv_coeff, double h_coeff) {
     HdWork *w = c->hd;
     const swm::HdGeo g = hd_geo(c);
     const long long total = 6LL * c->n * c->n;
@@ -479,7 +567,17 @@ int enqueue_diff(swm_ctx *c, const double *yin, const double *y0, double *yo, do
     const int ns = 24 * c->N;
     int rc;
     hipLaunchKernelGGL(swm::hd_to_cart, grd, blk, 0, c->stream, g, yin, w->WA);
-    if ((rc = enqueue_laplace(c, 4, w->WA, w->WB)) || (rc = enqueue_laplace(c, 4, w->WB, w->WA))) return rc;
+    // h and the three Cartesian velocity components share the launches of one family
+    for (int pass = 0; pass < 2; ++pass) {
+        const double *src = pass ? w->WB : w->WA;
+        double *dst = pass ? w->WA : w->WB;
+        if (uv_order == h_order) {
+            if ((rc = enqueue_laplace(c, h_order, 0, 4, src, dst))) return rc;
+        } else if ((rc = enqueue_laplace(c, h_order, 0, 1, src, dst)) ||
+                   (rc = enqueue_laplace(c, uv_order, 1, 3, src, dst))) {
+            return rc;
+        }
+    }
     // hordiff_factory_mod.f90:158-188: diff_coeff = hordiff_coeff*scale*hx, tend coefficient
     // (-1)**3 * diff_coeff**4 (the power as (c*c)*(c*c))
     const double du = uv_coeff * c->scale * c->hx, dh = h_coeff * c->scale * c->hx;
@@ -500,13 +598,14 @@ void free_hd_work(HdWork *w) {
 
 }  // namespace
 
-extern "C" int swm_laplace(swm_ctx *c, const void *f, void *out) {
+extern "C" int swm_laplace(swm_ctx *c, const char *laplace_name, const void *f, void *out) {
     if (!c || !f || !out) return fail(SWM_EARG, "NULL argument");
     if (f == out) return fail(SWM_EARG, "out must differ from f");
-    int rc = ensure_hd(c);
-    if (rc) return rc;
+    int order = 0;
+    int rc = parse_laplace(laplace_name, order);
+    if (rc || (rc = ensure_hd(c, order))) return rc;
     HIP_TRY(hipMemcpyAsync(c->hd->WA, f, c->field_bytes(), hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = enqueue_laplace(c, 1, c->hd->WA, c->hd->WB))) return rc;
+    if ((rc = enqueue_laplace(c, order, 0, 1, c->hd->WA, c->hd->WB))) return rc;
     HIP_TRY(hipMemcpyAsync(out, c->hd->WB, c->field_bytes(), hipMemcpyDeviceToDevice, c->stream));
     return SWM_OK;
 }
@@ -514,17 +613,20 @@ extern "C" int swm_laplace(swm_ctx *c, const void *f, void *out) {
 extern "C" int swm_diff_step(swm_ctx *c, void *state, double dt, const char *hordiff_uv_name, double uv_diff_coeff,
                              const char *hordiff_h_name, double h_diff_coeff) {
     if (!c || !state) return fail(SWM_EARG, "NULL argument");
-    int rc = parse_hordiff(hordiff_uv_name, hordiff_h_name);
-    if (rc || (rc = ensure_hd(c))) return rc;
+    int uv_order = 0, h_order = 0;
+    int rc = parse_hordiff(hordiff_uv_name, hordiff_h_name, uv_order, h_order);
+    if (rc || (rc = ensure_hd(c, uv_order)) || (rc = ensure_hd(c, h_order))) return rc;
     double *y = (double *)state;
-    return enqueue_diff(c, y, y, y, dt, uv_diff_coeff, h_diff_coeff);
+    return enqueue_diff(c, uv_order, h_order, y, y, y, dt, uv_diff_coeff, h_diff_coeff);
 }
 
 extern "C" int swm_diff_apply(swm_ctx *c, const void *vin, void *vout, const char *hordiff_uv_name,
                               double uv_diff_coeff, const char *hordiff_h_name, double h_diff_coeff) {
     if (!c || !vin || !vout) return fail(SWM_EARG, "NULL argument");
     if (vin == vout) return fail(SWM_EARG, "vout must differ from vin");
-    int rc = parse_hordiff(hordiff_uv_name, hordiff_h_name);
-    if (rc || (rc = ensure_hd(c))) return rc;
-    return enqueue_diff(c, (const double *)vin, nullptr, (double *)vout, 0.0, uv_diff_coeff, h_diff_coeff);
+    int uv_order = 0, h_order = 0;
+    int rc = parse_hordiff(hordiff_uv_name, hordiff_h_name, uv_order, h_order);
+    if (rc || (rc = ensure_hd(c, uv_order)) || (rc = ensure_hd(c, h_order))) return rc;
+    return enqueue_diff(c, uv_order, h_order, (const double *)vin, nullptr, (double *)vout, 0.0, uv_diff_coeff,
+                        h_diff_coeff);
 }

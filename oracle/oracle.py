@@ -60,11 +60,14 @@ def lib():
         L.ora_maxabs.restype = ctypes.c_double
         L.ora_maxabs.argtypes = [ctypes.c_void_p, _dp]
         L.ora_project_vector.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
-        L.ora_laplace42.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.ora_hordiff_scalar42.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp]
-        L.ora_hordiff_vec42.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, _dp]
-        L.ora_diff_step42.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
-                                      ctypes.c_double]
+        L.ora_narrow_min_n.restype = ctypes.c_int
+        L.ora_narrow_min_n.argtypes = [ctypes.c_int]
+        L.ora_laplace_narrow.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.ora_hordiff_scalar_narrow.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.ora_hordiff_vec_narrow.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp,
+                                             _dp]
+        L.ora_diff_step_narrow.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.ora_rk4_iomega.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double]
         _lib = L
     return _lib
@@ -197,24 +200,34 @@ class Grid:
     def rk4(self, h, u, v, dt, nsteps, hs=None):
         lib().ora_rk4(self.h, _p(h), _p(u), _p(v), _pn(hs), dt, nsteps)
 
-    # narrow-stencil SBP Laplacian (sbp42) and hyperdiffusion; N+1 >= 12
-    def laplace42(self, f):
+    # narrow-stencil SBP Laplacians (order 42 or 63) and hyperdiffusion; N+1 >= 12 (42), 18 (63)
+    def _check_narrow(self, order):
+        if order not in (42, 63) or self.n < lib().ora_narrow_min_n(order):
+            raise ValueError(f"narrow Laplacian of order {order} needs order 42/63 and N+1 large enough")
+
+    def laplace(self, f, order=42):
+        self._check_narrow(order)
         out = self.zeros()
-        lib().ora_laplace42(self.h, _p(f), _p(out))
+        lib().ora_laplace_narrow(self.h, order, _p(f), _p(out))
         return out
 
-    def hordiff_scalar42(self, coeff, f):
+    def hordiff_scalar(self, coeff, f, order=42):
+        self._check_narrow(order)
         out = self.zeros()
-        lib().ora_hordiff_scalar42(self.h, float(coeff), _p(f), _p(out))
+        lib().ora_hordiff_scalar_narrow(self.h, order, float(coeff), _p(f), _p(out))
         return out
 
-    def hordiff_vec42(self, coeff, u, v):
+    def hordiff_vec(self, coeff, u, v, order=42):
+        self._check_narrow(order)
         ut, vt = self.zeros(), self.zeros()
-        lib().ora_hordiff_vec42(self.h, float(coeff), _p(u), _p(v), _p(ut), _p(vt))
+        lib().ora_hordiff_vec_narrow(self.h, order, float(coeff), _p(u), _p(v), _p(ut), _p(vt))
         return ut, vt
 
-    def diff_step42(self, h, u, v, dt, uv_coeff, h_coeff):
-        lib().ora_diff_step42(self.h, _p(h), _p(u), _p(v), float(dt), float(uv_coeff), float(h_coeff))
+    def diff_step(self, h, u, v, dt, uv_coeff, h_coeff, uv_order=42, h_order=42):
+        self._check_narrow(uv_order)
+        self._check_narrow(h_order)
+        lib().ora_diff_step_narrow(self.h, uv_order, h_order, _p(h), _p(u), _p(v), float(dt),
+                                   float(uv_coeff), float(h_coeff))
 
     def diagnostics(self, quad_order, h, u, v, hs=None):
         out = np.zeros(7)

@@ -17,8 +17,10 @@
  */
 #include "swm_oracle.h"
 #include "../cubedsphere_sbp_swm_amd/csrc/laplace_sbp42_narrow.h"
+#include "../cubedsphere_sbp_swm_amd/csrc/laplace_sbp63_narrow.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <complex.h>
@@ -898,36 +900,61 @@ void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *h
     for (int c = 0; c < 3; c++) free(y[c]);
 }
 
-/* ---- narrow-stencil SBP Laplacian, laplace_Ah_sbp42_narrow_t (differential_operators/
- * horizontal/laplace/laplace_Ah_sbp42_narrow_mod.f90): calc_laplace :23-58, calc_tile_d1x_d1y
- * :60-93, calc_laplace_tile :94-763.  The closure and interior coefficients come from the
- * generated table laplace_sbp42_narrow.h (terms in the reference's source order). ---- */
+/* ---- narrow-stencil SBP Laplacians, laplace_Ah_sbp42_narrow_t and laplace_Ah_sbp63_narrow_t
+ * (differential_operators/horizontal/laplace/laplace_Ah_sbp42_narrow_mod.f90: calc_laplace
+ * :23-58, calc_tile_d1x_d1y :60-93, calc_laplace_tile :94-763; laplace_Ah_sbp63_narrow_mod.f90:
+ * calc_laplace :494-529, calc_tile_d1x_d1y :531-564, calc_laplace_tile :565-1717; the
+ * first-derivative operators d42 / d63 from laplace_factory_mod.f90:176-250).  The closure and
+ * interior coefficients come from the generated tables laplace_sbp42_narrow.h /
+ * laplace_sbp63_narrow.h (terms in the reference's source order). ---- */
+
+typedef struct {
+    int order, nb, ncol, R, neg_inv_h;
+    const double *H, *tcoef, *icoef;
+    const int *ncols, *mstart, *tsign, *tb, *istart, *isign, *ib;
+    double pc;
+} narrow_tab;
+
+static const narrow_tab NARROW42 = {42, LN42_NB, LN42_NCOL, LN42_R, LN42_ROW_NEG_INV_H, LN42_H, LN42_TCOEF,
+                                    LN42_ICOEF, LN42_NCOLS, &LN42_MSTART[0][0], LN42_TSIGN, LN42_TB,
+                                    LN42_ISTART, LN42_ISIGN, LN42_IB, LN42_PC};
+static const narrow_tab NARROW63 = {63, LN63_NB, LN63_NCOL, LN63_R, LN63_ROW_NEG_INV_H, LN63_H, LN63_TCOEF,
+                                    LN63_ICOEF, LN63_NCOLS, &LN63_MSTART[0][0], LN63_TSIGN, LN63_TB,
+                                    LN63_ISTART, LN63_ISIGN, LN63_IB, LN63_PC};
+
+static const narrow_tab *narrow_table(int order) {
+    if (order == 42) return &NARROW42;
+    if (order == 63) return &NARROW63;
+    fprintf(stderr, "oracle: no narrow Laplacian of order %d\n", order);
+    abort();
+}
 
 /* closure row r: b[k], fv[c] = values k, c points from the panel edge */
-static double ln42_row(int r, const double *b, const double *fv) {
+static double narrow_row(const narrow_tab *T, int r, const double *b, const double *fv) {
+    const int *ms = T->mstart + r * (T->ncol + 1);
     double s = 0.0;
-    for (int c = 0; c < LN42_NCOLS[r]; c++) {
+    for (int c = 0; c < T->ncols[r]; c++) {
         double m = 0.0;
-        for (int t = LN42_MSTART[r][c]; t < LN42_MSTART[r][c + 1]; t++) {
-            double term = (LN42_TNUM[t] / LN42_TDEN[t]) * b[LN42_TB[t]];
-            if (t == LN42_MSTART[r][c]) m = LN42_TSIGN[t] < 0 ? -term : term;
-            else m = LN42_TSIGN[t] < 0 ? m - term : m + term;
+        for (int t = ms[c]; t < ms[c + 1]; t++) {
+            double term = T->tcoef[t] * b[T->tb[t]];
+            if (t == ms[c]) m = T->tsign[t] < 0 ? -term : term;
+            else m = T->tsign[t] < 0 ? m - term : m + term;
         }
         double pr = m * fv[c];
         s = c == 0 ? pr : s + pr;
     }
-    /* -1.0_8/h_rr*(...) */
-    return -((1.0 / (LN42_HNUM[r] / LN42_HDEN[r])) * s);
+    /* sbp42: -1.0_8/h_rr*(...); sbp63: 1.0_8*(...), exact */
+    return T->neg_inv_h ? -((1.0 / T->H[r]) * s) : s;
 }
-/* interior row: b[o], fv[o] at offsets o-2, o = 0..4 */
-static double ln42_interior(const double *b, const double *fv) {
+/* interior row: b[o], fv[o] at offsets o-R, o = 0..2R */
+static double narrow_interior(const narrow_tab *T, const double *b, const double *fv) {
     double s = 0.0;
-    for (int o = 0; o < 5; o++) {
+    for (int o = 0; o <= 2 * T->R; o++) {
         double cf = 0.0;
-        for (int t = LN42_ISTART[o]; t < LN42_ISTART[o + 1]; t++) {
-            double term = (LN42_INUM[t] / LN42_IDEN[t]) * b[LN42_IB[t] + 2];
-            if (t == LN42_ISTART[o]) cf = LN42_ISIGN[t] < 0 ? -term : term;
-            else cf = LN42_ISIGN[t] < 0 ? cf - term : cf + term;
+        for (int t = T->istart[o]; t < T->istart[o + 1]; t++) {
+            double term = T->icoef[t] * b[T->ib[t] + T->R];
+            if (t == T->istart[o]) cf = T->isign[t] < 0 ? -term : term;
+            else cf = T->isign[t] < 0 ? cf - term : cf + term;
         }
         double pr = cf * fv[o];
         s = o == 0 ? pr : s + pr;
@@ -936,36 +963,40 @@ static double ln42_interior(const double *b, const double *fv) {
 }
 
 /* D2(b) f along x ('x') or y at the point (i, j) of panel p, b = Qi11 J (x) or Qi22 J (y) */
-static double ln42_d2(const ora_grid *g, int p, int i, int j, char dir, const double *f) {
+static double narrow_d2(const narrow_tab *T, const ora_grid *g, int p, int i, int j, char dir, const double *f) {
     const int n = g->n;
     const double *J = g->geo[0], *Q = dir == 'x' ? g->geo[1] : g->geo[3];
     const int t = dir == 'x' ? i : j;
 #define AT(a) (dir == 'x' ? IDX(g, p, (a), j) : IDX(g, p, i, (a)))
-    double b[LN42_NCOL], fv[LN42_NCOL];
-    if (t <= LN42_NB || t >= n + 1 - LN42_NB) {
-        const int left = t <= LN42_NB;
+    double b[16], fv[16];
+    if (t <= T->nb || t >= n + 1 - T->nb) {
+        const int left = t <= T->nb;
         const int r = left ? t - 1 : n - t;
-        for (int k = 0; k < LN42_NCOL; k++) {
+        for (int k = 0; k < T->ncol; k++) {
             long id = AT(left ? 1 + k : n - k);
             b[k] = Q[id] * J[id];
             fv[k] = f[id];
         }
-        return ln42_row(r, b, fv);
+        return narrow_row(T, r, b, fv);
     }
-    for (int o = 0; o < 5; o++) {
-        long id = AT(t - 2 + o);
+    for (int o = 0; o <= 2 * T->R; o++) {
+        long id = AT(t - T->R + o);
         b[o] = Q[id] * J[id];
         fv[o] = f[id];
     }
 #undef AT
-    return ln42_interior(b, fv);
+    return narrow_interior(T, b, fv);
 }
 
+int ora_narrow_min_n(int order) { return 2 * narrow_table(order)->nb; }
+
 /* out = L f (in place allowed: f is read completely before out is written) */
-void ora_laplace42(const ora_grid *g, const double *f, double *out) {
+void ora_laplace_narrow(const ora_grid *g, int order, const double *f, double *out) {
+    const narrow_tab *T = narrow_table(order);
     const int n = g->n;
+    if (n < 2 * T->nb) { fprintf(stderr, "oracle: narrow Laplacian needs n >= %d\n", 2 * T->nb); abort(); }
     sbp_op d1;
-    sbp_make(&d1, 42);
+    sbp_make(&d1, order);
     const double *J = g->geo[0], *Q12 = g->geo[2];
     double *d1x = alloc_field(g), *d1y = alloc_field(g), *d2x = alloc_field(g), *d2y = alloc_field(g);
     double *d2xy = alloc_field(g), *d2yx = alloc_field(g);
@@ -980,17 +1011,17 @@ void ora_laplace42(const ora_grid *g, const double *f, double *out) {
                 d1x[id] = d1x[id] * J[id] * Q12[id];
                 d1y[id] = d1y[id] * J[id] * Q12[id];
             }
-    /* second derivatives: closure rows and interior (:127-714) */
+    /* second derivatives: closure rows and interior */
     #pragma omp parallel for collapse(2) schedule(static)
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
                 long id = IDX(g, p, i, j);
-                d2x[id] = ln42_d2(g, p, i, j, 'x', f);
-                d2y[id] = ln42_d2(g, p, i, j, 'y', f);
+                d2x[id] = narrow_d2(T, g, p, i, j, 'x', f);
+                d2y[id] = narrow_d2(T, g, p, i, j, 'y', f);
             }
-    /* boundary penalties of the cross terms, all into d2f_x (:716-742) */
-    const double pc = LN42_PNUM / LN42_PDEN;
+    /* boundary penalties of the cross terms, all into d2f_x (sbp42 :716-742, sbp63 :1671-1697) */
+    const double pc = T->pc;
     for (int p = 0; p < 6; p++) {
         for (int j = 1; j <= n; j++) {
             long id = IDX(g, p, 1, j);
@@ -1033,14 +1064,15 @@ void ora_laplace42(const ora_grid *g, const double *f, double *out) {
 /* hordiff_scalar_t%calc_diff (hordiff/hordiff_scalar_mod.f90:34-59) with diff_order 2 and
  * diff_coeff = hordiff_coeff*scale*hx (hordiff_factory_mod.f90:158-188):
  * tend = (-1)**3 * diff_coeff**4 * L(L(f)); the integer power as (c*c)*(c*c) */
-void ora_hordiff_scalar42(const ora_grid *g, double hordiff_coeff, const double *f, double *tend) {
+void ora_hordiff_scalar_narrow(const ora_grid *g, int order, double hordiff_coeff, const double *f,
+                               double *tend) {
     const int n = g->n;
     double *w = alloc_field(g);
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) w[IDX(g, p, i, j)] = 1.0 * f[IDX(g, p, i, j)];
-    ora_laplace42(g, w, w);
-    ora_laplace42(g, w, w);
+    ora_laplace_narrow(g, order, w, w);
+    ora_laplace_narrow(g, order, w, w);
     const double dc = hordiff_coeff * g->scale * g->hx;
     const double coeff = -1.0 * ((dc * dc) * (dc * dc));
     for (int p = 0; p < 6; p++)
@@ -1052,8 +1084,8 @@ void ora_hordiff_scalar42(const ora_grid *g, double hordiff_coeff, const double 
 /* hordiff_colocated_xyz_t%calc_diff_vec (hordiff/hordiff_colocated_mod.f90:39-58): covariant
  * (u, v) to Cartesian with b1, b2 (vec_math_mod.f90:107-126), the scalar hyperdiffusion of each
  * component, back with b1, b2 (vec_math_mod.f90:144-165), covariant edge sync */
-void ora_hordiff_vec42(const ora_grid *g, double hordiff_coeff, const double *u, const double *v,
-                       double *ut, double *vt) {
+void ora_hordiff_vec_narrow(const ora_grid *g, int order, double hordiff_coeff, const double *u,
+                            const double *v, double *ut, double *vt) {
     const int n = g->n;
     const long fl = g->flen;
     const double *b1 = g->geo[14], *b2 = g->geo[17];
@@ -1065,7 +1097,7 @@ void ora_hordiff_vec42(const ora_grid *g, double hordiff_coeff, const double *u,
                 long id = IDX(g, p, i, j);
                 for (int c = 0; c < 3; c++) vx[c][id] = b1[id + c * fl] * u[id] + b2[id + c * fl] * v[id];
             }
-    for (int c = 0; c < 3; c++) ora_hordiff_scalar42(g, hordiff_coeff, vx[c], tx[c]);
+    for (int c = 0; c < 3; c++) ora_hordiff_scalar_narrow(g, order, hordiff_coeff, vx[c], tx[c]);
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {
@@ -1078,13 +1110,15 @@ void ora_hordiff_vec42(const ora_grid *g, double hordiff_coeff, const double *u,
 }
 
 /* operator_swm_diff_t%apply (operator_swm_diff_mod.f90:25-46) followed by explicit_Eul1
- * (time_schemes/explicit_Eul1_mod.f90:20-30): y = y + dt * diff(y) */
-void ora_diff_step42(const ora_grid *g, double *h, double *u, double *v, double dt,
-                     double uv_diff_coeff, double h_diff_coeff) {
+ * (time_schemes/explicit_Eul1_mod.f90:20-30): y = y + dt * diff(y); uv_order / h_order pick
+ * the narrow Laplacian under hordiff_vec_xyz_Ah_sbp_{42,63}_narrow and
+ * hordiff_scalar_Ah_sbp_{42,63}_narrow (hordiff_factory_mod.f90:25-50) */
+void ora_diff_step_narrow(const ora_grid *g, int uv_order, int h_order, double *h, double *u, double *v,
+                          double dt, double uv_diff_coeff, double h_diff_coeff) {
     const int n = g->n;
     double *ht = alloc_field(g), *ut = alloc_field(g), *vt = alloc_field(g);
-    ora_hordiff_vec42(g, uv_diff_coeff, u, v, ut, vt);
-    ora_hordiff_scalar42(g, h_diff_coeff, h, ht);
+    ora_hordiff_vec_narrow(g, uv_order, uv_diff_coeff, u, v, ut, vt);
+    ora_hordiff_scalar_narrow(g, h_order, h_diff_coeff, h, ht);
     for (int p = 0; p < 6; p++)
         for (int j = 1; j <= n; j++)
             for (int i = 1; i <= n; i++) {

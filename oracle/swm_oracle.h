@@ -58,13 +58,16 @@ void ora_grad(const ora_grid *g, double *f, double *gx, double *gy);
 void ora_curl(const ora_grid *g, double *u, double *v, double *curl);
 void ora_scalar_sync(const ora_grid *g, double *f);
 void ora_vec_sync_covariant(const ora_grid *g, double *gx, double *gy);
-/* narrow-stencil SBP Laplacian (sbp42) and the hyperdiffusion built on it (N+1 >= 12) */
-void ora_laplace42(const ora_grid *g, const double *f, double *out);
-void ora_hordiff_scalar42(const ora_grid *g, double hordiff_coeff, const double *f, double *tend);
-void ora_hordiff_vec42(const ora_grid *g, double hordiff_coeff, const double *u, const double *v,
-                       double *ut, double *vt);
-void ora_diff_step42(const ora_grid *g, double *h, double *u, double *v, double dt,
-                     double uv_diff_coeff, double h_diff_coeff);
+/* narrow-stencil SBP Laplacians (order 42 or 63) and the hyperdiffusion built on them
+ * (N+1 >= ora_narrow_min_n(order): 12 for sbp42, 18 for sbp63) */
+int ora_narrow_min_n(int order);
+void ora_laplace_narrow(const ora_grid *g, int order, const double *f, double *out);
+void ora_hordiff_scalar_narrow(const ora_grid *g, int order, double hordiff_coeff, const double *f,
+                               double *tend);
+void ora_hordiff_vec_narrow(const ora_grid *g, int order, double hordiff_coeff, const double *u,
+                            const double *v, double *ut, double *vt);
+void ora_diff_step_narrow(const ora_grid *g, int uv_order, int h_order, double *h, double *u, double *v,
+                          double dt, double uv_diff_coeff, double h_diff_coeff);
 
 /* full RHS: out = RHS(in); in/out are [h,u,v] each a stored field; hs may be NULL */
 void ora_rhs(const ora_grid *g, double *in_h, double *in_u, double *in_v,

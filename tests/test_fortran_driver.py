@@ -117,15 +117,17 @@ def test_driver_equals_python_path(tmp_path, N, order, nsteps):
 
 
 @pytest.mark.gpu
-def test_driver_with_hyperdiffusion_equals_python_path(tmp_path):
-    """&shallow_water_model with the sbp42 narrow-stencil hyperdiffusion: the driver's RK4 +
-    explicit_Eul1 loop equals the Python path step for step, bit for bit."""
+@pytest.mark.parametrize("order", [42, 63])
+def test_driver_with_hyperdiffusion_equals_python_path(tmp_path, order):
+    """&shallow_water_model with the sbp42 / sbp63 narrow-stencil hyperdiffusion (the operator
+    family of the same order): the driver's RK4 + explicit_Eul1 loop equals the Python path step
+    for step, bit for bit."""
     _need_driver()
     import cubedsphere_sbp_swm_amd as swm
-    N, order, nsteps, dt, seed = 24, 42, 5, 600.0, 3
-    uvn, hn = "hordiff_vec_xyz_Ah_sbp_42_narrow", "hordiff_scalar_Ah_sbp_42_narrow"
+    N, nsteps, dt, seed = 24, 5, 600.0, 3
+    uvn, hn = f"hordiff_vec_xyz_Ah_sbp_{order}_narrow", f"hordiff_scalar_Ah_sbp_{order}_narrow"
     nml, out = _write_nml(tmp_path, N, order, dt, nsteps, seed, uvc=0.03, hc=0.01, uvn=uvn, hn=hn)
-    print(f"[fortran driver + diffusion] N={N} steps={nsteps}", flush=True)
+    print(f"[fortran driver + diffusion] N={N} order={order} steps={nsteps}", flush=True)
     r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
@@ -148,7 +150,7 @@ def test_driver_with_hyperdiffusion_equals_python_path(tmp_path):
 def test_driver_rejects_other_hyperdiffusion(tmp_path):
     _need_driver()
     nml, _ = _write_nml(tmp_path, 16, 42, 600.0, 2, 1, uvc=0.03, hc=0.01,
-                        uvn="hordiff_vec_xyz_Ah_sbp_63_narrow", hn="hordiff_scalar_Ah_sbp_63_narrow")
+                        uvn="hordiff_vec_xyz_Ah_sbp_21_narrow", hn="hordiff_scalar_Ah_sbp_42_narrow")
     r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=60, cwd=tmp_path)
     assert r.returncode != 0
     assert "hyperdiffusion" in r.stdout
