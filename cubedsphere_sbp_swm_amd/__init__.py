@@ -14,13 +14,15 @@ Host arrays use the reference tile layout: shape (6, n+2*hw, n+2*hw), index [pan
 with reference indices i,j = 1..n stored at [.., j-1+hw, i-1+hw], n = N+1.
 """
 import ctypes
+import os
 
 import numpy as np
 
 from . import _capi
 from ._capi import SwmError, check
 
-__all__ = ["SwmError", "operator_names", "create_swm_operator", "create_stvec_swm", "create_timescheme",
+__all__ = ["SwmError", "operator_names", "create_latlon_outputer", "create_latlon_vec_outputer",
+           "create_master_paneled_outputer", "create_swm_operator", "create_stvec_swm", "create_timescheme",
            "OperatorSWM", "StvecSWM", "RK4", "Ensemble", "EARTH"]
 
 # const_mod.f90:6-9
@@ -194,6 +196,57 @@ class OperatorSWM:
         return out
 
 
+class Outputer:
+    """Writers of the reference's on-disk formats (outputer_factory_mod.f90): REAL*4 direct-access
+    records, record r at byte (r-1) * 4 * (values per record).
+
+    create_latlon_outputer(op, Nlat, Nlon, "Ah")                 -> .write(x.field("h"), file, rec)
+    create_latlon_vec_outputer(op, Nlat, Nlon, "Ah", "covariant") -> .write_vec(x, file_u, file_v, rec)
+    create_master_paneled_outputer(op, "xy")                     -> .write(x.field("h"), file, rec)
+    """
+
+    def __init__(self, op, handle=None, paneled=False):
+        self.op, self.handle, self.paneled = op, handle, paneled
+
+    def write(self, field, file_name, rec_num=1):
+        if self.paneled:
+            check(_capi.lib().swm_paneled_write(self.op.ctx, field, os.fsencode(file_name), int(rec_num)))
+        else:
+            check(_capi.lib().swm_latlon_write(self.handle, field, os.fsencode(file_name), int(rec_num)))
+
+    def write_vec(self, x, file_name_u, file_name_v, rec_num=1):
+        check(_capi.lib().swm_latlon_vec_write(self.handle, x.field("u"), x.field("v"), os.fsencode(file_name_u),
+                                               os.fsencode(file_name_v), int(rec_num)))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _capi.lib().swm_outputer_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def create_latlon_outputer(op, Nlat, Nlon, scalar_grid_type="Ah"):
+    h = ctypes.c_void_p()
+    check(_capi.lib().swm_latlon_outputer_create(op.ctx, int(Nlat), int(Nlon), scalar_grid_type.encode(),
+                                                 ctypes.byref(h)))
+    return Outputer(op, h)
+
+
+def create_latlon_vec_outputer(op, Nlat, Nlon, staggering="Ah", components_type="covariant"):
+    h = ctypes.c_void_p()
+    check(_capi.lib().swm_latlon_vec_outputer_create(op.ctx, int(Nlat), int(Nlon), staggering.encode(),
+                                                     components_type.encode(), ctypes.byref(h)))
+    return Outputer(op, h)
+
+
+def create_master_paneled_outputer(op, points_type="xy"):
+    if points_type != "xy":
+        raise SwmError(f"the device path writes Ah (xy) points only, not {points_type}")
+    return Outputer(op, paneled=True)
+
+
 class StvecSWM:
     """Device-resident state vector (h, u, v) of one operator's domain (stvec_swm_t)."""
 
@@ -216,6 +269,11 @@ class StvecSWM:
 
     def create_similar(self):
         return StvecSWM(self.op)
+
+    def field(self, name):
+        """Device pointer of the h, u or v field (as the reference's state%h, state%u, state%v)."""
+        k = {"h": 0, "u": 1, "v": 2}[name]
+        return ctypes.c_void_p(self.ptr + k * self.op.field_elems * self.op.elem_bytes)
 
     def upload(self, h, u, v, halo_width=0):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (h, u, v)]

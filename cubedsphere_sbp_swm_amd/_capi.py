@@ -61,6 +61,12 @@ SIGNATURES = {
     "swm_ens_bench_stage": (_i, [_vp, _i]),
     "swm_ecs_halo_xy": (_i, [_vp, _vp, _i, _vp]),
     "swm_ecs_halo_xy_host": (_i, [_vp, _vp, _i, _dp]),
+    "swm_latlon_outputer_create": (_i, [_vp, _i, _i, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "swm_latlon_vec_outputer_create": (_i, [_vp, _i, _i, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "swm_outputer_destroy": (_i, [_vp]),
+    "swm_latlon_write": (_i, [_vp, _vp, ctypes.c_char_p, _i]),
+    "swm_latlon_vec_write": (_i, [_vp, _vp, _vp, ctypes.c_char_p, ctypes.c_char_p, _i]),
+    "swm_paneled_write": (_i, [_vp, _vp, ctypes.c_char_p, _i]),
 }
 
 # host set-up routines (geometry.h), exported by the same library; they need no device
@@ -73,6 +79,7 @@ HOST_SIGNATURES = {
     "swmh_edge_sync_matrix": (_i, [_i, _i, _ip, _ip, _dp, _i]),
     "swmh_ecs_xy_weights": (_i, [_i, _i, _dp, _ip]),
     "swmh_basis_b": (None, [_i, _i, _dp, _dp]),
+    "swmh_latlon_table": (None, [_i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp]),
 }
 
 _lib = None

@@ -145,6 +145,75 @@ extern "C" void swmh_basis_b(int N, int pitch, double *b1, double *b2) {
         }
 }
 
+// create_latlon_interp (regrid_factory_mod.f90:183-301) on the Ah (xy) mesh: stencil bounds
+// 1..n, hx as the mesh, shift 0; transform_cartesian_to_native_ecs (ecs_metric_mod.f90:480-506)
+// for the panel and (alpha, beta); cubic Lagrange weights; for the rotation the covariant branch
+// of create_latlon_vector_regrid (:159-176) with b1, b2 of calculate_ecs_b1_2d / _b2_2d.
+extern "C" void swmh_latlon_table(int N, int Nlat, int Nlon, int *panel, int *x0, int *y0, double *wx, double *wy,
+                                  double *rot) {
+    const int n = N + 1;
+    const double pi = pi_(), hx = grid_hx(N), alpha0 = -0.25 * pi;
+    const double hlon = 2.0 * pi / (double)Nlon, hlat = pi / ((double)Nlat - 1.0);
+    for (int j = 1; j <= Nlat; ++j) {
+        const double lat = -0.5 * pi + hlat * (double)(j - 1);
+        for (int i = 1; i <= Nlon; ++i) {
+            const double lon = hlon * (double)(i - 1);
+            const double r[3] = {std::cos(lat) * std::cos(lon), std::cos(lat) * std::sin(lon), std::sin(lat)};
+            // panel with the largest projection on the outer normal (first one on ties)
+            int pi_ind = 0;
+            double dmax = 0.0;
+            for (int p = 0; p < 6; ++p) {
+                double s = r[0] * (double)NV[p][0];
+                s = s + r[1] * (double)NV[p][1];
+                s = s + r[2] * (double)NV[p][2];
+                const double d = -s;
+                if (d > dmax) {
+                    pi_ind = p;
+                    dmax = d;
+                }
+            }
+            double sx = r[0] * (double)EX[pi_ind][0];
+            sx = sx + r[1] * (double)EX[pi_ind][1];
+            sx = sx + r[2] * (double)EX[pi_ind][2];
+            double sy = r[0] * (double)EY[pi_ind][0];
+            sy = sy + r[1] * (double)EY[pi_ind][1];
+            sy = sy + r[2] * (double)EY[pi_ind][2];
+            const double alpha = std::atan(sx / dmax), beta = std::atan(sy / dmax);
+            const double zdx = (alpha - alpha0) / hx + 1.0 - 0.0;
+            const double zdy = (beta - alpha0) / hx + 1.0 - 0.0;
+            const int xi = std::max(std::min((int)std::floor(zdx) + 2, n) - 4 + 1, 1);
+            const int yi = std::max(std::min((int)std::floor(zdy) + 2, n) - 4 + 1, 1);
+            const long q = (long)(j - 1) * Nlon + (i - 1);
+            panel[q] = pi_ind;
+            x0[q] = xi - 1;
+            y0[q] = yi - 1;
+            const double zx = zdx - (double)xi, zy = zdy - (double)yi;
+            const double z2[2] = {zx, zy};
+            double *w2[2] = {wx + 4 * q, wy + 4 * q};
+            for (int d = 0; d < 2; ++d) {
+                const double z = z2[d];
+                w2[d][0] = -((z - 1.0) * (z - 2.0) * (z - 3.0) / 6.0);
+                w2[d][1] = z * (z - 2.0) * (z - 3.0) / 2.0;
+                w2[d][2] = -(z * (z - 1.0) * (z - 3.0) / 2.0);
+                w2[d][3] = z * (z - 1.0) * (z - 2.0) / 6.0;
+            }
+            if (rot) {
+                const double iv[3] = {-std::sin(lon), std::cos(lon), 0.0};
+                const double jv[3] = {-std::sin(lat) * std::cos(lon), -std::sin(lat) * std::sin(lon), std::cos(lat)};
+                double v[3], b1[3], b2[3];
+                b1_proto(alpha, beta, v);
+                to_panel(pi_ind, v, b1);
+                b2_proto(alpha, beta, v);
+                to_panel(pi_ind, v, b2);
+                rot[4 * q + 0] = dot3(iv, b1);
+                rot[4 * q + 1] = dot3(jv, b1);
+                rot[4 * q + 2] = dot3(iv, b2);
+                rot[4 * q + 3] = dot3(jv, b2);
+            }
+        }
+    }
+}
+
 extern "C" void swmh_sync_weights(int N, double *qb, double *qt, double *ql, double *qr) {
     const int n = N + 1;
     const double pi = pi_();

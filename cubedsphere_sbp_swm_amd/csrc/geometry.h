@@ -49,6 +49,16 @@ int swmh_edge_sync_matrix(int N, int vector, int *rows, int *cols, double *w, in
 // j = 1..hw (init_halo_xy_interp, ecs_halo_factory_mod.f90:228-269).  Returns 0 on success.
 int swmh_ecs_xy_weights(int N, int hw, double *w, int *ind);
 
+// Cubic lat-lon regrid of Ah fields (create_latlon_interp, regridders/regrid_factory_mod.f90:
+// 183-301, scalar_grid_type "Ah", interp_type "cubic"; covariant components as in
+// create_latlon_vector_regrid :80-181).  Target q = (j-1)*Nlon + (i-1), lat = -pi/2 +
+// (j-1) pi/(Nlat-1), lon = (i-1) 2 pi/Nlon.  panel[q] (0-based), x0[q], y0[q] (0-based first
+// stencil point, the 4 x 4 stencil inside the panel's mesh points), wx[4q..4q+3], wy[..] (cubic
+// Lagrange weights) and, when rot is not NULL, rot[4q..4q+3] = (u2u, u2v, v2u, v2v): the
+// lat-lon components of covariant (u, v) are (u2u u + v2u v, u2v u + v2v v).
+void swmh_latlon_table(int N, int Nlat, int Nlon, int *panel, int *x0, int *y0, double *wx, double *wy,
+                       double *rot);
+
 // Seeded initial state: h = h0 + h' (max|h'| = h_amp), V = r x grad(psi) (max|V| = v_max),
 // h', psi random real spherical-harmonic series of degree 1..lmax with N(0,1)*l^-2
 // coefficients drawn from a splitmix64/Box-Muller stream of `seed`; covariant components

@@ -13,7 +13,10 @@
 !
 ! Loop as in run_ts2 (ts2_mod.f90:141-156): an rk4 step, then, when a diffusion coefficient is
 ! nonzero, the explicit hyperdiffusion step (:145-146); key/value diagnostics every tau_diagnostics.
-! The final state is written to output_file as raw FP64 h, u, v, each real(8) f(n, n, 6).
+! The final state is written to output_file as raw FP64 h, u, v, each real(8) f(n, n, 6).  With
+! latlon_output = .true. the run also writes h.dat, u.dat, v.dat as run_ts2 does (ts2_mod.f90:
+! 108-112,134-138,174-179): REAL*4 records of the cubic (2N+1) x 4N lat-lon regrid, record 1 at
+! the start and record it/nstep_write + 1 every nstep_write = nint(tau_write/dt) steps.
 program swm_gpu_driver
 
 use swm_gpu_mod
@@ -46,6 +49,7 @@ integer(kind=8)    :: seed = 0
 integer(kind=4)    :: lmax = 8, precision = 64, autotune_reps = 0
 real(kind=8)       :: h0 = 1.0e4_8, h_amp = 100.0_8, v_max = 40.0_8, grav = 9.80616_8
 character(len=255) :: output_file = ""
+logical            :: latlon_output = .false.
 
 namelist /domain/ N, Nz, staggering_type, topology_type, metric_type, vertical_staggering, h_top
 namelist /metric/ scale, omega, rotation_matrix, rotation_axis, metric_2d_type, vertical_transform_name
@@ -55,7 +59,8 @@ namelist /shallow_water_model/ swm_op_type, v_components_type, div_op_name, grad
                                uv_diff_coeff, hordiff_h_name, h_diff_coeff, dt, tau_write, tau_diagnostics, &
                                simulation_time_sec, simulation_time_min, simulation_time_hours, &
                                simulation_time_days
-namelist /swm_gpu_run/ seed, lmax, h0, h_amp, v_max, grav, precision, autotune_reps, output_file
+namelist /swm_gpu_run/ seed, lmax, h0, h_amp, v_max, grav, precision, autotune_reps, output_file, &
+                       latlon_output
 
 type(operator_swm_gpu_t) :: operator
 type(stvec_swm_gpu_t)    :: state
@@ -63,11 +68,13 @@ type(rk4_gpu_t)          :: timescheme
 type(operator_swm_diff_gpu_t) :: operator_diff
 type(explicit_eul1_gpu_t)     :: timescheme_diff
 logical                       :: use_diff
+type(outputer_gpu_t)            :: outputer
+type(latlon_vec_outputer_gpu_t) :: outputer_vec
 type(key_value_r8_gpu_t) :: diagnostics
 character(len=1024) :: fname
 real(kind=8), allocatable :: h(:,:,:), u(:,:,:), v(:,:,:)
 real(kind=8) :: simulation_time, best_us
-integer(kind=4) :: nsteps, nstep_diagnostics, it, chunk, unit_nml, ios, n_pts, k
+integer(kind=4) :: nsteps, nstep_diagnostics, nstep_write, it, chunk, unit_nml, ios, n_pts, k
 
 fname = "namelist_swm"
 if (command_argument_count() >= 1) call get_command_argument(1, fname)
@@ -103,6 +110,7 @@ simulation_time = simulation_time_days*86400.0_8 + simulation_time_hours*3600.0_
                   simulation_time_min*60.0_8 + simulation_time_sec
 nsteps = int(simulation_time/dt)
 nstep_diagnostics = max(1, nint(tau_diagnostics/dt))
+nstep_write = max(1, nint(tau_write/dt))
 
 print '(A,I6,A,A,A,A,A,A,A,I3)', "N = ", N, "  div = ", trim(div_op_name), "  grad = ", trim(grad_op_name), &
       "  quadrature = ", trim(quadrature_name), "  precision = ", precision
@@ -130,9 +138,17 @@ end if
 
 diagnostics = operator%get_diagnostics(state)
 call diagnostics%print()
+if (latlon_output) then
+    ! lat-lon output as run_ts2 writes it (ts2_mod.f90:108-112,134-138,174-179)
+    call create_latlon_outputer_gpu(outputer, operator, 2*N+1, 4*N, "Ah")
+    call create_latlon_vec_outputer_gpu(outputer_vec, operator, 2*N+1, 4*N, "Ah", v_components_type)
+    call outputer%write(state, "h", "h.dat", 1)
+    call outputer_vec%write(state, "u.dat", "v.dat", 1)
+end if
 it = 0
 do while (it < nsteps)
     chunk = min(nstep_diagnostics - mod(it, nstep_diagnostics), nsteps - it)
+    if (latlon_output) chunk = min(chunk, nstep_write - mod(it, nstep_write))
     if (use_diff) then
         do k = 1, chunk
             call timescheme%step(state, operator, dt)
@@ -147,7 +163,15 @@ do while (it < nsteps)
         print '(A,F12.4)', "Hours = ", real(it*dt/3600.0_8, 4)
         call diagnostics%print()
     end if
+    if (latlon_output .and. mod(it, nstep_write) == 0) then
+        call outputer%write(state, "h", "h.dat", it/nstep_write + 1)
+        call outputer_vec%write(state, "u.dat", "v.dat", it/nstep_write + 1)
+    end if
 end do
+if (latlon_output) then
+    call outputer%destroy()
+    call outputer_vec%destroy()
+end if
 
 if (len_trim(output_file) > 0) then
     call state%download(h, u, v, 0)

@@ -17,6 +17,8 @@ private
 public :: operator_swm_gpu_t, stvec_swm_gpu_t, rk4_gpu_t, key_value_r8_gpu_t
 public :: create_swm_gpu_operator, create_stvec_swm_gpu, create_rk4_gpu, swm_gpu_check
 public :: operator_swm_diff_gpu_t, explicit_eul1_gpu_t, create_swm_diff_gpu_operator, create_explicit_eul1_gpu
+public :: outputer_gpu_t, latlon_vec_outputer_gpu_t, create_latlon_outputer_gpu, create_latlon_vec_outputer_gpu
+public :: create_master_paneled_outputer_gpu
 
 interface
     function c_swm_last_error() bind(C, name="swm_last_error") result(p)
@@ -121,6 +123,44 @@ interface
         character(kind=c_char), intent(in) :: uv_name(*), h_name(*)
         real(c_double), value :: uv_coeff, h_coeff
     end function
+    integer(c_int) function c_swm_latlon_outputer_create(ctx, nlat, nlon, grid, out) &
+                                                          bind(C, name="swm_latlon_outputer_create")
+        import :: c_ptr, c_int, c_char
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: nlat, nlon
+        character(kind=c_char), intent(in) :: grid(*)
+        type(c_ptr), intent(out) :: out
+    end function
+    integer(c_int) function c_swm_latlon_vec_outputer_create(ctx, nlat, nlon, stag, comp, out) &
+                                                              bind(C, name="swm_latlon_vec_outputer_create")
+        import :: c_ptr, c_int, c_char
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: nlat, nlon
+        character(kind=c_char), intent(in) :: stag(*), comp(*)
+        type(c_ptr), intent(out) :: out
+    end function
+    integer(c_int) function c_swm_outputer_destroy(o) bind(C, name="swm_outputer_destroy")
+        import :: c_ptr, c_int
+        type(c_ptr), value :: o
+    end function
+    integer(c_int) function c_swm_latlon_write(o, f, file_name, rec) bind(C, name="swm_latlon_write")
+        import :: c_ptr, c_int, c_char
+        type(c_ptr), value :: o, f
+        character(kind=c_char), intent(in) :: file_name(*)
+        integer(c_int), value :: rec
+    end function
+    integer(c_int) function c_swm_latlon_vec_write(o, u, v, file_u, file_v, rec) bind(C, name="swm_latlon_vec_write")
+        import :: c_ptr, c_int, c_char
+        type(c_ptr), value :: o, u, v
+        character(kind=c_char), intent(in) :: file_u(*), file_v(*)
+        integer(c_int), value :: rec
+    end function
+    integer(c_int) function c_swm_paneled_write(ctx, f, file_name, rec) bind(C, name="swm_paneled_write")
+        import :: c_ptr, c_int, c_char
+        type(c_ptr), value :: ctx, f
+        character(kind=c_char), intent(in) :: file_name(*)
+        integer(c_int), value :: rec
+    end function
     function c_strlen(s) bind(C, name="strlen") result(n)
         import :: c_ptr, c_size_t
         type(c_ptr), value :: s
@@ -173,6 +213,25 @@ type explicit_eul1_gpu_t
 contains
     procedure, public :: step => step_explicit_eul1_gpu
 end type explicit_eul1_gpu_t
+
+! latlon_outputer_t (outputer/latlon_outputer_mod.f90:16-24) and master_paneled_outputer_t
+! (master_paneled_outputer_mod.f90:11-21): write(state, field, file_name, rec_num) with field
+! 'h', 'u' or 'v' of the state in place of a grid_field_t
+type outputer_gpu_t
+    type(c_ptr) :: ctx = c_null_ptr, handle = c_null_ptr
+    logical     :: paneled = .false.
+contains
+    procedure, public :: write => write_outputer_gpu
+    procedure, public :: destroy => destroy_outputer_gpu
+end type outputer_gpu_t
+
+! latlon_vec_outputer_t (latlon_outputer_mod.f90:26-38): write(state, file_name_u, file_name_v, rec_num)
+type latlon_vec_outputer_gpu_t
+    type(c_ptr) :: ctx = c_null_ptr, handle = c_null_ptr
+contains
+    procedure, public :: write => write_latlon_vec_gpu
+    procedure, public :: destroy => destroy_latlon_vec_gpu
+end type latlon_vec_outputer_gpu_t
 
 type rk4_gpu_t
 contains
@@ -338,8 +397,8 @@ subroutine steps_rk4_gpu(this, v0, operator, dt, nsteps)
 end subroutine steps_rk4_gpu
 
 ! create_swm_diff_operator (operator_swm_diff_factory_mod.f90:11-34) for the names of the
-! &shallow_water_model namelist; the device implements the sbp42 narrow-stencil pair
-! ("hordiff_vec_xyz_Ah_sbp_42_narrow", "hordiff_scalar_Ah_sbp_42_narrow")
+! &shallow_water_model namelist; the device implements the narrow-stencil names
+! "hordiff_vec_xyz_Ah_sbp_{42,63}_narrow" and "hordiff_scalar_Ah_sbp_{42,63}_narrow"
 subroutine create_swm_diff_gpu_operator(op_diff, op, hordiff_uv_name, uv_diff_coeff, hordiff_h_name, h_diff_coeff)
     type(operator_swm_diff_gpu_t), intent(out) :: op_diff
     type(operator_swm_gpu_t),      intent(in)  :: op
@@ -375,6 +434,106 @@ subroutine step_explicit_eul1_gpu(this, v0, operator, dt)
                                        operator%uv_diff_coeff, trim(operator%hordiff_h_name)//c_null_char, &
                                        operator%h_diff_coeff), "timescheme_diff%step")
 end subroutine step_explicit_eul1_gpu
+
+! device pointer of field 'h', 'u' or 'v' of a state vector
+function field_ptr(ctx, state, field) result(p)
+    type(c_ptr),      intent(in) :: ctx, state
+    character(len=*), intent(in) :: field
+    type(c_ptr) :: p
+    integer(c_int) :: n_, pitch, eb, k
+    integer(c_long_long) :: fe
+    call swm_gpu_check(c_swm_ctx_info(ctx, n_, pitch, fe, eb), "outputer%write")
+    select case (field)
+    case ("h"); k = 0
+    case ("u"); k = 1
+    case ("v"); k = 2
+    case default
+        k = -1
+        call swm_gpu_check(-1_c_int, "outputer%write: field must be 'h', 'u' or 'v'")
+    end select
+    p = transfer(transfer(state, 0_c_intptr_t) + int(k, c_intptr_t) * int(fe, c_intptr_t) * int(eb, c_intptr_t), &
+                 c_null_ptr)
+end function field_ptr
+
+! create_latlon_outputer(outputer, Nlat, Nlon, scalar_grid_type, domain)
+! (outputer/outputer_factory_mod.f90:45-136); scalar_grid_type "Ah"
+subroutine create_latlon_outputer_gpu(outputer, op, Nlat, Nlon, scalar_grid_type)
+    type(outputer_gpu_t),     intent(out) :: outputer
+    type(operator_swm_gpu_t), intent(in)  :: op
+    integer(kind=4),          intent(in)  :: Nlat, Nlon
+    character(len=*),         intent(in)  :: scalar_grid_type
+    outputer%ctx = op%ctx
+    call swm_gpu_check(c_swm_latlon_outputer_create(op%ctx, int(Nlat, c_int), int(Nlon, c_int), &
+                                                    trim(scalar_grid_type)//c_null_char, outputer%handle), &
+                       "create_latlon_outputer")
+end subroutine create_latlon_outputer_gpu
+
+! create_master_paneled_outputer(outputer, points_type, domain) (outputer_factory_mod.f90:11-43);
+! points_type "xy"
+subroutine create_master_paneled_outputer_gpu(outputer, op, points_type)
+    type(outputer_gpu_t),     intent(out) :: outputer
+    type(operator_swm_gpu_t), intent(in)  :: op
+    character(len=*),         intent(in)  :: points_type
+    if (trim(points_type) /= "xy") &
+        call swm_gpu_check(-2_c_int, "create_master_paneled_outputer: the device path writes xy points only")
+    outputer%ctx = op%ctx
+    outputer%paneled = .true.
+end subroutine create_master_paneled_outputer_gpu
+
+! create_latlon_vec_outputer(outputer, Nlat, Nlon, staggering, components_type, domain)
+! (outputer_factory_mod.f90:138-190); staggering "Ah", components_type "covariant"
+subroutine create_latlon_vec_outputer_gpu(outputer, op, Nlat, Nlon, staggering, components_type)
+    type(latlon_vec_outputer_gpu_t), intent(out) :: outputer
+    type(operator_swm_gpu_t),        intent(in)  :: op
+    integer(kind=4),                 intent(in)  :: Nlat, Nlon
+    character(len=*),                intent(in)  :: staggering, components_type
+    outputer%ctx = op%ctx
+    call swm_gpu_check(c_swm_latlon_vec_outputer_create(op%ctx, int(Nlat, c_int), int(Nlon, c_int), &
+                                                        trim(staggering)//c_null_char, &
+                                                        trim(components_type)//c_null_char, outputer%handle), &
+                       "create_latlon_vec_outputer")
+end subroutine create_latlon_vec_outputer_gpu
+
+! latlon_outputer_t%write / master_paneled_outputer_t%write (f, domain, file_name, rec_num)
+subroutine write_outputer_gpu(this, state, field, file_name, rec_num)
+    class(outputer_gpu_t),  intent(inout) :: this
+    class(stvec_swm_gpu_t), intent(in)    :: state
+    character(len=*),       intent(in)    :: field, file_name
+    integer(kind=4),        intent(in)    :: rec_num
+    if (this%paneled) then
+        call swm_gpu_check(c_swm_paneled_write(this%ctx, field_ptr(this%ctx, state%state, field), &
+                                               trim(file_name)//c_null_char, int(rec_num, c_int)), "outputer%write")
+    else
+        call swm_gpu_check(c_swm_latlon_write(this%handle, field_ptr(this%ctx, state%state, field), &
+                                              trim(file_name)//c_null_char, int(rec_num, c_int)), "outputer%write")
+    end if
+end subroutine write_outputer_gpu
+
+subroutine destroy_outputer_gpu(this)
+    class(outputer_gpu_t), intent(inout) :: this
+    integer(c_int) :: ierr
+    if (c_associated(this%handle)) ierr = c_swm_outputer_destroy(this%handle)
+    this%handle = c_null_ptr
+end subroutine destroy_outputer_gpu
+
+! latlon_vec_outputer_t%write(u, v, domain, file_name_u, file_name_v, rec_num)
+subroutine write_latlon_vec_gpu(this, state, file_name_u, file_name_v, rec_num)
+    class(latlon_vec_outputer_gpu_t), intent(inout) :: this
+    class(stvec_swm_gpu_t),           intent(in)    :: state
+    character(len=*),                 intent(in)    :: file_name_u, file_name_v
+    integer(kind=4),                  intent(in)    :: rec_num
+    call swm_gpu_check(c_swm_latlon_vec_write(this%handle, field_ptr(this%ctx, state%state, "u"), &
+                                              field_ptr(this%ctx, state%state, "v"), &
+                                              trim(file_name_u)//c_null_char, trim(file_name_v)//c_null_char, &
+                                              int(rec_num, c_int)), "outputer_vec%write")
+end subroutine write_latlon_vec_gpu
+
+subroutine destroy_latlon_vec_gpu(this)
+    class(latlon_vec_outputer_gpu_t), intent(inout) :: this
+    integer(c_int) :: ierr
+    if (c_associated(this%handle)) ierr = c_swm_outputer_destroy(this%handle)
+    this%handle = c_null_ptr
+end subroutine destroy_latlon_vec_gpu
 
 subroutine print_key_value(this)
     class(key_value_r8_gpu_t), intent(in) :: this

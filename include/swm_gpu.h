@@ -31,6 +31,7 @@ extern "C" {
 #define SWM_ENAME (-2)   /* unknown or unsupported operator name */
 #define SWM_EHIP (-3)    /* HIP runtime error */
 #define SWM_ESTATE (-4)  /* object in the wrong state (e.g. wrong context) */
+#define SWM_EIO (-5)     /* output file cannot be opened or written */
 
 typedef struct swm_ctx swm_ctx;
 
@@ -118,6 +119,38 @@ int swm_diff_apply(swm_ctx *ctx, const void *vin, void *vout, const char *hordif
                    double uv_diff_coeff, const char *hordiff_h_name, double h_diff_coeff);
 int swm_diff_step(swm_ctx *ctx, void *state, double dt, const char *hordiff_uv_name,
                   double uv_diff_coeff, const char *hordiff_h_name, double h_diff_coeff);
+
+/* ---- Output in the reference's on-disk formats ----
+ * Records are REAL*4, written as direct-access record rec_num (1-based) of a file that is
+ * created if absent and otherwise kept (other records unchanged); a record of W values starts at
+ * byte (rec_num-1) * 4 * W (record lengths in 4-byte words, as the reference's Intel Fortran
+ * build writes them).  Field pointers are one field of the context layout (field_elems
+ * elements), e.g. the h, u or v plane of a state vector.
+ * swm_latlon_outputer_create: create_latlon_outputer(outputer, Nlat, Nlon, scalar_grid_type,
+ * domain) (outputer/outputer_factory_mod.f90:45-136) with scalar_grid_type "Ah" (others return
+ * SWM_ENAME): cubic regrid to the Nlat x Nlon lat-lon grid (lat from -pi/2 to pi/2 inclusive,
+ * lon from 0 in steps of 2 pi/Nlon; regrid_factory_mod.f90:9-60,183-301).
+ * swm_latlon_vec_outputer_create: create_latlon_vec_outputer(outputer, Nlat, Nlon, staggering,
+ * components_type, domain) (outputer_factory_mod.f90:138-190) with staggering "Ah" and
+ * components_type "covariant".
+ * swm_latlon_write: latlon_outputer_t%write(f, domain, file_name, rec_num)
+ * (latlon_outputer_mod.f90:44-98): one record of Nlon x Nlat values, lon fastest.
+ * swm_latlon_vec_write: latlon_vec_outputer_t%write(u, v, domain, file_name_u, file_name_v,
+ * rec_num) (latlon_outputer_mod.f90:100-190): the zonal and meridional components.
+ * swm_paneled_write: master_paneled_outputer_t%write(f, domain, file_name, rec_num) of an
+ * outputer made by create_master_paneled_outputer(outputer, "xy", domain)
+ * (master_paneled_outputer_mod.f90:24-77): one record of (N+1) x (N+1) x 6 values, i fastest,
+ * then j, then panel. */
+typedef struct swm_outputer swm_outputer;
+int swm_latlon_outputer_create(swm_ctx *ctx, int Nlat, int Nlon, const char *scalar_grid_type,
+                               swm_outputer **out);
+int swm_latlon_vec_outputer_create(swm_ctx *ctx, int Nlat, int Nlon, const char *staggering,
+                                   const char *components_type, swm_outputer **out);
+int swm_outputer_destroy(swm_outputer *outputer);
+int swm_latlon_write(swm_outputer *outputer, const void *f, const char *file_name, int rec_num);
+int swm_latlon_vec_write(swm_outputer *outputer, const void *u, const void *v, const char *file_name_u,
+                         const char *file_name_v, int rec_num);
+int swm_paneled_write(swm_ctx *ctx, const void *f, const char *file_name, int rec_num);
 
 /* Tile configuration of the fused stage kernel (no counterpart in the reference): tile height
  * 8, 16 or 32 rows (tiles of 64 x 8, 32 x 16 or 16 x 32 points, 512 threads; the height must be

@@ -68,6 +68,8 @@ def lib():
                                              _dp]
         L.ora_diff_step_narrow.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.ora_latlon_regrid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip,
+                                         _ip, _dp]
         L.ora_rk4_iomega.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double]
         _lib = L
     return _lib
@@ -228,6 +230,24 @@ class Grid:
         self._check_narrow(h_order)
         lib().ora_diff_step_narrow(self.h, uv_order, h_order, _p(h), _p(u), _p(v), float(dt),
                                    float(uv_coeff), float(h_coeff))
+
+    def latlon_regrid(self, Nlat, Nlon, f, fv=None, table=False):
+        """Cubic lat-lon regrid (Nlat, Nlon) of an Ah scalar f, or of covariant (f, fv) to (zonal,
+        meridional) components; with table=True also (panel, x0, y0, wx, wy) per target point."""
+        a = np.zeros((Nlat, Nlon))
+        b = np.zeros((Nlat, Nlon)) if fv is not None else None
+        ip = lambda x: x.ctypes.data_as(_ip)
+        tab = None
+        if table:
+            tab = (np.zeros((Nlat, Nlon), np.int32), np.zeros((Nlat, Nlon), np.int32),
+                   np.zeros((Nlat, Nlon), np.int32), np.zeros((Nlat, Nlon, 8)))
+        lib().ora_latlon_regrid(self.h, int(Nlat), int(Nlon), _p(f), _pn(fv), _p(a), _pn(b),
+                                ip(tab[0]) if tab else None, ip(tab[1]) if tab else None,
+                                ip(tab[2]) if tab else None, _p(tab[3]) if tab else None)
+        out = (a,) if fv is None else (a, b)
+        if table:
+            out = out + (tab[0], tab[1], tab[2], tab[3][..., :4], tab[3][..., 4:])
+        return out
 
     def diagnostics(self, quad_order, h, u, v, hs=None):
         out = np.zeros(7)

@@ -1130,6 +1130,100 @@ void ora_diff_step_narrow(const ora_grid *g, int uv_order, int h_order, double *
     free(ht); free(ut); free(vt);
 }
 
+/* ---- cubic lat-lon regrid of Ah fields: create_latlon_interp (regridders/regrid_factory_mod.f90:
+ * 183-301) with stencil bounds 1..n of the xy mesh, transform_cartesian_to_native_ecs
+ * (equiang_cs/ecs_metric_mod.f90:480-506), latlon_interpolate (regrid_to_latlon_mod.f90:138-160);
+ * vectors: covariant branch of create_latlon_vector_regrid (regrid_factory_mod.f90:159-176) and
+ * transform_components_to_latlon_uv (regrid_to_latlon_mod.f90:119-136).  out[(j-1)*Nlon + i-1]
+ * for lat index j, lon index i; fv == NULL: scalar f -> out_a; else (f, fv) = covariant (u, v)
+ * -> zonal out_a, meridional out_b.  panel/x0/y0/w (when not NULL) receive the target table:
+ * 0-based panel and first stencil point, weights wx[4], wy[4] per point. ---- */
+static void latlon_native(double lat, double lon, int *panel, double *alpha, double *beta, double r[3]) {
+    r[0] = cos(lat) * cos(lon);
+    r[1] = cos(lat) * sin(lon);
+    r[2] = sin(lat);
+    int pan = 0;
+    double dmax = 0.0;
+    for (int p = 0; p < 6; p++) {
+        /* minus sign: the topology stores inner normals */
+        double s = r[0] * NV[p][0];
+        s = s + r[1] * NV[p][1];
+        s = s + r[2] * NV[p][2];
+        double d = -s;
+        if (d > dmax) { pan = p; dmax = d; }
+    }
+    double sx = r[0] * EX[pan][0];
+    sx = sx + r[1] * EX[pan][1];
+    sx = sx + r[2] * EX[pan][2];
+    double sy = r[0] * EY[pan][0];
+    sy = sy + r[1] * EY[pan][1];
+    sy = sy + r[2] * EY[pan][2];
+    *panel = pan;
+    *alpha = atan(sx / dmax);
+    *beta = atan(sy / dmax);
+}
+
+static void cubic_weights(double z, double w[4]) {
+    w[0] = -((z - 1.0) * (z - 2.0) * (z - 3.0) / 6.0);
+    w[1] = z * (z - 2.0) * (z - 3.0) / 2.0;
+    w[2] = -(z * (z - 1.0) * (z - 3.0) / 2.0);
+    w[3] = z * (z - 1.0) * (z - 2.0) / 6.0;
+}
+
+static int stencil_start(double z, int n) {
+    int iright = (int)floor(z) + 4 / 2;
+    if (iright > n) iright = n;
+    int s = iright - 4 + 1;
+    return s < 1 ? 1 : s;
+}
+
+void ora_latlon_regrid(const ora_grid *g, int Nlat, int Nlon, const double *f, const double *fv, double *out_a,
+                       double *out_b, int *panel, int *x0, int *y0, double *w) {
+    const int n = g->n;
+    const double pi = pi_(), alpha0 = -0.25 * pi;
+    const double hlon = 2.0 * pi / (double)Nlon, hlat = pi / (Nlat - 1.0);
+    #pragma omp parallel for schedule(static)
+    for (int j = 1; j <= Nlat; j++) {
+        const double lat = -0.5 * pi + hlat * (j - 1);
+        for (int i = 1; i <= Nlon; i++) {
+            const double lon = hlon * (i - 1);
+            int pan;
+            double alpha, beta, r[3];
+            latlon_native(lat, lon, &pan, &alpha, &beta, r);
+            const double zdx = (alpha - alpha0) / g->hx + 1 - 0.0;
+            const double zdy = (beta - alpha0) / g->hx + 1 - 0.0;
+            const int xi = stencil_start(zdx, n), yi = stencil_start(zdy, n);
+            double wx[4], wy[4];
+            cubic_weights(zdx - xi, wx);
+            cubic_weights(zdy - yi, wy);
+            const long q = (long)(j - 1) * Nlon + (i - 1);
+            if (panel) { panel[q] = pan; x0[q] = xi - 1; y0[q] = yi - 1; }
+            if (w) for (int k = 0; k < 4; k++) { w[8 * q + k] = wx[k]; w[8 * q + 4 + k] = wy[k]; }
+            double za = 0.0, zb = 0.0;
+            for (int j1 = 1; j1 <= 4; j1++)
+                for (int i1 = 1; i1 <= 4; i1++) {
+                    long id = IDX(g, pan, xi + i1 - 1, yi + j1 - 1);
+                    za = za + wx[i1 - 1] * wy[j1 - 1] * f[id];
+                    if (fv) zb = zb + wx[i1 - 1] * wy[j1 - 1] * fv[id];
+                }
+            if (!fv) { out_a[q] = za; continue; }
+            const double iv[3] = {-sin(lon), cos(lon), 0.0};
+            const double jv[3] = {-sin(lat) * cos(lon), -sin(lat) * sin(lon), cos(lat)};
+            double v[3], b1[3], b2[3];
+            ecs_b1_proto(alpha, beta, v);
+            proto2real(pan, v, b1);
+            ecs_b2_proto(alpha, beta, v);
+            proto2real(pan, v, b2);
+            const double u2u = iv[0] * b1[0] + iv[1] * b1[1] + iv[2] * b1[2];
+            const double u2v = jv[0] * b1[0] + jv[1] * b1[1] + jv[2] * b1[2];
+            const double v2u = iv[0] * b2[0] + iv[1] * b2[1] + iv[2] * b2[2];
+            const double v2v = jv[0] * b2[0] + jv[1] * b2[1] + jv[2] * b2[2];
+            out_a[q] = u2u * za + v2u * zb;
+            out_b[q] = u2v * za + v2v * zb;
+        }
+    }
+}
+
 /* ---- quadrature: quadrature/quadrature_factory_mod.f90:39-86,158-177, sbp_quadrature_mod.f90:21-40 ---- */
 void ora_quad_weights(const ora_grid *g, int order, double *A) {
     int n = g->n, ne;

@@ -69,6 +69,11 @@ void ora_hordiff_vec_narrow(const ora_grid *g, int order, double hordiff_coeff, 
 void ora_diff_step_narrow(const ora_grid *g, int uv_order, int h_order, double *h, double *u, double *v,
                           double dt, double uv_diff_coeff, double h_diff_coeff);
 
+/* cubic lat-lon regrid of an Ah scalar (fv == NULL) or covariant vector (f, fv) onto Nlat x Nlon
+ * points, lon fastest; optional target table (panel, x0, y0: 0-based; w: wx[4], wy[4] per point) */
+void ora_latlon_regrid(const ora_grid *g, int Nlat, int Nlon, const double *f, const double *fv, double *out_a,
+                       double *out_b, int *panel, int *x0, int *y0, double *w);
+
 /* full RHS: out = RHS(in); in/out are [h,u,v] each a stored field; hs may be NULL */
 void ora_rhs(const ora_grid *g, double *in_h, double *in_u, double *in_v,
              const double *hs, double *dh, double *du, double *dv);

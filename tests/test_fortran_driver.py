@@ -41,7 +41,7 @@ NAMELIST = """
     h_diff_coeff      =  {hc},
     hordiff_h_name    = '{hn}',
     dt={dt},
-    tau_write = 86400.0,
+    tau_write = {twrite},
     tau_diagnostics = {tau}
     simulation_time_days  = 0.0,
     simulation_time_hours = 0.0,
@@ -49,7 +49,7 @@ NAMELIST = """
     simulation_time_sec   = {tsec},
 /
 &swm_gpu_run
-    seed = {seed}, lmax = 8, grav = 9.80616, output_file = '{out}'
+    seed = {seed}, lmax = 8, grav = 9.80616, output_file = '{out}', latlon_output = {llo}
 /
 """
 
@@ -60,14 +60,14 @@ def _need_driver():
 
 
 def _write_nml(tmp_path, N, order, dt, nsteps, seed, stag="Ah", uvc=0.0, hc=0.0,
-               uvn="hordiff_empty", hn="hordiff_empty"):
+               uvn="hordiff_empty", hn="hordiff_empty", twrite=86400.0, llo=".false."):
     import cubedsphere_sbp_swm_amd as swm
     names = swm.operator_names(order)
     out = tmp_path / "state.bin"
     text = NAMELIST.format(N=N, stag=stag, omega=swm.EARTH["omega"], div=names["div_op_name"],
                            grad=names["grad_op_name"], curl=names["curl_op_name"], quad=names["quadrature_name"],
                            dt=dt, tau=dt * 3, tsec=dt * nsteps, seed=seed, out=out, uvc=uvc, hc=hc, uvn=uvn,
-                           hn=hn)
+                           hn=hn, twrite=twrite, llo=llo)
     p = tmp_path / "namelist_swm"
     p.write_text(text)
     return p, out
@@ -145,6 +145,40 @@ def test_driver_with_hyperdiffusion_equals_python_path(tmp_path, order):
         op.diff_step(x, dt, 0.03, 0.01, uvn, hn)
     for a, b in zip((fh, fu, fv), x.download()):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_driver_latlon_output_equals_python_outputers(tmp_path):
+    """latlon_output = .true.: the driver writes h.dat, u.dat, v.dat as run_ts2 does (record 1 at
+    the start, then one every tau_write), equal byte for byte to the Python outputers on the same
+    states."""
+    _need_driver()
+    import cubedsphere_sbp_swm_amd as swm
+    N, order, nsteps, dt, seed = 16, 42, 4, 600.0, 2
+    nml, _ = _write_nml(tmp_path, N, order, dt, nsteps, seed, twrite=2 * dt, llo=".true.")
+    print(f"[fortran driver latlon output] N={N} steps={nsteps}", flush=True)
+    r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    op = swm.create_swm_operator(N, **swm.operator_names(order), scale=swm.EARTH["radius"],
+                                 omega=swm.EARTH["omega"], grav=swm.EARTH["grav"])
+    h, u, v = op.initial_state(seed=seed, lmax=8)
+    x = swm.create_stvec_swm(op)
+    x.upload(h, u, v)
+    ref = tmp_path / "ref"
+    ref.mkdir()
+    out = swm.create_latlon_outputer(op, 2 * N + 1, 4 * N, "Ah")
+    outv = swm.create_latlon_vec_outputer(op, 2 * N + 1, 4 * N, "Ah", "covariant")
+    ts = swm.create_timescheme(op, "rk4")
+    for rec in (1, 2, 3):
+        if rec > 1:
+            ts.steps(x, dt, 2)
+        out.write(x.field("h"), str(ref / "h.dat"), rec)
+        outv.write_vec(x, str(ref / "u.dat"), str(ref / "v.dat"), rec)
+    for name in ("h.dat", "u.dat", "v.dat"):
+        got, want = (tmp_path / name).read_bytes(), (ref / name).read_bytes()
+        assert len(got) == 3 * 4 * (2 * N + 1) * 4 * N
+        assert got == want, name
 
 
 def test_driver_rejects_other_hyperdiffusion(tmp_path):
