@@ -157,6 +157,18 @@ class OperatorSWM:
         check(_capi.lib().swm_ecs_halo_xy_host(self.ctx, field, int(halo_width), _dp(out)))
         return out
 
+    def ecs_halo_o(self, f_o, halo_width):
+        """ECS cubic halo on O points (the reference's "ECS_O" halo procedure) of a cell-centre
+        field f_o (6, N, N) [panel, j, i]: host array (6, N+2hw, N+2hw) with i, j from 1-hw to
+        N+hw; NaN at the corner-square points the procedure does not write (hw >= 3)."""
+        f_o = np.ascontiguousarray(f_o, dtype=np.float64)
+        if f_o.shape != (6, self.N, self.N):
+            raise ValueError(f"expected a (6, {self.N}, {self.N}) cell-centre field")
+        L = self.N + 2 * halo_width
+        out = np.zeros((6, L, L))
+        check(_capi.lib().swm_ecs_halo_o_host(self.ctx, _dp(f_o), int(halo_width), _dp(out)))
+        return out
+
     # transposed linearised right-hand side (adjoint form)
     def rhs_vjp(self, out, y, mu):
         """out = (dF/dy)^T mu at state y (FP64, SBP families 42/43/63)."""

@@ -594,6 +594,32 @@ extern "C" int swmh_edge_sync_matrix(int N, int vector, int *rows, int *cols, do
 // ECS cubic halo on xy points: interpolation weights of init_halo_xy_interp
 // (equiang_cs/ecs_halo_factory_mod.f90:228-269) for target indices i = 1-hw .. n+hw and halo
 // rows j = 1..hw: w[((j-1)*(n+2hw) + i-1+hw)*4 + s+1], s = -1..2, base index ind (1-based).
+// init_halo_interp (ecs_halo_factory_mod.f90:193-226): target O point (t, r) of the halo row r
+// beyond a panel edge at (alpha, beta) = (-pi/4 + (t-1/2) hx, pi/4 + (r-1/2) hx), projected on
+// the neighbour panel, 4-point Lagrange weights about the source cell to its left
+extern "C" void swmh_ecs_o_weights(int N, int rows, double *w, int *ind) {
+    const double pi = pi_(), dxa = grid_hx(N);
+    const int T = N + 4;
+    for (int j = 1; j <= rows; ++j) {
+        const double zb = 0.25 * pi + ((double)j - 0.5) * dxa;
+        for (int i = -1; i <= N + 2; ++i) {
+            double za = -0.25 * pi + ((double)i - 0.5) * dxa;
+            double zx = std::tan(za);
+            const double zz = std::tan(zb);
+            zx = zx / zz;
+            za = std::atan(zx);
+            const int id = (int)std::floor((za + 0.25 * pi - 0.5 * dxa) / dxa) + 1;
+            const double zxi = (za + 0.25 * pi) / dxa - ((double)id - 0.5);
+            const int o = (j - 1) * T + (i + 1);
+            ind[o] = id;
+            w[4 * o + 0] = -zxi * (zxi - 1.0) * (zxi - 2.0) / 6.0;
+            w[4 * o + 1] = (zxi + 1.0) * (zxi - 1.0) * (zxi - 2.0) / 2.0;
+            w[4 * o + 2] = -(zxi + 1.0) * zxi * (zxi - 2.0) / 2.0;
+            w[4 * o + 3] = (zxi + 1.0) * zxi * (zxi - 1.0) / 6.0;
+        }
+    }
+}
+
 extern "C" int swmh_ecs_xy_weights(int N, int hw, double *w, int *ind) {
     const int n = N + 1, L = n + 2 * hw;
     if (hw < 1 || n < 4) return -1;

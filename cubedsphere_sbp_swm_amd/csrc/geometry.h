@@ -49,6 +49,11 @@ int swmh_edge_sync_matrix(int N, int vector, int *rows, int *cols, double *w, in
 // j = 1..hw (init_halo_xy_interp, ecs_halo_factory_mod.f90:228-269).  Returns 0 on success.
 int swmh_ecs_xy_weights(int N, int hw, double *w, int *ind);
 
+// ECS cubic halo on O points (cell centres): weights w[((r-1)*(N+4) + t+1)*4 + s+1] (s = -1..2)
+// and 1-based stencil base indices ind[(r-1)*(N+4) + t+1] for positions t = -1..N+2 along an
+// edge and halo rows r = 1..rows (init_halo_interp, ecs_halo_factory_mod.f90:193-226).
+void swmh_ecs_o_weights(int N, int rows, double *w, int *ind);
+
 // Cubic lat-lon regrid of Ah fields (create_latlon_interp, regridders/regrid_factory_mod.f90:
 // 183-301, scalar_grid_type "Ah", interp_type "cubic"; covariant components as in
 // create_latlon_vector_regrid :80-181).  Target q = (j-1)*Nlon + (i-1), lat = -pi/2 +

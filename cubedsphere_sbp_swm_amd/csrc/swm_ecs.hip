@@ -1,4 +1,4 @@
-// ECS cubic halo on xy (Ah) points (included into swm_capi.hip).
+// ECS cubic halo on xy (Ah) points and on O points (included into swm_capi.hip).
 //
 // get_ecs_halo / ext_ecs_tile_halo (equiang_cs/ecs_halo_xy_mod.f90:68-219) for a field of the
 // context: the panel is extended by `hw` rows of virtual points beyond each edge.  The
@@ -97,6 +97,142 @@ __global__ void ecs_fill(DevGeo<T> g, const T *__restrict__ f, int hw, const T *
     out[t] = v;
 }
 
+// ---- ECS cubic halo on O points (cell centres of the A grid) ----
+// get_ecs_halo / ext_ecs_tile_halo (equiang_cs/ecs_halo_mod.f90:71-166) after the width-8 'full'
+// O-point exchange (parallel/exchange_factory_mod.f90:13-28,45-155), which fills the strip beyond
+// each panel edge with the edge neighbour's cells and leaves the halo corner squares alone.  Every
+// value the procedure writes is a function of the panel and those strips only:
+//  - halo row r of an edge at position t along it: 4-point Lagrange interpolation along the
+//    neighbour's row at depth r-1 (ecs_ext_halo_1e, :451-475), positions 2..N-1 in row 1 and
+//    1..N beyond;
+//  - the special points at each cube corner (ecs_halo_1corner, :215-285, placed by
+//    ecs_halo_corners_put, :287-334): row 1 at positions 1 and N, the corner-square diagonal and,
+//    for hw >= 2, the rest of the 2 x 2 block next to the corner.
+// One thread computes one output point from f directly.  Points of the corner squares the
+// procedure does not write (only for hw >= 3) are set to NaN.
+template <typename T>
+struct EcsO {
+    const T *f;        // [6][N][N], i fastest
+    const T *w;        // [rows][N+4][4]: positions -1..N+2, stencil offsets -1..2
+    const int *ind;    // [rows][N+4]: 1-based stencil base
+    const int *edge;   // [6][4][4]: neighbour panel, normal axis is x, far side, orientation
+    int N, hw;
+
+    __device__ T at(int p, int i, int j) const { return f[((long long)p * N + j - 1) * N + i - 1]; }
+    // exchanged value beyond edge e (0 W, 1 E, 2 S, 3 N) at depth d (0 = first cell row of the
+    // neighbour), position t = 1..N along the edge
+    __device__ T raw(int p, int e, int d, int t) const {
+        const int *ed = edge + (p * 4 + e) * 4;
+        const int nidx = ed[2] == 0 ? d : N - 1 - d, tidx = ed[3] > 0 ? t - 1 : N - t;
+        const int qi = ed[1] ? nidx : tidx, qj = ed[1] ? tidx : nidx;
+        return f[((long long)ed[0] * N + qj) * N + qi];
+    }
+    // the exchanged array pf(I, J) where one of I, J lies in 1..N
+    __device__ T pf(int p, int I, int J) const {
+        if (I >= 1 && I <= N && J >= 1 && J <= N) return at(p, I, J);
+        if (J < 1) return raw(p, 2, -J, I);
+        if (J > N) return raw(p, 3, J - N - 1, I);
+        if (I < 1) return raw(p, 0, -I, J);
+        return raw(p, 1, I - N - 1, J);
+    }
+    __device__ T W(int s, int t, int r) const { return w[((r - 1) * (N + 4) + t + 1) * 4 + s + 1]; }
+};
+
+// special value k (0..5) of corner c (0..3: (1,1), (N,1), (N,N), (1,N) sides), ecs_halo_1corner
+template <typename T>
+__device__ T ecs_o_special(const EcsO<T> &o, int p, int c, int k) {
+    const int N = o.N;
+    const int ist = (c == 1 || c == 2) ? N + 1 : 0, idir = (c == 1 || c == 2) ? -1 : 1;
+    const int jst = c >= 2 ? N + 1 : 0, jdir = c >= 2 ? -1 : 1;
+    auto FC = [&](int a, int b) { return o.pf(p, ist + idir * a, jst + jdir * b); };
+    // weights of corner 1; mirrored, w(2:-1:-1, N+2:N-1:-1, :), where the corner is on the right
+    auto PW = [&](int s, int t, int r) { return idir == 1 ? o.W(s, t, r) : o.W(1 - s, N + 1 - t, r); };
+    const T z0 = PW(0, 1, 1), z1 = PW(1, 1, 1), z2 = PW(2, 1, 1);
+    const T zw = PW(-1, 1, 1), zw2 = zw * zw, zw3 = zw2 * zw;
+    if (k == 0 || k == 2) {
+        const T zx = z0 * FC(0, 1) + z1 * FC(0, 2) + z2 * FC(0, 3);
+        const T zy = z0 * FC(1, 0) + z1 * FC(1, -1) + z2 * FC(1, -2);
+        const T zz = z0 * FC(1, 1) + z1 * FC(2, 1) + z2 * FC(3, 1);
+        if (k == 0) return (zx + zw * zy + zw2 * zz) / (T(1) - zw3);
+        const T zfy1 = (zy + zw * zz + zw2 * zx) / (T(1) - zw3);
+        const T xx = z0 * FC(1, 0) + z1 * FC(2, 0) + z2 * FC(3, 0);
+        const T yy = z0 * FC(0, 1) + z1 * FC(-1, 1) + z2 * FC(-2, 1);
+        const T zz2 = z0 * FC(1, 1) + z1 * FC(1, 2) + z2 * FC(1, 3);
+        const T zfy2 = (yy + zw * zz2 + zw2 * xx) / (T(1) - zw3);
+        const T zff1 = PW(-1, 1, 2) * FC(2, 0) + PW(0, 1, 2) * FC(2, -1) + PW(1, 1, 2) * FC(2, -2) + PW(2, 1, 2) * FC(2, -3);
+        const T zff2 = PW(-1, 1, 2) * FC(0, 2) + PW(0, 1, 2) * FC(-1, 2) + PW(1, 1, 2) * FC(-2, 2) + PW(2, 1, 2) * FC(-3, 2);
+        return T(0.5) * (PW(-1, 0, 1) * zff1 + PW(0, 0, 1) * zfy1 + PW(1, 0, 1) * FC(0, 1) + PW(2, 0, 1) * FC(0, 2) +
+                         PW(-1, 0, 1) * zff2 + PW(0, 0, 1) * zfy2 + PW(1, 0, 1) * FC(1, 0) + PW(2, 0, 1) * FC(2, 0));
+    }
+    if (k == 1) {
+        const T zx = z0 * FC(1, 0) + z1 * FC(2, 0) + z2 * FC(3, 0);
+        const T zy = z0 * FC(0, 1) + z1 * FC(-1, 1) + z2 * FC(-2, 1);
+        const T zz = z0 * FC(1, 1) + z1 * FC(1, 2) + z2 * FC(1, 3);
+        return (zx + zw * zy + zw2 * zz) / (T(1) - zw3);
+    }
+    const T zff1 = PW(-1, 2, 1) * FC(1, 0) + PW(0, 2, 1) * FC(1, -1) + PW(1, 2, 1) * FC(1, -2) + PW(2, 2, 1) * FC(1, -3);
+    const T zff2 = PW(-1, 2, 1) * FC(0, 1) + PW(0, 2, 1) * FC(-1, 1) + PW(1, 2, 1) * FC(-2, 1) + PW(2, 2, 1) * FC(-3, 1);
+    if (k == 3) return PW(-1, 0, 2) * zff1 + PW(0, 0, 2) * FC(-1, 1) + PW(1, 0, 2) * FC(-1, 2) + PW(2, 0, 2) * FC(-1, 3);
+    if (k == 4) return PW(-1, 0, 2) * zff2 + PW(0, 0, 2) * FC(1, -1) + PW(1, 0, 2) * FC(2, -1) + PW(2, 0, 2) * FC(3, -1);
+    const T zff11 = PW(-1, 2, 2) * FC(2, -1) + PW(0, 2, 2) * FC(2, -2) + PW(1, 2, 2) * FC(2, -3) + PW(2, 2, 2) * FC(2, -4);
+    const T zff22 = PW(-1, 2, 2) * FC(-1, 2) + PW(0, 2, 2) * FC(-2, 2) + PW(1, 2, 2) * FC(-3, 2) + PW(2, 2, 2) * FC(-4, 2);
+    return T(0.5) * (PW(-1, -1, 2) * zff11 + PW(0, -1, 2) * zff1 + PW(1, -1, 2) * FC(-1, 1) + PW(2, -1, 2) * FC(-1, 2) +
+                     PW(-1, -1, 2) * zff22 + PW(0, -1, 2) * zff2 + PW(1, -1, 2) * FC(1, -1) + PW(2, -1, 2) * FC(2, -1));
+}
+
+// out[(p*L + jj)*L + ii], L = N+2hw, for i, j = ii+1-hw, jj+1-hw
+template <typename T>
+__global__ void ecs_o_fill(EcsO<T> o, T *__restrict__ out) {
+    const int N = o.N, hw = o.hw, L = N + 2 * hw;
+    const long long total = 6LL * L * L;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int ii = (int)(t % L), jj = (int)((t / L) % L), p = (int)(t / ((long long)L * L));
+    const int i = ii + 1 - hw, j = jj + 1 - hw;
+    const bool in_i = i >= 1 && i <= N, in_j = j >= 1 && j <= N;
+    // corner and special index of a point next to a cube corner (-1 if not special)
+    auto special = [&](int &c, int &k) {
+        const bool left = i <= 1, bottom = j <= 1;
+        c = bottom ? (left ? 0 : 1) : (left ? 3 : 2);
+        const int ist = left ? 0 : N + 1, idir = left ? 1 : -1, jst = bottom ? 0 : N + 1, jdir = bottom ? 1 : -1;
+        const int ci = idir * (i - ist), cj = jdir * (j - jst);
+        k = -1;
+        if (ci == 0 && cj == 1) k = 0;
+        else if (ci == 1 && cj == 0) k = 1;
+        else if (ci == 0 && cj == 0) k = 2;
+        else if (hw > 1 && ci == -1 && cj == 0) k = 3;
+        else if (hw > 1 && ci == 0 && cj == -1) k = 4;
+        else if (hw > 1 && ci == -1 && cj == -1) k = 5;
+    };
+    T v;
+    if (in_i && in_j) {
+        v = o.at(p, i, j);
+    } else if (in_i || in_j) {
+        int e, r, pos;
+        if (j < 1) { e = 2; r = 1 - j; pos = i; }
+        else if (j > N) { e = 3; r = j - N; pos = i; }
+        else if (i < 1) { e = 0; r = 1 - i; pos = j; }
+        else { e = 1; r = i - N; pos = j; }
+        if (r == 1 && (pos == 1 || pos == N)) {
+            int c, k;
+            special(c, k);
+            v = ecs_o_special(o, p, c, k);
+        } else {
+            const int b = o.ind[(r - 1) * (N + 4) + pos + 1];
+            T s = o.W(-1, pos, r) * o.raw(p, e, r - 1, b - 1);
+            s = s + o.W(0, pos, r) * o.raw(p, e, r - 1, b);
+            s = s + o.W(1, pos, r) * o.raw(p, e, r - 1, b + 1);
+            s = s + o.W(2, pos, r) * o.raw(p, e, r - 1, b + 2);
+            v = s;
+        }
+    } else {
+        int c, k;
+        special(c, k);
+        v = k >= 0 ? ecs_o_special(o, p, c, k) : T(__builtin_nan(""));
+    }
+    out[t] = v;
+}
+
 }  // namespace swm
 
 namespace {
@@ -160,4 +296,102 @@ extern "C" int swm_ecs_halo_xy_host(swm_ctx *c, const void *field, int halo_widt
     }
     HIP_TRY(hipFree(dev));
     return SWM_OK;
+}
+
+namespace {
+
+template <typename T>
+int ecs_o_run(swm_ctx *c, const void *field, int hw, void *out) {
+    const int N = c->N, rows = hw > 2 ? hw : 2, TT = N + 4;
+    if (c->ecs_o_rows < rows) {
+        std::vector<double> w((size_t)4 * rows * TT);
+        std::vector<int> ind((size_t)rows * TT);
+        swmh_ecs_o_weights(N, rows, w.data(), ind.data());
+        // the positions the procedure interpolates take their stencils inside the panel
+        for (int r = 1; r <= rows; ++r)
+            for (int t = r == 1 ? 2 : 1; t <= (r == 1 ? N - 1 : N); ++t) {
+                const int b = ind[(size_t)(r - 1) * TT + t + 1];
+                if (b - 1 < 1 || b + 2 > N) return fail(SWM_EARG, "ECS O-point halo stencil leaves the panel");
+            }
+        std::vector<T> wt(w.begin(), w.end());
+        if (c->ecs_o_w) { HIP_TRY(hipFree(c->ecs_o_w)); c->ecs_o_w = nullptr; }
+        if (c->ecs_o_ind) { HIP_TRY(hipFree(c->ecs_o_ind)); c->ecs_o_ind = nullptr; }
+        HIP_TRY(hipMalloc(&c->ecs_o_w, wt.size() * sizeof(T)));
+        HIP_TRY(hipMalloc(&c->ecs_o_ind, ind.size() * sizeof(int)));
+        HIP_TRY(hipMemcpy(c->ecs_o_w, wt.data(), wt.size() * sizeof(T), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->ecs_o_ind, ind.data(), ind.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!c->ecs_o_edge) {
+            // O-point edge map from the Ah edge descriptors: cell k between Ah points k and k+1
+            // maps to cell k (orientation +1) or N-1-k (-1); depth d to d or N-1-d
+            std::vector<int> ei((size_t)6 * 4 * 8), eo((size_t)6 * 4 * 4);
+            HIP_TRY(hipMemcpy(ei.data(), c->einfo, ei.size() * sizeof(int), hipMemcpyDeviceToHost));
+            for (int k = 0; k < 24; ++k) {
+                eo[4 * k + 0] = ei[8 * k + 0];
+                eo[4 * k + 1] = ei[8 * k + 1];
+                eo[4 * k + 2] = ei[8 * k + 2] == 0 ? 0 : 1;
+                eo[4 * k + 3] = ei[8 * k + 4];
+            }
+            HIP_TRY(hipMalloc(&c->ecs_o_edge, eo.size() * sizeof(int)));
+            HIP_TRY(hipMemcpy(c->ecs_o_edge, eo.data(), eo.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        HIP_TRY(hipDeviceSynchronize());  // null-stream copies before the context stream reads them
+        c->ecs_o_rows = rows;
+    }
+    const swm::EcsO<T> o{(const T *)field, (const T *)c->ecs_o_w, (const int *)c->ecs_o_ind,
+                         (const int *)c->ecs_o_edge, N, hw};
+    const long long total = 6LL * (N + 2 * hw) * (N + 2 * hw);
+    hipLaunchKernelGGL(swm::ecs_o_fill<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, o, (T *)out);
+    HIP_TRY(hipGetLastError());
+    return SWM_OK;
+}
+
+int ecs_o_check(swm_ctx *c, int hw) {
+    if (hw < 1 || hw > 8) return fail(SWM_EARG, "halo_width must be in [1, 8]");
+    if (c->N < 8) return fail(SWM_EARG, "the ECS O-point halo exchanges 8 cell rows: N >= 8 required");
+    return SWM_OK;
+}
+
+}  // namespace
+
+extern "C" int swm_ecs_halo_o(swm_ctx *c, const void *field_o, int halo_width, void *out) {
+    if (!c || !field_o || !out) return fail(SWM_EARG, "NULL argument");
+    int rc = ecs_o_check(c, halo_width);
+    if (rc) return rc;
+    return c->prec == 64 ? ecs_o_run<double>(c, field_o, halo_width, out) : ecs_o_run<float>(c, field_o, halo_width, out);
+}
+
+extern "C" int swm_ecs_halo_o_host(swm_ctx *c, const double *field_o, int halo_width, double *out) {
+    if (!c || !field_o || !out) return fail(SWM_EARG, "NULL argument");
+    int rc = ecs_o_check(c, halo_width);
+    if (rc) return rc;
+    const size_t N = (size_t)c->N, L = N + 2 * halo_width, nin = 6 * N * N, nout = 6 * L * L, es = c->esz();
+    void *din = nullptr, *dout = nullptr;
+    HIP_TRY(hipMalloc(&din, nin * es));
+    if (hipMalloc(&dout, nout * es) != hipSuccess) {
+        (void)hipFree(din);
+        return fail(SWM_EHIP, "hipMalloc failed");
+    }
+    auto done = [&](int r) {
+        (void)hipFree(din);
+        (void)hipFree(dout);
+        return r;
+    };
+    if (c->prec == 64) {
+        if (hipMemcpyAsync(din, field_o, nin * es, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return done(fail(SWM_EHIP, "upload failed"));
+        if ((rc = swm_ecs_halo_o(c, din, halo_width, dout))) return done(rc);
+        if (hipMemcpyAsync(out, dout, nout * es, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return done(fail(SWM_EHIP, "download failed"));
+    } else {
+        std::vector<float> bin(field_o, field_o + nin), bout(nout);
+        if (hipMemcpyAsync(din, bin.data(), nin * es, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return done(fail(SWM_EHIP, "upload failed"));
+        if ((rc = swm_ecs_halo_o(c, din, halo_width, dout))) return done(rc);
+        if (hipMemcpyAsync(bout.data(), dout, nout * es, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return done(fail(SWM_EHIP, "download failed"));
+        for (size_t k = 0; k < nout; ++k) out[k] = bout[k];
+    }
+    return done(SWM_OK);
 }

@@ -223,6 +223,19 @@ int swm_ens_bench_stage(swm_ens *ens, int reps);
 int swm_ecs_halo_xy(swm_ctx *ctx, const void *field, int halo_width, void *out);
 int swm_ecs_halo_xy_host(swm_ctx *ctx, const void *field, int halo_width, double *out);
 
+/* ECS cubic halo on O points (A-grid cell centres): ecs_halo_t%get_halo_scalar(f, domain,
+ * halo_width) of the halo procedure create_halo_procedure(halo, domain, halo_width, "ECS_O")
+ * (equiang_cs/ecs_halo_mod.f90:71-166, ecs_halo_factory_mod.f90:24-44,116-226), including its
+ * width-8 'full' O-point exchange.  field_o: N x N x 6 cell values of the context precision
+ * (i fastest, then j, then panel); out: (N+2hw) x (N+2hw) x 6, i, j from 1-hw to N+hw.
+ * 1 <= halo_width <= 8, N >= 8.  Edge strips and the cube-corner special points are computed as
+ * the reference computes them (weights as for a halo object created with width max(hw, 2); the
+ * corner step reads the second row).  The other points of the halo corner squares, which the
+ * reference leaves as they were (only for hw >= 3), are NaN.  swm_ecs_halo_o takes and writes
+ * device memory; swm_ecs_halo_o_host FP64 host memory. */
+int swm_ecs_halo_o(swm_ctx *ctx, const void *field_o, int halo_width, void *out);
+int swm_ecs_halo_o_host(swm_ctx *ctx, const double *field_o, int halo_width, double *out);
+
 /* Device-to-device copies between a state vector and three dense device arrays of shape
  * (6, n, n) (panel, j, i) in the context precision (e.g. the storage of framework tensors); no
  * counterpart in the reference. */

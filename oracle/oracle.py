@@ -70,6 +70,9 @@ def lib():
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.ora_latlon_regrid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip,
                                          _ip, _dp]
+        L.ora_ecs_o_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip]
+        L.ora_ecs_halo_o.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
+        L.ora_ecs_halo_o.restype = ctypes.c_int
         L.ora_rk4_iomega.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double]
         _lib = L
     return _lib
@@ -230,6 +233,25 @@ class Grid:
         self._check_narrow(h_order)
         lib().ora_diff_step_narrow(self.h, uv_order, h_order, _p(h), _p(u), _p(v), float(dt),
                                    float(uv_coeff), float(h_coeff))
+
+    def ecs_o_weights(self, rows):
+        """init_halo_interp weights (rows, N+4, 4) for positions -1..N+2 and stencil base indices."""
+        T = self.N + 4
+        w = np.zeros((rows, T, 4))
+        ind = np.zeros((rows, T), dtype=np.int32)
+        lib().ora_ecs_o_weights(self.h, int(rows), _p(w), ind.ctypes.data_as(_ip))
+        return w, ind
+
+    def ecs_halo_o(self, f, hw):
+        """ECS_O halo of a cell-centre field f (6, N, N) -> (6, N+2hw, N+2hw); NaN where the
+        procedure writes nothing."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == (6, self.N, self.N)
+        L = self.N + 2 * hw
+        out = np.zeros((6, L, L))
+        if lib().ora_ecs_halo_o(self.h, _p(f), int(hw), _p(out)) != 0:
+            raise ValueError("halo width out of range (1..8, N >= 8)")
+        return out
 
     def latlon_regrid(self, Nlat, Nlon, f, fv=None, table=False):
         """Cubic lat-lon regrid (Nlat, Nlon) of an Ah scalar f, or of covariant (f, fv) to (zonal,

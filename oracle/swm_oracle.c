@@ -1130,6 +1130,180 @@ void ora_diff_step_narrow(const ora_grid *g, int uv_order, int h_order, double *
     free(ht); free(ut); free(vt);
 }
 
+/* ---- ECS cubic halo on O points (cell centres of the A grid), ECS_O ----
+ * Fields: f[(p*N + j-1)*N + i-1], i, j = 1..N; output window out[(p*L + jj)*L + ii], L = N+2hw,
+ * jj, ii = 0..L-1 for j, i = jj+1-hw, ii+1-hw.
+ * Weights: init_halo_interp (equiang_cs/ecs_halo_factory_mod.f90:193-226) for positions
+ * t = -1..N+2 and rows 1..max(hw,2): w[((r-1)*(N+4) + t+1)*4 + s+1], ind[(r-1)*(N+4) + t+1].
+ * Exchange: create_o_points_halo_exchange(..., 8, 'full') (parallel/exchange_factory_mod.f90:
+ * 13-28,45-155,618-660) with one tile per panel copies each edge neighbour's cells into the strip
+ * beyond the shared edge (rows 1..8, along the edge only); restated here by folding doubled
+ * lattice coordinates over the cube edge.  The halo corner squares are not filled by it.
+ * Procedure: ext_ecs_tile_halo (equiang_cs/ecs_halo_mod.f90:98-166) on that array in the
+ * reference's order: corner special points from the exchanged values (ecs_halo_corners,
+ * ecs_halo_1corner :168-285), edges along x then y (ecs_halo_edges_x/_y, ecs_ext_halo_1e
+ * :336-475), then the special points (ecs_halo_corners_put :287-334).  Corner-square points the
+ * procedure does not write (hw >= 3) are returned as NaN. ---- */
+void ora_ecs_o_weights(const ora_grid *g, int rows, double *w, int *ind) {
+    const double pi = pi_();
+    const int N = g->N, T = N + 4;
+    const double dxa = g->hx;
+    for (int j = 1; j <= rows; j++) {
+        double zb = 0.25 * pi + (j - 0.5) * dxa;
+        for (int i = -1; i <= N + 2; i++) {
+            double za = -0.25 * pi + (i - 0.5) * dxa;
+            double zx = tan(za);
+            double zz = tan(zb);
+            zx = zx / zz;
+            za = atan(zx);
+            int id = (int)floor((za + 0.25 * pi - 0.5 * dxa) / dxa) + 1;
+            double zxi = (za + 0.25 * pi) / dxa - (id - 0.5);
+            int o = (j - 1) * T + (i + 1);
+            ind[o] = id;
+            w[4 * o + 0] = -zxi * (zxi - 1.0) * (zxi - 2.0) / 6.0;
+            w[4 * o + 1] = (zxi + 1.0) * (zxi - 1.0) * (zxi - 2.0) / 2.0;
+            w[4 * o + 2] = -(zxi + 1.0) * zxi * (zxi - 2.0) / 2.0;
+            w[4 * o + 3] = (zxi + 1.0) * zxi * (zxi - 1.0) / 6.0;
+        }
+    }
+}
+
+/* exchanged value at O index (I, J) of panel p beyond one edge: the cell of the panel whose face
+ * holds the point folded over that edge (doubled lattice units, cube [0, 2N]^3) */
+static double o_fold(const ora_grid *g, const double *f, int p, int oi, int oj) {
+    const int N = g->N;
+    int P[3];
+    for (int k = 0; k < 3; k++) {
+        int x = 2 * oi - 1, y = 2 * oj - 1, d = 0;
+        if (oi < 1) { d = -x; x = 0; }
+        else if (oi > N) { d = x - 2 * N; x = 2 * N; }
+        else if (oj < 1) { d = -y; y = 0; }
+        else { d = y - 2 * N; y = 2 * N; }
+        P[k] = x * EX[p][k] + y * EY[p][k] + 2 * N * RV[p][k] + d * NV[p][k];
+    }
+    for (int q = 0; q < 6; q++) {
+        if (q == p) continue;
+        int D[3];
+        for (int k = 0; k < 3; k++) D[k] = P[k] - 2 * N * RV[q][k];
+        if (D[0] * NV[q][0] + D[1] * NV[q][1] + D[2] * NV[q][2] != 0) continue;
+        int a = D[0] * EX[q][0] + D[1] * EX[q][1] + D[2] * EX[q][2];
+        int b = D[0] * EY[q][0] + D[1] * EY[q][1] + D[2] * EY[q][2];
+        if (a <= 0 || a >= 2 * N || b <= 0 || b >= 2 * N) continue;
+        return f[((long)q * N + (b - 1) / 2) * N + (a - 1) / 2];
+    }
+    abort();
+}
+
+int ora_ecs_halo_o(const ora_grid *g, const double *f, int hw, double *out) {
+    const int N = g->N, E = 8, LE = N + 2 * E, L = N + 2 * hw, T = N + 4;
+    if (hw < 1 || hw > E || N < E) return -1;
+    const int rows = hw > 2 ? hw : 2;
+    double *w = malloc(sizeof(double) * 4 * rows * T);
+    int *ind = malloc(sizeof(int) * rows * T);
+    double *pf = malloc(sizeof(double) * LE * LE);
+    ora_ecs_o_weights(g, rows, w, ind);
+#define W(s, t, r) w[(((r) - 1) * T + (t) + 1) * 4 + (s) + 1]
+#define IND(t, r) ind[((r) - 1) * T + (t) + 1]
+#define PF(i, j) pf[((j) - 1 + E) * LE + (i) - 1 + E]
+    for (int p = 0; p < 6; p++) {
+        /* exchange of width 8, corner squares left unfilled (NaN) */
+        for (int j = 1 - E; j <= N + E; j++)
+            for (int i = 1 - E; i <= N + E; i++) {
+                int ii = i >= 1 && i <= N, jj = j >= 1 && j <= N;
+                if (ii && jj) PF(i, j) = f[((long)p * N + j - 1) * N + i - 1];
+                else if (ii || jj) PF(i, j) = o_fold(g, f, p, i, j);
+                else PF(i, j) = NAN;
+            }
+        /* corner special points from the exchanged values */
+        double csp[4][6];
+        const int ist[4] = {0, N + 1, N + 1, 0}, idir[4] = {1, -1, -1, 1};
+        const int jst[4] = {0, 0, N + 1, N + 1}, jdir[4] = {1, 1, -1, -1};
+        for (int c = 0; c < 4; c++) {
+#define FC(i, j) PF(ist[c] + idir[c] * (i), jst[c] + jdir[c] * (j))
+            /* weights of corner 1; mirrored (w(2:-1:-1, N+2:N-1:-1, :)) where idir = -1 */
+#define PW(s, t, r) (idir[c] == 1 ? W((s), (t), (r)) : W(1 - (s), N + 1 - (t), (r)))
+            double zpw[4];
+            for (int s2 = -1; s2 <= 2; s2++) zpw[s2 + 1] = PW(s2, 1, 1);
+            double zw = PW(-1, 1, 1), zw2 = zw * zw, zw3 = zw2 * zw;
+            double zx, zy, zz, zfy1, zfy2, zff1, zff2, zff11, zff22;
+            zx = zpw[1] * FC(0, 1) + zpw[2] * FC(0, 2) + zpw[3] * FC(0, 3);
+            zy = zpw[1] * FC(1, 0) + zpw[2] * FC(1, -1) + zpw[3] * FC(1, -2);
+            zz = zpw[1] * FC(1, 1) + zpw[2] * FC(2, 1) + zpw[3] * FC(3, 1);
+            csp[c][0] = (zx + zw * zy + zw2 * zz) / (1 - zw3);
+            zfy1 = (zy + zw * zz + zw2 * zx) / (1 - zw3);
+            zx = zpw[1] * FC(1, 0) + zpw[2] * FC(2, 0) + zpw[3] * FC(3, 0);
+            zy = zpw[1] * FC(0, 1) + zpw[2] * FC(-1, 1) + zpw[3] * FC(-2, 1);
+            zz = zpw[1] * FC(1, 1) + zpw[2] * FC(1, 2) + zpw[3] * FC(1, 3);
+            csp[c][1] = (zx + zw * zy + zw2 * zz) / (1 - zw3);
+            zfy2 = (zy + zw * zz + zw2 * zx) / (1 - zw3);
+            zff1 = PW(-1, 1, 2) * FC(2, 0) + PW(0, 1, 2) * FC(2, -1) + PW(1, 1, 2) * FC(2, -2) + PW(2, 1, 2) * FC(2, -3);
+            zff2 = PW(-1, 1, 2) * FC(0, 2) + PW(0, 1, 2) * FC(-1, 2) + PW(1, 1, 2) * FC(-2, 2) + PW(2, 1, 2) * FC(-3, 2);
+            csp[c][2] = 0.5 * (PW(-1, 0, 1) * zff1 + PW(0, 0, 1) * zfy1 + PW(1, 0, 1) * FC(0, 1) + PW(2, 0, 1) * FC(0, 2) +
+                               PW(-1, 0, 1) * zff2 + PW(0, 0, 1) * zfy2 + PW(1, 0, 1) * FC(1, 0) + PW(2, 0, 1) * FC(2, 0));
+            if (hw > 1) {
+                zff1 = PW(-1, 2, 1) * FC(1, 0) + PW(0, 2, 1) * FC(1, -1) + PW(1, 2, 1) * FC(1, -2) + PW(2, 2, 1) * FC(1, -3);
+                zff11 = PW(-1, 2, 2) * FC(2, -1) + PW(0, 2, 2) * FC(2, -2) + PW(1, 2, 2) * FC(2, -3) + PW(2, 2, 2) * FC(2, -4);
+                zff2 = PW(-1, 2, 1) * FC(0, 1) + PW(0, 2, 1) * FC(-1, 1) + PW(1, 2, 1) * FC(-2, 1) + PW(2, 2, 1) * FC(-3, 1);
+                zff22 = PW(-1, 2, 2) * FC(-1, 2) + PW(0, 2, 2) * FC(-2, 2) + PW(1, 2, 2) * FC(-3, 2) + PW(2, 2, 2) * FC(-4, 2);
+                csp[c][3] = PW(-1, 0, 2) * zff1 + PW(0, 0, 2) * FC(-1, 1) + PW(1, 0, 2) * FC(-1, 2) + PW(2, 0, 2) * FC(-1, 3);
+                csp[c][4] = PW(-1, 0, 2) * zff2 + PW(0, 0, 2) * FC(1, -1) + PW(1, 0, 2) * FC(2, -1) + PW(2, 0, 2) * FC(3, -1);
+                csp[c][5] = 0.5 * (PW(-1, -1, 2) * zff11 + PW(0, -1, 2) * zff1 + PW(1, -1, 2) * FC(-1, 1) + PW(2, -1, 2) * FC(-1, 2) +
+                                   PW(-1, -1, 2) * zff22 + PW(0, -1, 2) * zff2 + PW(1, -1, 2) * FC(1, -1) + PW(2, -1, 2) * FC(2, -1));
+            }
+#undef PW
+#undef FC
+        }
+        /* edges along x (rows below j = 1, above j = N), then along y: ecs_ext_halo_1e on the
+         * exchanged strip, positions 2..N-1 in the first row, 1..N further out */
+        double zh[4096 + 16];
+        for (int e = 0; e < 4; e++)
+            for (int r = 1; r <= hw; r++) {
+                const int t0 = r == 1 ? 2 : 1, t1 = r == 1 ? N - 1 : N;
+#define STRIP(t) (e == 0 ? PF((t), 1 - r) : e == 1 ? PF((t), N + r) : e == 2 ? PF(1 - r, (t)) : PF(N + r, (t)))
+                for (int t = t0; t <= t1; t++) {
+                    int ii = IND(t, r);
+                    double sacc = W(-1, t, r) * STRIP(ii - 1);
+                    sacc = sacc + W(0, t, r) * STRIP(ii);
+                    sacc = sacc + W(1, t, r) * STRIP(ii + 1);
+                    sacc = sacc + W(2, t, r) * STRIP(ii + 2);
+                    zh[t] = sacc;
+                }
+#undef STRIP
+                for (int t = t0; t <= t1; t++) {
+                    if (e == 0) PF(t, 1 - r) = zh[t];
+                    else if (e == 1) PF(t, N + r) = zh[t];
+                    else if (e == 2) PF(1 - r, t) = zh[t];
+                    else PF(N + r, t) = zh[t];
+                }
+            }
+        /* special points */
+        const int sx[4][6] = {{0, 1, 0, -1, 0, -1}, {N + 1, N, N + 1, N + 2, N + 1, N + 2},
+                              {N + 1, N, N + 1, N + 2, N + 1, N + 2}, {0, 1, 0, -1, 0, -1}};
+        const int sy[4][6] = {{1, 0, 0, 0, -1, -1}, {1, 0, 0, 0, -1, -1},
+                              {N, N + 1, N + 1, N + 1, N + 2, N + 2}, {N, N + 1, N + 1, N + 1, N + 2, N + 2}};
+        for (int c = 0; c < 4; c++)
+            for (int k = 0; k < (hw > 1 ? 6 : 3); k++) PF(sx[c][k], sy[c][k]) = csp[c][k];
+        /* points of the corner squares the procedure does not write */
+        for (int j = 1 - E; j <= N + E; j++)
+            for (int i = 1 - E; i <= N + E; i++) {
+                int ii = i >= 1 && i <= N, jj = j >= 1 && j <= N;
+                if (ii || jj) continue;
+                int special = 0;
+                for (int c = 0; c < 4; c++)
+                    for (int k = 0; k < (hw > 1 ? 6 : 3); k++)
+                        if (sx[c][k] == i && sy[c][k] == j) special = 1;
+                if (!special) PF(i, j) = NAN;
+            }
+        for (int jj = 0; jj < L; jj++)
+            for (int ii = 0; ii < L; ii++) out[((long)p * L + jj) * L + ii] = PF(ii + 1 - hw, jj + 1 - hw);
+    }
+#undef W
+#undef IND
+#undef PF
+    free(w); free(ind); free(pf);
+    return 0;
+}
+
 /* ---- cubic lat-lon regrid of Ah fields: create_latlon_interp (regridders/regrid_factory_mod.f90:
  * 183-301) with stencil bounds 1..n of the xy mesh, transform_cartesian_to_native_ecs
  * (equiang_cs/ecs_metric_mod.f90:480-506), latlon_interpolate (regrid_to_latlon_mod.f90:138-160);

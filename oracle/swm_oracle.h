@@ -98,6 +98,10 @@ void ora_project_vector(const ora_grid *g, int kind, const double *vx,
  * w[((j-1)*(n+2hw) + i-1+hw)*4 + s+1] and base indices for i = 1-hw..n+hw, j = 1..hw */
 void ora_ecs_xy_weights(const ora_grid *g, int hw, double *w, int *ind);
 int ora_ecs_halo_xy(const ora_grid *g, double *f, int hw);
+/* ECS cubic halo of an O-point (cell-centre) field f[6][N][N] into out[6][N+2hw][N+2hw]
+ * (1 <= hw <= 8, N >= 8); weights for rows 1..rows, positions -1..N+2 */
+void ora_ecs_o_weights(const ora_grid *g, int rows, double *w, int *ind);
+int ora_ecs_halo_o(const ora_grid *g, const double *f, int hw, double *out);
 
 /* generic classical RK4 on a diagonal complex linear ODE df/dt = w f (the
  * reference's iomega mock model); returns f after one step */
