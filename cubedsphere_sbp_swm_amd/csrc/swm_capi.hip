@@ -76,6 +76,8 @@ struct swm_ctx {
     void *ecs_w = nullptr, *ecs_ind = nullptr, *ecs_src = nullptr;
     // ECS cubic halo on O points: weight rows set up so far, weights, base indices, edge map
     int ecs_o_rows = 0;
+    // every panel holds bitwise the same J, Qi11, Qi12, Qi22 (checked at creation)
+    bool metric_shared = false;
     void *ecs_o_w = nullptr, *ecs_o_ind = nullptr, *ecs_o_edge = nullptr;
 
     size_t esz() const { return prec == 64 ? 8 : 4; }
@@ -101,6 +103,7 @@ swm::DevGeo<T> make_geo(const swm_ctx *c) {
     g.n = c->n;
     g.pitch = c->pitch;
     g.fs = c->fs;
+    g.mps = c->metric_shared ? 0 : (long long)c->n * c->pitch;
     g.hxs = (T)(c->hx * c->scale);
     g.hx = (T)c->hx;
     g.scale = (T)c->scale;
@@ -266,6 +269,16 @@ extern "C" int swm_ctx_create(swm_ctx **out, int N, const char *div, const char 
     const double axis[3] = {0.0, 0.0, 1.0};
     swmh_geometry(N, c->pitch, omega, axis, J.data(), Qi11.data(), Qi12.data(), Qi22.data(), fc.data(), nullptr,
                   nullptr, nullptr);
+    {
+        // the ECS metric depends on (alpha, beta) only: when all six panels agree bit for bit
+        // the stage kernels read panel 0's copy for every panel, a sixth of the metric traffic
+        const size_t slab = (size_t)c->n * c->pitch;
+        bool same = true;
+        for (const auto *m : {&J, &Qi11, &Qi12, &Qi22})
+            for (int p = 1; p < 6 && same; ++p)
+                same = std::memcmp(m->data(), m->data() + p * slab, slab * sizeof(double)) == 0;
+        c->metric_shared = same;
+    }
     std::vector<double> qw[4];
     for (auto &v : qw) v.assign(c->n, 0.0);
     swmh_sync_weights(N, qw[0].data(), qw[1].data(), qw[2].data(), qw[3].data());

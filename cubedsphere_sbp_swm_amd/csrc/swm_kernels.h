@@ -29,6 +29,9 @@ namespace swm {
 template <typename T>
 struct DevGeo {
     const T *J, *Qi11, *Qi12, *Qi22, *f;   // [panel][j][i], row pitch `pitch`
+    // panel stride of J, Qi11, Qi12, Qi22 in elements: n*pitch, or 0 where every panel holds the
+    // same values (the ECS metric depends on alpha, beta only) and panel 0's copy serves all six
+    long long mps;
     const T *qb, *qt, *ql, *qr;            // covariant sync weights, n values each
     const int *einfo;                      // per (panel, edge): q, nu_is_x, tnu, a_base, s, code
     const double *tanA;                    // tan(alpha_i), i = 0..n-1 (metric evaluated on the fly)
@@ -120,6 +123,11 @@ template <typename T>
 __device__ __forceinline__ long long pidx(const DevGeo<T> &g, int p, int i, int j) {
     return ((long long)p * g.n + j) * g.pitch + i;
 }
+// index of the metric terms J, Qi of point (i, j) of panel p
+template <typename T>
+__device__ __forceinline__ long long midx(const DevGeo<T> &g, int p, int i, int j) {
+    return (long long)p * g.mps + (long long)j * g.pitch + i;
+}
 
 // Metric terms of Ah point (i,j) from tan(alpha_i), tan(beta_j), with the operation order of
 // the host set-up (ecs_metric_mod.f90:413-463); FP64 only.
@@ -138,9 +146,9 @@ __device__ __forceinline__ Metric metric_otf(const double *tanA, int i, int j) {
 }
 
 template <bool OTF, typename T>
-__device__ __forceinline__ T metric_J(const DevGeo<T> &g, long long id, int i, int j) {
+__device__ __forceinline__ T metric_J(const DevGeo<T> &g, int p, int i, int j) {
     if constexpr (OTF) return metric_otf(g.tanA, i, j).J;
-    else return g.J[id];
+    else return g.J[midx(g, p, i, j)];
 }
 
 template <bool OTF, typename T, int WHICH = 0>
@@ -151,8 +159,9 @@ __device__ __forceinline__ Derived<T> derived_global(const DevGeo<T> &g, const T
         return make_derived<T, WHICH>(y[id], y[id + g.fs], y[id + 2 * g.fs], (T)m.J, (T)m.Qi11, (T)m.Qi12,
                                       (T)m.Qi22, hs ? hs[id] : T(0), g.grav);
     } else {
-        return make_derived<T, WHICH>(y[id], y[id + g.fs], y[id + 2 * g.fs], g.J[id], g.Qi11[id], g.Qi12[id],
-                                      g.Qi22[id], hs ? hs[id] : T(0), g.grav);
+        const long long mid = midx(g, p, i, j);
+        return make_derived<T, WHICH>(y[id], y[id + g.fs], y[id + 2 * g.fs], g.J[mid], g.Qi11[mid], g.Qi12[mid],
+                                      g.Qi22[mid], hs ? hs[id] : T(0), g.grav);
     }
 }
 
@@ -174,7 +183,8 @@ __device__ __forceinline__ RawPoint<T> load_point(const DevGeo<T> &g, const T *y
         const Metric m = metric_otf(g.tanA, i, j);
         r.J = (T)m.J; r.q11 = (T)m.Qi11; r.q12 = (T)m.Qi12; r.q22 = (T)m.Qi22;
     } else {
-        r.J = g.J[id]; r.q11 = g.Qi11[id]; r.q12 = g.Qi12[id]; r.q22 = g.Qi22[id];
+        const long long mid = midx(g, p, i, j);
+        r.J = g.J[mid]; r.q11 = g.Qi11[mid]; r.q12 = g.Qi12[mid]; r.q22 = g.Qi22[mid];
     }
     r.hs = hs ? hs[id] : T(0);
     return r;

@@ -241,7 +241,8 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         const Metric m = metric_otf(g.tanA, i, j);
         Jv = (T)m.J; q11 = (T)m.Qi11; q12 = (T)m.Qi12; q22 = (T)m.Qi22;
     } else {
-        Jv = g.J[id]; q11 = g.Qi11[id]; q12 = g.Qi12[id]; q22 = g.Qi22[id];
+        const long long mid = midx(g, p, i, j);
+        Jv = g.J[mid]; q11 = g.Qi11[mid]; q12 = g.Qi12[mid]; q22 = g.Qi22[mid];
     }
     const T Jd = Jv * g.hx * g.scale;
     const T gx = dxK / g.hxs;
@@ -314,7 +315,7 @@ __device__ __forceinline__ void stage_edge(const DevGeo<T> &g, const StageArgs<T
         if (d == 0) {
             T *dst = slot < 0 ? sm.e0[side][idx] : sm.c0[slot][idx];
             dst[0] = dv.K; dst[1] = dv.Gxd; dst[2] = dv.Gyd; dst[3] = dv.Gxc; dst[4] = dv.Gyc;
-            dst[5] = metric_J<OTF>(g, pidx(g, ei.q, qi, qj), qi, qj);
+            dst[5] = metric_J<OTF>(g, ei.q, qi, qj);
         } else {
             const bool nu_x = ei.nu_x != 0;
             T *dst = slot < 0 ? sm.en[side][d - 1][idx] : sm.cn[slot][d - 1];
@@ -383,7 +384,8 @@ __device__ __forceinline__ void stage_edge(const DevGeo<T> &g, const StageArgs<T
         const Metric m = metric_otf(g.tanA, i, j);
         Jv = (T)m.J; q11 = (T)m.Qi11; q12 = (T)m.Qi12; q22 = (T)m.Qi22;
     } else {
-        Jv = g.J[id]; q11 = g.Qi11[id]; q12 = g.Qi12[id]; q22 = g.Qi22[id];
+        const long long mid = midx(g, p, i, j);
+        Jv = g.J[mid]; q11 = g.Qi11[mid]; q12 = g.Qi12[mid]; q22 = g.Qi22[mid];
     }
     finish_point<T, MODE>(g, a, id, y[id], y[id + g.fs], y[id + 2 * g.fs], Jv, q11, q12, q22, gx, gy, dv, cu, pre);
 }
