@@ -169,6 +169,19 @@ class OperatorSWM:
         check(_capi.lib().swm_ecs_halo_o_host(self.ctx, _dp(f_o), int(halo_width), _dp(out)))
         return out
 
+    def ecs_halo_vec_a(self, u_o, v_o, halo_width):
+        """ECS vector halo on O points (the reference's A-grid vector halo procedure) of cell-centre
+        contravariant components u_o, v_o (6, N, N): host arrays (6, N+2hw, N+2hw) each."""
+        u_o = np.ascontiguousarray(u_o, dtype=np.float64)
+        v_o = np.ascontiguousarray(v_o, dtype=np.float64)
+        for a in (u_o, v_o):
+            if a.shape != (6, self.N, self.N):
+                raise ValueError(f"expected (6, {self.N}, {self.N}) cell-centre fields")
+        L = self.N + 2 * halo_width
+        ou, ov = np.zeros((6, L, L)), np.zeros((6, L, L))
+        check(_capi.lib().swm_ecs_halo_vec_a_host(self.ctx, _dp(u_o), _dp(v_o), int(halo_width), _dp(ou), _dp(ov)))
+        return ou, ov
+
     # transposed linearised right-hand side (adjoint form)
     def rhs_vjp(self, out, y, mu):
         """out = (dF/dy)^T mu at state y (FP64, SBP families 42/43/63)."""

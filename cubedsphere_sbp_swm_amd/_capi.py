@@ -63,6 +63,8 @@ SIGNATURES = {
     "swm_ecs_halo_xy_host": (_i, [_vp, _vp, _i, _dp]),
     "swm_ecs_halo_o": (_i, [_vp, _vp, _i, _vp]),
     "swm_ecs_halo_o_host": (_i, [_vp, _dp, _i, _dp]),
+    "swm_ecs_halo_vec_a": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
+    "swm_ecs_halo_vec_a_host": (_i, [_vp, _dp, _dp, _i, _dp, _dp]),
     "swm_latlon_outputer_create": (_i, [_vp, _i, _i, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "swm_latlon_vec_outputer_create": (_i, [_vp, _i, _i, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "swm_outputer_destroy": (_i, [_vp]),
@@ -82,6 +84,7 @@ HOST_SIGNATURES = {
     "swmh_ecs_xy_weights": (_i, [_i, _i, _dp, _ip]),
     "swmh_basis_b": (None, [_i, _i, _dp, _dp]),
     "swmh_ecs_o_weights": (None, [_i, _i, _dp, _ip]),
+    "swmh_ecs_vec_a_tm": (None, [_i, _i, _dp]),
     "swmh_latlon_table": (None, [_i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp]),
 }
 

@@ -620,6 +620,49 @@ extern "C" void swmh_ecs_o_weights(int N, int rows, double *w, int *ind) {
     }
 }
 
+// init_transform_matrix (ecs_halo_vec_a_factory_mod.f90:114-175): a1, a2 of the source point and
+// b1, b2 of the target point in the frame of the prototype face, rotated per edge
+extern "C" void swmh_ecs_vec_a_tm(int N, int rows, double *tm) {
+    // rot[edge][c][k]: component c of the rotated vector is sum_k rot[edge][c][k] * v[k]
+    static const int rot[4][3][3] = {{{1, 0, 0}, {0, 0, -1}, {0, 1, 0}},
+                                     {{1, 0, 0}, {0, 0, 1}, {0, -1, 0}},
+                                     {{0, 0, -1}, {0, 1, 0}, {1, 0, 0}},
+                                     {{0, 0, 1}, {0, 1, 0}, {-1, 0, 0}}};
+    static const int ab_shift[4] = {0, 0, 1, 1}, b_sign[4] = {1, -1, 1, -1};
+    const double pi = pi_(), hx = grid_hx(N);
+    for (int e = 0; e < 4; ++e)
+        for (int j = 1; j <= rows; ++j)
+            for (int i = 1; i <= N; ++i) {
+                const double alpha = -0.25 * pi + ((double)i - 0.5) * hx;
+                const double beta = (double)b_sign[e] * (0.25 * pi - ((double)j - 0.5) * hx);
+                const double as = (double)(1 - ab_shift[e]) * alpha + (double)ab_shift[e] * beta;
+                const double bs = (double)ab_shift[e] * alpha + (double)(1 - ab_shift[e]) * beta;
+                auto rotate = [&](const double v[3], double out[3]) {
+                    for (int c = 0; c < 3; ++c) {
+                        double acc = (double)rot[e][c][0] * v[0];
+                        acc = acc + (double)rot[e][c][1] * v[1];
+                        acc = acc + (double)rot[e][c][2] * v[2];
+                        out[c] = acc;
+                    }
+                };
+                double v[3], a1s[3], a2s[3], r[3], b1t[3], b2t[3];
+                a1_proto(as, bs, v);
+                rotate(v, a1s);
+                a2_proto(as, bs, v);
+                rotate(v, a2s);
+                r_proto(as, bs, v);
+                rotate(v, r);
+                const double at = std::atan(r[0] / r[2]), bt = std::atan(r[1] / r[2]);
+                b1_proto(at, bt, b1t);
+                b2_proto(at, bt, b2t);
+                double *o = tm + (((size_t)e * rows + (j - 1)) * N + (i - 1)) * 4;
+                o[0] = dot3(a1s, b1t);
+                o[1] = dot3(a2s, b1t);
+                o[2] = dot3(a1s, b2t);
+                o[3] = dot3(a2s, b2t);
+            }
+}
+
 extern "C" int swmh_ecs_xy_weights(int N, int hw, double *w, int *ind) {
     const int n = N + 1, L = n + 2 * hw;
     if (hw < 1 || n < 4) return -1;

@@ -54,6 +54,12 @@ int swmh_ecs_xy_weights(int N, int hw, double *w, int *ind);
 // edge and halo rows r = 1..rows (init_halo_interp, ecs_halo_factory_mod.f90:193-226).
 void swmh_ecs_o_weights(int N, int rows, double *w, int *ind);
 
+// ECS vector halo on O points (A grid): transform matrices of init_transform_matrix
+// (ecs_halo_vec_a_factory_mod.f90:114-175) from the exchanged neighbour components to this panel's
+// contravariant components, tm[(((edge-1)*rows + r-1)*N + t-1)*4 + k] for edge 1 (below j = 1),
+// 2 (above j = N), 3 (left of i = 1), 4 (right of i = N), halo row r, position t = 1..N.
+void swmh_ecs_vec_a_tm(int N, int rows, double *tm);
+
 // Cubic lat-lon regrid of Ah fields (create_latlon_interp, regridders/regrid_factory_mod.f90:
 // 183-301, scalar_grid_type "Ah", interp_type "cubic"; covariant components as in
 // create_latlon_vector_regrid :80-181).  Target q = (j-1)*Nlon + (i-1), lat = -pi/2 +

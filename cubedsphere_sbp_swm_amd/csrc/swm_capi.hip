@@ -2,6 +2,7 @@
 // kernel launches, HIP-graph captured RK4 stepping and FP64 diagnostics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -74,11 +75,12 @@ struct swm_ctx {
     // ECS cubic halo: weights / base indices of the last halo width used, neighbour rows
     int ecs_hw = 0;
     void *ecs_w = nullptr, *ecs_ind = nullptr, *ecs_src = nullptr;
-    // ECS cubic halo on O points: weight rows set up so far, weights, base indices, edge map
-    int ecs_o_rows = 0;
+    // ECS cubic halo on O points: weight rows set up so far, weights, base indices, edge map;
+    // transform matrices of the vector halo and their rows
+    int ecs_o_rows = 0, ecs_tm_rows = 0;
+    void *ecs_o_w = nullptr, *ecs_o_ind = nullptr, *ecs_o_edge = nullptr, *ecs_tm = nullptr;
     // every panel holds bitwise the same J, Qi11, Qi12, Qi22 (checked at creation)
     bool metric_shared = false;
-    void *ecs_o_w = nullptr, *ecs_o_ind = nullptr, *ecs_o_edge = nullptr;
 
     size_t esz() const { return prec == 64 ? 8 : 4; }
     size_t field_bytes() const { return (size_t)fs * esz(); }
@@ -226,7 +228,8 @@ void free_ctx(swm_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     void *ptrs[] = {c->J, c->Qi11, c->Qi12, c->Qi22, c->f, c->q[0], c->q[1], c->q[2], c->q[3], c->nbr, c->einfo, c->tanA, c->hs,
                     c->J64, c->Qi64[0], c->Qi64[1], c->Qi64[2], c->f64, c->A64, c->hs64, c->yA, c->yB, c->acc,
-                    c->curl, c->red, c->ecs_w, c->ecs_ind, c->ecs_src, c->ecs_o_w, c->ecs_o_ind, c->ecs_o_edge};
+                    c->curl, c->red, c->ecs_w, c->ecs_ind, c->ecs_src, c->ecs_o_w, c->ecs_o_ind, c->ecs_o_edge,
+                    c->ecs_tm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_adj_work(c->adj);

@@ -236,6 +236,16 @@ int swm_ecs_halo_xy_host(swm_ctx *ctx, const void *field, int halo_width, double
 int swm_ecs_halo_o(swm_ctx *ctx, const void *field_o, int halo_width, void *out);
 int swm_ecs_halo_o_host(swm_ctx *ctx, const double *field_o, int halo_width, double *out);
 
+/* ECS vector halo on O points (A grid, contravariant components): halo_vec_t%get_halo_vector(u, v,
+ * domain, halo_width) of create_ecs_A_vec_halo_procedure (equiang_cs/ecs_halo_vec_a_factory_mod.f90:
+ * 20-175, ecs_halo_vec_a_mod.f90:53-192), with its width-8 O-point vector exchange.  u_o, v_o and
+ * out_u, out_v laid out as field_o / out of swm_ecs_halo_o, the same widths and NaN points.
+ * Currently switched off on the device: returns SWM_ESTATE (an unresolved GPU memory fault in its
+ * first run); the algorithm is restated and tested in the oracle. */
+int swm_ecs_halo_vec_a(swm_ctx *ctx, const void *u_o, const void *v_o, int halo_width, void *out_u, void *out_v);
+int swm_ecs_halo_vec_a_host(swm_ctx *ctx, const double *u_o, const double *v_o, int halo_width, double *out_u,
+                            double *out_v);
+
 /* Device-to-device copies between a state vector and three dense device arrays of shape
  * (6, n, n) (panel, j, i) in the context precision (e.g. the storage of framework tensors); no
  * counterpart in the reference. */

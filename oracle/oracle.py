@@ -73,6 +73,9 @@ def lib():
         L.ora_ecs_o_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip]
         L.ora_ecs_halo_o.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
         L.ora_ecs_halo_o.restype = ctypes.c_int
+        L.ora_ecs_vec_a_tm.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.ora_ecs_halo_vec_a.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.ora_ecs_halo_vec_a.restype = ctypes.c_int
         L.ora_rk4_iomega.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double]
         _lib = L
     return _lib
@@ -252,6 +255,22 @@ class Grid:
         if lib().ora_ecs_halo_o(self.h, _p(f), int(hw), _p(out)) != 0:
             raise ValueError("halo width out of range (1..8, N >= 8)")
         return out
+
+    def ecs_vec_a_tm(self, rows):
+        """init_transform_matrix (4 edges, rows, N positions, 4 entries)."""
+        tm = np.zeros((4, rows, self.N, 4))
+        lib().ora_ecs_vec_a_tm(self.h, int(rows), _p(tm))
+        return tm
+
+    def ecs_halo_vec_a(self, u, v, hw):
+        """ECS vector halo (A grid, contravariant) of cell-centre (u, v) (6, N, N) each."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        L = self.N + 2 * hw
+        ou, ov = np.zeros((6, L, L)), np.zeros((6, L, L))
+        if lib().ora_ecs_halo_vec_a(self.h, _p(u), _p(v), int(hw), _p(ou), _p(ov)) != 0:
+            raise ValueError("halo width out of range (1..8, N >= 8)")
+        return ou, ov
 
     def latlon_regrid(self, Nlat, Nlon, f, fv=None, table=False):
         """Cubic lat-lon regrid (Nlat, Nlon) of an Ah scalar f, or of covariant (f, fv) to (zonal,

@@ -1194,113 +1194,268 @@ static double o_fold(const ora_grid *g, const double *f, int p, int oi, int oj) 
     abort();
 }
 
+/* ext_ecs_tile_halo on one panel's exchanged array pf (width E = 8 around the N x N cells):
+ * corner special points, edges along x then y, special points put; corner-square points not
+ * written set to NaN; the (N+2hw)^2 window copied to out */
+#define O_E 8
+static void ecs_o_procedure(const ora_grid *g, double *pf, int hw, const double *w, const int *ind, double *out) {
+    const int N = g->N, E = O_E, LE = N + 2 * E, L = N + 2 * hw, T = N + 4;
+#define W(s, t, r) w[(((r) - 1) * T + (t) + 1) * 4 + (s) + 1]
+#define IND(t, r) ind[((r) - 1) * T + (t) + 1]
+#define PF(i, j) pf[((j) - 1 + E) * LE + (i) - 1 + E]
+    /* corner special points from the exchanged values */
+    double csp[4][6];
+    const int ist[4] = {0, N + 1, N + 1, 0}, idir[4] = {1, -1, -1, 1};
+    const int jst[4] = {0, 0, N + 1, N + 1}, jdir[4] = {1, 1, -1, -1};
+    for (int c = 0; c < 4; c++) {
+#define FC(i, j) PF(ist[c] + idir[c] * (i), jst[c] + jdir[c] * (j))
+        /* weights of corner 1; mirrored (w(2:-1:-1, N+2:N-1:-1, :)) where idir = -1 */
+#define PW(s, t, r) (idir[c] == 1 ? W((s), (t), (r)) : W(1 - (s), N + 1 - (t), (r)))
+        double zpw[4];
+        for (int s2 = -1; s2 <= 2; s2++) zpw[s2 + 1] = PW(s2, 1, 1);
+        double zw = PW(-1, 1, 1), zw2 = zw * zw, zw3 = zw2 * zw;
+        double zx, zy, zz, zfy1, zfy2, zff1, zff2, zff11, zff22;
+        zx = zpw[1] * FC(0, 1) + zpw[2] * FC(0, 2) + zpw[3] * FC(0, 3);
+        zy = zpw[1] * FC(1, 0) + zpw[2] * FC(1, -1) + zpw[3] * FC(1, -2);
+        zz = zpw[1] * FC(1, 1) + zpw[2] * FC(2, 1) + zpw[3] * FC(3, 1);
+        csp[c][0] = (zx + zw * zy + zw2 * zz) / (1 - zw3);
+        zfy1 = (zy + zw * zz + zw2 * zx) / (1 - zw3);
+        zx = zpw[1] * FC(1, 0) + zpw[2] * FC(2, 0) + zpw[3] * FC(3, 0);
+        zy = zpw[1] * FC(0, 1) + zpw[2] * FC(-1, 1) + zpw[3] * FC(-2, 1);
+        zz = zpw[1] * FC(1, 1) + zpw[2] * FC(1, 2) + zpw[3] * FC(1, 3);
+        csp[c][1] = (zx + zw * zy + zw2 * zz) / (1 - zw3);
+        zfy2 = (zy + zw * zz + zw2 * zx) / (1 - zw3);
+        zff1 = PW(-1, 1, 2) * FC(2, 0) + PW(0, 1, 2) * FC(2, -1) + PW(1, 1, 2) * FC(2, -2) + PW(2, 1, 2) * FC(2, -3);
+        zff2 = PW(-1, 1, 2) * FC(0, 2) + PW(0, 1, 2) * FC(-1, 2) + PW(1, 1, 2) * FC(-2, 2) + PW(2, 1, 2) * FC(-3, 2);
+        csp[c][2] = 0.5 * (PW(-1, 0, 1) * zff1 + PW(0, 0, 1) * zfy1 + PW(1, 0, 1) * FC(0, 1) + PW(2, 0, 1) * FC(0, 2) +
+                           PW(-1, 0, 1) * zff2 + PW(0, 0, 1) * zfy2 + PW(1, 0, 1) * FC(1, 0) + PW(2, 0, 1) * FC(2, 0));
+        if (hw > 1) {
+            zff1 = PW(-1, 2, 1) * FC(1, 0) + PW(0, 2, 1) * FC(1, -1) + PW(1, 2, 1) * FC(1, -2) + PW(2, 2, 1) * FC(1, -3);
+            zff11 = PW(-1, 2, 2) * FC(2, -1) + PW(0, 2, 2) * FC(2, -2) + PW(1, 2, 2) * FC(2, -3) + PW(2, 2, 2) * FC(2, -4);
+            zff2 = PW(-1, 2, 1) * FC(0, 1) + PW(0, 2, 1) * FC(-1, 1) + PW(1, 2, 1) * FC(-2, 1) + PW(2, 2, 1) * FC(-3, 1);
+            zff22 = PW(-1, 2, 2) * FC(-1, 2) + PW(0, 2, 2) * FC(-2, 2) + PW(1, 2, 2) * FC(-3, 2) + PW(2, 2, 2) * FC(-4, 2);
+            csp[c][3] = PW(-1, 0, 2) * zff1 + PW(0, 0, 2) * FC(-1, 1) + PW(1, 0, 2) * FC(-1, 2) + PW(2, 0, 2) * FC(-1, 3);
+            csp[c][4] = PW(-1, 0, 2) * zff2 + PW(0, 0, 2) * FC(1, -1) + PW(1, 0, 2) * FC(2, -1) + PW(2, 0, 2) * FC(3, -1);
+            csp[c][5] = 0.5 * (PW(-1, -1, 2) * zff11 + PW(0, -1, 2) * zff1 + PW(1, -1, 2) * FC(-1, 1) + PW(2, -1, 2) * FC(-1, 2) +
+                               PW(-1, -1, 2) * zff22 + PW(0, -1, 2) * zff2 + PW(1, -1, 2) * FC(1, -1) + PW(2, -1, 2) * FC(2, -1));
+        }
+#undef PW
+#undef FC
+    }
+    /* edges along x (rows below j = 1, above j = N), then along y: ecs_ext_halo_1e on the
+     * exchanged strip, positions 2..N-1 in the first row, 1..N further out */
+    double *zh = malloc(sizeof(double) * (N + 2));
+    for (int e = 0; e < 4; e++)
+        for (int r = 1; r <= hw; r++) {
+            const int t0 = r == 1 ? 2 : 1, t1 = r == 1 ? N - 1 : N;
+#define STRIP(t) (e == 0 ? PF((t), 1 - r) : e == 1 ? PF((t), N + r) : e == 2 ? PF(1 - r, (t)) : PF(N + r, (t)))
+            for (int t = t0; t <= t1; t++) {
+                int ii = IND(t, r);
+                double sacc = W(-1, t, r) * STRIP(ii - 1);
+                sacc = sacc + W(0, t, r) * STRIP(ii);
+                sacc = sacc + W(1, t, r) * STRIP(ii + 1);
+                sacc = sacc + W(2, t, r) * STRIP(ii + 2);
+                zh[t] = sacc;
+            }
+#undef STRIP
+            for (int t = t0; t <= t1; t++) {
+                if (e == 0) PF(t, 1 - r) = zh[t];
+                else if (e == 1) PF(t, N + r) = zh[t];
+                else if (e == 2) PF(1 - r, t) = zh[t];
+                else PF(N + r, t) = zh[t];
+            }
+        }
+    free(zh);
+    /* special points */
+    const int sx[4][6] = {{0, 1, 0, -1, 0, -1}, {N + 1, N, N + 1, N + 2, N + 1, N + 2},
+                          {N + 1, N, N + 1, N + 2, N + 1, N + 2}, {0, 1, 0, -1, 0, -1}};
+    const int sy[4][6] = {{1, 0, 0, 0, -1, -1}, {1, 0, 0, 0, -1, -1},
+                          {N, N + 1, N + 1, N + 1, N + 2, N + 2}, {N, N + 1, N + 1, N + 1, N + 2, N + 2}};
+    for (int c = 0; c < 4; c++)
+        for (int k = 0; k < (hw > 1 ? 6 : 3); k++) PF(sx[c][k], sy[c][k]) = csp[c][k];
+    /* points of the corner squares the procedure does not write */
+    for (int j = 1 - E; j <= N + E; j++)
+        for (int i = 1 - E; i <= N + E; i++) {
+            int ii = i >= 1 && i <= N, jj = j >= 1 && j <= N;
+            if (ii || jj) continue;
+            int special = 0;
+            for (int c = 0; c < 4; c++)
+                for (int k = 0; k < (hw > 1 ? 6 : 3); k++)
+                    if (sx[c][k] == i && sy[c][k] == j) special = 1;
+            if (!special) PF(i, j) = NAN;
+        }
+    for (int jj = 0; jj < L; jj++)
+        for (int ii = 0; ii < L; ii++) out[(long)jj * L + ii] = PF(ii + 1 - hw, jj + 1 - hw);
+#undef W
+#undef IND
+#undef PF
+}
+
 int ora_ecs_halo_o(const ora_grid *g, const double *f, int hw, double *out) {
-    const int N = g->N, E = 8, LE = N + 2 * E, L = N + 2 * hw, T = N + 4;
+    const int N = g->N, E = O_E, LE = N + 2 * E, L = N + 2 * hw, T = N + 4;
     if (hw < 1 || hw > E || N < E) return -1;
     const int rows = hw > 2 ? hw : 2;
     double *w = malloc(sizeof(double) * 4 * rows * T);
     int *ind = malloc(sizeof(int) * rows * T);
     double *pf = malloc(sizeof(double) * LE * LE);
     ora_ecs_o_weights(g, rows, w, ind);
-#define W(s, t, r) w[(((r) - 1) * T + (t) + 1) * 4 + (s) + 1]
-#define IND(t, r) ind[((r) - 1) * T + (t) + 1]
-#define PF(i, j) pf[((j) - 1 + E) * LE + (i) - 1 + E]
     for (int p = 0; p < 6; p++) {
         /* exchange of width 8, corner squares left unfilled (NaN) */
         for (int j = 1 - E; j <= N + E; j++)
             for (int i = 1 - E; i <= N + E; i++) {
                 int ii = i >= 1 && i <= N, jj = j >= 1 && j <= N;
-                if (ii && jj) PF(i, j) = f[((long)p * N + j - 1) * N + i - 1];
-                else if (ii || jj) PF(i, j) = o_fold(g, f, p, i, j);
-                else PF(i, j) = NAN;
+                double v;
+                if (ii && jj) v = f[((long)p * N + j - 1) * N + i - 1];
+                else if (ii || jj) v = o_fold(g, f, p, i, j);
+                else v = NAN;
+                pf[(j - 1 + E) * LE + i - 1 + E] = v;
             }
-        /* corner special points from the exchanged values */
-        double csp[4][6];
-        const int ist[4] = {0, N + 1, N + 1, 0}, idir[4] = {1, -1, -1, 1};
-        const int jst[4] = {0, 0, N + 1, N + 1}, jdir[4] = {1, 1, -1, -1};
-        for (int c = 0; c < 4; c++) {
-#define FC(i, j) PF(ist[c] + idir[c] * (i), jst[c] + jdir[c] * (j))
-            /* weights of corner 1; mirrored (w(2:-1:-1, N+2:N-1:-1, :)) where idir = -1 */
-#define PW(s, t, r) (idir[c] == 1 ? W((s), (t), (r)) : W(1 - (s), N + 1 - (t), (r)))
-            double zpw[4];
-            for (int s2 = -1; s2 <= 2; s2++) zpw[s2 + 1] = PW(s2, 1, 1);
-            double zw = PW(-1, 1, 1), zw2 = zw * zw, zw3 = zw2 * zw;
-            double zx, zy, zz, zfy1, zfy2, zff1, zff2, zff11, zff22;
-            zx = zpw[1] * FC(0, 1) + zpw[2] * FC(0, 2) + zpw[3] * FC(0, 3);
-            zy = zpw[1] * FC(1, 0) + zpw[2] * FC(1, -1) + zpw[3] * FC(1, -2);
-            zz = zpw[1] * FC(1, 1) + zpw[2] * FC(2, 1) + zpw[3] * FC(3, 1);
-            csp[c][0] = (zx + zw * zy + zw2 * zz) / (1 - zw3);
-            zfy1 = (zy + zw * zz + zw2 * zx) / (1 - zw3);
-            zx = zpw[1] * FC(1, 0) + zpw[2] * FC(2, 0) + zpw[3] * FC(3, 0);
-            zy = zpw[1] * FC(0, 1) + zpw[2] * FC(-1, 1) + zpw[3] * FC(-2, 1);
-            zz = zpw[1] * FC(1, 1) + zpw[2] * FC(1, 2) + zpw[3] * FC(1, 3);
-            csp[c][1] = (zx + zw * zy + zw2 * zz) / (1 - zw3);
-            zfy2 = (zy + zw * zz + zw2 * zx) / (1 - zw3);
-            zff1 = PW(-1, 1, 2) * FC(2, 0) + PW(0, 1, 2) * FC(2, -1) + PW(1, 1, 2) * FC(2, -2) + PW(2, 1, 2) * FC(2, -3);
-            zff2 = PW(-1, 1, 2) * FC(0, 2) + PW(0, 1, 2) * FC(-1, 2) + PW(1, 1, 2) * FC(-2, 2) + PW(2, 1, 2) * FC(-3, 2);
-            csp[c][2] = 0.5 * (PW(-1, 0, 1) * zff1 + PW(0, 0, 1) * zfy1 + PW(1, 0, 1) * FC(0, 1) + PW(2, 0, 1) * FC(0, 2) +
-                               PW(-1, 0, 1) * zff2 + PW(0, 0, 1) * zfy2 + PW(1, 0, 1) * FC(1, 0) + PW(2, 0, 1) * FC(2, 0));
-            if (hw > 1) {
-                zff1 = PW(-1, 2, 1) * FC(1, 0) + PW(0, 2, 1) * FC(1, -1) + PW(1, 2, 1) * FC(1, -2) + PW(2, 2, 1) * FC(1, -3);
-                zff11 = PW(-1, 2, 2) * FC(2, -1) + PW(0, 2, 2) * FC(2, -2) + PW(1, 2, 2) * FC(2, -3) + PW(2, 2, 2) * FC(2, -4);
-                zff2 = PW(-1, 2, 1) * FC(0, 1) + PW(0, 2, 1) * FC(-1, 1) + PW(1, 2, 1) * FC(-2, 1) + PW(2, 2, 1) * FC(-3, 1);
-                zff22 = PW(-1, 2, 2) * FC(-1, 2) + PW(0, 2, 2) * FC(-2, 2) + PW(1, 2, 2) * FC(-3, 2) + PW(2, 2, 2) * FC(-4, 2);
-                csp[c][3] = PW(-1, 0, 2) * zff1 + PW(0, 0, 2) * FC(-1, 1) + PW(1, 0, 2) * FC(-1, 2) + PW(2, 0, 2) * FC(-1, 3);
-                csp[c][4] = PW(-1, 0, 2) * zff2 + PW(0, 0, 2) * FC(1, -1) + PW(1, 0, 2) * FC(2, -1) + PW(2, 0, 2) * FC(3, -1);
-                csp[c][5] = 0.5 * (PW(-1, -1, 2) * zff11 + PW(0, -1, 2) * zff1 + PW(1, -1, 2) * FC(-1, 1) + PW(2, -1, 2) * FC(-1, 2) +
-                                   PW(-1, -1, 2) * zff22 + PW(0, -1, 2) * zff2 + PW(1, -1, 2) * FC(1, -1) + PW(2, -1, 2) * FC(2, -1));
+        ecs_o_procedure(g, pf, hw, w, ind, out + (long)p * L * L);
+    }
+    free(w); free(ind); free(pf);
+    return 0;
+}
+
+/* ---- ECS vector halo on O points (A grid, contravariant components), create_ecs_A_vec_halo_procedure
+ * (equiang_cs/ecs_halo_vec_a_factory_mod.f90:20-175) and ext_ecs_a_vector_tile_halo
+ * (ecs_halo_vec_a_mod.f90:66-192): the O-point vector exchange (pack_to_buf_vec,
+ * parallel/buffer_mod.f90:171-277: components swapped and negated by the panel pair's
+ * orientation, the code the Ah exchange uses), then in each edge strip the 2 x 2 matrix TM of
+ * init_transform_matrix (:114-175) at rows 1..hw for positions ish..ieh and at rows hw+1..5 near
+ * the corners (positions ish..2 and N-1..ieh), then the scalar procedure on u and v.
+ * tm[(((edge-1)*rows + r-1)*N + t-1)*4 + k] for edge 1 (j < 1), 2 (j > N), 3 (i < 1), 4 (i > N). ---- */
+void ora_ecs_vec_a_tm(const ora_grid *g, int rows, double *tm) {
+    static const int rot[4][3][3] = {{{1, 0, 0}, {0, 0, -1}, {0, 1, 0}},
+                                     {{1, 0, 0}, {0, 0, 1}, {0, -1, 0}},
+                                     {{0, 0, -1}, {0, 1, 0}, {1, 0, 0}},
+                                     {{0, 0, 1}, {0, 1, 0}, {-1, 0, 0}}};
+    static const int ab_shift[4] = {0, 0, 1, 1}, b_sign[4] = {1, -1, 1, -1};
+    const int N = g->N;
+    const double pi = pi_(), hx = g->hx;
+    for (int e = 1; e <= 4; e++)
+        for (int j = 1; j <= rows; j++)
+            for (int i = 1; i <= N; i++) {
+                const double alpha = -0.25 * pi + (i - 0.5) * hx;
+                const double beta = b_sign[e - 1] * (0.25 * pi - (j - 0.5) * hx);
+                const double as = (1 - ab_shift[e - 1]) * alpha + ab_shift[e - 1] * beta;
+                const double bs = ab_shift[e - 1] * alpha + (1 - ab_shift[e - 1]) * beta;
+                double v[3], a1s[3], a2s[3], r[3], b1t[3], b2t[3];
+                /* x(c) = sum_k rot(k, c) v(k), rot(k, c) = rot[e][c][k] */
+#define ROT(dst)                                                                                  \
+    for (int c = 0; c < 3; c++) {                                                                 \
+        double sacc = rot[e - 1][c][0] * v[0];                                                    \
+        sacc = sacc + rot[e - 1][c][1] * v[1];                                                    \
+        sacc = sacc + rot[e - 1][c][2] * v[2];                                                    \
+        dst[c] = sacc;                                                                            \
+    }
+                ecs_a1_proto(as, bs, v);
+                ROT(a1s)
+                ecs_a2_proto(as, bs, v);
+                ROT(a2s)
+                ecs_r_proto(as, bs, v);
+                ROT(r)
+#undef ROT
+                const double at = atan(r[0] / r[2]), bt = atan(r[1] / r[2]);
+                ecs_b1_proto(at, bt, b1t);
+                ecs_b2_proto(at, bt, b2t);
+                double *o = tm + ((((long)e - 1) * rows + j - 1) * N + i - 1) * 4;
+                o[0] = a1s[0] * b1t[0] + a1s[1] * b1t[1] + a1s[2] * b1t[2];
+                o[1] = a2s[0] * b1t[0] + a2s[1] * b1t[1] + a2s[2] * b1t[2];
+                o[2] = a1s[0] * b2t[0] + a1s[1] * b2t[1] + a1s[2] * b2t[2];
+                o[3] = a2s[0] * b2t[0] + a2s[1] * b2t[1] + a2s[2] * b2t[2];
             }
-#undef PW
-#undef FC
-        }
-        /* edges along x (rows below j = 1, above j = N), then along y: ecs_ext_halo_1e on the
-         * exchanged strip, positions 2..N-1 in the first row, 1..N further out */
-        double zh[4096 + 16];
-        for (int e = 0; e < 4; e++)
-            for (int r = 1; r <= hw; r++) {
-                const int t0 = r == 1 ? 2 : 1, t1 = r == 1 ? N - 1 : N;
-#define STRIP(t) (e == 0 ? PF((t), 1 - r) : e == 1 ? PF((t), N + r) : e == 2 ? PF(1 - r, (t)) : PF(N + r, (t)))
-                for (int t = t0; t <= t1; t++) {
-                    int ii = IND(t, r);
-                    double sacc = W(-1, t, r) * STRIP(ii - 1);
-                    sacc = sacc + W(0, t, r) * STRIP(ii);
-                    sacc = sacc + W(1, t, r) * STRIP(ii + 1);
-                    sacc = sacc + W(2, t, r) * STRIP(ii + 2);
-                    zh[t] = sacc;
-                }
-#undef STRIP
-                for (int t = t0; t <= t1; t++) {
-                    if (e == 0) PF(t, 1 - r) = zh[t];
-                    else if (e == 1) PF(t, N + r) = zh[t];
-                    else if (e == 2) PF(1 - r, t) = zh[t];
-                    else PF(N + r, t) = zh[t];
-                }
-            }
-        /* special points */
-        const int sx[4][6] = {{0, 1, 0, -1, 0, -1}, {N + 1, N, N + 1, N + 2, N + 1, N + 2},
-                              {N + 1, N, N + 1, N + 2, N + 1, N + 2}, {0, 1, 0, -1, 0, -1}};
-        const int sy[4][6] = {{1, 0, 0, 0, -1, -1}, {1, 0, 0, 0, -1, -1},
-                              {N, N + 1, N + 1, N + 1, N + 2, N + 2}, {N, N + 1, N + 1, N + 1, N + 2, N + 2}};
-        for (int c = 0; c < 4; c++)
-            for (int k = 0; k < (hw > 1 ? 6 : 3); k++) PF(sx[c][k], sy[c][k]) = csp[c][k];
-        /* points of the corner squares the procedure does not write */
+}
+
+/* source cell (q, qi, qj), 0-based, of the exchanged point (oi, oj) of panel p beyond one edge */
+static void o_fold_src(const ora_grid *g, int p, int oi, int oj, int *q_out, int *qi, int *qj) {
+    const int N = g->N;
+    int P[3];
+    for (int k = 0; k < 3; k++) {
+        int x = 2 * oi - 1, y = 2 * oj - 1, d = 0;
+        if (oi < 1) { d = -x; x = 0; }
+        else if (oi > N) { d = x - 2 * N; x = 2 * N; }
+        else if (oj < 1) { d = -y; y = 0; }
+        else { d = y - 2 * N; y = 2 * N; }
+        P[k] = x * EX[p][k] + y * EY[p][k] + 2 * N * RV[p][k] + d * NV[p][k];
+    }
+    for (int q = 0; q < 6; q++) {
+        if (q == p) continue;
+        int D[3];
+        for (int k = 0; k < 3; k++) D[k] = P[k] - 2 * N * RV[q][k];
+        if (D[0] * NV[q][0] + D[1] * NV[q][1] + D[2] * NV[q][2] != 0) continue;
+        int a = D[0] * EX[q][0] + D[1] * EX[q][1] + D[2] * EX[q][2];
+        int b = D[0] * EY[q][0] + D[1] * EY[q][1] + D[2] * EY[q][2];
+        if (a <= 0 || a >= 2 * N || b <= 0 || b >= 2 * N) continue;
+        *q_out = q; *qi = (a - 1) / 2; *qj = (b - 1) / 2;
+        return;
+    }
+    abort();
+}
+
+int ora_ecs_halo_vec_a(const ora_grid *g, const double *u, const double *v, int hw, double *out_u, double *out_v) {
+    const int N = g->N, E = O_E, LE = N + 2 * E, L = N + 2 * hw, T = N + 4, CHW = 5;
+    if (hw < 1 || hw > E || N < E) return -1;
+    const int rows = hw > 2 ? hw : 2, tmrows = hw > CHW ? hw : CHW;
+    double *w = malloc(sizeof(double) * 4 * rows * T);
+    int *ind = malloc(sizeof(int) * rows * T);
+    double *tm = malloc(sizeof(double) * 4 * 4 * tmrows * N);
+    double *pfu = malloc(sizeof(double) * LE * LE), *pfv = malloc(sizeof(double) * LE * LE);
+    ora_ecs_o_weights(g, rows, w, ind);
+    ora_ecs_vec_a_tm(g, tmrows, tm);
+    /* ish..ieh of init_ecs_tile_halo_vect (single tile per panel: is = 1, ie = N) */
+    int ish = 1 - hw < 1 ? 1 : 1 - hw, ieh = N + hw > N ? N : N + hw;
+    int mn = 1 << 30, mx = -(1 << 30);
+    for (int r = 1; r <= hw; r++) {
+        int a1 = ind[(r - 1) * T + ish + 1], a2 = ind[(r - 1) * T + ieh + 1];
+        if (a1 < mn) mn = a1;
+        if (a2 > mx) mx = a2;
+    }
+    ish = mn - 1 > 1 ? mn - 1 : 1;
+    ieh = mx + 2 < N ? mx + 2 : N;
+    /* Ah halo point of each edge (0 W, 1 E, 2 S, 3 N) whose exchange code the O points share */
+    for (int p = 0; p < 6; p++) {
+        int code[4];
+        const int n = g->n;
+        const int ci[4] = {0, n + 1, 2, 2}, cj[4] = {2, 2, 0, n + 1};
+        for (int e = 0; e < 4; e++) code[e] = g->hvec[IDX(g, p, ci[e], cj[e])];
         for (int j = 1 - E; j <= N + E; j++)
             for (int i = 1 - E; i <= N + E; i++) {
                 int ii = i >= 1 && i <= N, jj = j >= 1 && j <= N;
-                if (ii || jj) continue;
-                int special = 0;
-                for (int c = 0; c < 4; c++)
-                    for (int k = 0; k < (hw > 1 ? 6 : 3); k++)
-                        if (sx[c][k] == i && sy[c][k] == j) special = 1;
-                if (!special) PF(i, j) = NAN;
+                double uu, vv;
+                if (ii && jj) {
+                    uu = u[((long)p * N + j - 1) * N + i - 1];
+                    vv = v[((long)p * N + j - 1) * N + i - 1];
+                } else if (ii || jj) {
+                    int q, qi, qj;
+                    o_fold_src(g, p, i, j, &q, &qi, &qj);
+                    const double us = u[((long)q * N + qj) * N + qi], vs = v[((long)q * N + qj) * N + qi];
+                    /* edge: 0 W (i < 1), 1 E, 2 S (j < 1), 3 N; reference edge number and row, position */
+                    int e, edge, r, t;
+                    if (j < 1) { e = 2; edge = 1; r = 1 - j; t = i; }
+                    else if (j > N) { e = 3; edge = 2; r = j - N; t = i; }
+                    else if (i < 1) { e = 0; edge = 3; r = 1 - i; t = j; }
+                    else { e = 1; edge = 4; r = i - N; t = j; }
+                    const int c = code[e];
+                    const double a = (c & 1) ? vs : us, b = (c & 1) ? us : vs;
+                    const double zu = (c & 2) ? -a : a, zv = (c & 4) ? -b : b;
+                    uu = zu;
+                    vv = zv;
+                    const int inr = t >= ish && t <= ieh;
+                    if ((r <= hw && inr) || (r > hw && r <= CHW && inr && (t <= 2 || t >= N - 1))) {
+                        const double *o = tm + ((((long)edge - 1) * tmrows + r - 1) * N + t - 1) * 4;
+                        uu = o[0] * zu + o[1] * zv;
+                        vv = o[2] * zu + o[3] * zv;
+                    }
+                } else {
+                    uu = vv = NAN;
+                }
+                pfu[(j - 1 + E) * LE + i - 1 + E] = uu;
+                pfv[(j - 1 + E) * LE + i - 1 + E] = vv;
             }
-        for (int jj = 0; jj < L; jj++)
-            for (int ii = 0; ii < L; ii++) out[((long)p * L + jj) * L + ii] = PF(ii + 1 - hw, jj + 1 - hw);
+        ecs_o_procedure(g, pfu, hw, w, ind, out_u + (long)p * L * L);
+        ecs_o_procedure(g, pfv, hw, w, ind, out_v + (long)p * L * L);
     }
-#undef W
-#undef IND
-#undef PF
-    free(w); free(ind); free(pf);
+    free(w); free(ind); free(tm); free(pfu); free(pfv);
     return 0;
 }
 

@@ -102,6 +102,11 @@ int ora_ecs_halo_xy(const ora_grid *g, double *f, int hw);
  * (1 <= hw <= 8, N >= 8); weights for rows 1..rows, positions -1..N+2 */
 void ora_ecs_o_weights(const ora_grid *g, int rows, double *w, int *ind);
 int ora_ecs_halo_o(const ora_grid *g, const double *f, int hw, double *out);
+/* ECS vector halo on O points (A grid, contravariant u, v): transform matrices for rows
+ * 1..rows (edges 1 j<1, 2 j>N, 3 i<1, 4 i>N; tm[(((edge-1)*rows + r-1)*N + t-1)*4 + k]) and the
+ * halo of (u, v) [6][N][N] into out_u, out_v [6][N+2hw][N+2hw] */
+void ora_ecs_vec_a_tm(const ora_grid *g, int rows, double *tm);
+int ora_ecs_halo_vec_a(const ora_grid *g, const double *u, const double *v, int hw, double *out_u, double *out_v);
 
 /* generic classical RK4 on a diagonal complex linear ODE df/dt = w f (the
  * reference's iomega mock model); returns f after one step */

@@ -114,3 +114,76 @@ def test_host_weights_equal_oracle(oracle_lib, N, rows):
     _capi.lib().swmh_ecs_o_weights(N, rows, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                    ind.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     assert np.array_equal(w, w_ref) and np.array_equal(ind, ind_ref)
+
+
+# ---- vector halo on O points (A grid, contravariant components) ----
+
+def _contravariant(G, hw, V):
+    """contravariant components (u, v) of the tangent field V(r) at the O points and virtual
+    points of the extended panels: V = u a1 + v a2 with a1 = dr/dalpha, a2 = dr/dbeta of the
+    panel's gnomonic map (derivatives taken analytically, components from the 2 x 2 Gram system)"""
+    N = G.N
+    idx = np.arange(1 - hw, N + hw + 1)
+    a = -0.25 * np.pi + (idx - 0.5) * (0.5 * np.pi / N)
+    A, B = np.meshgrid(a, a)
+    X, Y = np.tan(A), np.tan(B)
+    d = np.sqrt(1 + X * X + Y * Y)
+    P = np.stack([X, Y, np.ones_like(X)], -1) / d[..., None]
+    e1, e2 = np.zeros(P.shape), np.zeros(P.shape)
+    e1[..., 0], e2[..., 1] = 1.0, 1.0
+    a1p = (1 + X * X)[..., None] * (e1 / d[..., None] - X[..., None] * P / (d * d)[..., None])
+    a2p = (1 + Y * Y)[..., None] * (e2 / d[..., None] - Y[..., None] * P / (d * d)[..., None])
+    us, vs = [], []
+    for M in _frames(G):
+        r, a1, a2 = P @ M, a1p @ M, a2p @ M
+        Vr = V(r)
+        g11, g12, g22 = (a1 * a1).sum(-1), (a1 * a2).sum(-1), (a2 * a2).sum(-1)
+        r1, r2 = (Vr * a1).sum(-1), (Vr * a2).sum(-1)
+        det = g11 * g22 - g12 * g12
+        us.append((g22 * r1 - g12 * r2) / det)
+        vs.append((g11 * r2 - g12 * r1) / det)
+    return np.stack(us), np.stack(vs)
+
+
+def _flow(r):
+    x, y, z = r[..., 0], r[..., 1], r[..., 2]
+    rot = np.stack([-y, x, np.zeros_like(x)], -1) + 0.5 * np.stack([np.zeros_like(x), -z, y], -1)
+    # plus the tangential part of a gradient field: grad(x y + z^2) projected on the sphere
+    gr = np.stack([y, x, 2 * z], -1)
+    gr = gr - (gr * r).sum(-1)[..., None] * r
+    return rot + 0.3 * gr
+
+
+@pytest.mark.parametrize("hw", [1, 2, 3])
+def test_vector_halo_converges_to_the_field(oracle_lib, hw):
+    errs = []
+    for N in (16, 32, 64):
+        G = oracle_lib.Grid(N, 42, 42, 1.0, 0.0, 1.0)
+        eu, ev = _contravariant(G, hw, _flow)
+        u, v = eu[:, hw:hw + N, hw:hw + N].copy(), ev[:, hw:hw + N, hw:hw + N].copy()
+        ou, ov = G.ecs_halo_vec_a(u, v, hw)
+        assert np.array_equal(ou[:, hw:hw + N, hw:hw + N], u) and np.array_equal(ov[:, hw:hw + N, hw:hw + N], v)
+        assert np.array_equal(np.isnan(ou), np.isnan(ov))
+        cm = _corner_mask(N, hw)
+        defined = ~np.isnan(ou)
+        assert defined[:, ~cm].all()
+        e = np.maximum(np.abs(ou - eu), np.abs(ov - ev))
+        strip = e[:, ~cm & ~np.pad(np.ones((N, N), bool), hw)]
+        corner = e[:, cm & defined[0]]
+        errs.append((strip.max(), corner.max()))
+    print(f"vector hw={hw}: strips / corner points max errors", errs)
+    (s0, c0), (s1, c1), (s2, c2) = errs
+    assert s0 / s1 > 10.0 and s1 / s2 > 10.0
+    assert c0 / c1 > 8.0 and c1 / c2 > 8.0
+    assert s2 < 1e-6
+
+
+@pytest.mark.parametrize("N,rows", [(8, 5), (33, 6)])
+def test_host_transform_matrices_equal_oracle(oracle_lib, N, rows):
+    import ctypes
+    from cubedsphere_sbp_swm_amd import _capi
+    G = oracle_lib.Grid(N, 42, 42, 1.0, 0.0, 1.0)
+    ref = G.ecs_vec_a_tm(rows)
+    tm = np.zeros_like(ref)
+    _capi.lib().swmh_ecs_vec_a_tm(N, rows, tm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert np.array_equal(tm, ref)
