@@ -174,42 +174,54 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         if (oi > 0 && oj > 0 && oi < n - 1 && oj < n - 1) pre = prefetch_point<T, MODE>(g, a, pidx(g, p, oi, oj));
     }
 
-    // ---- load phase: stencil inputs of the tile rows (incl. x-halo) and of the y-halo rows.
-    // A thread owns at most NXR row points and NYH y-halo points; all their global loads are
-    // issued before any derived value is formed, so that they are in flight together. ----
-    constexpr int NXR = (TY * LX + NT - 1) / NT, NYH = (2 * R * TX + NT - 1) / NT;
-    RawPoint<T> rx[NXR], ry[NYH];
-    bool okx[NXR], oky[NYH];
+    // ---- load phase: every thread loads its own tile point (kept in registers for the output
+    // phase), plus at most NXH points of the x-halo columns and NYH points of the y-halo rows;
+    // all global loads are issued before any derived value is formed, so that they are in flight
+    // together. ----
+    constexpr int NXH = (2 * R * TY + NT - 1) / NT, NYH = (2 * R * TX + NT - 1) / NT;
+    const int tx = tid % TX, ty = tid / TX;
+    const int i = tx0 + tx, j = ty0 + ty;
+    const bool own_ok = i < n && j < n;
+    RawPoint<T> own{};
+    if (own_ok) own = load_point<OTF>(g, y, a.hs, p, i, j);
+    RawPoint<T> rx[NXH], ry[NYH];
+    bool okx[NXH], oky[NYH];
 #pragma unroll
-    for (int k = 0; k < NXR; ++k) {
+    for (int k = 0; k < NXH; ++k) {
         const int c = tid + k * NT;
-        const int r = c / LX, lx = c - r * LX;
-        const int i = tx0 - R + lx, j = ty0 + r;
-        okx[k] = c < TY * LX && i >= 0 && i < n && j < n;
-        if (okx[k]) rx[k] = load_point<OTF>(g, y, a.hs, p, i, j);
+        const int r = c / (2 * R), q = c - r * (2 * R);
+        const int lx = q < R ? q : q + TX;
+        const int xi = tx0 - R + lx, xj = ty0 + r;
+        okx[k] = c < 2 * R * TY && xi >= 0 && xi < n && xj < n;
+        if (okx[k]) rx[k] = load_point<OTF>(g, y, a.hs, p, xi, xj);
     }
 #pragma unroll
     for (int k = 0; k < NYH; ++k) {
         const int c = tid + k * NT;
         const int r = c / TX, cx = c - r * TX;
         const int ly = r < R ? r : r + TY;
-        const int i = tx0 + cx, j = ty0 - R + ly;
-        oky[k] = c < 2 * R * TX && i < n && j >= 0 && j < n;
-        if (oky[k]) ry[k] = load_point<OTF>(g, y, a.hs, p, i, j);
+        const int yi = tx0 + cx, yj = ty0 - R + ly;
+        oky[k] = c < 2 * R * TX && yi < n && yj >= 0 && yj < n;
+        if (oky[k]) ry[k] = load_point<OTF>(g, y, a.hs, p, yi, yj);
+    }
+    if (own_ok) {
+        const Derived<T> d = derived_of(g, own);
+        sm.K[ty + R][tx + R] = d.K;
+        sm.Gxd[ty][tx + R] = d.Gxd;
+        sm.Gxc[ty][tx + R] = d.Gxc;
+        sm.Gyd[ty + R][tx] = d.Gyd;
+        sm.Gyc[ty + R][tx] = d.Gyc;
     }
 #pragma unroll
-    for (int k = 0; k < NXR; ++k) {
+    for (int k = 0; k < NXH; ++k) {
         if (!okx[k]) continue;
         const int c = tid + k * NT;
-        const int r = c / LX, lx = c - r * LX;
+        const int r = c / (2 * R), q = c - r * (2 * R);
+        const int lx = q < R ? q : q + TX;
         const Derived<T> d = derived_of(g, rx[k]);
         sm.K[r + R][lx] = d.K;
         sm.Gxd[r][lx] = d.Gxd;
         sm.Gxc[r][lx] = d.Gxc;
-        if (lx >= R && lx < R + TX) {
-            sm.Gyd[r + R][lx - R] = d.Gyd;
-            sm.Gyc[r + R][lx - R] = d.Gyc;
-        }
     }
 #pragma unroll
     for (int k = 0; k < NYH; ++k) {
@@ -224,9 +236,7 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     }
     __syncthreads();
 
-    // ---- output phase: one point per thread ----
-    const int tx = tid % TX, ty = tid / TX;
-    const int i = tx0 + tx, j = ty0 + ty;
+    // ---- output phase: the thread's own point ----
     if (i <= 0 || j <= 0 || i >= n - 1 || j >= n - 1) return;
     const int lx = tx + R, ly = ty + R;
     const T dxK = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.K[ly][ai - tx0 + R]; });
@@ -236,21 +246,13 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     const T dxC = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.Gxc[ty][ai - tx0 + R]; });
     const T dyC = sbp_row<ORD, T>(j, n, [&](int aj) { return sm.Gyc[aj - ty0 + R][tx]; });
     const long long id = pidx(g, p, i, j);
-    T Jv, q11, q12, q22;
-    if constexpr (OTF) {
-        const Metric m = metric_otf(g.tanA, i, j);
-        Jv = (T)m.J; q11 = (T)m.Qi11; q12 = (T)m.Qi12; q22 = (T)m.Qi22;
-    } else {
-        const long long mid = midx(g, p, i, j);
-        Jv = g.J[mid]; q11 = g.Qi11[mid]; q12 = g.Qi12[mid]; q22 = g.Qi22[mid];
-    }
-    const T Jd = Jv * g.hx * g.scale;
+    const T Jd = own.J * g.hx * g.scale;
     const T gx = dxK / g.hxs;
     const T gy = dyK / g.hxs;
     const T dv = (dxD + dyD) / Jd;
     const T cu = (dxC + dyC) / Jd;
 
-    finish_point<T, MODE>(g, a, id, y[id], y[id + g.fs], y[id + 2 * g.fs], Jv, q11, q12, q22, gx, gy, dv, cu, pre);
+    finish_point<T, MODE>(g, a, id, own.h, own.u, own.v, own.J, own.q11, own.q12, own.q22, gx, gy, dv, cu, pre);
 }
 
 // Edge segment `seg` of the cube (edge_segs(n, PERSIST) per cube): positions k0..k1-1 of one
