@@ -16,7 +16,8 @@
 // Step, operator_swm_diff_t%apply + explicit_Eul1 (operator_swm_diff_mod.f90:25-46,
 // time_schemes/explicit_Eul1_mod.f90:20-30): y += dt * tend.
 //
-// One Laplacian is two launches, hd_lap_tile and hd_sync, over several fields (blockIdx.y):
+// One Laplacian is two launches, hd_lap_tile (one flat grid over tiles x fields) and hd_sync
+// (fields on blockIdx.y), over several fields:
 // h and the three Cartesian velocity components share every launch.  Panel-edge results are
 // written to a compact edge buffer first and synchronised from there, so that no kernel reads
 // a value another block of the same launch overwrites.
@@ -221,15 +222,18 @@ __device__ __forceinline__ void hd_rows(int a0, int ae, int n, int &lo, int &hi)
 
 template <int ORD, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__restrict__ W,
-                                                       double *__restrict__ Wout, double *__restrict__ E) {
+                                                       double *__restrict__ Wout, double *__restrict__ E, int nf) {
     using C = HdTile<ORD, TX, TY>;
     using L = Ln<ORD>;
     __shared__ double sf[C::LH][C::LW];                  // f on the box
     __shared__ double sbx[TY][C::LW], sdy[TY][C::LW];    // tile rows: Qi11 J, d1y
     __shared__ double sby[C::LH][TX], sdx[C::LH][TX];    // tile columns: Qi22 J, d1x
-    const int n = g.n, k = blockIdx.y;
+    const int n = g.n;
     const int ntx = (n + TX - 1) / TX, nty = (n + TY - 1) / TY;
-    const int tile = xcd_tile(blockIdx.x, 6 * ntx * nty);
+    // one flat block index over (tile, field), field fastest: the nf blocks of a tile sit in the
+    // same XCD's contiguous range, so the metric they all read is fetched into one L2
+    const int u = xcd_tile(blockIdx.x, 6 * ntx * nty * nf);
+    const int tile = u / nf, k = u - tile * nf;
     const int p = tile / (ntx * nty), rem = tile - p * ntx * nty;
     const int x0 = (rem % ntx) * TX, y0 = (rem / ntx) * TY;
     const int xe = x0 + TX < n ? x0 + TX : n, ye = y0 + TY < n ? y0 + TY : n;
@@ -517,11 +521,11 @@ int enqueue_laplace(swm_ctx *c, int order, int k0, int nf, const double *in, dou
     const double *ik = in + (long long)k0 * c->fs;
     double *ok = out + (long long)k0 * c->fs, *ek = w->E + (long long)k0 * ns;
     if (order == 63)
-        hipLaunchKernelGGL((swm::hd_lap_tile<63, LTX, LTY>), dim3(ltiles, nf), dim3(LTX * LTY), 0, c->stream, g, ik,
-                           ok, ek);
+        hipLaunchKernelGGL((swm::hd_lap_tile<63, LTX, LTY>), dim3(ltiles * nf), dim3(LTX * LTY), 0, c->stream, g, ik,
+                           ok, ek, nf);
     else
-        hipLaunchKernelGGL((swm::hd_lap_tile<42, LTX, LTY>), dim3(ltiles, nf), dim3(LTX * LTY), 0, c->stream, g, ik,
-                           ok, ek);
+        hipLaunchKernelGGL((swm::hd_lap_tile<42, LTX, LTY>), dim3(ltiles * nf), dim3(LTX * LTY), 0, c->stream, g, ik,
+                           ok, ek, nf);
     hipLaunchKernelGGL(swm::hd_sync, dim3((ns + 255) / 256, nf), blk, 0, c->stream, g, (const double *)ek, ok);
     HIP_TRY(hipGetLastError());
     return SWM_OK;
