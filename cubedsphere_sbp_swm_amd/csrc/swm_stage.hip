@@ -160,7 +160,7 @@ template <int ORD, typename T, int MODE, int TX, int TY, bool OTF, bool PERSIST 
 __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T> &a, int p, int tx0, int ty0,
                                            TileSmem<ORD, T, TX, TY> &sm) {
     using C = Tile<ORD, T, TX, TY>;
-    constexpr int R = C::R, D = C::D, LX = C::LX, LY = C::LY, NT = C::NT;
+    constexpr int R = C::R, NT = C::NT;
     const int n = g.n;
     // in the persistent ensemble loop the thread index is made opaque per tile, so that the
     // thread-index arithmetic is recomputed per tile instead of being hoisted out of the loop
