@@ -68,7 +68,8 @@ def parse():
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--tiling", default="auto",
-                    help="stage-kernel tiling: auto (autotuner) or ROWS[,otf] with ROWS 8 or 16")
+                    help="stage-kernel tiling: auto (autotuner) or COLSxROWS[,otf] with COLSxROWS one of 64x8, "
+                         "32x16, 16x32, 32x8")
     return ap.parse_args()
 
 
@@ -79,7 +80,8 @@ def profiled_traffic(N, order, prec, tiling):
     path = os.path.join(ROOT, "profiles", f"stage23_c{N}_sbp{order}_f{prec}.md")
     if not os.path.exists(path):
         return None
-    key = f"C{N} sbp{order} f{prec} rows={tiling['tile_rows']} otf={int(tiling['metric_on_the_fly'])}"
+    key = (f"C{N} sbp{order} f{prec} tile={tiling['tile_cols']}x{tiling['tile_rows']} "
+           f"otf={int(tiling['metric_on_the_fly'])}")
     for line in open(path):
         m = re.match(r"traffic_bytes_per_launch (.*): (\d+)\s*$", line)
         if m and m.group(1) == key:
@@ -434,8 +436,9 @@ def main():
     if args.tiling == "auto":
         tuned_us = op.autotune(x, reps=30)
     else:
-        rows, _, otf = args.tiling.partition(",")
-        op.set_tiling(int(rows), otf == "otf")
+        shape, _, otf = args.tiling.partition(",")
+        cols, _, rows = shape.partition("x")
+        op.set_tiling(int(rows), otf == "otf", tile_cols=int(cols))
         tuned_us = None
     tiling = op.get_tiling()
     ts = swm.create_timescheme(op, "rk4")
@@ -511,7 +514,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(N, order, args.precision, tiling),
                      "kernel": "stage_kernel<%d, %s, MODE_STAGE23, %d, %d, %s> (RK stages 2 and 3)" % (
-                         order, "double" if args.precision == 64 else "float", 512 // tiling["tile_rows"],
+                         order, "double" if args.precision == 64 else "float", tiling["tile_cols"],
                          tiling["tile_rows"], str(tiling["metric_on_the_fly"]).lower()),
                      "kernel_avg_us": k_us, "algorithmic_bytes_per_launch": stage_bytes,
                      "step_effective_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,

@@ -115,19 +115,22 @@ class OperatorSWM:
                                                      float(v_max), _dp(h), _dp(u), _dp(v), int(halo_width)))
         return h, u, v
 
-    def set_tiling(self, tile_rows=8, metric_on_the_fly=False):
-        """Tile shape of the fused stage kernel by its height: 8, 16 or 32 rows (64 x 8, 32 x 16,
-        16 x 32 points); FP64 may evaluate the metric terms on the fly instead of reading them
+    def set_tiling(self, tile_rows=8, metric_on_the_fly=False, tile_cols=None):
+        """Tile shape of the fused stage kernel: tile_cols x tile_rows = 64 x 8, 32 x 16, 16 x 32
+        (512 threads) or 32 x 8 (256 threads) points; tile_cols=None takes the 512-thread shape of
+        tile_rows.  FP64 may evaluate the metric terms on the fly instead of reading them
         (identical values)."""
-        check(_capi.lib().swm_ctx_set_tiling(self.ctx, int(tile_rows), int(bool(metric_on_the_fly))))
+        cols = 512 // int(tile_rows) if tile_cols is None else int(tile_cols)
+        check(_capi.lib().swm_ctx_set_tile_shape(self.ctx, cols, int(tile_rows), int(bool(metric_on_the_fly))))
 
     def get_tiling(self):
-        r, m = ctypes.c_int(), ctypes.c_int()
-        check(_capi.lib().swm_ctx_get_tiling(self.ctx, ctypes.byref(r), ctypes.byref(m)))
-        return {"tile_rows": r.value, "metric_on_the_fly": bool(m.value)}
+        c, r, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_capi.lib().swm_ctx_get_tile_shape(self.ctx, ctypes.byref(c), ctypes.byref(r), ctypes.byref(m)))
+        return {"tile_cols": c.value, "tile_rows": r.value, "metric_on_the_fly": bool(m.value)}
 
     def autotune(self, v, reps=50):
-        """Time every tile configuration on state `v` and keep the fastest (returns us/launch)."""
+        """Time every tile configuration with RK4 steps on state `v` (left unchanged) and keep the
+        fastest (returns us per step)."""
         best = ctypes.c_double()
         check(_capi.lib().swm_ctx_autotune(self.ctx, ctypes.c_void_p(v.ptr), int(reps), ctypes.byref(best)))
         return best.value
@@ -392,13 +395,17 @@ class Ensemble:
             raise ValueError("one time step per member")
         check(_capi.lib().swm_ens_rk4_steps(self.h, _dp(d), int(nsteps)))
 
-    def set_tiling(self, tile_rows=8, metric_on_the_fly=False, blocks_per_slot=1):
-        check(_capi.lib().swm_ens_set_tiling(self.h, int(tile_rows), int(bool(metric_on_the_fly)), int(blocks_per_slot)))
+    def set_tiling(self, tile_rows=8, metric_on_the_fly=False, blocks_per_slot=1, tile_cols=None):
+        cols = 512 // int(tile_rows) if tile_cols is None else int(tile_cols)
+        check(_capi.lib().swm_ens_set_tile_shape(self.h, cols, int(tile_rows), int(bool(metric_on_the_fly)),
+                                                 int(blocks_per_slot)))
 
     def get_tiling(self):
-        r, m, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        check(_capi.lib().swm_ens_get_tiling(self.h, ctypes.byref(r), ctypes.byref(m), ctypes.byref(b)))
-        return {"tile_rows": r.value, "metric_on_the_fly": bool(m.value), "blocks_per_slot": b.value}
+        c, r, m, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_capi.lib().swm_ens_get_tile_shape(self.h, ctypes.byref(c), ctypes.byref(r), ctypes.byref(m),
+                                                 ctypes.byref(b)))
+        return {"tile_cols": c.value, "tile_rows": r.value, "metric_on_the_fly": bool(m.value),
+                "blocks_per_slot": b.value}
 
     def autotune(self, reps=20):
         best = ctypes.c_double()

@@ -35,6 +35,8 @@ SIGNATURES = {
     "swm_diagnostics": (_i, [_vp, _vp, _dp]),
     "swm_set_orography": (_i, [_vp, _dp, _i]),
     "swm_ic_spherical_harmonics": (_i, [_vp, ctypes.c_ulonglong, _i, _d, _d, _d, _dp, _dp, _dp, _i]),
+    "swm_ctx_set_tile_shape": (_i, [_vp, _i, _i, _i]),
+    "swm_ctx_get_tile_shape": (_i, [_vp, _ip, _ip, _ip]),
     "swm_ctx_set_tiling": (_i, [_vp, _i, _i]),
     "swm_ctx_get_tiling": (_i, [_vp, _ip, _ip]),
     "swm_ctx_autotune": (_i, [_vp, _vp, _i, _dp]),
@@ -55,6 +57,8 @@ SIGNATURES = {
     "swm_ens_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
     "swm_ens_destroy": (_i, [_vp]),
     "swm_ens_rk4_steps": (_i, [_vp, _dp, _i]),
+    "swm_ens_set_tile_shape": (_i, [_vp, _i, _i, _i, _i]),
+    "swm_ens_get_tile_shape": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "swm_ens_set_tiling": (_i, [_vp, _i, _i, _i]),
     "swm_ens_get_tiling": (_i, [_vp, _ip, _ip, _ip]),
     "swm_ens_autotune": (_i, [_vp, _i, _dp]),

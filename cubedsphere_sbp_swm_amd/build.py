@@ -21,7 +21,8 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-function"]
-HIP_SOURCES = ["swm_capi.hip", "stage_o21.hip", "stage_o42.hip", "stage_o43.hip", "stage_o63.hip"]
+HIP_SOURCES = ["swm_capi.hip", "stage_o21.hip"] + [f"{k}_o{o}{s}.hip" for o in (42, 43, 63)
+                                                for k, s in (("stage", ""), ("stage", "_f32"), ("ens", ""))]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
 FORTRAN = os.path.join(HERE, "fortran")

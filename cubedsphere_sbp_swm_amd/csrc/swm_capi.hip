@@ -22,6 +22,8 @@ namespace {
 
 thread_local std::string g_err;
 
+constexpr int RK4_GRAPH_STEPS = 50;  // steps per replay of the multi-step graph
+
 int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
@@ -56,8 +58,9 @@ struct swm_ctx {
     int *einfo = nullptr;
     double *tanA = nullptr;
     bool otf = false;   // FP64: metric terms evaluated on the fly from tan(alpha)
-    int tile_ty = 8;    // tile height of the stage kernel (tile width 64)
+    int tile_tx = 64, tile_ty = 8;  // tile shape of the stage kernel (swm::tile_shape_ok)
     void *hs = nullptr;
+    unsigned hs_gen = 0;  // bumped whenever hs is replaced (ensemble tables hold the pointer)
     // FP64 copies used by the diagnostics
     double *J64 = nullptr, *Qi64[3] = {nullptr, nullptr, nullptr}, *f64 = nullptr, *A64 = nullptr;
     double *hs64 = nullptr;
@@ -65,8 +68,8 @@ struct swm_ctx {
     void *yA = nullptr, *yB = nullptr, *acc = nullptr, *curl = nullptr;
     double *red = nullptr;
     int red_blocks = 0;
-    // captured RK4 step
-    hipGraphExec_t graph = nullptr;
+    // captured RK4 steps: one step, and RK4_GRAPH_STEPS steps in a row (fewer graph launches)
+    hipGraphExec_t graph = nullptr, graph_multi = nullptr;
     void *graph_state = nullptr;
     double graph_dt = 0.0;
     std::vector<int> host_nbr;
@@ -118,6 +121,7 @@ int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, vo
     swm::StageLaunch r{};
     r.prec = c->prec;
     r.mode = mode;
+    r.tx = c->tile_tx;
     r.ty = c->tile_ty;
     r.otf = c->otf && c->prec == 64;
     r.stream = c->stream;
@@ -142,14 +146,15 @@ int run_stage(swm_ctx *c, int mode, const void *y, void *out, const void *v0, vo
     return SWM_OK;
 }
 
-// one RK4 step = 4 fused stage launches (rk4_mod.f90:28-40)
-int enqueue_rk4_step(swm_ctx *c, void *state, double dt) {
+// one RK4 step = 4 fused stage launches (rk4_mod.f90:28-40); `out` = state, or another buffer
+// for a timing run that leaves the state unchanged
+int enqueue_rk4_step(swm_ctx *c, void *state, double dt, void *out = nullptr) {
     const double c1 = 0.5 * dt, c2 = dt / 6.0, c3 = dt / 3.0;
     int rc;
     if ((rc = run_stage(c, swm::MODE_STAGE1, state, c->yA, nullptr, c->acc, c1, c2))) return rc;
     if ((rc = run_stage(c, swm::MODE_STAGE23, c->yA, c->yB, state, c->acc, c1, c3))) return rc;
     if ((rc = run_stage(c, swm::MODE_STAGE23, c->yB, c->yA, state, c->acc, dt, c3))) return rc;
-    if ((rc = run_stage(c, swm::MODE_STAGE4, c->yA, state, nullptr, c->acc, c2, 0.0))) return rc;
+    if ((rc = run_stage(c, swm::MODE_STAGE4, c->yA, out ? out : state, nullptr, c->acc, c2, 0.0))) return rc;
     return SWM_OK;
 }
 
@@ -223,9 +228,15 @@ int parse_names(swm_ctx *c, const char *div, const char *grad, const char *curl,
     return SWM_OK;
 }
 
+void drop_graphs(swm_ctx *c) {
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    if (c->graph_multi) { (void)hipGraphExecDestroy(c->graph_multi); c->graph_multi = nullptr; }
+    c->graph_state = nullptr;
+}
+
 void free_ctx(swm_ctx *c) {
     if (!c) return;
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    drop_graphs(c);
     void *ptrs[] = {c->J, c->Qi11, c->Qi12, c->Qi22, c->f, c->q[0], c->q[1], c->q[2], c->q[3], c->nbr, c->einfo, c->tanA, c->hs,
                     c->J64, c->Qi64[0], c->Qi64[1], c->Qi64[2], c->f64, c->A64, c->hs64, c->yA, c->yB, c->acc,
                     c->curl, c->red, c->ecs_w, c->ecs_ind, c->ecs_src, c->ecs_o_w, c->ecs_o_ind, c->ecs_o_edge,
@@ -382,7 +393,7 @@ extern "C" int swm_ctx_info(const swm_ctx *c, int *n, int *pitch, long long *fe,
 extern "C" int swm_ctx_set_stream(swm_ctx *c, void *s) {
     if (!c) return fail(SWM_EARG, "ctx is NULL");
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graphs(c);
     if (s) {
         if (c->own_stream) { (void)hipStreamDestroy(c->stream); c->own_stream = false; }
         c->stream = (hipStream_t)s;
@@ -413,7 +424,7 @@ extern "C" int swm_state_free(swm_ctx *c, void *state) {
     if (!c) return fail(SWM_EARG, "ctx is NULL");
     if (!state) return SWM_OK;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->graph_state == state && c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; c->graph_state = nullptr; }
+    if (c->graph_state == state) drop_graphs(c);
     HIP_TRY(hipFree(state));
     return SWM_OK;
 }
@@ -475,21 +486,52 @@ extern "C" int swm_rk4_steps(swm_ctx *c, void *state, double dt, int nsteps) {
     if (!c || !state) return fail(SWM_EARG, "NULL argument");
     if (nsteps < 0) return fail(SWM_EARG, "nsteps must be >= 0");
     if (nsteps == 0) return SWM_OK;
-    if (!c->graph || c->graph_state != state || c->graph_dt != dt) {
-        if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    if (c->graph_state != state || c->graph_dt != dt) drop_graphs(c);
+    // whole runs of RK4_GRAPH_STEPS steps replay the multi-step graph (4 * RK4_GRAPH_STEPS
+    // kernel nodes, one graph launch), the rest the one-step graph
+    const int nmulti = nsteps / RK4_GRAPH_STEPS, nsingle = nsteps - nmulti * RK4_GRAPH_STEPS;
+    for (int which = 0; which < 2; ++which) {
+        hipGraphExec_t &ge = which == 0 ? c->graph : c->graph_multi;
+        const int per = which == 0 ? 1 : RK4_GRAPH_STEPS;
+        if (ge || (which == 0 ? nsingle : nmulti) == 0) continue;
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_rk4_step(c, state, dt);
+        int rc = SWM_OK;
+        for (int k = 0; k < per && !rc; ++k) rc = enqueue_rk4_step(c, state, dt);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
         if (e != hipSuccess) return fail(SWM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (e != hipSuccess) { c->graph = nullptr; return fail(SWM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+        if (e != hipSuccess) { ge = nullptr; return fail(SWM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
         c->graph_state = state;
         c->graph_dt = dt;
     }
-    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    for (int s = 0; s < nmulti; ++s) HIP_TRY(hipGraphLaunch(c->graph_multi, c->stream));
+    for (int s = 0; s < nsingle; ++s) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_set_tile_shape(swm_ctx *c, int tile_cols, int tile_rows, int metric_on_the_fly) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (!swm::tile_shape_ok(tile_cols, tile_rows))
+        return fail(SWM_EARG, "tile shape must be 64 x 8, 32 x 16, 16 x 32 or 32 x 8 points");
+    if (metric_on_the_fly && c->prec != 64) return fail(SWM_EARG, "metric on the fly is FP64 only");
+    // a tile must not touch both the j=1 and the j=n edge of a panel
+    if (c->n <= tile_rows) return fail(SWM_EARG, "tile_rows must be smaller than N+1");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    c->tile_tx = tile_cols;
+    c->tile_ty = tile_rows;
+    c->otf = metric_on_the_fly != 0;
+    return SWM_OK;
+}
+
+extern "C" int swm_ctx_get_tile_shape(const swm_ctx *c, int *tile_cols, int *tile_rows, int *metric_on_the_fly) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (tile_cols) *tile_cols = c->tile_tx;
+    if (tile_rows) *tile_rows = c->tile_ty;
+    if (metric_on_the_fly) *metric_on_the_fly = c->otf ? 1 : 0;
     return SWM_OK;
 }
 
@@ -497,31 +539,24 @@ extern "C" int swm_ctx_set_tiling(swm_ctx *c, int tile_rows, int metric_on_the_f
     if (!c) return fail(SWM_EARG, "ctx is NULL");
     if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32)
         return fail(SWM_EARG, "tile_rows must be 8, 16 or 32 (tiles of 64 x 8, 32 x 16, 16 x 32 points)");
-    if (metric_on_the_fly && c->prec != 64) return fail(SWM_EARG, "metric on the fly is FP64 only");
-    // a tile must not touch both the j=1 and the j=n edge of a panel
-    if (c->n <= tile_rows) return fail(SWM_EARG, "tile_rows must be smaller than N+1");
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
-    c->tile_ty = tile_rows;
-    c->otf = metric_on_the_fly != 0;
-    return SWM_OK;
+    return swm_ctx_set_tile_shape(c, swm::tile_width(tile_rows), tile_rows, metric_on_the_fly);
 }
 
 extern "C" int swm_ctx_get_tiling(const swm_ctx *c, int *tile_rows, int *metric_on_the_fly) {
-    if (!c) return fail(SWM_EARG, "ctx is NULL");
-    if (tile_rows) *tile_rows = c->tile_ty;
-    if (metric_on_the_fly) *metric_on_the_fly = c->otf ? 1 : 0;
-    return SWM_OK;
+    return swm_ctx_get_tile_shape(c, nullptr, tile_rows, metric_on_the_fly);
 }
 
-// Autotuner: times `reps` launches of the RK stage-2 kernel on `state` for every tile
-// configuration (tile height 8/16, and for FP64 metric read or evaluated on the fly) with HIP
-// events on the context stream, and keeps the fastest.
+// Autotuner: times `reps` RK4 steps on `state` (the four stage kernels with the buffers of the
+// step, the last writing yB so that the state is unchanged) for every tile shape and, for FP64,
+// metric read or evaluated on the fly, with HIP events on the context stream; keeps the fastest
+// and returns its time per step.
 extern "C" int swm_ctx_autotune(swm_ctx *c, void *state, int reps, double *best_us) {
     if (!c || !state || reps < 1) return fail(SWM_EARG, "bad argument");
-    struct Cand { int ty; bool otf; };
-    std::vector<Cand> cands = {{8, false}, {16, false}, {32, false}};
-    if (c->prec == 64) { cands.push_back({8, true}); cands.push_back({16, true}); cands.push_back({32, true}); }
+    struct Cand { int tx, ty; bool otf; };
+    std::vector<Cand> cands;
+    for (int otf = 0; otf < (c->prec == 64 ? 2 : 1); ++otf)
+        for (auto sh : {std::make_pair(64, 8), std::make_pair(32, 16), std::make_pair(16, 32), std::make_pair(32, 8)})
+            cands.push_back({sh.first, sh.second, otf != 0});
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -532,13 +567,14 @@ extern "C" int swm_ctx_autotune(swm_ctx *c, void *state, int reps, double *best_
         for (size_t q = 0; q < cands.size(); ++q) {
             const Cand &cd = cands[q];
             if (c->n <= cd.ty) continue;
+            c->tile_tx = cd.tx;
             c->tile_ty = cd.ty;
             c->otf = cd.otf;
             int rc = 0;
-            for (int r = 0; r < 3 && !rc; ++r) rc = run_stage(c, swm::MODE_STAGE23, state, c->yB, state, c->acc, 1.0, 0.0);
+            for (int r = 0; r < 2 && !rc; ++r) rc = enqueue_rk4_step(c, state, 1.0, c->yB);
             if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
             HIP_TRY(hipEventRecord(e0, c->stream));
-            for (int r = 0; r < reps && !rc; ++r) rc = run_stage(c, swm::MODE_STAGE23, state, c->yB, state, c->acc, 1.0, 0.0);
+            for (int r = 0; r < reps && !rc; ++r) rc = enqueue_rk4_step(c, state, 1.0, c->yB);
             HIP_TRY(hipEventRecord(e1, c->stream));
             HIP_TRY(hipEventSynchronize(e1));
             if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
@@ -552,23 +588,27 @@ extern "C" int swm_ctx_autotune(swm_ctx *c, void *state, int reps, double *best_
         if (t[q] < best) { best = t[q]; pick = cands[q]; }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    c->tile_tx = pick.tx;
     c->tile_ty = pick.ty;
     c->otf = pick.otf;
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graphs(c);
     if (best_us) *best_us = best;
     return SWM_OK;
 }
 
 // Measurement helper: `reps` back-to-back launches of one fused stage kernel on the context
-// stream (stage 1, 2 (= 2 and 3) or 4) reading `state`; results go to the work buffers.
+// stream (stage 1, 2 (= 2 and 3) or 4) with the buffers the RK4 step gives it: stage 1 reads
+// `state` and writes yA, acc; stage 2 reads yA, v0 = state, acc and writes yB, acc; stage 4
+// reads yA, acc and writes yB (in place of the state).  yA starts as a copy of `state`.
 extern "C" int swm_bench_stage(swm_ctx *c, void *state, int stage_id, int reps) {
     if (!c || !state || reps < 0) return fail(SWM_EARG, "bad argument");
+    if (stage_id != 1 && stage_id != 2 && stage_id != 4) return fail(SWM_EARG, "stage_id must be 1, 2 or 4");
+    if (stage_id != 1) HIP_TRY(hipMemcpyAsync(c->yA, state, 3 * c->field_bytes(), hipMemcpyDeviceToDevice, c->stream));
     for (int r = 0; r < reps; ++r) {
         int rc;
         if (stage_id == 1) rc = run_stage(c, swm::MODE_STAGE1, state, c->yA, nullptr, c->acc, 1.0, 1.0);
-        else if (stage_id == 2) rc = run_stage(c, swm::MODE_STAGE23, state, c->yB, state, c->acc, 1.0, 0.0);
-        else if (stage_id == 4) rc = run_stage(c, swm::MODE_STAGE4, state, c->yB, nullptr, c->acc, 1.0, 0.0);
-        else return fail(SWM_EARG, "stage_id must be 1, 2 or 4");
+        else if (stage_id == 2) rc = run_stage(c, swm::MODE_STAGE23, c->yA, c->yB, state, c->acc, 1.0, 0.0);
+        else rc = run_stage(c, swm::MODE_STAGE4, c->yA, c->yB, nullptr, c->acc, 1.0, 0.0);
         if (rc) return rc;
     }
     return SWM_OK;
@@ -577,9 +617,10 @@ extern "C" int swm_bench_stage(swm_ctx *c, void *state, int stage_id, int reps) 
 extern "C" int swm_set_orography(swm_ctx *c, const double *hs, int hw) {
     if (!c) return fail(SWM_EARG, "ctx is NULL");
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    drop_graphs(c);
     if (c->hs) { HIP_TRY(hipFree(c->hs)); c->hs = nullptr; }
     if (c->hs64) { HIP_TRY(hipFree(c->hs64)); c->hs64 = nullptr; }
+    ++c->hs_gen;
     if (!hs) return SWM_OK;
     HIP_TRY(hipMalloc(&c->hs, c->field_bytes()));
     HIP_TRY(hipMalloc((void **)&c->hs64, (size_t)c->fs * sizeof(double)));

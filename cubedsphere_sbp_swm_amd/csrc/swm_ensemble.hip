@@ -1,8 +1,8 @@
 // Batched ensemble form (included into swm_capi.hip): cubes of different resolutions, each
 // with its own context (geometry, work buffers) and time step, advanced together.  One RK4
 // step of the whole ensemble = 4 persistent stage launches (ens_stage_kernel), each walking
-// the concatenated tile list of all members; the 4 launches are captured once in a hipGraph
-// and replayed per step.  The tile height, the on-the-fly metric and the size of the
+// the concatenated tile list of all members; the launches of one step and of RK4_GRAPH_STEPS
+// steps are captured once in hipGraphs and replayed.  The tile height, the on-the-fly metric and the size of the
 // persistent grid are chosen by swm_ens_autotune, which times the stage-2 launch of the
 // whole ensemble.
 
@@ -12,14 +12,15 @@ struct swm_ens {
     int prec = 64, order = 42;
     hipStream_t stream = nullptr;
     int num_cu = 0;
-    int ty = 8, bps = 1;  // tile rows, persistent blocks per resident slot
+    int tx = 64, ty = 8, bps = 1;  // tile shape (swm::tile_shape_ok), persistent blocks per resident slot
     bool otf = false;
     void *tables[4] = {nullptr, nullptr, nullptr, nullptr};  // EnsMember<T>[m] per RK stage
-    int *tile_start = nullptr;
-    int ntiles = 0;
+    int2 *tasks = nullptr;  // per task of the persistent loop: (member, task index within the member)
+    int ntiles = 0, ntiles_cap = 0;
     std::vector<double> dt;
+    std::vector<unsigned> hs_gen;  // the members' orography generations the tables were built with
     bool tables_ok = false;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr, graph_multi = nullptr;  // one step; RK4_GRAPH_STEPS steps
     hipEvent_t order_ev = nullptr;  // orders the members' own streams around ensemble steps
 };
 
@@ -27,6 +28,7 @@ namespace {
 
 void ens_drop_graph(swm_ens *e) {
     if (e->graph) { (void)hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+    if (e->graph_multi) { (void)hipGraphExecDestroy(e->graph_multi); e->graph_multi = nullptr; }
 }
 
 void free_ens(swm_ens *e) {
@@ -34,7 +36,7 @@ void free_ens(swm_ens *e) {
     ens_drop_graph(e);
     for (void *t : e->tables)
         if (t) (void)hipFree(t);
-    if (e->tile_start) (void)hipFree(e->tile_start);
+    if (e->tasks) (void)hipFree(e->tasks);
     if (e->order_ev) (void)hipEventDestroy(e->order_ev);
     delete e;
 }
@@ -77,30 +79,47 @@ int ens_build_tables(swm_ens *e) {
             HIP_TRY(hipMemcpy(e->tables[s], tab.data(), M * sizeof(tab[0]), hipMemcpyHostToDevice));
         }
     }
-    // task prefix sums for the current tile height: each member's tiles, then its edge segments
-    std::vector<int> ts(M + 1, 0);
+    // task table for the current tile height: each member's tiles, then its edge segments
+    std::vector<int2> tk;
     for (size_t k = 0; k < M; ++k) {
         const int n = e->m[k]->n;
-        const int tx = swm::tile_width(e->ty);
-        ts[k + 1] = ts[k] + 6 * swm::tiles_along(n, tx) * swm::tiles_along(n, e->ty) + swm::edge_segs(n, true);
+        const int tx = e->tx;
+        const int cnt = 6 * swm::tiles_along(n, tx) * swm::tiles_along(n, e->ty) + swm::edge_segs(n, true);
+        for (int t = 0; t < cnt; ++t) tk.push_back(make_int2((int)k, t));
     }
-    if (!e->tile_start) HIP_TRY(hipMalloc((void **)&e->tile_start, (M + 1) * sizeof(int)));
-    HIP_TRY(hipMemcpy(e->tile_start, ts.data(), (M + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (e->tasks && (int)tk.size() > e->ntiles_cap) { HIP_TRY(hipFree(e->tasks)); e->tasks = nullptr; }
+    if (!e->tasks) {
+        HIP_TRY(hipMalloc((void **)&e->tasks, tk.size() * sizeof(int2)));
+        e->ntiles_cap = (int)tk.size();
+    }
+    HIP_TRY(hipMemcpy(e->tasks, tk.data(), tk.size() * sizeof(int2), hipMemcpyHostToDevice));
     HIP_TRY(hipDeviceSynchronize());  // the non-blocking stream does not wait for null-stream copies
-    e->ntiles = ts[M];
+    e->ntiles = (int)tk.size();
+    e->hs_gen.resize(M);
+    for (size_t k = 0; k < M; ++k) e->hs_gen[k] = e->m[k]->hs_gen;
     e->tables_ok = true;
     return SWM_OK;
+}
+
+// a member's orography was replaced (swm_set_orography frees the old buffer) since the tables
+// were built: they hold a stale pointer
+bool ens_tables_stale(const swm_ens *e) {
+    if (!e->tables_ok || e->hs_gen.size() != e->m.size()) return true;
+    for (size_t k = 0; k < e->m.size(); ++k)
+        if (e->m[k]->hs_gen != e->hs_gen[k]) return true;
+    return false;
 }
 
 int ens_launch(swm_ens *e, int stage) {
     swm::EnsLaunch r{};
     r.prec = e->prec;
     r.mode = stage == 0 ? swm::MODE_STAGE1 : stage == 3 ? swm::MODE_STAGE4 : swm::MODE_STAGE23;
+    r.tx = e->tx;
     r.ty = e->ty;
     r.otf = e->otf && e->prec == 64;
     r.stream = e->stream;
     r.members = e->tables[stage];
-    r.tile_start = e->tile_start;
+    r.tasks = e->tasks;
     r.nmembers = (int)e->m.size();
     r.ntiles = e->ntiles;
     r.num_cu = e->num_cu;
@@ -183,15 +202,18 @@ extern "C" int swm_ens_destroy(swm_ens *e) {
     return SWM_OK;
 }
 
-extern "C" int swm_ens_set_tiling(swm_ens *e, int tile_rows, int metric_on_the_fly, int blocks_per_slot) {
+extern "C" int swm_ens_set_tile_shape(swm_ens *e, int tile_cols, int tile_rows, int metric_on_the_fly,
+                                      int blocks_per_slot) {
     if (!e) return fail(SWM_EARG, "ensemble is NULL");
-    if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32) return fail(SWM_EARG, "tile_rows must be 8, 16 or 32");
+    if (!swm::tile_shape_ok(tile_cols, tile_rows))
+        return fail(SWM_EARG, "tile shape must be 64 x 8, 32 x 16, 16 x 32 or 32 x 8 points");
     if (metric_on_the_fly && e->prec != 64) return fail(SWM_EARG, "metric on the fly is FP64 only");
     if (blocks_per_slot < 1 || blocks_per_slot > 64) return fail(SWM_EARG, "blocks_per_slot must be in [1, 64]");
     for (swm_ctx *c : e->m)
         if (c->n <= tile_rows) return fail(SWM_EARG, "tile_rows must be smaller than every member's N+1");
     HIP_TRY(hipStreamSynchronize(e->stream));
     ens_drop_graph(e);
+    e->tx = tile_cols;
     e->ty = tile_rows;
     e->otf = metric_on_the_fly != 0;
     e->bps = blocks_per_slot;
@@ -199,12 +221,24 @@ extern "C" int swm_ens_set_tiling(swm_ens *e, int tile_rows, int metric_on_the_f
     return SWM_OK;
 }
 
-extern "C" int swm_ens_get_tiling(const swm_ens *e, int *tile_rows, int *metric_on_the_fly, int *blocks_per_slot) {
+extern "C" int swm_ens_get_tile_shape(const swm_ens *e, int *tile_cols, int *tile_rows, int *metric_on_the_fly,
+                                      int *blocks_per_slot) {
     if (!e) return fail(SWM_EARG, "ensemble is NULL");
+    if (tile_cols) *tile_cols = e->tx;
     if (tile_rows) *tile_rows = e->ty;
     if (metric_on_the_fly) *metric_on_the_fly = e->otf ? 1 : 0;
     if (blocks_per_slot) *blocks_per_slot = e->bps;
     return SWM_OK;
+}
+
+extern "C" int swm_ens_set_tiling(swm_ens *e, int tile_rows, int metric_on_the_fly, int blocks_per_slot) {
+    if (!e) return fail(SWM_EARG, "ensemble is NULL");
+    if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32) return fail(SWM_EARG, "tile_rows must be 8, 16 or 32");
+    return swm_ens_set_tile_shape(e, swm::tile_width(tile_rows), tile_rows, metric_on_the_fly, blocks_per_slot);
+}
+
+extern "C" int swm_ens_get_tiling(const swm_ens *e, int *tile_rows, int *metric_on_the_fly, int *blocks_per_slot) {
+    return swm_ens_get_tile_shape(e, nullptr, tile_rows, metric_on_the_fly, blocks_per_slot);
 }
 
 extern "C" int swm_ens_rk4_steps(swm_ens *e, const double *dt, int nsteps) {
@@ -212,7 +246,7 @@ extern "C" int swm_ens_rk4_steps(swm_ens *e, const double *dt, int nsteps) {
     if (nsteps < 0) return fail(SWM_EARG, "nsteps must be >= 0");
     bool same = true;
     for (size_t k = 0; k < e->m.size(); ++k) same = same && e->dt[k] == dt[k];
-    if (!same || !e->tables_ok) {
+    if (!same || ens_tables_stale(e)) {
         HIP_TRY(hipStreamSynchronize(e->stream));
         ens_drop_graph(e);
         e->dt.assign(dt, dt + e->m.size());
@@ -223,21 +257,30 @@ extern "C" int swm_ens_rk4_steps(swm_ens *e, const double *dt, int nsteps) {
     int rc0;
     for (swm_ctx *c : e->m)
         if ((rc0 = ens_order(e, c->stream, e->stream))) return rc0;
-    if (!e->graph) {
+    if (!e->graph && !e->graph_multi) {
         int rc = ens_step(e);  // eager first step: sets up the launch configuration
         if (rc) return rc;
         --nsteps;
+    }
+    // whole runs of RK4_GRAPH_STEPS steps replay the multi-step graph, the rest the one-step graph
+    const int nmulti = nsteps / RK4_GRAPH_STEPS, nsingle = nsteps - nmulti * RK4_GRAPH_STEPS;
+    for (int which = 0; which < 2; ++which) {
+        hipGraphExec_t &ge = which == 0 ? e->graph : e->graph_multi;
+        const int per = which == 0 ? 1 : RK4_GRAPH_STEPS;
+        if (ge || (which == 0 ? nsingle : nmulti) == 0) continue;
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        rc = ens_step(e);
+        int rc = SWM_OK;
+        for (int k = 0; k < per && !rc; ++k) rc = ens_step(e);
         hipError_t err = hipStreamEndCapture(e->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
         if (err != hipSuccess) return fail(SWM_EHIP, std::string("graph capture: ") + hipGetErrorString(err));
-        err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+        err = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (err != hipSuccess) { e->graph = nullptr; return fail(SWM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(err)); }
+        if (err != hipSuccess) { ge = nullptr; return fail(SWM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(err)); }
     }
-    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(e->graph, e->stream));
+    for (int s = 0; s < nmulti; ++s) HIP_TRY(hipGraphLaunch(e->graph_multi, e->stream));
+    for (int s = 0; s < nsingle; ++s) HIP_TRY(hipGraphLaunch(e->graph, e->stream));
     // later work on a member's own stream (downloads, diagnostics) follows the ensemble steps
     for (swm_ctx *c : e->m)
         if ((rc0 = ens_order(e, e->stream, c->stream))) return rc0;
@@ -255,14 +298,14 @@ extern "C" int swm_ens_autotune(swm_ens *e, int reps, double *best_us) {
         // coefficients do not matter for timing; tables need some dt
         for (auto &d : e->dt) d = 1.0;
     }
-    struct Cand { int ty; bool otf; int bps; };
+    struct Cand { int tx, ty; bool otf; int bps; };
     std::vector<Cand> cands;
     int min_n = 1 << 30;
     for (swm_ctx *c : e->m) min_n = std::min(min_n, c->n);
-    for (int ty : {8, 16, 32}) {
-        if (min_n <= ty) continue;  // a tile may not span both y-edges of a panel
+    for (auto sh : {std::make_pair(64, 8), std::make_pair(32, 16), std::make_pair(16, 32), std::make_pair(32, 8)}) {
+        if (min_n <= sh.second) continue;  // a tile may not span both y-edges of a panel
         for (int otf = 0; otf < (e->prec == 64 ? 2 : 1); ++otf)
-            for (int bps : {1, 2, 4}) cands.push_back({ty, otf != 0, bps});
+            for (int bps : {1, 2, 4}) cands.push_back({sh.first, sh.second, otf != 0, bps});
     }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -272,7 +315,7 @@ extern "C" int swm_ens_autotune(swm_ens *e, int reps, double *best_us) {
     int rc = SWM_OK;
     for (const Cand &cd : cands) {
         ens_drop_graph(e);
-        e->ty = cd.ty; e->otf = cd.otf; e->bps = cd.bps;
+        e->tx = cd.tx; e->ty = cd.ty; e->otf = cd.otf; e->bps = cd.bps;
         if ((rc = ens_build_tables(e))) break;
         for (int r = 0; r < 2 && !rc; ++r) rc = ens_launch(e, 1);
         if (rc) break;
@@ -288,7 +331,7 @@ extern "C" int swm_ens_autotune(swm_ens *e, int reps, double *best_us) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    e->ty = pick.ty; e->otf = pick.otf; e->bps = pick.bps;
+    e->tx = pick.tx; e->ty = pick.ty; e->otf = pick.otf; e->bps = pick.bps;
     e->tables_ok = false;
     if (best_us) *best_us = best;
     return SWM_OK;
@@ -297,7 +340,7 @@ extern "C" int swm_ens_autotune(swm_ens *e, int reps, double *best_us) {
 // Measurement helper: `reps` back-to-back stage-2 launches of the whole ensemble.
 extern "C" int swm_ens_bench_stage(swm_ens *e, int reps) {
     if (!e || reps < 0) return fail(SWM_EARG, "bad argument");
-    if (!e->tables_ok) {
+    if (ens_tables_stale(e)) {
         if (e->dt[0] == 0.0)
             for (auto &d : e->dt) d = 1.0;
         int rc = ens_build_tables(e);

@@ -60,9 +60,13 @@ struct StageArgs {
     T *zeta;       // optional: the synced relative vorticity of y is also written here
 };
 
-// one cube of an ensemble: its geometry and the arguments of the current RK stage
-// tile width of a tile of `ty` rows (512 points per tile)
+// Tile shapes (columns x rows of points, one thread per point) of the stage kernels: 64 x 8,
+// 32 x 16, 16 x 32 (512 threads) and 32 x 8 (256 threads).  tile_width(ty) is the 512-thread
+// shape of `ty` rows.
 SWM_HD constexpr int tile_width(int ty) { return 512 / ty; }
+SWM_HD constexpr bool tile_shape_ok(int tx, int ty) {
+    return (tx == 64 && ty == 8) || (tx == 32 && ty == 16) || (tx == 16 && ty == 32) || (tx == 32 && ty == 8);
+}
 // The tiles cover the points 1 .. n-2 of a panel line (the edge segments own 0 and n-1):
 // tiles_along(n, t) tiles of t points, the b-th starting at point 1 + b*t.
 SWM_HD constexpr int tiles_along(int n, int t) { return (n - 2 + t - 1) / t; }
@@ -97,6 +101,7 @@ struct EnsMember {
 template <typename T>
 struct Derived {
     T K, Gxd, Gyd, Gxc, Gyc;
+    T hu, hv;  // h*ut, h*vt (kept in registers for the point's own output)
 };
 
 // WHICH: 0 all five inputs, 2 only those of the y-derivative (K, J*hv, J*(-u/J))
@@ -110,6 +115,8 @@ __device__ __forceinline__ Derived<T> make_derived(T h, T u, T v, T J, T q11, T 
     T K = (u * ut + v * vt) / T(2);
     K = K + grav * h + grav * hsv;
     d.K = K;
+    d.hu = hu;
+    d.hv = hv;
     if constexpr (WHICH != 2) {
         d.Gxd = J * hu;
         d.Gxc = J * (v / J);

@@ -111,17 +111,15 @@ __device__ __forceinline__ OutPre<T> prefetch_point(const DevGeo<T> &g, const St
 
 // Coriolis term, tendencies and RK stage combination of one point
 // (coriolis_colocated_mod.f90:97-98, operator_swm_mod.f90:95-101, rk4_mod.f90:28-40)
+// hu = h*ut, hv = h*vt: the mass fluxes of the point (u, v are read by stage 1 only)
 template <typename T, int MODE>
-__device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs<T> &a, long long id, T h, T u, T v,
-                                             T Jv, T q11, T q12, T q22, T gx, T gy, T dv, T cu, const OutPre<T> &pre) {
+__device__ __forceinline__ void finish_point_hv(const DevGeo<T> &g, const StageArgs<T> &a, long long id, T h, T u, T v,
+                                                T hu, T hv, T Jv, T gx, T gy, T dv, T cu, const OutPre<T> &pre) {
     if constexpr (MODE == MODE_CURL) {
         a.out[id] = cu;
         return;
     }
     if (a.zeta) a.zeta[id] = cu;
-    const T ut = q11 * u + q12 * v;
-    const T vt = q12 * u + q22 * v;
-    const T hu = h * ut, hv = h * vt;
     const T fz = pre.f + cu;
     const T cor_u = fz * hv / h * Jv;
     const T cor_v = -(fz * hu / h * Jv);
@@ -154,6 +152,14 @@ __device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs
     }
 }
 
+template <typename T, int MODE>
+__device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs<T> &a, long long id, T h, T u, T v,
+                                             T Jv, T q11, T q12, T q22, T gx, T gy, T dv, T cu, const OutPre<T> &pre) {
+    const T ut = q11 * u + q12 * v;
+    const T vt = q12 * u + q22 * v;
+    finish_point_hv<T, MODE>(g, a, id, h, u, v, h * ut, h * vt, Jv, gx, gy, dv, cu, pre);
+}
+
 // One TX x TY tile of panel p starting at (tx0, ty0) >= (1, 1): the whole RHS + RK stage update
 // of its points off the panel edges (the panel-edge points belong to the edge segments).
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF, bool PERSIST = false>
@@ -169,10 +175,6 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     if constexpr (PERSIST) asm volatile("" : "+v"(tid));
     const T *y = a.y;
     OutPre<T> pre{};
-    {
-        const int oi = tx0 + tid % TX, oj = ty0 + tid / TX;
-        if (oi > 0 && oj > 0 && oi < n - 1 && oj < n - 1) pre = prefetch_point<T, MODE>(g, a, pidx(g, p, oi, oj));
-    }
 
     // ---- load phase: every thread loads its own tile point (kept in registers for the output
     // phase), plus at most NXH points of the x-halo columns and NYH points of the y-halo rows;
@@ -204,8 +206,13 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         oky[k] = c < 2 * R * TX && yi < n && yj >= 0 && yj < n;
         if (oky[k]) ry[k] = load_point<OTF>(g, y, a.hs, p, yi, yj);
     }
+    // of the own point only h, u, v, J and the mass fluxes h*ut, h*vt stay live for the output
+    // phase (the same products the Coriolis term forms)
+    T own_hu = T(0), own_hv = T(0);
     if (own_ok) {
         const Derived<T> d = derived_of(g, own);
+        own_hu = d.hu;
+        own_hv = d.hv;
         sm.K[ty + R][tx + R] = d.K;
         sm.Gxd[ty][tx + R] = d.Gxd;
         sm.Gxc[ty][tx + R] = d.Gxc;
@@ -234,6 +241,10 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         sm.Gyd[ly][cx] = d.Gyd;
         sm.Gyc[ly][cx] = d.Gyc;
     }
+    // the output point's Coriolis parameter and RK vectors: issued after the load phase (issued
+    // with it, they raised its register peak from 74 to 88 VGPRs at FP64, one wave per SIMD
+    // less), in flight across the barrier and the stencil phase
+    if (i > 0 && j > 0 && i < n - 1 && j < n - 1) pre = prefetch_point<T, MODE>(g, a, pidx(g, p, i, j));
     __syncthreads();
 
     // ---- output phase: the thread's own point ----
@@ -252,7 +263,7 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     const T dv = (dxD + dyD) / Jd;
     const T cu = (dxC + dyC) / Jd;
 
-    finish_point<T, MODE>(g, a, id, own.h, own.u, own.v, own.J, own.q11, own.q12, own.q22, gx, gy, dv, cu, pre);
+    finish_point_hv<T, MODE>(g, a, id, own.h, own.u, own.v, own_hu, own_hv, own.J, gx, gy, dv, cu, pre);
 }
 
 // Edge segment `seg` of the cube (edge_segs(n, PERSIST) per cube): positions k0..k1-1 of one
@@ -425,28 +436,22 @@ __device__ __forceinline__ S uniform_copy(const S *src) {
 }
 
 // Ensemble of cubes of different sizes in one persistent launch: the blocks walk the
-// concatenated task list of all members (tile_start[m] = first task of member m, prefix sums
-// over the members' 6*ntx*nty tiles (tiles_along) followed by their edge_segs(n, true) edge
-// segments),
-// XCD-contiguous, and take each task's geometry and stage arguments from the member table.
+// concatenated task list of all members (per member its 6*ntx*nty tiles (tiles_along), then its
+// edge_segs(n, true) edge segments), XCD-contiguous.  tasks[t] = (member, task index within the
+// member): one load gives a block its member, whose geometry and stage arguments it then takes
+// from the member table.
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
 __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *__restrict__ members,
-                                                            const int *__restrict__ tile_start, int nmembers,
-                                                            int ntiles) {
+                                                            const int2 *__restrict__ tasks, int ntiles) {
     __shared__ Smem<ORD, T, TX, TY> sm;
     for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
         const int tile = xcd_tile(it, ntiles);
-        int lo = 0, hi = nmembers - 1;  // last member with tile_start <= tile
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (tile_start[mid] <= tile) lo = mid;
-            else hi = mid - 1;
-        }
-        lo = __builtin_amdgcn_readfirstlane(lo);  // block-uniform member index
-        const DevGeo<T> g = uniform_copy(&members[lo].g);
-        const StageArgs<T> a = uniform_copy(&members[lo].a);
+        const int2 tk = tasks[tile];
+        const int m = __builtin_amdgcn_readfirstlane(tk.x);   // block-uniform member index
+        const int lt = __builtin_amdgcn_readfirstlane(tk.y);
+        const DevGeo<T> g = uniform_copy(&members[m].g);
+        const StageArgs<T> a = uniform_copy(&members[m].a);
         const int ntx = tiles_along(g.n, TX), nty = tiles_along(g.n, TY);
-        const int lt = tile - tile_start[lo];
         __syncthreads();  // the previous task's LDS reads are done
         if (lt >= 6 * ntx * nty) {
             stage_edge<ORD, T, MODE, OTF, true>(g, a, lt - 6 * ntx * nty, sm.e, TX * TY);
@@ -462,9 +467,8 @@ __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *
 
 namespace swm {
 
-template <int ORD, typename T, int TY, bool OTF>
+template <int ORD, typename T, int TX, int TY, bool OTF>
 hipError_t launch_one(int mode, const DevGeo<T> &g, const StageArgs<T> &a, hipStream_t s) {
-    constexpr int TX = tile_width(TY);
     const int ntx = tiles_along(g.n, TX), nty = tiles_along(g.n, TY);
     const dim3 grid(edge_blocks(g.n) + 6 * ntx * nty), block(TX * TY);
     switch (mode) {
@@ -478,9 +482,8 @@ hipError_t launch_one(int mode, const DevGeo<T> &g, const StageArgs<T> &a, hipSt
     return hipGetLastError();
 }
 
-template <int ORD, typename T, int TY, bool OTF, int MODE>
+template <int ORD, typename T, int TX, int TY, bool OTF, int MODE>
 hipError_t launch_ens_mode(const EnsLaunch &r) {
-    constexpr int TX = tile_width(TY);
     auto kern = ens_stage_kernel<ORD, T, MODE, TX, TY, OTF>;
     static int per_cu = 0;  // resident blocks per CU of this instantiation
     if (per_cu == 0) {
@@ -496,49 +499,58 @@ hipError_t launch_ens_mode(const EnsLaunch &r) {
     if (grid > r.ntiles) grid = (r.ntiles + 7) / 8 * 8;
     if (grid < 8) grid = 8;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(TX * TY), 0, r.stream, (const EnsMember<T> *)r.members,
-                       r.tile_start, r.nmembers, r.ntiles);
+                       r.tasks, r.ntiles);
     return hipGetLastError();
 }
 
-template <int ORD, typename T, int TY, bool OTF>
+template <int ORD, typename T, int TX, int TY, bool OTF>
 hipError_t launch_ens_one(const EnsLaunch &r) {
     switch (r.mode) {
-    case MODE_STAGE1: return launch_ens_mode<ORD, T, TY, OTF, MODE_STAGE1>(r);
-    case MODE_STAGE23: return launch_ens_mode<ORD, T, TY, OTF, MODE_STAGE23>(r);
-    case MODE_STAGE4: return launch_ens_mode<ORD, T, TY, OTF, MODE_STAGE4>(r);
+    case MODE_STAGE1: return launch_ens_mode<ORD, T, TX, TY, OTF, MODE_STAGE1>(r);
+    case MODE_STAGE23: return launch_ens_mode<ORD, T, TX, TY, OTF, MODE_STAGE23>(r);
+    case MODE_STAGE4: return launch_ens_mode<ORD, T, TX, TY, OTF, MODE_STAGE4>(r);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <int ORD>
-hipError_t launch_ens_order(const EnsLaunch &r) {
-    if (r.prec == 64) {
-        if (r.ty == 32)
-            return r.otf ? launch_ens_one<ORD, double, 32, true>(r) : launch_ens_one<ORD, double, 32, false>(r);
-        if (r.ty == 16)
-            return r.otf ? launch_ens_one<ORD, double, 16, true>(r) : launch_ens_one<ORD, double, 16, false>(r);
-        return r.otf ? launch_ens_one<ORD, double, 8, true>(r) : launch_ens_one<ORD, double, 8, false>(r);
-    }
-    if (r.ty == 32) return launch_ens_one<ORD, float, 32, false>(r);
-    if (r.ty == 16) return launch_ens_one<ORD, float, 16, false>(r);
-    return launch_ens_one<ORD, float, 8, false>(r);
+// the tile shapes of the stage kernels (tile_shape_ok): 64 x 8, 32 x 16, 16 x 32 (512 threads)
+// and 32 x 8 (256 threads)
+template <int ORD, typename T, bool OTF>
+hipError_t launch_ens_shape(const EnsLaunch &r) {
+    if (r.tx == 64 && r.ty == 8) return launch_ens_one<ORD, T, 64, 8, OTF>(r);
+    if (r.tx == 32 && r.ty == 16) return launch_ens_one<ORD, T, 32, 16, OTF>(r);
+    if (r.tx == 16 && r.ty == 32) return launch_ens_one<ORD, T, 16, 32, OTF>(r);
+    if (r.tx == 32 && r.ty == 8) return launch_ens_one<ORD, T, 32, 8, OTF>(r);
+    return hipErrorInvalidValue;
 }
 
 template <int ORD>
-hipError_t launch_stage_order(const StageLaunch &r) {
-    if (r.prec == 64) {
-        if (r.ty == 32)
-            return r.otf ? launch_one<ORD, double, 32, true>(r.mode, r.gd, r.ad, r.stream)
-                         : launch_one<ORD, double, 32, false>(r.mode, r.gd, r.ad, r.stream);
-        if (r.ty == 16)
-            return r.otf ? launch_one<ORD, double, 16, true>(r.mode, r.gd, r.ad, r.stream)
-                         : launch_one<ORD, double, 16, false>(r.mode, r.gd, r.ad, r.stream);
-        return r.otf ? launch_one<ORD, double, 8, true>(r.mode, r.gd, r.ad, r.stream)
-                     : launch_one<ORD, double, 8, false>(r.mode, r.gd, r.ad, r.stream);
-    }
-    if (r.ty == 32) return launch_one<ORD, float, 32, false>(r.mode, r.gf, r.af, r.stream);
-    if (r.ty == 16) return launch_one<ORD, float, 16, false>(r.mode, r.gf, r.af, r.stream);
-    return launch_one<ORD, float, 8, false>(r.mode, r.gf, r.af, r.stream);
+hipError_t launch_ens_f64(const EnsLaunch &r) {
+    return r.otf ? launch_ens_shape<ORD, double, true>(r) : launch_ens_shape<ORD, double, false>(r);
+}
+
+template <int ORD>
+hipError_t launch_ens_f32(const EnsLaunch &r) {
+    return launch_ens_shape<ORD, float, false>(r);
+}
+
+template <int ORD, typename T, bool OTF>
+hipError_t launch_stage_shape(const StageLaunch &r, const DevGeo<T> &g, const StageArgs<T> &a) {
+    if (r.tx == 64 && r.ty == 8) return launch_one<ORD, T, 64, 8, OTF>(r.mode, g, a, r.stream);
+    if (r.tx == 32 && r.ty == 16) return launch_one<ORD, T, 32, 16, OTF>(r.mode, g, a, r.stream);
+    if (r.tx == 16 && r.ty == 32) return launch_one<ORD, T, 16, 32, OTF>(r.mode, g, a, r.stream);
+    if (r.tx == 32 && r.ty == 8) return launch_one<ORD, T, 32, 8, OTF>(r.mode, g, a, r.stream);
+    return hipErrorInvalidValue;
+}
+
+template <int ORD>
+hipError_t launch_stage_f64(const StageLaunch &r) {
+    return r.otf ? launch_stage_shape<ORD, double, true>(r, r.gd, r.ad) : launch_stage_shape<ORD, double, false>(r, r.gd, r.ad);
+}
+
+template <int ORD>
+hipError_t launch_stage_f32(const StageLaunch &r) {
+    return launch_stage_shape<ORD, float, false>(r, r.gf, r.af);
 }
 
 }  // namespace swm

@@ -152,19 +152,25 @@ int swm_latlon_vec_write(swm_outputer *outputer, const void *u, const void *v, c
                          const char *file_name_v, int rec_num);
 int swm_paneled_write(swm_ctx *ctx, const void *f, const char *file_name, int rec_num);
 
-/* Tile configuration of the fused stage kernel (no counterpart in the reference): tile height
- * 8, 16 or 32 rows (tiles of 64 x 8, 32 x 16 or 16 x 32 points, 512 threads; the height must be
- * below N+1) and, for FP64, whether the metric terms J, Qi are read from memory (0) or evaluated
- * on the fly from tan(alpha) (1; identical values).  swm_ctx_autotune times every
- * configuration with `reps` launches of the RK stage-2 kernel on `state`, in two interleaved
- * rounds, keeps the fastest (minimum over the rounds) and returns its time per launch. */
+/* Tile configuration of the fused stage kernel (no counterpart in the reference): the tile shape
+ * in points, tile_cols x tile_rows = 64 x 8, 32 x 16, 16 x 32 (512 threads) or 32 x 8 (256
+ * threads; the row count must be below N+1; the sbp21 kernel uses 64 x 16 tiles for 16 rows and
+ * 64 x 8 otherwise), and, for FP64, whether the metric terms J, Qi are read from memory (0) or
+ * evaluated on the fly from tan(alpha) (1; identical values).  swm_ctx_set_tiling /
+ * swm_ctx_get_tiling name the 512-thread shapes by their row count (8, 16, 32).
+ * swm_ctx_autotune times `reps` RK4 steps on `state` (the state is left unchanged) for every
+ * configuration, in two interleaved rounds, keeps the fastest (minimum over the rounds) and
+ * returns its time per step. */
+int swm_ctx_set_tile_shape(swm_ctx *ctx, int tile_cols, int tile_rows, int metric_on_the_fly);
+int swm_ctx_get_tile_shape(const swm_ctx *ctx, int *tile_cols, int *tile_rows, int *metric_on_the_fly);
 int swm_ctx_set_tiling(swm_ctx *ctx, int tile_rows, int metric_on_the_fly);
 int swm_ctx_get_tiling(const swm_ctx *ctx, int *tile_rows, int *metric_on_the_fly);
 int swm_ctx_autotune(swm_ctx *ctx, void *state, int reps, double *best_us);
 
 /* Measurement helper (no counterpart in the reference): `reps` back-to-back launches of one
- * fused RK stage kernel (stage_id 1, 2 or 4) reading `state`, on the context stream; outputs go
- * to internal work buffers. */
+ * fused RK stage kernel (stage_id 1, 2 or 4) on the context stream, with the buffers the RK4 step
+ * gives it (stage 2: y = a work copy of `state`, v0 = `state`); outputs go to internal work
+ * buffers, `state` is unchanged. */
 int swm_bench_stage(swm_ctx *ctx, void *state, int stage_id, int reps);
 
 /* ---- Adjoint form (no counterpart in the reference, which has no adjoint) ----
@@ -197,16 +203,19 @@ int swm_adj_set_tiling(swm_ctx *ctx, int tile_x);
 /* ---- Batched ensemble form (no counterpart in the reference) ----
  * Cubes of different resolutions, one context and one state vector each (same precision and
  * SBP family 42/43/63, N >= 16), advanced together: every RK4 stage of the whole ensemble is
- * one persistent kernel launch over all members' tiles, and one ensemble step (4 launches) is
- * captured once in a hipGraph.  dt holds one time step per member.  The ensemble runs on the
- * first member's stream; the states stay owned by the caller.  swm_ens_autotune times the
- * stage-2 launch over tile shapes (64 x 8, 32 x 16, 16 x 32), the metric source (FP64: read or on the fly) and
- * the persistent grid size (1, 2, 4 blocks per resident slot) and keeps the fastest;
- * swm_ens_bench_stage runs `reps` stage-2 launches (measurement helper). */
+ * one persistent kernel launch over all members' tiles, and one ensemble step (4 launches) and a
+ * run of steps are captured once in hipGraphs.  dt holds one time step per member.  The ensemble
+ * runs on the first member's stream; the states stay owned by the caller.  swm_ens_autotune
+ * times the stage-2 launch over the tile shapes of swm_ctx_set_tile_shape, the metric source
+ * (FP64: read or on the fly) and the persistent grid size (1, 2, 4 blocks per resident slot) and
+ * keeps the fastest; swm_ens_bench_stage runs `reps` stage-2 launches (measurement helper). */
 typedef struct swm_ens swm_ens;
 int swm_ens_create(swm_ctx *const *members, void *const *states, int nmembers, swm_ens **ens);
 int swm_ens_destroy(swm_ens *ens);
 int swm_ens_rk4_steps(swm_ens *ens, const double *dt, int nsteps);
+int swm_ens_set_tile_shape(swm_ens *ens, int tile_cols, int tile_rows, int metric_on_the_fly, int blocks_per_slot);
+int swm_ens_get_tile_shape(const swm_ens *ens, int *tile_cols, int *tile_rows, int *metric_on_the_fly,
+                           int *blocks_per_slot);
 int swm_ens_set_tiling(swm_ens *ens, int tile_rows, int metric_on_the_fly, int blocks_per_slot);
 int swm_ens_get_tiling(const swm_ens *ens, int *tile_rows, int *metric_on_the_fly, int *blocks_per_slot);
 int swm_ens_autotune(swm_ens *ens, int reps, double *best_us);

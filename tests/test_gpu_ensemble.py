@@ -28,7 +28,8 @@ def _dt(N):
     return 0.5 * A * (np.pi / 2) / N / (np.sqrt(G0 * 1.0e4) + 40.0)
 
 
-@pytest.mark.parametrize("tiling", [(8, False, 1), (16, True, 2), (16, False, 4), "auto"])
+@pytest.mark.parametrize("tiling", [(8, False, 1), (16, True, 2), (16, False, 4), (8, False, 2, 32), (8, True, 1, 32),
+                                    "auto"])
 def test_ensemble_equals_single_cube_path(tiling):
     Ns = [16, 24, 33, 48]
     swm, ops, xs, ics = _members(Ns, 42)
@@ -76,3 +77,41 @@ def test_ensemble_rejects_mixed_families_and_shared_members():
         swm.Ensemble([a, b], [xa, xb])
     with pytest.raises(swm.SwmError):
         swm.Ensemble([a, a], [xa, swm.create_stvec_swm(a)])
+
+
+def test_ensemble_follows_orography_changes():
+    """swm_set_orography replaces a member's orography buffer: ensemble steps before and after
+    the change use the member's current orography (the single-cube path with the same
+    orography sequence gives the same states)."""
+    Ns = [20, 28]
+    swm, ops, xs, ics = _members(Ns, 42, seed0=30)
+    ens = swm.Ensemble(ops, xs)
+    dts = [_dt(N) for N in Ns]
+
+    def mountain(op, amp):
+        f = np.zeros(op.host_shape())
+        f[2, op.n // 3: 2 * op.n // 3, op.n // 4: op.n // 2] = amp
+        return f
+
+    ens.steps(dts, 2)
+    ops[1].set_orography(mountain(ops[1], 1500.0))
+    ens.steps(dts, 2)
+    ops[1].set_orography(mountain(ops[1], 700.0))
+    ops[0].set_orography(mountain(ops[0], 300.0))
+    ens.steps(dts, 2)
+    refs = []
+    for k, (op, ic, dt) in enumerate(zip(ops, ics, dts)):
+        op.set_orography(None)
+        y = swm.create_stvec_swm(op)
+        y.upload(*ic)
+        ts = swm.create_timescheme(op, "rk4")
+        ts.steps(y, dt, 2)
+        if k == 1:
+            op.set_orography(mountain(op, 1500.0))
+        ts.steps(y, dt, 2)
+        op.set_orography(mountain(op, 700.0 if k == 1 else 300.0))
+        ts.steps(y, dt, 2)
+        refs.append(y.download())
+    for x, ref in zip(xs, refs):
+        for a, b in zip(x.download(), ref):
+            assert np.array_equal(a, b)

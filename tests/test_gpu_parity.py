@@ -32,7 +32,7 @@ def _relerr(G, a, b):
     return float(np.max(np.abs(ai - bi)) / max(np.max(np.abs(bi)), 1e-300))
 
 
-@pytest.mark.parametrize("tiling", [(8, False), (16, True), (32, False)])
+@pytest.mark.parametrize("tiling", [(8, False), (16, True), (32, False), (8, False, 32)])
 @pytest.mark.parametrize("N,order", [(32, 42), (17, 42), (8, 42), (24, 43), (24, 63), (70, 63), (32, 21), (9, 21),
                                      (70, 21)])
 def test_rhs_matches_oracle(oracle_lib, N, order, tiling):
@@ -53,7 +53,8 @@ def test_rhs_matches_oracle(oracle_lib, N, order, tiling):
     assert max(errs) < 1e-13
 
 
-@pytest.mark.parametrize("tiling", [(8, False), (16, False), (8, True), (16, True), (32, False), (32, True)])
+@pytest.mark.parametrize("tiling", [(8, False), (16, False), (8, True), (16, True), (32, False), (32, True),
+                                    (8, False, 32), (8, True, 32)])
 @pytest.mark.parametrize("N,order,nsteps,dt", [(32, 42, 10, 900.0), (24, 63, 4, 900.0), (40, 43, 3, 600.0),
                                                (32, 21, 10, 900.0)])
 def test_rk4_matches_oracle(oracle_lib, N, order, nsteps, dt, tiling):
@@ -154,3 +155,95 @@ def test_bad_operator_name_is_an_error():
     with pytest.raises(swm.SwmError):
         swm.create_swm_operator(16, "divergence_ah99", "gradient_ah42_sbp_ecs", "curl_divergence_ah99",
                                 "SBP_Ah42_quadrature")
+
+
+def test_assign_update_match_numpy(oracle_lib):
+    """stvec assign / update (stvec_swm_mod.f90:38-106 -> grid_field_mod.f90:252-323,430-485):
+    y = s1*v1 + s2*v2 and y = y + s1*v1 + s2*v2 on every mesh point, bit for bit with the same
+    FP64 expressions evaluated by numpy (no fused multiply-add on either side)."""
+    swm, op, G, h, u, v = _setup(oracle_lib, 24, 42, seed=12)
+    rng = np.random.default_rng(3)
+    n = G.n
+    fields = [[rng.standard_normal((6, n, n)) * s for s in (1e4, 40.0, 40.0)] for _ in range(3)]
+    xs = []
+    for f in fields:
+        x = swm.create_stvec_swm(op)
+        x.upload(*f)
+        xs.append(x)
+    y, v1, v2 = xs
+    s1, s2 = 0.3712, -1.0 / 6.0
+    y.assign(s1, v1, s2, v2)
+    for got, a, b in zip(y.download(), fields[1], fields[2]):
+        assert np.array_equal(got, s1 * a + s2 * b)
+    y0 = [s1 * a + s2 * b for a, b in zip(fields[1], fields[2])]
+    t1, t2 = 75.0 / 3.0, 75.0 / 6.0
+    y.update(t1, v1, t2, v2)
+    for got, c, a, b in zip(y.download(), y0, fields[1], fields[2]):
+        assert np.array_equal(got, c + t1 * a + t2 * b)
+    # a state combined with itself (v1 == v2 == y) as the RK4 composition does with y and v0
+    y.assign(1.0, v1, 0.5, v1)
+    for got, a in zip(y.download(), fields[1]):
+        assert np.array_equal(got, 1.0 * a + 0.5 * a)
+
+
+def _orography(G, amp=2000.0):
+    """A smooth mountain (Williamson test 5 shape, 2000 m) at mesh points."""
+    x, y, z = G.geometry("r")
+    r = np.sqrt(x * x + y * y + z * z)
+    lam0, phi0, R = 1.5 * np.pi, np.pi / 6, np.pi / 9
+    c = (np.cos(phi0) * np.cos(lam0), np.cos(phi0) * np.sin(lam0), np.sin(phi0))
+    d = np.arccos(np.clip((x * c[0] + y * c[1] + z * c[2]) / r, -1.0, 1.0))
+    return G.embed(G.interior(amp * np.maximum(0.0, 1.0 - d / R)))
+
+
+@pytest.mark.parametrize("N,order", [(32, 42), (24, 63), (32, 21)])
+def test_orography_rhs_rk4_diagnostics_match_oracle(oracle_lib, N, order):
+    """g*hs enters the kinetic-energy term (operator_swm_mod.f90:85) and the potential energy
+    of the diagnostics (operator_swm_mod.f90:210-230): RHS and RK4 bit for bit, diagnostics
+    to 1e-12 with the orography set."""
+    swm, op, G, h, u, v = _setup(oracle_lib, N, order, seed=21)
+    hs = _orography(G)
+    op.set_orography(hs, G.hw)
+    print(f"[orography] N={N} order={order}", flush=True)
+    x = swm.create_stvec_swm(op)
+    k = x.create_similar()
+    x.upload(h, u, v, G.hw)
+    op.apply(k, x)
+    ref = G.rhs(h.copy(), u.copy(), v.copy(), hs=hs)
+    for a, b in zip(k.download(G.hw), ref):
+        assert np.array_equal(G.interior(a), G.interior(b))
+    # the orography changes the tendencies (the test is not vacuous)
+    ref0 = G.rhs(h.copy(), u.copy(), v.copy())
+    assert not np.array_equal(G.interior(ref0[1]), G.interior(ref[1]))
+    d = op.get_diagnostics(x)
+    dref = G.diagnostics({42: 42, 63: 63, 21: 21}[order], h.copy(), u.copy(), v.copy(), hs=hs)
+    ours = [d[key] for key in ("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy")]
+    for a, b in zip(ours, dref):
+        assert abs(a - b) <= 1e-12 * abs(b)
+    swm.create_timescheme(op, "rk4").steps(x, 600.0, 4)
+    G.rk4(h, u, v, 600.0, 4, hs=hs)
+    for a, b in zip(x.download(G.hw), (h, u, v)):
+        assert np.array_equal(G.interior(a), G.interior(b))
+    # removing the orography restores the plain path
+    op.set_orography(None)
+    x.upload(h, u, v, G.hw)
+    op.apply(k, x)
+    for a, b in zip(k.download(G.hw), G.rhs(h.copy(), u.copy(), v.copy())):
+        assert np.array_equal(G.interior(a), G.interior(b))
+
+
+def test_multi_step_graph_equals_single_steps(oracle_lib):
+    """Runs of 10 steps replay a 10-step graph; 23 steps in one call (two 10-step replays and
+    three single steps) equal 23 calls of one step and the oracle."""
+    swm, op, G, h, u, v = _setup(oracle_lib, 16, 42, seed=8)
+    a, b = swm.create_stvec_swm(op), swm.create_stvec_swm(op)
+    a.upload(h, u, v, G.hw)
+    b.upload(h, u, v, G.hw)
+    ts = swm.create_timescheme(op, "rk4")
+    ts.steps(a, 900.0, 23)
+    for _ in range(23):
+        ts.steps(b, 900.0, 1)
+    G.rk4(h, u, v, 900.0, 23)
+    for fa, fb, fr in zip(a.download(G.hw), b.download(G.hw), (h, u, v)):
+        assert np.array_equal(G.interior(fa), G.interior(fb))
+        assert np.array_equal(G.interior(fa), G.interior(fr))
