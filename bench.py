@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--lmax", type=int, default=32)
     ap.add_argument("--precision", type=int, default=64, choices=(32, 64))
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--autotune-reps", type=int, default=30, help="RK4 steps per tiling candidate in the autotuner")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,adjoint,batched,hyperdiffusion",
@@ -434,7 +435,7 @@ def main():
     x = swm.create_stvec_swm(op)
     x.upload(h, u, v)
     if args.tiling == "auto":
-        tuned_us = op.autotune(x, reps=30)
+        tuned_us = op.autotune(x, reps=args.autotune_reps)
     else:
         shape, _, otf = args.tiling.partition(",")
         cols, _, rows = shape.partition("x")
