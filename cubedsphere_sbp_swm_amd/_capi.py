@@ -33,6 +33,7 @@ SIGNATURES = {
     "swm_update": (_i, [_vp, _vp, _d, _vp, _d, _vp]),
     "swm_rk4_steps": (_i, [_vp, _vp, _d, _i]),
     "swm_diagnostics": (_i, [_vp, _vp, _dp]),
+    "swm_diagnostics_noalloc": (_i, [_vp, _vp, _vp]),
     "swm_set_orography": (_i, [_vp, _dp, _i]),
     "swm_ic_spherical_harmonics": (_i, [_vp, ctypes.c_ulonglong, _i, _d, _d, _d, _dp, _dp, _dp, _i]),
     "swm_ctx_set_tile_shape": (_i, [_vp, _i, _i, _i]),

@@ -95,7 +95,81 @@ __global__ void __launch_bounds__(64) diag_final(const double *part, int nblocks
     out[NQ + 1] = hmax;
 }
 
+// the seven diagnostics in the order of swm_diagnostics from the sums of diag_final (the same
+// FP64 additions swm_diagnostics makes on the host)
+__global__ void __launch_bounds__(64) diag_order(const double *r, double *out) {
+    if (threadIdx.x != 0) return;
+    const double ke = r[1] + r[2];
+    out[0] = r[5];
+    out[1] = r[6];
+    out[2] = r[0];
+    out[3] = ke + r[3];
+    out[4] = ke;
+    out[5] = r[3];
+    out[6] = r[4];
+}
+
+// Scratch of swm_diagnostics_noalloc: the per-block partial sums of diag_partial (16 doubles per
+// block, at most NOALLOC_BLOCKS blocks) and the 8 sums of diag_final, 128.1 KiB per device.  Every
+// call on a device shares it.
+constexpr int NOALLOC_BLOCKS = 1024;
+constexpr int NOALLOC_DOUBLES = 16 * NOALLOC_BLOCKS + 16;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
 }  // namespace swm
+
+namespace {
+
+// device address of swm::scratch on each device, taken once per device with that device current
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+int noalloc_scratch(double **ptr) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(SWM_EARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(swm::scratch)));
+        if (!p) return fail(SWM_EHIP, "hipGetSymbolAddress returned NULL for the diagnostics scratch");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return SWM_OK;
+}
+
+}  // namespace
+
+// The diagnostics with the partial sums in the device scratch above: the same kernels as
+// swm_diagnostics, no allocation and no synchronisation; the seven values are written to the
+// device array `out` in stream order on the context stream.
+extern "C" int swm_diagnostics_noalloc(swm_ctx *c, const void *state, double *out) {
+    if (!c || !state || !out) return fail(SWM_EARG, "NULL argument");
+    if (c->red_blocks > swm::NOALLOC_BLOCKS)
+        return fail(SWM_EARG, "diagnostics use more partial sums than the no-allocation scratch holds (1024 blocks)");
+    double *scr = nullptr;
+    int rc = noalloc_scratch(&scr);
+    if (rc) return rc;
+    rc = run_stage(c, swm::MODE_CURL, state, c->curl, nullptr, nullptr, 0.0, 0.0);
+    if (rc) return rc;
+    double *part = scr, *res = scr + 16 * swm::NOALLOC_BLOCKS;
+    if (c->prec == 64)
+        hipLaunchKernelGGL(swm::diag_partial<double>, dim3(c->red_blocks), dim3(256), 0, c->stream, (const double *)state,
+                           (const double *)c->curl, c->J64, c->Qi64[0], c->Qi64[1], c->Qi64[2], c->f64, c->A64, c->hs64,
+                           c->n, c->pitch, c->fs, c->hx, c->grav, part);
+    else
+        hipLaunchKernelGGL(swm::diag_partial<float>, dim3(c->red_blocks), dim3(256), 0, c->stream, (const float *)state,
+                           (const float *)c->curl, c->J64, c->Qi64[0], c->Qi64[1], c->Qi64[2], c->f64, c->A64, c->hs64,
+                           c->n, c->pitch, c->fs, c->hx, c->grav, part);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(swm::diag_final, dim3(1), dim3(64), 0, c->stream, part, c->red_blocks, res);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(swm::diag_order, dim3(1), dim3(64), 0, c->stream, res, out);
+    HIP_TRY(hipGetLastError());
+    return SWM_OK;
+}
 
 extern "C" int swm_diagnostics(swm_ctx *c, const void *state, double out[7]) {
     if (!c || !state || !out) return fail(SWM_EARG, "NULL argument");

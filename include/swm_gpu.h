@@ -84,6 +84,13 @@ int swm_rk4_steps(swm_ctx *ctx, void *state, double dt, int nsteps);
  * out = {hmin, hmax, mass, TE, KE, PE, enstrophy}; integrals use the context's quadrature
  * with compensated (Neumaier) FP64 summation */
 int swm_diagnostics(swm_ctx *ctx, const void *state, double out[7]);
+/* No counterpart in the reference: swm_diagnostics for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops).  Same kernels and the same seven values, written to
+ * the device array `out` (7 doubles) in stream order on the context stream, with no allocation
+ * and no synchronisation: the partial sums live in a library-owned device scratch of 128.1 KiB
+ * per GPU (at most 1024 blocks of partial sums; a context asking for more is rejected) that
+ * every call on that GPU shares, so calls that overlap on one GPU race. */
+int swm_diagnostics_noalloc(swm_ctx *ctx, const void *state, double *out);
 
 /* orography h_surf (operator_swm_factory_mod.f90:115-116); NULL removes it */
 int swm_set_orography(swm_ctx *ctx, const double *hs, int halo_width);
