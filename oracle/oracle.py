@@ -77,6 +77,28 @@ def lib():
         L.ora_ecs_halo_vec_a.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
         L.ora_ecs_halo_vec_a.restype = ctypes.c_int
         L.ora_rk4_iomega.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double]
+        L.ora_cgrid_create.restype = ctypes.c_void_p
+        L.ora_cgrid_create.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.ora_cgrid_destroy.argtypes = [ctypes.c_void_p]
+        L.ora_cgrid_grid.restype = ctypes.c_void_p
+        L.ora_cgrid_grid.argtypes = [ctypes.c_void_p]
+        L.ora_cgrid_geometry.restype = ctypes.c_int
+        L.ora_cgrid_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.ora_c_co2contra.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.ora_c_massflux.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.ora_c_KE.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.ora_c_grad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.ora_c_curl.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.ora_c_coriolis.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ora_c_div.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.ora_c_exchange_o.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        L.ora_c_exchange_vec.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.ora_c_rhs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ora_c_rk4.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int]
+        L.ora_c_l2norm.restype = ctypes.c_double
+        L.ora_c_l2norm.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.ora_c_maxabs.restype = ctypes.c_double
+        L.ora_c_maxabs.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -320,3 +342,119 @@ def rk4_iomega(w, f, dt):
     fr, fi = np.ascontiguousarray(f.real), np.ascontiguousarray(f.imag)
     lib().ora_rk4_iomega(len(w), _p(wr), _p(wi), _p(fr), _p(fi), dt)
     return fr + 1j * fi
+
+
+C_TYPES = {"o": 0, "x": 1, "y": 2, "q": 3}
+C_GEO = {"J": 0, "Qi11": 1, "Qi12": 2, "Qi22": 3, "f": 4, "r": 5, "a1": 6, "a2": 7, "b1": 8, "b2": 9}
+
+
+class CGrid:
+    """C-grid (staggering 'C') storage on one cubed sphere of N cells per panel side: h at o
+    points (N x N), u at x points (N+1 x N), v at y points (N x N+1), curl at q points
+    (N+1 x N+1).  Arrays have the (6, ld, ld) layout of the Ah grid of the same N; index (i, j)
+    of every point type sits at [.., j-1+hw, i-1+hw]."""
+
+    def __init__(self, N, scale=1.0, omega=1.0, grav=1.0):
+        self.N = N
+        self.scale, self.omega, self.grav = scale, omega, grav
+        self.h = lib().ora_cgrid_create(N, scale, omega, grav)
+        if not self.h:
+            raise RuntimeError("ora_cgrid_create failed")
+        g = lib().ora_cgrid_grid(self.h)
+        self.hw = lib().ora_hw(g)
+        self.ld = lib().ora_ld(g)
+        self.shape = (6, self.ld, self.ld)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_cgrid_destroy(self.h)
+            self.h = None
+
+    def zeros(self):
+        return np.zeros(self.shape)
+
+    def dims(self, t):
+        t = C_TYPES[t]
+        return self.N + (t in (1, 3)), self.N + (t in (2, 3))   # (nx, ny)
+
+    def interior(self, t, f):
+        nx, ny = self.dims(t)
+        return f[:, self.hw:self.hw + ny, self.hw:self.hw + nx]
+
+    def embed(self, t, a):
+        f = self.zeros()
+        self.interior(t, f)[...] = a
+        return f
+
+    def geometry(self, t, name):
+        tt, w = C_TYPES[t], C_GEO[name]
+        if name in ("r", "a1", "a2", "b1", "b2"):
+            out = [self.zeros() for _ in range(3)]
+            for c in range(3):
+                lib().ora_cgrid_geometry(self.h, tt, w, c, _p(out[c]))
+            return out
+        out = self.zeros()
+        lib().ora_cgrid_geometry(self.h, tt, w, 0, _p(out))
+        return out
+
+    def project(self, t, kind, vx, vy, vz, comp):
+        """Component `comp` (0 along x, 1 along y) of the cartesian vectors at points of type t:
+        sum([vx,vy,vz]*a1) for covariant, *b1 for contravariant (test_fields_mod.f90:397-433)."""
+        e = self.geometry(t, ("a1" if comp == 0 else "a2") if kind == "covariant" else
+                          ("b1" if comp == 0 else "b2"))
+        return vx * e[0] + vy * e[1] + vz * e[2]
+
+    def co2contra(self, u, v):
+        ut, vt = self.zeros(), self.zeros()
+        lib().ora_c_co2contra(self.h, _p(u), _p(v), _p(ut), _p(vt))
+        return ut, vt
+
+    def massflux(self, h, ut, vt):
+        hu, hv = self.zeros(), self.zeros()
+        lib().ora_c_massflux(self.h, _p(h), _p(ut), _p(vt), _p(hu), _p(hv))
+        return hu, hv
+
+    def KE(self, u, v, ut, vt):
+        out = self.zeros()
+        lib().ora_c_KE(self.h, _p(u), _p(v), _p(ut), _p(vt), _p(out))
+        return out
+
+    def grad(self, f):
+        gx, gy = self.zeros(), self.zeros()
+        lib().ora_c_grad(self.h, _p(f), _p(gx), _p(gy))
+        return gx, gy
+
+    def curl(self, u, v):
+        out = self.zeros()
+        lib().ora_c_curl(self.h, _p(u), _p(v), _p(out))
+        return out
+
+    def coriolis(self, hu, hv, h, curl):
+        cu, cv = self.zeros(), self.zeros()
+        lib().ora_c_coriolis(self.h, _p(hu), _p(hv), _p(h), _p(curl), _p(cu), _p(cv))
+        return cu, cv
+
+    def div(self, u, v):
+        out = self.zeros()
+        lib().ora_c_div(self.h, _p(u), _p(v), _p(out))
+        return out
+
+    def exchange_o(self, f, depth):
+        lib().ora_c_exchange_o(self.h, _p(f), int(depth))
+
+    def exchange_vec(self, u, v, depth):
+        lib().ora_c_exchange_vec(self.h, _p(u), _p(v), int(depth))
+
+    def rhs(self, h, u, v, hs=None):
+        dh, du, dv = self.zeros(), self.zeros(), self.zeros()
+        lib().ora_c_rhs(self.h, _p(h), _p(u), _p(v), _pn(hs), _p(dh), _p(du), _p(dv))
+        return dh, du, dv
+
+    def rk4(self, h, u, v, dt, nsteps, hs=None):
+        lib().ora_c_rk4(self.h, _p(h), _p(u), _p(v), _pn(hs), dt, nsteps)
+
+    def l2norm(self, t, f):
+        return lib().ora_c_l2norm(self.h, C_TYPES[t], _p(f))
+
+    def maxabs(self, t, f):
+        return lib().ora_c_maxabs(self.h, C_TYPES[t], _p(f))

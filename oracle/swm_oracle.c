@@ -1742,3 +1742,705 @@ void ora_rk4_iomega(int n, const double *wr, const double *wi, double *fr, doubl
     free(v0); free(y);
     for (int s = 0; s < 4; s++) free(k[s]);
 }
+
+/* =====================================================================================
+ * C grid (staggering 'C'), SBP42 vector-invariant family: the &shallow_water_model_c set
+ * (namelist_swm:80-102) of divergence_c_sbp42, gradient_c_sbp42, curl_c_sbp42,
+ * coriolis_Cgrid_sbp42, KE_Cgrid_sbp42, co2contra_c_sbp42, massflux_c_sbp42.
+ *
+ * Points (mesh_factory_mod.f90:34-47, mesh_mod.f90:83-92): alpha = alpha0 + (i-1+shift_i)*hx,
+ * type 0 = o (cell centres, shifts 1/2, 1/2; i, j = 1..N), 1 = x (u points, 0, 1/2; i = 1..N+1),
+ * 2 = y (v points, 1/2, 0; j = 1..N+1), 3 = q (xy corners, 0, 0; i, j = 1..N+1).  Fields use the
+ * storage of the Ah grid of the same N (halo 4): index (i, j) of any type at IDX(g, p, i, j).
+ * ===================================================================================== */
+
+struct ora_cgrid {
+    ora_grid *g;                          /* Ah grid of the same N: layout, q-point scalar sync */
+    int N;
+    double *J[4], *Qi[4][3], *f[4];       /* per point type */
+    double *r[4][3], *a1[4][3], *a2[4][3], *b1[4][3], *b2[4][3];
+};
+
+static int c_nx(int N, int t) { return N + (t == 1 || t == 3); }
+static int c_ny(int N, int t) { return N + (t == 2 || t == 3); }
+
+static void c_build_geometry(ora_cgrid *c) {
+    ora_grid *g = c->g;
+    const double pi = pi_(), alpha0 = -0.25 * pi;
+    for (int t = 0; t < 4; t++) {
+        const double si = (t == 0 || t == 2) ? 0.5 : 0.0, sj = (t == 0 || t == 1) ? 0.5 : 0.0;
+        c->J[t] = alloc_field(g); c->f[t] = alloc_field(g);
+        for (int k = 0; k < 3; k++) {
+            c->Qi[t][k] = alloc_field(g);
+            c->r[t][k] = alloc_field(g); c->a1[t][k] = alloc_field(g); c->a2[t][k] = alloc_field(g);
+            c->b1[t][k] = alloc_field(g); c->b2[t][k] = alloc_field(g);
+        }
+        for (int p = 0; p < 6; p++)
+            for (int j = 1 - g->hw; j <= g->n + g->hw; j++) {
+                const double beta = alpha0 + ((double)(j - 1) + sj) * g->hx;
+                for (int i = 1 - g->hw; i <= g->n + g->hw; i++) {
+                    const double alpha = alpha0 + ((double)(i - 1) + si) * g->hx;
+                    const long id = IDX(g, p, i, j);
+                    const double ta = tan(alpha), tb = tan(beta);
+                    const double sg = sqrt(1.0 + ta * ta + tb * tb);
+                    c->J[t][id] = (1.0 + ta * ta) * (1.0 + tb * tb) / (sg * sg * sg);
+                    c->Qi[t][0][id] = sg * sg / (1.0 + ta * ta);
+                    c->Qi[t][1][id] = sg * sg * ta * tb / ((1 + ta * ta) * (1 + tb * tb));
+                    c->Qi[t][2][id] = sg * sg / (1.0 + tb * tb);
+                    double r0[3], r[3], v[3], w3[3];
+                    ecs_r_proto(alpha, beta, r0);
+                    proto2real(p, r0, r);
+                    for (int k = 0; k < 3; k++) c->r[t][k][id] = r[k];
+                    /* calc_coriolis_parameter (coriolis_factory_mod.f90:151-172) */
+                    double dn = g->axis[0] * r[0] + g->axis[1] * r[1] + g->axis[2] * r[2];
+                    c->f[t][id] = 2 * g->omega * (dn / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
+                    ecs_a1_proto(alpha, beta, v); proto2real(p, v, w3);
+                    for (int k = 0; k < 3; k++) c->a1[t][k][id] = w3[k];
+                    ecs_a2_proto(alpha, beta, v); proto2real(p, v, w3);
+                    for (int k = 0; k < 3; k++) c->a2[t][k][id] = w3[k];
+                    ecs_b1_proto(alpha, beta, v); proto2real(p, v, w3);
+                    for (int k = 0; k < 3; k++) c->b1[t][k][id] = w3[k];
+                    ecs_b2_proto(alpha, beta, v); proto2real(p, v, w3);
+                    for (int k = 0; k < 3; k++) c->b2[t][k][id] = w3[k];
+                }
+            }
+    }
+}
+
+/* Fold over a cube edge: the point at doubled depth d2 >= 0 beyond edge e (0 i < 1, 1 i > nx,
+ * 2 j < 1, 3 j > ny) of panel p, at doubled tangential coordinate T2 along that edge, lies on
+ * panel q at doubled coordinates (a, b) in [0, 2N] (cube of doubled lattice units, corner 2N*RV,
+ * axes EX, EY, inner normal NV).  Points on the edge itself (d2 = 0) give the neighbour's copy;
+ * points on the neighbour's far edges (T2 = 0 or 2N) stay on the neighbour across e. */
+static void c_fold(int N, int p, int e, int d2, int T2, int *q_out, int *a_out, int *b_out) {
+    int P[3];
+    const int X = e == 0 ? 0 : e == 1 ? 2 * N : T2, Y = e == 2 ? 0 : e == 3 ? 2 * N : T2;
+    for (int k = 0; k < 3; k++) P[k] = X * EX[p][k] + Y * EY[p][k] + 2 * N * RV[p][k] + d2 * NV[p][k];
+    /* the neighbour across edge e: inner normal +EX[p] (e = 0), -EX[p], +EY[p], -EY[p] (e = 3) */
+    int nq[3];
+    for (int k = 0; k < 3; k++) nq[k] = e == 0 ? EX[p][k] : e == 1 ? -EX[p][k] : e == 2 ? EY[p][k] : -EY[p][k];
+    for (int q = 0; q < 6; q++) {
+        if (q == p || NV[q][0] != nq[0] || NV[q][1] != nq[1] || NV[q][2] != nq[2]) continue;
+        int D[3];
+        for (int k = 0; k < 3; k++) D[k] = P[k] - 2 * N * RV[q][k];
+        if (D[0] * NV[q][0] + D[1] * NV[q][1] + D[2] * NV[q][2] != 0) continue;
+        const int a = D[0] * EX[q][0] + D[1] * EX[q][1] + D[2] * EX[q][2];
+        const int b = D[0] * EY[q][0] + D[1] * EY[q][1] + D[2] * EY[q][2];
+        if (a < 0 || a > 2 * N || b < 0 || b > 2 * N) continue;
+        *q_out = q; *a_out = a; *b_out = b;
+        return;
+    }
+    abort();
+}
+
+/* component of a local axis (0 x, 1 y) at a point beyond edge e of panel p on the neighbour q:
+ * the axis normal to the edge points towards p (-NV[p] on q) beyond edges 0 and 2, away from it
+ * beyond 1 and 3; the tangential axis is unchanged.  Returns +-1 and the neighbour's axis. */
+static void c_axis_map(int p, int e, int axis, int q, int *sign, int *qaxis) {
+    int d[3];
+    const int normal = (e < 2) ? 0 : 1;
+    for (int k = 0; k < 3; k++) {
+        if (axis == normal) d[k] = (e == 0 || e == 2) ? -NV[p][k] : NV[p][k];
+        else d[k] = axis == 0 ? EX[p][k] : EY[p][k];
+    }
+    const int sx = idot(d, EX[q]), sy = idot(d, EY[q]);
+    if (sx != 0) { *sign = sx; *qaxis = 0; }
+    else { *sign = sy; *qaxis = 1; }
+}
+
+/* O-point halo (create_o_points_halo_exchange, 'cross'): cells at depth 1..depth beyond each
+ * edge <- the neighbour's cells at the same place */
+static void c_exchange_o(const ora_cgrid *c, double *f, int depth) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    for (int p = 0; p < 6; p++)
+        for (int e = 0; e < 4; e++)
+            for (int d = 1; d <= depth; d++)
+                for (int t = 1; t <= N; t++) {
+                    int q, a, b;
+                    c_fold(N, p, e, 2 * d - 1, 2 * t - 1, &q, &a, &b);
+                    const int i = e == 0 ? 1 - d : e == 1 ? N + d : t, j = e == 2 ? 1 - d : e == 3 ? N + d : t;
+                    f[IDX(g, p, i, j)] = f[IDX(g, q, (a + 1) / 2, (b + 1) / 2)];
+                }
+}
+
+/* C-grid vector halo (create_symmetric_halo_vec_exchange_C with pack_to_buf_vec): points of u
+ * (x points) and v (y points) in the strips beyond each edge <- the neighbour's component along
+ * the same direction, at the same place.  The points on faces normal to an edge are shifted by
+ * one: index 0 (and nx+1) of the x points beyond the x edges holds the neighbour's copy of the
+ * shared face, index -1 its first face inside (the halo of the face-staggered points is one
+ * wider across panels, exchange_factory_mod.f90:402-403,420-423,544-545). */
+static void c_exchange_vec(const ora_cgrid *c, double *u, double *v, int depth) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    double *fld[2] = {u, v};
+    for (int comp = 0; comp < 2; comp++) {
+        const int t = comp == 0 ? 1 : 2;
+        const int nx = c_nx(N, t), ny = c_ny(N, t);
+        for (int p = 0; p < 6; p++)
+            for (int e = 0; e < 4; e++) {
+                const int xn = e < 2;                                 /* edge normal is x */
+                const int faces = (xn && t == 1) || (!xn && t == 2);  /* points on faces normal to the edge */
+                const int ntan = xn ? ny : nx;
+                for (int d = 1; d <= depth; d++)
+                    for (int k = 1; k <= ntan; k++) {
+                        const int d2 = faces ? 2 * (d - 1) : 2 * d - 1;
+                        const int T2 = xn ? (t == 1 ? 2 * k - 1 : 2 * (k - 1)) : (t == 1 ? 2 * (k - 1) : 2 * k - 1);
+                        int i, j;
+                        if (e == 0) { i = 1 - d; j = k; }
+                        else if (e == 1) { i = nx + d; j = k; }
+                        else if (e == 2) { i = k; j = 1 - d; }
+                        else { i = k; j = ny + d; }
+                        int q, a, b, sgn, qaxis;
+                        c_fold(N, p, e, d2, T2, &q, &a, &b);
+                        c_axis_map(p, e, comp, q, &sgn, &qaxis);
+                        if (qaxis == 0 ? (a % 2 != 0 || b % 2 != 1) : (a % 2 != 1 || b % 2 != 0)) abort();
+                        const int qi = qaxis == 0 ? a / 2 + 1 : (a + 1) / 2, qj = qaxis == 0 ? (b + 1) / 2 : b / 2 + 1;
+                        /* only mesh points of the neighbour are read, never its halo */
+                        const double val = fld[qaxis][IDX(g, q, qi, qj)];
+                        fld[comp][IDX(g, p, i, j)] = sgn < 0 ? -val : val;
+                    }
+            }
+    }
+}
+
+/* ---- staggered SBP42 operators: sbp_operators_collection_mod.f90:100-191 (opt1 interpolators,
+ * staggered differences), built as sbp_factory_mod.f90:104-200 ---- */
+typedef struct {
+    int nw, ne, win, inshift, dnx, nproj, nA;
+    double Ql[6 * 4], Qr[6 * 4], Qin[4], proj_l[3], proj_r[3], Al_out[5], Ar_out[5];
+    int last[4], rshift[4];
+} c_sop;
+
+static void c_sop_finish(c_sop *o, double sign) {
+    for (int ir = 1; ir <= o->ne; ir++) {
+        for (int m = 1; m <= o->nw; m++)
+            o->Qr[(m - 1) + o->nw * (ir - 1)] = sign * o->Ql[(o->nw - m) + o->nw * (o->ne - ir)];
+        o->rshift[ir - 1] = -o->last[o->ne - ir] + 1;
+    }
+    for (int k = 0; k < o->nproj; k++) o->proj_r[k] = o->proj_l[o->nproj - 1 - k];
+    for (int k = 0; k < o->nA; k++) o->Ar_out[k] = o->Al_out[o->nA - 1 - k];
+}
+
+/* kind: 0 W42 interfaces->centres (i2c), 1 W42 centres->interfaces (c2i), 2 D42 c2i, 3 D42 i2c */
+static void c_sop_make(c_sop *o, int kind) {
+    memset(o, 0, sizeof *o);
+    if (kind == 0) {
+        static const double W[24] = {
+            0.4892964885611128, 0.48123107353632727, 0.0696483872440123, -0.040175949341450676, 0.0, 0.0,
+            -0.07913099957454726, 0.6127279550862234, 0.5119370885511979, -0.045534044062872786, 0.0, 0.0,
+            -0.0869982983587545, 0.1691760860218048, 0.3626427230326682, 0.6151794893042863, -0.06, 0.0,
+            0.018680545032460714, -0.10869571330417979, 0.09884979151099299, 0.4911653767607321, 0.5625, -0.0625};
+        static const int L[4] = {4, 4, 5, 6};
+        o->nw = 6; o->ne = 4; memcpy(o->Ql, W, sizeof W); memcpy(o->last, L, sizeof L);
+        o->inshift = -1; o->dnx = -1;
+        o->win = 4; o->Qin[0] = -1.0 / 16.0; o->Qin[1] = 9.0 / 16.0; o->Qin[2] = 9.0 / 16.0; o->Qin[3] = -1.0 / 16.0;
+        c_sop_finish(o, 1.0);
+    } else if (kind == 1) {
+        static const double W[20] = {
+            1.3630402181345285, -0.17804474904273135, -0.23303115631809243, 0.048035687226327554, 0.0,
+            0.46340770044238916, 0.4765661872892848, 0.15664452409426372, -0.09661841182593758, 0.0,
+            0.07545241951434666, 0.44794495248229815, 0.37775283649236274, 0.09884979151099299, 0.0,
+            -0.04413695843145285, -0.04040344754874627, 0.649837488701711, 0.4980831989686297, -0.06338028169014084};
+        static const int L[4] = {4, 4, 4, 5};
+        o->nw = 5; o->ne = 4; memcpy(o->Ql, W, sizeof W); memcpy(o->last, L, sizeof L);
+        o->inshift = -2; o->dnx = 1;
+        o->win = 4; o->Qin[0] = -1.0 / 16.0; o->Qin[1] = 9.0 / 16.0; o->Qin[2] = 9.0 / 16.0; o->Qin[3] = -1.0 / 16.0;
+        c_sop_finish(o, 1.0);
+    } else {
+        const double ci2[20] = {-2.0, 3.0, -1.0, 0.0, 0.0,
+                                -1.0, 1.0, 0.0, 0.0, 0.0,
+                                1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0, 0.0,
+                                -1.0 / 71.0, 6.0 / 71.0, -83.0 / 71.0, 81.0 / 71.0, -3.0 / 71.0};
+        const double ic2[15] = {-79.0 / 78.0, 27.0 / 26.0, -1.0 / 26.0, 1.0 / 78.0, 0.0,
+                                2.0 / 21.0, -9.0 / 7.0, 9.0 / 7.0, -2.0 / 21.0, 0.0,
+                                1.0 / 75.0, 0.0, -27.0 / 25.0, 83.0 / 75.0, -1.0 / 25.0};
+        const double Ai[5] = {7.0 / 18.0, 9.0 / 8.0, 1.0, 71.0 / 72.0, 1.0};
+        const double Ac[4] = {13.0 / 12.0, 7.0 / 8.0, 25.0 / 24.0, 1.0};
+        o->nw = 5;
+        o->win = 4; o->Qin[0] = 1.0 / 24.0; o->Qin[1] = -9.0 / 8.0; o->Qin[2] = 9.0 / 8.0; o->Qin[3] = -1.0 / 24.0;
+        o->nproj = 3; o->proj_l[0] = 15.0 / 8.0; o->proj_l[1] = -5.0 / 4.0; o->proj_l[2] = 3.0 / 8.0;
+        if (kind == 2) {
+            static const int L[4] = {3, 2, 4, 5};
+            o->ne = 4; memcpy(o->Ql, ci2, sizeof ci2); memcpy(o->last, L, sizeof L);
+            o->inshift = -2; o->dnx = 1; o->nA = 5; memcpy(o->Al_out, Ai, sizeof Ai);
+        } else {
+            static const int L[3] = {4, 4, 5};
+            o->ne = 3; memcpy(o->Ql, ic2, sizeof ic2); memcpy(o->last, L, sizeof L);
+            o->inshift = -1; o->dnx = -1; o->nA = 4; memcpy(o->Al_out, Ac, sizeof Ac);
+        }
+        c_sop_finish(o, -1.0);
+    }
+}
+
+/* one output row of sbp_apply (sbp_operator_mod.f90:152-228): f(k) = F[k * s], k = 1..nsrc */
+static double c_sop_row(const c_sop *o, int i, int nsrc, const double *F, long s) {
+    const int ntarg = nsrc + o->dnx;
+#define FF(k) F[(long)(k) * s]
+    double r;
+    if (i <= o->ne) {
+        r = o->Ql[o->nw * (i - 1)] * FF(1);
+        for (int m = 2; m <= o->last[i - 1]; m++) r = r + o->Ql[(m - 1) + o->nw * (i - 1)] * FF(m);
+    } else if (i <= ntarg - o->ne) {
+        r = o->Qin[0] * FF(i + o->inshift);
+        for (int k = 2; k <= o->win; k++) r = r + o->Qin[k - 1] * FF(i + o->inshift + k - 1);
+    } else {
+        const int ir = i - (ntarg - o->ne);
+        const int mx = nsrc + o->rshift[ir - 1], nwst = o->nw + o->rshift[ir - 1];
+        r = o->Qr[(nwst - 1) + o->nw * (ir - 1)] * FF(mx);
+        for (int m = nwst + 1; m <= o->nw; m++) r = r + o->Qr[(m - 1) + o->nw * (ir - 1)] * FF(mx + m - nwst);
+    }
+#undef FF
+    return r;
+}
+
+/* apply along x (dir 'x') or y over the whole panel: out of type tout from f of type tin */
+static void c_apply(const ora_cgrid *c, const c_sop *o, char dir, int tin, int tout, const double *f, double *out) {
+    const ora_grid *g = c->g;
+    const int N = c->N, nxo = c_nx(N, tout), nyo = c_ny(N, tout);
+    const int nsrc = dir == 'x' ? c_nx(N, tin) : c_ny(N, tin);
+    const long sx = 1, sy = g->ld;
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= nyo; j++)
+            for (int i = 1; i <= nxo; i++) {
+                const double *F0 = dir == 'x' ? &f[IDX(g, p, 0, j)] : &f[IDX(g, p, i, 0)];
+                out[IDX(g, p, i, j)] = c_sop_row(o, dir == 'x' ? i : j, nsrc, F0, dir == 'x' ? sx : sy);
+            }
+}
+
+static double *c_field(const ora_cgrid *c) { return alloc_field(c->g); }
+
+/* ---- co2contra_c_sbp42: co2contra_Cgrid_mod.f90:292-399 ---- */
+static void c_co2contra(const ora_cgrid *c, const double *u, const double *v, double *ut, double *vt) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    c_sop i2c, c2i;
+    c_sop_make(&i2c, 0);
+    c_sop_make(&c2i, 1);
+    double *dyp = c_field(c), *dxp = c_field(c);
+    /* dy_at_p = v2h(v) along y, times J*Qi12 at o points; gx = h2v along x */
+    c_apply(c, &i2c, 'y', 2, 0, v, dyp);
+    c_apply(c, &i2c, 'x', 1, 0, u, dxp);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                dyp[id] = c->J[0][id] * c->Qi[0][1][id] * dyp[id];
+                dxp[id] = c->J[0][id] * c->Qi[0][1][id] * dxp[id];
+            }
+    c_apply(c, &c2i, 'x', 0, 1, dyp, ut);
+    c_apply(c, &c2i, 'y', 0, 2, dxp, vt);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) {
+                const long id = IDX(g, p, i, j);
+                ut[id] = c->Qi[1][0][id] * u[id] + ut[id] / c->J[1][id];
+            }
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                vt[id] = c->Qi[2][2][id] * v[id] + vt[id] / c->J[2][id];
+            }
+    }
+    free(dyp); free(dxp);
+}
+
+/* ---- massflux_c_sbp42: massflux_Cgrid_mod.f90:276-358 ---- */
+static void c_massflux(const ora_cgrid *c, const double *h, const double *ut, const double *vt, double *hu, double *hv) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    c_sop c2i;
+    c_sop_make(&c2i, 1);
+    c_apply(c, &c2i, 'x', 0, 1, h, hu);
+    c_apply(c, &c2i, 'y', 0, 2, h, hv);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) { const long id = IDX(g, p, i, j); hu[id] = ut[id] * hu[id]; }
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) { const long id = IDX(g, p, i, j); hv[id] = vt[id] * hv[id]; }
+    }
+}
+
+/* ---- KE_Cgrid_sbp42 (KE_Cgrid_mod.f90:21-77 with interp2d_uv2pvec_C_sbp42) ---- */
+static void c_KE(const ora_cgrid *c, const double *u, const double *v, const double *ut, const double *vt, double *KE) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    c_sop i2c;
+    c_sop_make(&i2c, 0);
+    double *ku = c_field(c), *kv = c_field(c), *kuh = c_field(c), *kvh = c_field(c);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) {
+                const long id = IDX(g, p, i, j);
+                ku[id] = c->J[1][id] * (u[id] * ut[id]) / 2.0;
+            }
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                kv[id] = c->J[2][id] * (v[id] * vt[id]) / 2.0;
+            }
+    }
+    c_apply(c, &i2c, 'x', 1, 0, ku, kuh);
+    c_apply(c, &i2c, 'y', 2, 0, kv, kvh);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                KE[id] = (kuh[id] + kvh[id]) / c->J[0][id];
+            }
+    free(ku); free(kv); free(kuh); free(kvh);
+}
+
+/* ---- gradient_c_sbp42: grad_c_sbp42_mod.f90:32-140 (f needs its O halo, depth 3) ---- */
+static void c_grad(const ora_cgrid *c, double *f, double *gx, double *gy) {
+    const ora_grid *g = c->g;
+    const int N = c->N, n = N + 1;
+    static const double dedge[4][5] = {{-2.0, 3.0, -1.0, 0.0, 0.0},
+                                       {-1.0, 1.0, 0.0, 0.0, 0.0},
+                                       {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0, 0.0},
+                                       {-1.0 / 71.0, 6.0 / 71.0, -83.0 / 71.0, 81.0 / 71.0, -3.0 / 71.0}};
+    static const int dlast[4] = {3, 2, 4, 5};
+    const double din[4] = {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0};
+    const double proj[3] = {15.0 / 8.0, -5.0 / 4.0, 3.0 / 8.0};
+    const double q1 = 7.0 / 18.0;
+    const double dx1 = 1.0 / (g->scale * g->hx);
+    c_exchange_o(c, f, 3);
+    for (int p = 0; p < 6; p++)
+        for (int dir = 0; dir < 2; dir++)
+            for (int l = 1; l <= N; l++)          /* line along the other axis */
+                for (int i = 1; i <= n; i++) {
+#define F(k) (dir == 0 ? f[IDX(g, p, (k), l)] : f[IDX(g, p, l, (k))])
+                    double s;
+                    if (i <= 4) {
+                        s = dedge[i - 1][0] * F(1);
+                        for (int m = 2; m <= dlast[i - 1]; m++) s = s + dedge[i - 1][m - 1] * F(m);
+                        s = s * dx1;
+                        if (i == 1) {
+                            const double dh = (proj[0] * (F(1) - F(0)) + proj[1] * (F(2) - F(-1)) +
+                                               proj[2] * (F(3) - F(-2))) * dx1;
+                            s = s + 0.5 * dh / q1;
+                        }
+                    } else if (i <= N + 1 - 4) {
+                        s = F(i - 2) * din[0];
+                        for (int m = 1; m < 4; m++) s = s + F(i - 2 + m) * din[m];
+                        s = s * dx1;
+                    } else {
+                        const int i2 = n - i + 1, i1 = dlast[i2 - 1];
+                        double t = dedge[i2 - 1][0] * F(n - 1);
+                        for (int m = 2; m <= i1; m++) t = t + dedge[i2 - 1][m - 1] * F(n - m);
+                        s = -t * dx1;
+                        if (i == n) {
+                            const double dh = (proj[0] * (F(n) - F(n - 1)) + proj[1] * (F(n + 1) - F(n - 2)) +
+                                               proj[2] * (F(n + 2) - F(n - 3))) * dx1;
+                            s = s + 0.5 * dh / q1;
+                        }
+                    }
+#undef F
+                    if (dir == 0) gx[IDX(g, p, i, l)] = s;
+                    else gy[IDX(g, p, l, i)] = s;
+                }
+}
+
+/* ---- curl_c_sbp42: curl_c_sbp_mod.f90:22-126 with D42_staggered_c2i, then the Ah scalar sync
+ * of the q points (halo_Ah_scalar_sync_mod.f90:52-93); u, v need their C halo, depth 4 ---- */
+static void c_curl(const ora_cgrid *c, double *u, double *v, double *curl) {
+    const ora_grid *g = c->g;
+    const int N = c->N, n = N + 1;
+    c_sop D;
+    c_sop_make(&D, 2);
+    c_exchange_vec(c, u, v, 4);
+    double *dvx = c_field(c), *duy = c_field(c);
+    c_apply(c, &D, 'x', 2, 3, v, dvx);
+    c_apply(c, &D, 'y', 1, 3, u, duy);
+    const int nw = D.nproj;
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= n; j++) {
+            double dv = 0.0;
+            for (int i = 1; i <= nw; i++) dv = dv + D.proj_l[i - 1] * (v[IDX(g, p, i, j)] - v[IDX(g, p, 1 - i, j)]);
+            dvx[IDX(g, p, 1, j)] = dvx[IDX(g, p, 1, j)] + 0.5 * dv / D.Al_out[0];
+            dv = 0.0;
+            for (int i = 0; i <= nw - 1; i++)
+                dv = dv + D.proj_r[nw - i - 1] * (v[IDX(g, p, n + i, j)] - v[IDX(g, p, n - i - 1, j)]);
+            dvx[IDX(g, p, n, j)] = dvx[IDX(g, p, n, j)] + 0.5 * dv / D.Ar_out[D.nA - 1];
+        }
+        for (int i = 1; i <= n; i++) {
+            double du = 0.0;
+            for (int j = 1; j <= nw; j++) du = du + D.proj_l[j - 1] * (u[IDX(g, p, i, j)] - u[IDX(g, p, i, 1 - j)]);
+            duy[IDX(g, p, i, 1)] = duy[IDX(g, p, i, 1)] + 0.5 * du / D.Al_out[0];
+            du = 0.0;
+            for (int j = 0; j <= nw - 1; j++)
+                du = du + D.proj_r[nw - j - 1] * (u[IDX(g, p, i, n + j)] - u[IDX(g, p, i, n - j - 1)]);
+            duy[IDX(g, p, i, n)] = duy[IDX(g, p, i, n)] + 0.5 * du / D.Ar_out[D.nA - 1];
+        }
+        for (int j = 1; j <= n; j++)
+            for (int i = 1; i <= n; i++) {
+                const long id = IDX(g, p, i, j);
+                curl[id] = (dvx[id] - duy[id]) / (c->J[3][id] * g->hx * g->scale);
+            }
+    }
+    ora_scalar_sync(g, curl);
+    free(dvx); free(duy);
+}
+
+/* ---- coriolis_Cgrid_sbp42: coriolis_Cgrid_mod.f90:39-103 with interp2d_uv2pvec_C_sbp42,
+ * interpolator_w2h (interpolator_w2h_mod.f90:23-68) and interp2d_pvec2uv_C_sbp42 ---- */
+static void c_coriolis(const ora_cgrid *c, const double *hu, const double *hv, const double *h, const double *curl,
+                       double *cor_u, double *cor_v) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    c_sop i2c, c2i;
+    c_sop_make(&i2c, 0);
+    c_sop_make(&c2i, 1);
+    double *Ghu = c_field(c), *Ghv = c_field(c), *Ghup = c_field(c), *Ghvp = c_field(c);
+    double *fu = c_field(c), *fv = c_field(c), *fuh = c_field(c), *fvh = c_field(c);
+    double *cup = c_field(c), *cvp = c_field(c);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) { const long id = IDX(g, p, i, j); Ghu[id] = c->J[1][id] * hu[id]; }
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) { const long id = IDX(g, p, i, j); Ghv[id] = c->J[2][id] * hv[id]; }
+    }
+    c_apply(c, &i2c, 'x', 1, 0, Ghu, Ghup);
+    c_apply(c, &i2c, 'y', 2, 0, Ghv, Ghvp);
+    /* w -> v points along x, w -> u points along y, then to o points, averaged */
+    c_apply(c, &i2c, 'x', 3, 2, curl, fv);
+    c_apply(c, &i2c, 'y', 3, 1, curl, fu);
+    c_apply(c, &i2c, 'x', 1, 0, fu, fuh);
+    c_apply(c, &i2c, 'y', 2, 0, fv, fvh);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                const double cp = 0.5 * fuh[id] + 0.5 * fvh[id];
+                cup[id] = (c->f[0][id] + cp) * Ghvp[id] / h[id];
+                cvp[id] = -(c->f[0][id] + cp) * Ghup[id] / h[id];
+            }
+    c_apply(c, &c2i, 'x', 0, 1, cup, cor_u);
+    c_apply(c, &c2i, 'y', 0, 2, cvp, cor_v);
+    free(Ghu); free(Ghv); free(Ghup); free(Ghvp); free(fu); free(fv); free(fuh); free(fvh); free(cup); free(cvp);
+}
+
+/* ---- divergence_c_sbp42: div_c_sbp42_mod.f90:31-130 with D42_staggered_i2c and
+ * add_penalty 'at_center' (sbp_operator_mod.f90:344-413) ---- */
+static void c_div(const ora_cgrid *c, const double *u, const double *v, double *div) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    c_sop D;
+    c_sop_make(&D, 3);
+    double *Gu = c_field(c), *Gv = c_field(c), *Dx = c_field(c), *Dy = c_field(c);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) { const long id = IDX(g, p, i, j); Gu[id] = u[id] * c->J[1][id]; }
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) { const long id = IDX(g, p, i, j); Gv[id] = v[id] * c->J[2][id]; }
+    }
+    c_exchange_vec(c, Gu, Gv, 3);
+    c_apply(c, &D, 'x', 1, 0, Gu, Dx);
+    c_apply(c, &D, 'y', 2, 0, Gv, Dy);
+    const int ne = D.nproj, nin = N + 1;  /* nx_in_grid = nx_out - dnx */
+    double pl[3], pr[3];
+    for (int k = 0; k < ne; k++) pl[k] = D.proj_l[k] / D.Al_out[k];
+    for (int k = 0; k < ne; k++) pr[k] = D.proj_r[k] / D.Ar_out[D.nA - ne + k];
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++) {
+            double df = Gu[IDX(g, p, 1, j)] - Gu[IDX(g, p, 0, j)];
+            for (int i = 1; i <= ne; i++) Dx[IDX(g, p, i, j)] = Dx[IDX(g, p, i, j)] + 0.5 * pl[i - 1] * df;
+            df = Gu[IDX(g, p, nin + 1, j)] - Gu[IDX(g, p, nin, j)];
+            for (int i = N - ne + 1; i <= N; i++)
+                Dx[IDX(g, p, i, j)] = Dx[IDX(g, p, i, j)] + 0.5 * pr[i - N + ne - 1] * df;
+        }
+        for (int j = 1; j <= ne; j++)
+            for (int i = 1; i <= N; i++) {
+                const double df = Gv[IDX(g, p, i, 1)] - Gv[IDX(g, p, i, 0)];
+                Dy[IDX(g, p, i, j)] = Dy[IDX(g, p, i, j)] + 0.5 * pl[j - 1] * df;
+            }
+        for (int j = N - ne + 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const double df = Gv[IDX(g, p, i, nin + 1)] - Gv[IDX(g, p, i, nin)];
+                Dy[IDX(g, p, i, j)] = Dy[IDX(g, p, i, j)] + 0.5 * pr[j - N + ne - 1] * df;
+            }
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                div[id] = (Dx[id] + Dy[id]) / (c->J[0][id] * g->hx * g->scale);
+            }
+    }
+    free(Gu); free(Gv); free(Dx); free(Dy);
+}
+
+/* ---- operator_swm_t%apply (operator_swm_mod.f90:61-113) with the C-grid operators; h at o
+ * points, u at x points, v at y points ---- */
+void ora_c_rhs(const ora_cgrid *c, double *h, double *u, double *v, const double *hs,
+               double *dh, double *du, double *dv) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    double *ut = c_field(c), *vt = c_field(c), *hu = c_field(c), *hv = c_field(c), *KE = c_field(c);
+    double *gx = c_field(c), *gy = c_field(c), *curl = c_field(c), *cu = c_field(c), *cv = c_field(c);
+    double *div = c_field(c);
+    c_co2contra(c, u, v, ut, vt);
+    c_massflux(c, h, ut, vt, hu, hv);
+    c_KE(c, u, v, ut, vt, KE);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                KE[id] = KE[id] + g->grav * h[id] + g->grav * (hs ? hs[id] : 0.0);
+            }
+    c_grad(c, KE, gx, gy);
+    c_curl(c, u, v, curl);
+    c_coriolis(c, hu, hv, h, curl, cu, cv);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) {
+                const long id = IDX(g, p, i, j);
+                du[id] = -1.0 * gx[id] + 1.0 * cu[id];
+            }
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                dv[id] = -1.0 * gy[id] + 1.0 * cv[id];
+            }
+    }
+    c_div(c, hu, hv, div);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                dh[id] = -1.0 * div[id];
+            }
+    free(ut); free(vt); free(hu); free(hv); free(KE); free(gx); free(gy); free(curl); free(cu); free(cv); free(div);
+}
+
+/* y = a*x + b*z over the points of type t (grid_field_t assign/update on one mesh) */
+static void c_axpy(const ora_cgrid *c, int t, double *y, double a, const double *x, double b, const double *z) {
+    const ora_grid *g = c->g;
+    const int nx = c_nx(c->N, t), ny = c_ny(c->N, t);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const long id = IDX(g, p, i, j);
+                y[id] = a * x[id] + b * z[id];
+            }
+}
+
+/* ---- rk4_t%step (time_schemes/rk4_mod.f90:20-42) over the three C-grid fields ---- */
+void ora_c_rk4(const ora_cgrid *c, double *h, double *u, double *v, const double *hs, double dt, int nsteps) {
+    double *k[4][3], *y[3];
+    for (int s = 0; s < 4; s++)
+        for (int m = 0; m < 3; m++) k[s][m] = c_field(c);
+    for (int m = 0; m < 3; m++) y[m] = c_field(c);
+    double *v0[3] = {h, u, v};
+    static const int T[3] = {0, 1, 2};
+    for (int step = 0; step < nsteps; step++) {
+        ora_c_rhs(c, v0[0], v0[1], v0[2], hs, k[0][0], k[0][1], k[0][2]);
+        for (int m = 0; m < 3; m++) c_axpy(c, T[m], y[m], 1.0, v0[m], 0.5 * dt, k[0][m]);
+        ora_c_rhs(c, y[0], y[1], y[2], hs, k[1][0], k[1][1], k[1][2]);
+        for (int m = 0; m < 3; m++) c_axpy(c, T[m], y[m], 1.0, v0[m], 0.5 * dt, k[1][m]);
+        ora_c_rhs(c, y[0], y[1], y[2], hs, k[2][0], k[2][1], k[2][2]);
+        for (int m = 0; m < 3; m++) c_axpy(c, T[m], y[m], 1.0, v0[m], dt, k[2][m]);
+        ora_c_rhs(c, y[0], y[1], y[2], hs, k[3][0], k[3][1], k[3][2]);
+        for (int m = 0; m < 3; m++) {
+            const ora_grid *g = c->g;
+            const int nx = c_nx(c->N, T[m]), ny = c_ny(c->N, T[m]);
+            const double c2 = dt / 6.0, c3 = dt / 3.0;
+            for (int p = 0; p < 6; p++)
+                for (int j = 1; j <= ny; j++)
+                    for (int i = 1; i <= nx; i++) {
+                        const long id = IDX(g, p, i, j);
+                        v0[m][id] = v0[m][id] + c2 * k[0][m][id] + c3 * k[1][m][id];
+                    }
+            for (int p = 0; p < 6; p++)
+                for (int j = 1; j <= ny; j++)
+                    for (int i = 1; i <= nx; i++) {
+                        const long id = IDX(g, p, i, j);
+                        v0[m][id] = v0[m][id] + c3 * k[2][m][id] + c2 * k[3][m][id];
+                    }
+        }
+    }
+    for (int s = 0; s < 4; s++)
+        for (int m = 0; m < 3; m++) free(k[s][m]);
+    for (int m = 0; m < 3; m++) free(y[m]);
+}
+
+/* ---- exported C-grid API ---- */
+ora_cgrid *ora_cgrid_create(int N, double scale, double omega, double grav) {
+    ora_cgrid *c = calloc(1, sizeof *c);
+    c->g = ora_create(N, 42, 42, scale, omega, grav);
+    c->N = N;
+    c_build_geometry(c);
+    return c;
+}
+
+void ora_cgrid_destroy(ora_cgrid *c) {
+    if (!c) return;
+    for (int t = 0; t < 4; t++) {
+        free(c->J[t]); free(c->f[t]);
+        for (int k = 0; k < 3; k++) {
+            free(c->Qi[t][k]); free(c->r[t][k]); free(c->a1[t][k]); free(c->a2[t][k]);
+            free(c->b1[t][k]); free(c->b2[t][k]);
+        }
+    }
+    ora_destroy(c->g);
+    free(c);
+}
+
+ora_grid *ora_cgrid_grid(ora_cgrid *c) { return c->g; }
+
+/* which: 0 J, 1 Qi11, 2 Qi12, 3 Qi22, 4 f, 5 r, 6 a1, 7 a2, 8 b1, 9 b2 (comp 0..2 for vectors) */
+int ora_cgrid_geometry(const ora_cgrid *c, int t, int which, int comp, double *out) {
+    if (t < 0 || t > 3 || comp < 0 || comp > 2) return -1;
+    const double *src = which == 0 ? c->J[t] : which <= 3 ? c->Qi[t][which - 1] : which == 4 ? c->f[t]
+                      : which == 5 ? c->r[t][comp] : which == 6 ? c->a1[t][comp] : which == 7 ? c->a2[t][comp]
+                      : which == 8 ? c->b1[t][comp] : which == 9 ? c->b2[t][comp] : NULL;
+    if (!src) return -1;
+    memcpy(out, src, sizeof(double) * c->g->flen);
+    return 0;
+}
+
+void ora_c_co2contra(const ora_cgrid *c, const double *u, const double *v, double *ut, double *vt) { c_co2contra(c, u, v, ut, vt); }
+void ora_c_massflux(const ora_cgrid *c, const double *h, const double *ut, const double *vt, double *hu, double *hv) {
+    c_massflux(c, h, ut, vt, hu, hv);
+}
+void ora_c_KE(const ora_cgrid *c, const double *u, const double *v, const double *ut, const double *vt, double *KE) {
+    c_KE(c, u, v, ut, vt, KE);
+}
+void ora_c_grad(const ora_cgrid *c, double *f, double *gx, double *gy) { c_grad(c, f, gx, gy); }
+void ora_c_curl(const ora_cgrid *c, double *u, double *v, double *curl) { c_curl(c, u, v, curl); }
+void ora_c_coriolis(const ora_cgrid *c, const double *hu, const double *hv, const double *h, const double *curl,
+                    double *cor_u, double *cor_v) {
+    c_coriolis(c, hu, hv, h, curl, cor_u, cor_v);
+}
+void ora_c_div(const ora_cgrid *c, const double *u, const double *v, double *div) { c_div(c, u, v, div); }
+void ora_c_exchange_o(const ora_cgrid *c, double *f, int depth) { c_exchange_o(c, f, depth); }
+void ora_c_exchange_vec(const ora_cgrid *c, double *u, double *v, int depth) { c_exchange_vec(c, u, v, depth); }
+
+/* vec_math_mod.f90:50-90 on the mesh of point type t */
+double ora_c_l2norm(const ora_cgrid *c, int t, const double *f) {
+    const ora_grid *g = c->g;
+    const int nx = c_nx(c->N, t), ny = c_ny(c->N, t);
+    double acc = 0.0;
+    for (int p = 0; p < 6; p++) {
+        double out = 0.0;
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const long id = IDX(g, p, i, j);
+                out = out + f[id] * f[id] * c->J[t][id] * g->hx * g->hx;
+            }
+        acc = acc + out;
+    }
+    return sqrt(acc);
+}
+
+double ora_c_maxabs(const ora_cgrid *c, int t, const double *f) {
+    const ora_grid *g = c->g;
+    const int nx = c_nx(c->N, t), ny = c_ny(c->N, t);
+    double m = 0.0;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const double a = fabs(f[IDX(g, p, i, j)]);
+                if (a > m) m = a;
+            }
+    return m;
+}

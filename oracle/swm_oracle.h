@@ -113,6 +113,36 @@ int ora_ecs_halo_vec_a(const ora_grid *g, const double *u, const double *v, int 
 void ora_rk4_iomega(int n, const double *w_re, const double *w_im,
                     double *f_re, double *f_im, double dt);
 
+/* ---- C grid, SBP42 vector-invariant family (divergence_c_sbp42, gradient_c_sbp42,
+ * curl_c_sbp42, coriolis_Cgrid_sbp42, KE_Cgrid_sbp42, co2contra_c_sbp42, massflux_c_sbp42).
+ * Point types t: 0 o (cell centres, N x N), 1 x (u, N+1 x N), 2 y (v, N x N+1), 3 q (corners,
+ * N+1 x N+1).  Every field uses the storage of the Ah grid of the same N (ora_cgrid_grid),
+ * index (i, j) of any type at the same place. ---- */
+typedef struct ora_cgrid ora_cgrid;
+ora_cgrid *ora_cgrid_create(int N, double scale, double omega, double grav);
+void ora_cgrid_destroy(ora_cgrid *c);
+ora_grid *ora_cgrid_grid(ora_cgrid *c);
+/* which: 0 J, 1 Qi11, 2 Qi12, 3 Qi22, 4 f, 5 r, 6 a1, 7 a2, 8 b1, 9 b2 (comp 0..2 for vectors) */
+int ora_cgrid_geometry(const ora_cgrid *c, int t, int which, int comp, double *out);
+void ora_c_co2contra(const ora_cgrid *c, const double *u, const double *v, double *ut, double *vt);
+void ora_c_massflux(const ora_cgrid *c, const double *h, const double *ut, const double *vt, double *hu, double *hv);
+void ora_c_KE(const ora_cgrid *c, const double *u, const double *v, const double *ut, const double *vt, double *KE);
+/* f's O halo (depth 3) is written */
+void ora_c_grad(const ora_cgrid *c, double *f, double *gx, double *gy);
+/* the C vector halo of u, v (depth 4) is written; curl at q points, corner copies averaged */
+void ora_c_curl(const ora_cgrid *c, double *u, double *v, double *curl);
+void ora_c_coriolis(const ora_cgrid *c, const double *hu, const double *hv, const double *h, const double *curl,
+                    double *cor_u, double *cor_v);
+void ora_c_div(const ora_cgrid *c, const double *u, const double *v, double *div);
+void ora_c_exchange_o(const ora_cgrid *c, double *f, int depth);
+void ora_c_exchange_vec(const ora_cgrid *c, double *u, double *v, int depth);
+/* RHS and RK4 of the C-grid SWM: h at o points, u at x points, v at y points; hs may be NULL */
+void ora_c_rhs(const ora_cgrid *c, double *h, double *u, double *v, const double *hs,
+               double *dh, double *du, double *dv);
+void ora_c_rk4(const ora_cgrid *c, double *h, double *u, double *v, const double *hs, double dt, int nsteps);
+double ora_c_l2norm(const ora_cgrid *c, int t, const double *f);
+double ora_c_maxabs(const ora_cgrid *c, int t, const double *f);
+
 #ifdef __cplusplus
 }
 #endif

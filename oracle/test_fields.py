@@ -91,3 +91,30 @@ def vsh_curl_free_10(x, y, z, level=1):
     """generate_VSH_curl_free_10, test_fields_mod.f90:755-774: v_phi = -cos(phi)."""
     lam, phi = _cart2sph(x, y, z)
     return _sph2cart_vec(lam, phi, 0.0 * lam, -np.cos(phi))
+
+
+def _cart2sph_vec(lam, phi, vx, vy, vz):
+    """sph_coords_mod.f90:43-51."""
+    vl = -np.sin(lam) * vx + np.cos(lam) * vy
+    vp = -np.cos(lam) * np.sin(phi) * vx - np.sin(lam) * np.sin(phi) * vy + np.cos(phi) * vz
+    return vl, vp
+
+
+def coriolis_force(field):
+    """coriolis_force_field_generator_t, test_fields_mod.f90:592-616: f x V with f = 2 sin(phi)
+    (unit rotation rate) of the vector field `field`."""
+    def gen(x, y, z, level=1):
+        vx, vy, vz = field(x, y, z, level)
+        lam, phi = _cart2sph(x, y, z)
+        vl, vp = _cart2sph_vec(lam, phi, vx, vy, vz)
+        pc = 2 * np.sin(phi)
+        return _sph2cart_vec(lam, phi, pc * vp, -pc * vl)
+    return gen
+
+
+def kinetic_energy(field):
+    """KE_scalar_field_t, test_fields_mod.f90:804-825: 0.5 |V|^2 of the vector field `field`."""
+    def gen(x, y, z, level=1):
+        vx, vy, vz = field(x, y, z, level)
+        return 0.5 * (vx ** 2 + vy ** 2 + vz ** 2)
+    return gen
