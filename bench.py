@@ -45,9 +45,10 @@ def parse():
     ap.add_argument("--autotune-reps", type=int, default=30, help="RK4 steps per tiling candidate in the autotuner")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,adjoint,batched,hyperdiffusion",
+    ap.add_argument("--c-steps", type=int, default=200, help="C-grid RK4 steps timed")
+    ap.add_argument("--forms", default="fp32,adjoint,batched,hyperdiffusion,cgrid",
                     help="comma-separated extra measurements beside the FP64 headline: fp32, adjoint, batched, "
-                         "hyperdiffusion")
+                         "hyperdiffusion, cgrid")
     ap.add_argument("--diff-steps", type=int, default=100, help="hyperdiffusion steps timed")
     # precision form: C768 sbp42, RK4 at gravity-wave Courant 0.9 (37.5 s), degree <= 64
     ap.add_argument("--fp32-N", type=int, default=768)
@@ -284,6 +285,76 @@ def hyperdiffusion_form(args, stream, op, x):
             "sbp63_us_per_step": us63}
 
 
+# Field passes of one C-grid RK4 step as implemented (each intermediate field read or written
+# once per kernel): RHS = 6 (contra_o) + 13 (flux) + 8 (ke_div) + 4 (curl) + 2 (qsync) +
+# 3 (w2uv) + 8 (cor_o) + 5 (mom) = 49, step = 4 RHS + 3 x 9 (stage axpy) + 18 (final) = 241.
+C_STEP_FIELD_PASSES = 241
+
+
+def cgrid_form(args, stream):
+    """C-grid SBP42 family (divergence_c_sbp42, gradient_c_sbp42, curl_c_sbp42,
+    coriolis_Cgrid_sbp42, KE_Cgrid_sbp42, co2contra_c_sbp42, massflux_c_sbp42), FP64 RK4 on the
+    same sphere and dt as the headline; the oracle's C-grid RK4 on the host as its CPU baseline."""
+    import numpy as np
+    import torch
+    import cubedsphere_sbp_swm_amd as swm
+    from oracle.oracle import CGrid, lib as oracle_lib
+    N, dt, steps = args.N, args.dt, args.c_steps
+    op = swm.create_swm_operator(N, **swm.operator_names("c42"), scale=swm.EARTH["radius"],
+                                 omega=swm.EARTH["omega"], grav=swm.EARTH["grav"])
+    C = CGrid(N, swm.EARTH["radius"], swm.EARTH["omega"], swm.EARTH["grav"])
+    # solid-body rotation in geostrophic balance plus a 100 m Gaussian bump (test case 2 + bump)
+    om, a, g = swm.EARTH["omega"], swm.EARTH["radius"], swm.EARTH["grav"]
+    u0 = 40.0
+    fields = []
+    for t in "oxy":
+        x, y, z = C.geometry(t, "r")
+        r = np.sqrt(x * x + y * y + z * z)
+        fields.append((x / r, y / r, z / r))
+    xo, yo, zo = fields[0]
+    h = 1.0e4 - (a * om * u0 + 0.5 * u0 ** 2) / g * zo ** 2 + 100.0 * np.exp(-((xo - 0.5) ** 2 + yo ** 2) / 0.05)
+    h = C.embed("o", C.interior("o", h))
+    u = C.embed("x", C.interior("x", C.project("x", "covariant", -u0 * fields[1][1], u0 * fields[1][0],
+                                               0 * fields[1][2], 0)))
+    v = C.embed("y", C.interior("y", C.project("y", "covariant", -u0 * fields[2][1], u0 * fields[2][0],
+                                               0 * fields[2][2], 1)))
+    xs = swm.create_stvec_swm(op)
+    xs.upload(h, u, v, C.hw)
+    ts = swm.create_timescheme(op, "rk4")
+    ts.steps(xs, dt, 5)
+    op.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ts.steps(xs, dt, steps)
+    op.synchronize()
+    secs = time.perf_counter() - t0
+    cells, points = 6 * N * N, 6 * (N + 1) ** 2
+    out = {"workload": f"C{N} C-grid SBP42 FP64 RK4 dt={dt:g}s x {steps} steps (test case 2 + 100 m bump)",
+           "cell_steps_per_s": cells * steps / secs, "ms_per_step": secs * 1e3 / steps,
+           "field_passes_per_step": C_STEP_FIELD_PASSES,
+           "implementation_GBs": C_STEP_FIELD_PASSES * 8 * points / (secs / steps) / 1e9,
+           "implementation_frac": C_STEP_FIELD_PASSES * 8 * points / (secs / steps) / 1e9 / HBM_PEAK_GBS,
+           "fused_minimum_GBs_equiv": STEP_VALUES * 8 * points / (secs / steps) / 1e9}
+    hd, _, _ = xs.download(C.hw)
+    out["h_range_after"] = [float(C.interior("o", hd).min()), float(C.interior("o", hd).max())]
+    if not args.no_cpu:
+        hc, uc, vc = h.copy(), u.copy(), v.copy()
+        t0 = time.perf_counter()
+        C.rk4(hc, uc, vc, dt, 1)
+        one = time.perf_counter() - t0
+        extra = max(0, int(args.cpu_seconds / 2 / max(one, 1e-6)) - 1)
+        t1 = time.perf_counter()
+        if extra:
+            C.rk4(hc, uc, vc, dt, extra)
+        total = one + time.perf_counter() - t1
+        threads = int(oracle_lib().ora_num_threads())
+        out["cpu_baseline"] = {"value": cells * (1 + extra) / total, "unit": "cell-steps/s", "cores": threads,
+                               "kind": "port", "sample": f"C{N} C-grid RK4, {1 + extra} steps ({total:.1f} s) "
+                                                         f"of the oracle (C restatement, OpenMP over {threads} threads)"}
+    op.close()
+    return out
+
+
 def batched_form(args, stream):
     """One persistent launch per RK stage over all members (hipGraph per step) vs the plain
     single-cube path run member after member; same states bit for bit."""
@@ -485,6 +556,8 @@ def main():
         forms["adjoint"] = adjoint_form(args, stream)
     if "batched" in wanted:
         forms["batched"] = batched_form(args, stream)
+    if "cgrid" in wanted and args.precision == 64:
+        forms["cgrid"] = cgrid_form(args, stream)
     if "hyperdiffusion" in wanted and args.precision == 64:
         forms["hyperdiffusion"] = hyperdiffusion_form(args, stream, op, y)
         forms["hyperdiffusion"]["rk4_step_us"] = ms * 1e3 / args.steps

@@ -23,15 +23,26 @@ from ._capi import SwmError, check
 
 __all__ = ["SwmError", "operator_names", "create_latlon_outputer", "create_latlon_vec_outputer",
            "create_master_paneled_outputer", "create_swm_operator", "create_stvec_swm", "create_timescheme",
-           "OperatorSWM", "StvecSWM", "RK4", "Ensemble", "EARTH"]
+           "OperatorSWM", "StvecSWM", "RK4", "Ensemble", "EARTH", "OperatorSWMC", "StvecSWMC", "RK4C",
+           "C_GRID_NAMES"]
 
 # const_mod.f90:6-9
 EARTH = {"radius": 0.6371229e7, "grav": 9.80616, "omega": 2.0 * np.pi / (23 * 60 * 60 + 56 * 60 + 4)}
 
 
+# the &shallow_water_model_c namelist group (namelist_swm:80-102)
+C_GRID_NAMES = {"div_op_name": "divergence_c_sbp42", "grad_op_name": "gradient_c_sbp42",
+                "curl_op_name": "curl_c_sbp42", "quadrature_name": "SBP_C42_quadrature",
+                "coriolis_op_name": "coriolis_Cgrid_sbp42", "KE_op_name": "KE_Cgrid_sbp42",
+                "co2contra_op_name": "co2contra_c_sbp42", "massflux_op_name": "massflux_c_sbp42"}
+
+
 def operator_names(order):
     """Operator names of the reference's sbp{21,42,43,63} run configurations
-    (models/shallow_water/test/scripts/ts2_ah/gen_namelist.sh:1-32)."""
+    (models/shallow_water/test/scripts/ts2_ah/gen_namelist.sh:1-32); "c42" gives the C-grid
+    SBP42 set of namelist_swm:80-102."""
+    if order == "c42":
+        return dict(C_GRID_NAMES)
     table = {
         21: ("divergence_ah2", "gradient_ah21_sbp_ecs", "SBP_Ah21_quadrature"),
         42: ("divergence_ah42_sbp", "gradient_ah42_sbp_ecs", "SBP_Ah42_quadrature"),
@@ -419,16 +430,137 @@ class Ensemble:
         self.ops[0].synchronize()
 
 
+class OperatorSWMC:
+    """C-grid (staggering 'C') domain + vector-invariant SWM operator with the SBP42 C-grid
+    operators (create_domain + create_swm_operator with the &shallow_water_model_c names), FP64.
+
+    Host arrays have the shape of the Ah grid of the same N, (6, N+1+2hw, N+1+2hw): h uses the
+    points i, j = 1..N, u (x faces) i = 1..N+1, j = 1..N, v (y faces) i = 1..N, j = 1..N+1."""
+
+    precision = 64
+
+    def __init__(self, N, div_op_name, grad_op_name, curl_op_name, quadrature_name, coriolis_op_name,
+                 KE_op_name, co2contra_op_name, massflux_op_name, scale=1.0, omega=1.0, grav=1.0):
+        L = _capi.lib()
+        h = ctypes.c_void_p()
+        names = (div_op_name, grad_op_name, curl_op_name, coriolis_op_name, KE_op_name, co2contra_op_name,
+                 massflux_op_name, quadrature_name)
+        check(L.swm_c_create(ctypes.byref(h), int(N), *(n.encode() for n in names), float(scale), float(omega),
+                             float(grav)))
+        self.ctx = h
+        self.N = N
+        self.n = N + 1
+        Nv, hw, rows, pitch, fe = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        check(L.swm_c_info(self.ctx, ctypes.byref(Nv), ctypes.byref(hw), ctypes.byref(rows), ctypes.byref(pitch),
+                           ctypes.byref(fe)))
+        self.halo, self.rows, self.pitch, self.field_elems = hw.value, rows.value, pitch.value, fe.value
+        self.elem_bytes = 8
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            _capi.lib().swm_c_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(_capi.lib().swm_c_synchronize(self.ctx))
+
+    def host_shape(self, halo_width=0):
+        ld = self.n + 2 * halo_width
+        return (6, ld, ld)
+
+    def _check_host(self, a, halo_width):
+        if a.shape != self.host_shape(halo_width):
+            raise ValueError(f"host array shape {a.shape} != {self.host_shape(halo_width)}")
+
+    # operator_t%apply (operator_swm_mod.f90:61-113); vin's u, v halos are overwritten
+    def apply(self, vout, vin):
+        check(_capi.lib().swm_c_rhs(self.ctx, ctypes.c_void_p(vin.ptr), ctypes.c_void_p(vout.ptr)))
+
+    def set_orography(self, hs, halo_width=0):
+        if hs is None:
+            check(_capi.lib().swm_c_set_orography(self.ctx, None, 0))
+            return
+        hs = np.ascontiguousarray(hs, dtype=np.float64)
+        self._check_host(hs, halo_width)
+        check(_capi.lib().swm_c_set_orography(self.ctx, _dp(hs), int(halo_width)))
+
+
+class StvecSWMC:
+    """Device-resident C-grid state vector (h at o points, u at x faces, v at y faces)."""
+
+    def __init__(self, op):
+        self.op = op
+        p = ctypes.c_void_p()
+        check(_capi.lib().swm_c_state_alloc(op.ctx, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def free(self):
+        if getattr(self, "ptr", None) and getattr(self.op, "ctx", None):
+            _capi.lib().swm_c_state_free(self.op.ctx, ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def create_similar(self):
+        return StvecSWMC(self.op)
+
+    def upload(self, h, u, v, halo_width=0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (h, u, v)]
+        for a in arrs:
+            self.op._check_host(a, halo_width)
+        check(_capi.lib().swm_c_state_upload(self.op.ctx, ctypes.c_void_p(self.ptr), *(_dp(a) for a in arrs),
+                                             int(halo_width)))
+
+    def download(self, halo_width=0):
+        shp = self.op.host_shape(halo_width)
+        h, u, v = np.zeros(shp), np.zeros(shp), np.zeros(shp)
+        check(_capi.lib().swm_c_state_download(self.op.ctx, ctypes.c_void_p(self.ptr), _dp(h), _dp(u), _dp(v),
+                                               int(halo_width)))
+        return h, u, v
+
+
+class RK4C:
+    """Classical RK4 (time_schemes/rk4_mod.f90) on the C-grid operator, HIP-graph captured."""
+
+    def __init__(self, op):
+        self.op = op
+
+    def step(self, v0, dt):
+        self.steps(v0, dt, 1)
+
+    def steps(self, v0, dt, nsteps):
+        check(_capi.lib().swm_c_rk4_steps(self.op.ctx, ctypes.c_void_p(v0.ptr), float(dt), int(nsteps)))
+
+
 def create_swm_operator(N, div_op_name, grad_op_name, curl_op_name, quadrature_name, scale=1.0, omega=1.0,
-                        grav=1.0, precision=64):
+                        grav=1.0, precision=64, coriolis_op_name=None, KE_op_name=None, co2contra_op_name=None,
+                        massflux_op_name=None):
+    """create_swm_operator (operator_swm_factory_mod.f90:14-41): the Ah-grid operator for the
+    &shallow_water_model names, the C-grid operator for the &shallow_water_model_c names."""
+    if div_op_name.startswith("divergence_c_"):
+        if precision != 64:
+            raise SwmError("the C-grid operators run in FP64 only")
+        return OperatorSWMC(N, div_op_name, grad_op_name, curl_op_name, quadrature_name,
+                            coriolis_op_name or "", KE_op_name or "", co2contra_op_name or "",
+                            massflux_op_name or "", scale, omega, grav)
     return OperatorSWM(N, div_op_name, grad_op_name, curl_op_name, quadrature_name, scale, omega, grav, precision)
 
 
 def create_stvec_swm(op):
-    return StvecSWM(op)
+    return StvecSWMC(op) if isinstance(op, OperatorSWMC) else StvecSWM(op)
 
 
 def create_timescheme(op, name="rk4"):
     if name != "rk4":
         raise SwmError(f"unknown or unsupported time scheme on the device path: {name}")
-    return RK4(op)
+    return RK4C(op) if isinstance(op, OperatorSWMC) else RK4(op)

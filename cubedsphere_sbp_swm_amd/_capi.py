@@ -76,6 +76,17 @@ SIGNATURES = {
     "swm_latlon_write": (_i, [_vp, _vp, ctypes.c_char_p, _i]),
     "swm_latlon_vec_write": (_i, [_vp, _vp, _vp, ctypes.c_char_p, ctypes.c_char_p, _i]),
     "swm_paneled_write": (_i, [_vp, _vp, ctypes.c_char_p, _i]),
+    "swm_c_create": (_i, [ctypes.POINTER(_vp), _i] + [ctypes.c_char_p] * 8 + [_d, _d, _d]),
+    "swm_c_destroy": (_i, [_vp]),
+    "swm_c_info": (_i, [_vp, _ip, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_longlong)]),
+    "swm_c_synchronize": (_i, [_vp]),
+    "swm_c_state_alloc": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "swm_c_state_free": (_i, [_vp, _vp]),
+    "swm_c_state_upload": (_i, [_vp, _vp, _dp, _dp, _dp, _i]),
+    "swm_c_state_download": (_i, [_vp, _vp, _dp, _dp, _dp, _i]),
+    "swm_c_set_orography": (_i, [_vp, _dp, _i]),
+    "swm_c_rhs": (_i, [_vp, _vp, _vp]),
+    "swm_c_rk4_steps": (_i, [_vp, _vp, _d, _i]),
 }
 
 # host set-up routines (geometry.h), exported by the same library; they need no device
@@ -91,6 +102,11 @@ HOST_SIGNATURES = {
     "swmh_ecs_o_weights": (None, [_i, _i, _dp, _ip]),
     "swmh_ecs_vec_a_tm": (None, [_i, _i, _dp]),
     "swmh_latlon_table": (None, [_i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp]),
+    "swmh_c_geometry": (None, [_i, _i, _i, _i, _i, _d, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "swmh_c_halo_lists": (_i, [_i, _i, _i, _i, ctypes.c_longlong, _i, _i] +
+                          [ctypes.POINTER(ctypes.c_longlong)] * 2 + [ctypes.c_void_p] +
+                          [ctypes.POINTER(ctypes.c_longlong)] * 2 + [ctypes.c_void_p] +
+                          [ctypes.POINTER(ctypes.c_longlong)] * 4 + [_ip]),
 }
 
 _lib = None

@@ -692,3 +692,136 @@ extern "C" int swmh_ecs_xy_weights(int N, int hw, double *w, int *ind) {
     }
     return 0;
 }
+
+// ---------------------------------------------------------------------------------------------
+// C grid (staggering 'C'): metric terms at the points of one type and the cube-edge maps of its
+// halos, both from the Ah machinery above on the same sphere.
+//
+// Points of type t (mesh_factory_mod.f90:34-47): alpha = alpha0 + (i-1+si)*hx, beta likewise with
+// sj; o (t=0): si = sj = 1/2, x (t=1): si = 0, sj = 1/2, y (t=2): si = 1/2, sj = 0, q (t=3): 0, 0.
+extern "C" void swmh_c_geometry(int N, int t, int H, int rows, int pitch, double omega, const double axis[3],
+                                double *J, double *Qi11, double *Qi12, double *Qi22, double *f) {
+    const double si = (t == 0 || t == 2) ? 0.5 : 0.0, sj = (t == 0 || t == 1) ? 0.5 : 0.0;
+    const double hx = grid_hx(N), alpha0 = -0.25 * pi_();
+    for (int p = 0; p < 6; ++p)
+        for (int j = 1; j <= N + 1; ++j) {
+            const double beta = alpha0 + ((double)(j - 1) + sj) * hx;
+            for (int i = 1; i <= N + 1; ++i) {
+                const double alpha = alpha0 + ((double)(i - 1) + si) * hx;
+                const long id = ((long)p * rows + (j - 1 + H)) * pitch + (i - 1 + H);
+                const double ta = std::tan(alpha), tb = std::tan(beta);
+                const double sg = std::sqrt(1.0 + ta * ta + tb * tb);
+                if (J) J[id] = (1.0 + ta * ta) * (1.0 + tb * tb) / (sg * sg * sg);
+                if (Qi11) Qi11[id] = sg * sg / (1.0 + ta * ta);
+                if (Qi12) Qi12[id] = sg * sg * ta * tb / ((1 + ta * ta) * (1 + tb * tb));
+                if (Qi22) Qi22[id] = sg * sg / (1.0 + tb * tb);
+                if (f) {
+                    double rp[3], rr[3];
+                    r_proto(alpha, beta, rp);
+                    to_panel(p, rp, rr);
+                    const double pr = (axis[0] * rr[0] + axis[1] * rr[1] + axis[2] * rr[2]) /
+                                      std::sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+                    f[id] = 2 * omega * pr;
+                }
+            }
+        }
+}
+
+// The C points of one panel side lie on the Ah lattice of 2N cells (doubled coordinates
+// 0..2N: faces at even, centres at odd values).  A point of a C halo at doubled depth d2 beyond
+// edge e of panel p, at doubled position T2 along the edge, is entry (k = T2, d = d2) of the Ah
+// neighbour table of that doubled lattice: the neighbour panel q across e, the doubled
+// coordinates (a, b) there, and the vector code of the components.  Face points normal to the
+// edge start at d2 = 0 (halo index 0 holds the neighbour's copy of the shared face:
+// exchange_factory_mod.f90:402-403,420-423 widen the exchange of those points by one), cell
+// centres at d2 = 1 (halo index 0 is the neighbour's centre nearest the edge).
+//
+// Entries of the gather lists are element offsets of the point in the stored C field layout
+// (+ fs for the second component of a vector pair); signs are +-1.
+namespace {
+long c_off(int H, int rows, int pitch, int p, int i, int j) {
+    return ((long)p * rows + (j - 1 + H)) * pitch + (i - 1 + H);
+}
+}  // namespace
+
+extern "C" int swmh_c_halo_lists(int N, int H, int rows, int pitch, long long fs, int depth_vec, int depth_o,
+                                 long long *vec_dst, long long *vec_src, signed char *vec_sign,
+                                 long long *face_dst, long long *face_src, signed char *face_sign,
+                                 long long *o_dst, long long *o_src, long long *q_own, long long *q_nb,
+                                 int *counts) {
+    const int N2 = 2 * N, n2 = N2 + 1, nd = 2 * std::max(depth_vec, depth_o);
+    if (depth_vec < 1 || depth_o < 1 || depth_vec >= H || depth_o >= H) return -1;
+    std::vector<int> tab((size_t)6 * 4 * n2 * nd * 4);
+    swmh_halo_table(N2, nd, tab.data());
+    auto entry = [&](int p, int e, int k2, int d2) { return &tab[((((size_t)p * 4 + e) * n2 + k2) * nd + d2) * 4]; };
+    int nv = 0, nf = 0, no = 0, nq = 0;
+    for (int p = 0; p < 6; ++p)
+        for (int e = 0; e < 4; ++e) {
+            const bool xn = e < 2;  // the edge's normal is the x axis
+            // vector components: comp 0 = u (x points), 1 = v (y points); tangential ones
+            // (u beyond y edges, v beyond x edges) to depth_vec, the face ones to depth 1
+            for (int comp = 0; comp < 2; ++comp) {
+                const bool faces = (xn && comp == 0) || (!xn && comp == 1);
+                const int ntan = faces ? N : N + 1;
+                const int depth = faces ? 1 : depth_vec;
+                for (int d = 1; d <= depth; ++d)
+                    for (int k = 1; k <= ntan; ++k) {
+                        const int d2 = faces ? 2 * (d - 1) : 2 * d - 1;
+                        const int T2 = faces ? 2 * k - 1 : 2 * (k - 1);
+                        const int *t = entry(p, e, T2, d2);
+                        const int q = t[0], a = t[1], b = t[2], code = t[3];
+                        const bool swap = code & 1;
+                        const int qcomp = swap ? 1 - comp : comp;
+                        const bool neg = comp == 0 ? (code & 2) : (code & 4);
+                        // the neighbour's point of component qcomp: faces even along its axis
+                        const bool ok = qcomp == 0 ? (a % 2 == 0 && b % 2 == 1) : (a % 2 == 1 && b % 2 == 0);
+                        if (!ok) return -2;
+                        const int qi = qcomp == 0 ? a / 2 + 1 : (a + 1) / 2, qj = qcomp == 0 ? (b + 1) / 2 : b / 2 + 1;
+                        int i, j;
+                        const int nx = N + (comp == 0), ny = N + (comp == 1);
+                        if (e == 0) { i = 1 - d; j = k; }
+                        else if (e == 1) { i = nx + d; j = k; }
+                        else if (e == 2) { i = k; j = 1 - d; }
+                        else { i = k; j = ny + d; }
+                        const long long dst = comp * fs + c_off(H, rows, pitch, p, i, j);
+                        const long long src = qcomp * fs + c_off(H, rows, pitch, q, qi, qj);
+                        if (faces) {
+                            face_dst[nf] = dst; face_src[nf] = src; face_sign[nf] = neg ? -1 : 1; ++nf;
+                        } else {
+                            vec_dst[nv] = dst; vec_src[nv] = src; vec_sign[nv] = neg ? -1 : 1; ++nv;
+                        }
+                    }
+            }
+            // O points (cell centres), cross halo to depth_o
+            for (int d = 1; d <= depth_o; ++d)
+                for (int k = 1; k <= N; ++k) {
+                    const int *t = entry(p, e, 2 * k - 1, 2 * d - 1);
+                    const int q = t[0], a = t[1], b = t[2];
+                    if (a % 2 != 1 || b % 2 != 1) return -3;
+                    int i, j;
+                    if (e == 0) { i = 1 - d; j = k; }
+                    else if (e == 1) { i = N + d; j = k; }
+                    else if (e == 2) { i = k; j = 1 - d; }
+                    else { i = k; j = N + d; }
+                    o_dst[no] = c_off(H, rows, pitch, p, i, j);
+                    o_src[no] = c_off(H, rows, pitch, q, (a + 1) / 2, (b + 1) / 2);
+                    ++no;
+                }
+            // q points on the edge: the neighbour's copy of each (k = 1..N+1)
+            for (int k = 1; k <= N + 1; ++k) {
+                const int *t = entry(p, e, 2 * (k - 1), 0);
+                const int q = t[0], a = t[1], b = t[2];
+                if (a % 2 != 0 || b % 2 != 0) return -4;
+                int i, j;
+                if (e == 0) { i = 1; j = k; }
+                else if (e == 1) { i = N + 1; j = k; }
+                else if (e == 2) { i = k; j = 1; }
+                else { i = k; j = N + 1; }
+                q_own[nq] = c_off(H, rows, pitch, p, i, j);
+                q_nb[nq] = c_off(H, rows, pitch, q, a / 2 + 1, b / 2 + 1);
+                ++nq;
+            }
+        }
+    counts[0] = nv; counts[1] = nf; counts[2] = no; counts[3] = nq;
+    return 0;
+}

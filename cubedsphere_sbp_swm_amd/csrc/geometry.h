@@ -77,6 +77,24 @@ void swmh_latlon_table(int N, int Nlat, int Nlon, int *panel, int *x0, int *y0, 
 int swmh_ic_sh(int N, int pitch, unsigned long long seed, int lmax, double h0, double h_amp,
                double v_max, double *h, double *u, double *v, int nthreads);
 
+// C grid: metric terms and Coriolis parameter at the points of type t (0 o, 1 x, 2 y, 3 q) in
+// the stored C layout [panel][j-1+H (rows)][i-1+H (pitch)], i, j = 1..N+1; any output may be NULL.
+void swmh_c_geometry(int N, int t, int H, int rows, int pitch, double omega, const double axis[3],
+                     double *J, double *Qi11, double *Qi12, double *Qi22, double *f);
+
+// C grid cube-edge gather lists (element offsets in the stored layout, the second component of a
+// vector pair at +fs): tangential vector halo to depth_vec (u beyond the y edges, v beyond the x
+// edges, 6*4*(N+1)*depth_vec entries), the face halo index 0 / nx+1 of the normal component
+// (6*4*N entries), the O-point cross halo to depth_o (6*4*N*depth_o) and, for each q point on
+// a panel edge, its own offset and the offset of the neighbour's copy across that edge
+// (6*4*(N+1) entries, edges in the order i=1, i=N+1, j=1, j=N+1).  counts = {vec, face, o, q}.
+// Returns 0, or a negative value when a neighbour point is not of the expected type.
+int swmh_c_halo_lists(int N, int H, int rows, int pitch, long long fs, int depth_vec, int depth_o,
+                      long long *vec_dst, long long *vec_src, signed char *vec_sign,
+                      long long *face_dst, long long *face_src, signed char *face_sign,
+                      long long *o_dst, long long *o_src, long long *q_own, long long *q_nb,
+                      int *counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -674,3 +674,4 @@ extern "C" int swm_debug_halo_table(swm_ctx *c, int *table) {
 #include "swm_ecs.hip"
 #include "swm_hordiff.hip"
 #include "swm_output.hip"
+#include "swm_cgrid.hip"

@@ -1,0 +1,768 @@
+// C grid (staggering 'C') vector-invariant shallow-water right-hand side and RK4, SBP42 family:
+// divergence_c_sbp42, gradient_c_sbp42, curl_c_sbp42, coriolis_Cgrid_sbp42, KE_Cgrid_sbp42,
+// co2contra_c_sbp42, massflux_c_sbp42 (the &shallow_water_model_c set, namelist_swm:80-102),
+// composed as operator_swm_t%apply (operator_swm_mod.f90:61-113), FP64.
+//
+// Points per panel: h at cell centres o (N x N), u at x faces (N+1 x N), v at y faces
+// (N x N+1), the curl at corners q (N+1 x N+1).  Every field is stored as 6 panels of
+// rows x pitch doubles with a halo of H = 4 on each side; point (i, j) of any type (1-based)
+// sits at ((p*rows + j-1+H)*pitch + i-1+H).
+//
+// The staggered operators are one-dimensional SBP stencils applied along x or y inside a
+// panel (sbp_operator_mod.f90:152-228); the panels couple only through
+//   - the tangential halo of u and v (curl penalties, curl_c_sbp_mod.f90:73-118),
+//   - the face halo of the mass fluxes J*hu, J*hv (divergence penalties,
+//     sbp_operator_mod.f90:344-413),
+//   - the O-point halo of the kinetic-energy potential (gradient, grad_c_sbp42_mod.f90:53-139),
+//   - the corner sync of the curl (halo_Ah_scalar_sync_mod.f90:52-93),
+// each a gather through precomputed offset lists.  One RHS is eleven launches, all
+// memory-bound stencil sweeps with one thread per output point (64 x 4 thread blocks,
+// coalesced along i); the RK4 step is captured into a HIP graph.
+//
+// Every expression keeps the reference's operation order with no fused multiply-adds
+// (-ffp-contract=off), so results match an FP64 CPU evaluation of the same algorithm bit for bit.
+
+namespace swmc {
+
+constexpr int H = 4;
+constexpr int BX = 64, BY = 4;
+
+// ---- staggered SBP42 operators, sbp_operators_collection_mod.f90:110-191 ----
+// W42 interpolation interfaces -> centres, optimised weights (W42_staggered_i2c_opt1)
+struct W42i2c {
+    static constexpr int NW = 6, NE = 4, INSHIFT = -1, DNX = -1;
+    static constexpr double SIGN = 1.0;
+    static constexpr int last[4] = {4, 4, 5, 6};
+    static constexpr double Ql[4][6] = {
+        {0.4892964885611128, 0.48123107353632727, 0.0696483872440123, -0.040175949341450676, 0.0, 0.0},
+        {-0.07913099957454726, 0.6127279550862234, 0.5119370885511979, -0.045534044062872786, 0.0, 0.0},
+        {-0.0869982983587545, 0.1691760860218048, 0.3626427230326682, 0.6151794893042863, -0.06, 0.0},
+        {0.018680545032460714, -0.10869571330417979, 0.09884979151099299, 0.4911653767607321, 0.5625, -0.0625}};
+    static constexpr double Qin[4] = {-1.0 / 16.0, 9.0 / 16.0, 9.0 / 16.0, -1.0 / 16.0};
+};
+// W42 interpolation centres -> interfaces (W42_staggered_c2i_opt1)
+struct W42c2i {
+    static constexpr int NW = 5, NE = 4, INSHIFT = -2, DNX = 1;
+    static constexpr double SIGN = 1.0;
+    static constexpr int last[4] = {4, 4, 4, 5};
+    static constexpr double Ql[4][5] = {
+        {1.3630402181345285, -0.17804474904273135, -0.23303115631809243, 0.048035687226327554, 0.0},
+        {0.46340770044238916, 0.4765661872892848, 0.15664452409426372, -0.09661841182593758, 0.0},
+        {0.07545241951434666, 0.44794495248229815, 0.37775283649236274, 0.09884979151099299, 0.0},
+        {-0.04413695843145285, -0.04040344754874627, 0.649837488701711, 0.4980831989686297, -0.06338028169014084}};
+    static constexpr double Qin[4] = {-1.0 / 16.0, 9.0 / 16.0, 9.0 / 16.0, -1.0 / 16.0};
+};
+// staggered first differences, centres -> interfaces (D42_staggered_c2i)
+struct D42c2i {
+    static constexpr int NW = 5, NE = 4, INSHIFT = -2, DNX = 1;
+    static constexpr double SIGN = -1.0;
+    static constexpr int last[4] = {3, 2, 4, 5};
+    static constexpr double Ql[4][5] = {{-2.0, 3.0, -1.0, 0.0, 0.0},
+                                        {-1.0, 1.0, 0.0, 0.0, 0.0},
+                                        {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0, 0.0},
+                                        {-1.0 / 71.0, 6.0 / 71.0, -83.0 / 71.0, 81.0 / 71.0, -3.0 / 71.0}};
+    static constexpr double Qin[4] = {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0};
+};
+// staggered first differences, interfaces -> centres (D42_staggered_i2c)
+struct D42i2c {
+    static constexpr int NW = 5, NE = 3, INSHIFT = -1, DNX = -1;
+    static constexpr double SIGN = -1.0;
+    static constexpr int last[3] = {4, 4, 5};
+    static constexpr double Ql[3][5] = {{-79.0 / 78.0, 27.0 / 26.0, -1.0 / 26.0, 1.0 / 78.0, 0.0},
+                                        {2.0 / 21.0, -9.0 / 7.0, 9.0 / 7.0, -2.0 / 21.0, 0.0},
+                                        {1.0 / 75.0, 0.0, -27.0 / 25.0, 83.0 / 75.0, -1.0 / 25.0}};
+    static constexpr double Qin[4] = {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0};
+};
+
+// D42 boundary projection and norms (D42_boundary_proj, D42_A_interfaces, D42_A_centers)
+constexpr double PROJ[3] = {15.0 / 8.0, -5.0 / 4.0, 3.0 / 8.0};
+constexpr double A_IF1 = 7.0 / 18.0;
+// add_penalty_center weights: proj_l(k)/Al_out(k) on the left, proj_r(k)/Ar_out(1+k) on the right
+// (Ar_out = A_centers reversed = [1, 25/24, 7/8, 13/12], proj_r = proj_l reversed)
+constexpr double PEN_L[3] = {(15.0 / 8.0) / (13.0 / 12.0), (-5.0 / 4.0) / (7.0 / 8.0), (3.0 / 8.0) / (25.0 / 24.0)};
+constexpr double PEN_R[3] = {(3.0 / 8.0) / (25.0 / 24.0), (-5.0 / 4.0) / (7.0 / 8.0), (15.0 / 8.0) / (13.0 / 12.0)};
+
+// one output row i (1-based) of sbp_apply along a line with f(k) = f0[k*s], k = 1..nsrc:
+// left rows sum(Ql(1:last,i) f(1:last)), inner rows sum(Qin f(i+inshift:...)), right rows the
+// left block reflected (sign SIGN), each accumulated left to right from the first product
+template <class O>
+__device__ __forceinline__ double sop(int i, int nsrc, const double *__restrict__ f0, long long s) {
+    const int ntarg = nsrc + O::DNX;
+    double r;
+    if (i <= O::NE) {
+        const int L = O::last[i - 1];
+        r = O::Ql[i - 1][0] * f0[s];
+#pragma unroll
+        for (int m = 2; m <= O::NW; ++m)
+            if (m <= L) r = r + O::Ql[i - 1][m - 1] * f0[(long long)m * s];
+    } else if (i <= ntarg - O::NE) {
+        const long long b = (long long)(i + O::INSHIFT) * s;
+        r = O::Qin[0] * f0[b];
+        r = r + O::Qin[1] * f0[b + s];
+        r = r + O::Qin[2] * f0[b + 2 * s];
+        r = r + O::Qin[3] * f0[b + 3 * s];
+    } else {
+        const int l = ntarg - i + 1;  // mirrored left row
+        const int L = O::last[l - 1];
+        const long long b = (long long)(nsrc + 1 - L) * s;
+        r = (O::SIGN * O::Ql[l - 1][L - 1]) * f0[b];
+#pragma unroll
+        for (int t = 1; t < O::NW; ++t)
+            if (t < L) r = r + (O::SIGN * O::Ql[l - 1][L - 1 - t]) * f0[b + (long long)t * s];
+    }
+    return r;
+}
+
+struct CGeo {
+    const double *Jo, *Jx, *Jy, *Jq, *Qi12o, *Qi11x, *Qi22y, *fo;
+    const double *hs;  // orography at o points, or NULL
+    int N, pitch;
+    long long plane, fs;
+    double hx, scale, grav, dx1;
+};
+
+__device__ __forceinline__ long long cid(const CGeo &g, int p, int i, int j) {
+    return (long long)p * g.plane + (long long)(j - 1 + H) * g.pitch + (i - 1 + H);
+}
+
+#define C_POINT                                   \
+    const int i = blockIdx.x * BX + threadIdx.x + 1; \
+    const int j = blockIdx.y * BY + threadIdx.y + 1; \
+    const int p = blockIdx.z;
+
+// co2contra_c_sbp42 first half (co2contra_Cgrid_mod.f90:361-378): at o points
+// a = J Qi12 * (v interpolated along y), b = J Qi12 * (u interpolated along x)
+__global__ void __launch_bounds__(256) k_contra_o(CGeo g, const double *__restrict__ u, const double *__restrict__ v,
+                                                  double *__restrict__ A, double *__restrict__ B) {
+    C_POINT
+    if (i > g.N || j > g.N) return;
+    const long long id = cid(g, p, i, j);
+    const double dy = sop<W42i2c>(j, g.N + 1, v + cid(g, p, i, 0), g.pitch);
+    const double dx = sop<W42i2c>(i, g.N + 1, u + cid(g, p, 0, j), 1);
+    A[id] = g.Jo[id] * g.Qi12o[id] * dy;
+    B[id] = g.Jo[id] * g.Qi12o[id] * dx;
+}
+
+// second half of co2contra (:370-398), massflux_c_sbp42 (massflux_Cgrid_mod.f90:330-355),
+// the J-weighted fluxes of the divergence and Coriolis terms (div_c_sbp42_mod.f90:101-130,
+// coriolis_Cgrid_mod.f90:69-84) and the kinetic-energy halves of KE_Cgrid (KE_Cgrid_mod.f90:
+// 41-57): G = (J hu, J hv), KEu = J u ut / 2, KEv = J v vt / 2
+__global__ void __launch_bounds__(256) k_flux(CGeo g, const double *__restrict__ h, const double *__restrict__ u,
+                                              const double *__restrict__ v, const double *__restrict__ A,
+                                              const double *__restrict__ B, double *__restrict__ G,
+                                              double *__restrict__ KEu, double *__restrict__ KEv) {
+    C_POINT
+    const int N = g.N;
+    if (i <= N + 1 && j <= N) {
+        const long long id = cid(g, p, i, j);
+        const double t = sop<W42c2i>(i, N, A + cid(g, p, 0, j), 1);
+        const double ut = g.Qi11x[id] * u[id] + t / g.Jx[id];
+        const double hh = sop<W42c2i>(i, N, h + cid(g, p, 0, j), 1);
+        const double hu = ut * hh;
+        G[id] = g.Jx[id] * hu;
+        KEu[id] = g.Jx[id] * (u[id] * ut) / 2.0;
+    }
+    if (i <= N && j <= N + 1) {
+        const long long id = cid(g, p, i, j);
+        const double t = sop<W42c2i>(j, N, B + cid(g, p, i, 0), g.pitch);
+        const double vt = g.Qi22y[id] * v[id] + t / g.Jy[id];
+        const double hh = sop<W42c2i>(j, N, h + cid(g, p, i, 0), g.pitch);
+        const double hv = vt * hh;
+        G[g.fs + id] = g.Jy[id] * hv;
+        KEv[id] = g.Jy[id] * (v[id] * vt) / 2.0;
+    }
+}
+
+// at o points: KE = (I_x KEu + I_y KEv)/J + g h + g hs (KE_Cgrid_mod.f90:59-75 and the update of
+// operator_swm_mod.f90:85), and dh = -div(J hu, J hv) with the centre penalties
+// (div_c_sbp42_mod.f90:70-99, sbp_operator_mod.f90:344-413); G holds its face halo
+__global__ void __launch_bounds__(256) k_ke_div(CGeo g, const double *__restrict__ h, const double *__restrict__ KEu,
+                                                const double *__restrict__ KEv, const double *__restrict__ G,
+                                                double *__restrict__ KE, double *__restrict__ kh) {
+    C_POINT
+    const int N = g.N;
+    if (i > N || j > N) return;
+    const long long id = cid(g, p, i, j);
+    const double kuh = sop<W42i2c>(i, N + 1, KEu + cid(g, p, 0, j), 1);
+    const double kvh = sop<W42i2c>(j, N + 1, KEv + cid(g, p, i, 0), g.pitch);
+    double ke = (kuh + kvh) / g.Jo[id];
+    ke = ke + g.grav * h[id] + g.grav * (g.hs ? g.hs[id] : 0.0);
+    KE[id] = ke;
+    const double *Gu = G, *Gv = G + g.fs;
+    double dx = sop<D42i2c>(i, N + 1, Gu + cid(g, p, 0, j), 1);
+    if (i <= 3) {
+        const double df = Gu[cid(g, p, 1, j)] - Gu[cid(g, p, 0, j)];
+        dx = dx + 0.5 * PEN_L[i - 1] * df;
+    }
+    if (i >= N - 2) {
+        const double df = Gu[cid(g, p, N + 2, j)] - Gu[cid(g, p, N + 1, j)];
+        dx = dx + 0.5 * PEN_R[i - N + 2] * df;
+    }
+    double dy = sop<D42i2c>(j, N + 1, Gv + cid(g, p, i, 0), g.pitch);
+    if (j <= 3) {
+        const double df = Gv[cid(g, p, i, 1)] - Gv[cid(g, p, i, 0)];
+        dy = dy + 0.5 * PEN_L[j - 1] * df;
+    }
+    if (j >= N - 2) {
+        const double df = Gv[cid(g, p, i, N + 2)] - Gv[cid(g, p, i, N + 1)];
+        dy = dy + 0.5 * PEN_R[j - N + 2] * df;
+    }
+    const double div = (dx + dy) / (g.Jo[id] * g.hx * g.scale);
+    kh[id] = -1.0 * div;
+}
+
+// curl_c_sbp42 at q points before the corner sync (curl_c_sbp_mod.f90:73-118): staggered
+// differences of v along x and u along y with the interface penalties; u, v hold their
+// tangential halo
+__global__ void __launch_bounds__(256) k_curl(CGeo g, const double *__restrict__ u, const double *__restrict__ v,
+                                              double *__restrict__ W) {
+    C_POINT
+    const int N = g.N, n = N + 1;
+    if (i > n || j > n) return;
+    const long long id = cid(g, p, i, j);
+    double dvx = sop<D42c2i>(i, N, v + cid(g, p, 0, j), 1);
+    if (i == 1) {
+        double dv = 0.0;
+        for (int k = 1; k <= 3; ++k) dv = dv + PROJ[k - 1] * (v[cid(g, p, k, j)] - v[cid(g, p, 1 - k, j)]);
+        dvx = dvx + 0.5 * dv / A_IF1;
+    }
+    if (i == n) {
+        double dv = 0.0;
+        for (int t = 0; t <= 2; ++t) dv = dv + PROJ[t] * (v[cid(g, p, n + t, j)] - v[cid(g, p, n - t - 1, j)]);
+        dvx = dvx + 0.5 * dv / A_IF1;
+    }
+    double duy = sop<D42c2i>(j, N, u + cid(g, p, i, 0), g.pitch);
+    if (j == 1) {
+        double du = 0.0;
+        for (int k = 1; k <= 3; ++k) du = du + PROJ[k - 1] * (u[cid(g, p, i, k)] - u[cid(g, p, i, 1 - k)]);
+        duy = duy + 0.5 * du / A_IF1;
+    }
+    if (j == n) {
+        double du = 0.0;
+        for (int t = 0; t <= 2; ++t) du = du + PROJ[t] * (u[cid(g, p, i, n + t)] - u[cid(g, p, i, n - t - 1)]);
+        duy = duy + 0.5 * du / A_IF1;
+    }
+    W[id] = (dvx - duy) / (g.Jq[id] * g.hx * g.scale);
+}
+
+// halo_Ah_scalar_sync on the q points (halo_Ah_scalar_sync_mod.f90:52-93): a point on a panel
+// edge takes the mean of its copies, own and neighbours' in the reference's order; qnb lists
+// the neighbour's copy per (panel, edge i=1 / i=n / j=1 / j=n, position 1..n)
+__global__ void __launch_bounds__(256) k_qsync(CGeo g, const double *__restrict__ W, const long long *__restrict__ qnb,
+                                               double *__restrict__ Ws) {
+    C_POINT
+    const int n = g.N + 1;
+    if (i > n || j > n) return;
+    const long long id = cid(g, p, i, j);
+    const double own = W[id];
+    auto nb = [&](int e, int k) { return W[qnb[((long long)p * 4 + e) * n + (k - 1)]]; };
+    double r;
+    if (i == 1 && j == 1) r = (own + nb(0, 1) + nb(2, 1)) / 3.0;
+    else if (i == 1 && j == n) r = (own + nb(0, n) + nb(3, 1)) / 3.0;
+    else if (i == n && j == 1) r = (own + nb(1, 1) + nb(2, n)) / 3.0;
+    else if (i == n && j == n) r = (own + nb(1, n) + nb(3, n)) / 3.0;
+    else if (i == 1) r = 0.5 * (nb(0, j) + own);
+    else if (i == n) r = 0.5 * (own + nb(1, j));
+    else if (j == 1) r = 0.5 * (own + nb(2, i));
+    else if (j == n) r = 0.5 * (own + nb(3, i));
+    else r = own;
+    Ws[id] = r;
+}
+
+// interpolator_w2h first half (interpolator_w2h_mod.f90:50-68): the curl at x points
+// (interpolated along y) and at y points (along x)
+__global__ void __launch_bounds__(256) k_w2uv(CGeo g, const double *__restrict__ Ws, double *__restrict__ fu,
+                                              double *__restrict__ fv) {
+    C_POINT
+    const int N = g.N;
+    if (i <= N + 1 && j <= N) fu[cid(g, p, i, j)] = sop<W42i2c>(j, N + 1, Ws + cid(g, p, i, 0), g.pitch);
+    if (i <= N && j <= N + 1) fv[cid(g, p, i, j)] = sop<W42i2c>(i, N + 1, Ws + cid(g, p, 0, j), 1);
+}
+
+// at o points: curl_p = (I_x fu + I_y fv)/2 (interpolator_w2h_mod.f90:40-46), the fluxes J hu, J hv
+// interpolated to o points, and the vector-invariant Coriolis terms (coriolis_Cgrid_mod.f90:86-103)
+__global__ void __launch_bounds__(256) k_cor_o(CGeo g, const double *__restrict__ h, const double *__restrict__ fu,
+                                               const double *__restrict__ fv, const double *__restrict__ G,
+                                               double *__restrict__ cup, double *__restrict__ cvp) {
+    C_POINT
+    const int N = g.N;
+    if (i > N || j > N) return;
+    const long long id = cid(g, p, i, j);
+    const double fuh = sop<W42i2c>(i, N + 1, fu + cid(g, p, 0, j), 1);
+    const double fvh = sop<W42i2c>(j, N + 1, fv + cid(g, p, i, 0), g.pitch);
+    const double ghup = sop<W42i2c>(i, N + 1, G + cid(g, p, 0, j), 1);
+    const double ghvp = sop<W42i2c>(j, N + 1, G + g.fs + cid(g, p, i, 0), g.pitch);
+    const double cp = 0.5 * fuh + 0.5 * fvh;
+    cup[id] = (g.fo[id] + cp) * ghvp / h[id];
+    cvp[id] = -(g.fo[id] + cp) * ghup / h[id];
+}
+
+// gradient_c_sbp42 along one line (grad_c_sbp42_mod.f90:61-139): F(k) = f0[k*s], k = -2..N+3
+__device__ __forceinline__ double grad_line(int i, int N, const double *__restrict__ f0, long long s, double dx1) {
+    constexpr double dedge[4][5] = {{-2.0, 3.0, -1.0, 0.0, 0.0},
+                                    {-1.0, 1.0, 0.0, 0.0, 0.0},
+                                    {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0, 0.0},
+                                    {-1.0 / 71.0, 6.0 / 71.0, -83.0 / 71.0, 81.0 / 71.0, -3.0 / 71.0}};
+    constexpr int dlast[4] = {3, 2, 4, 5};
+    constexpr double din[4] = {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0};
+    const int n = N + 1;
+    auto F = [&](int k) { return f0[(long long)k * s]; };
+    double r;
+    if (i <= 4) {
+        r = dedge[i - 1][0] * F(1);
+#pragma unroll
+        for (int m = 2; m <= 5; ++m)
+            if (m <= dlast[i - 1]) r = r + dedge[i - 1][m - 1] * F(m);
+        r = r * dx1;
+        if (i == 1) {
+            const double dh = (PROJ[0] * (F(1) - F(0)) + PROJ[1] * (F(2) - F(-1)) + PROJ[2] * (F(3) - F(-2))) * dx1;
+            r = r + 0.5 * dh / A_IF1;
+        }
+    } else if (i <= N + 1 - 4) {
+        r = F(i - 2) * din[0];
+        r = r + F(i - 1) * din[1];
+        r = r + F(i) * din[2];
+        r = r + F(i + 1) * din[3];
+        r = r * dx1;
+    } else {
+        const int i2 = n - i + 1, i1 = dlast[i2 - 1];
+        double t = dedge[i2 - 1][0] * F(n - 1);
+#pragma unroll
+        for (int m = 2; m <= 5; ++m)
+            if (m <= i1) t = t + dedge[i2 - 1][m - 1] * F(n - m);
+        r = -t * dx1;
+        if (i == n) {
+            const double dh =
+                (PROJ[0] * (F(n) - F(n - 1)) + PROJ[1] * (F(n + 1) - F(n - 2)) + PROJ[2] * (F(n + 2) - F(n - 3))) * dx1;
+            r = r + 0.5 * dh / A_IF1;
+        }
+    }
+    return r;
+}
+
+// momentum tendencies at x and y points: du = -grad_x(KE) + cor_u, dv = -grad_y(KE) + cor_v with
+// cor interpolated from o points (interp2d_pvec2uv_C_sbp42, interpolator_p2uv_mod.f90:90-113;
+// operator_swm_mod.f90:94-95); KE holds its O halo
+__global__ void __launch_bounds__(256) k_mom(CGeo g, const double *__restrict__ KE, const double *__restrict__ cup,
+                                             const double *__restrict__ cvp, double *__restrict__ ku,
+                                             double *__restrict__ kv) {
+    C_POINT
+    const int N = g.N;
+    if (i <= N + 1 && j <= N) {
+        const long long id = cid(g, p, i, j);
+        const double gx = grad_line(i, N, KE + cid(g, p, 0, j), 1, g.dx1);
+        const double cu = sop<W42c2i>(i, N, cup + cid(g, p, 0, j), 1);
+        ku[id] = -1.0 * gx + 1.0 * cu;
+    }
+    if (i <= N && j <= N + 1) {
+        const long long id = cid(g, p, i, j);
+        const double gy = grad_line(j, N, KE + cid(g, p, i, 0), g.pitch, g.dx1);
+        const double cv = sop<W42c2i>(j, N, cvp + cid(g, p, i, 0), g.pitch);
+        kv[id] = -1.0 * gy + 1.0 * cv;
+    }
+}
+
+// cube-edge halo: base[dst[k]] = +-base[src[k]] (sources are mesh points, targets halo points)
+__global__ void __launch_bounds__(256) k_gather(int n, const long long *__restrict__ dst, const long long *__restrict__ src,
+                                                const signed char *__restrict__ sgn, double *base) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double v = base[src[k]];
+    base[dst[k]] = (sgn && sgn[k] < 0) ? -v : v;
+}
+
+// y = 1.0*v0 + b*k over the points of h (o), u (x) and v (y) (stvec_swm_t%assign)
+__global__ void __launch_bounds__(256) k_rk_axpy(CGeo g, double *__restrict__ y, const double *__restrict__ v0, double b,
+                                                 const double *__restrict__ k) {
+    C_POINT
+    const int N = g.N;
+    for (int f = 0; f < 3; ++f) {
+        const int nx = N + (f == 1), ny = N + (f == 2);
+        if (i > nx || j > ny) continue;
+        const long long id = f * g.fs + cid(g, p, i, j);
+        y[id] = 1.0 * v0[id] + b * k[id];
+    }
+}
+
+// v0 = v0 + c2 k0 + c3 k1, then v0 = v0 + c3 k2 + c2 k3 (rk4_mod.f90:38-39)
+__global__ void __launch_bounds__(256) k_rk_final(CGeo g, double *__restrict__ v0, const double *__restrict__ k0,
+                                                  const double *__restrict__ k1, const double *__restrict__ k2,
+                                                  const double *__restrict__ k3, double c2, double c3) {
+    C_POINT
+    const int N = g.N;
+    for (int f = 0; f < 3; ++f) {
+        const int nx = N + (f == 1), ny = N + (f == 2);
+        if (i > nx || j > ny) continue;
+        const long long id = f * g.fs + cid(g, p, i, j);
+        double r = v0[id] + c2 * k0[id] + c3 * k1[id];
+        r = r + c3 * k2[id] + c2 * k3[id];
+        v0[id] = r;
+    }
+}
+
+#undef C_POINT
+
+}  // namespace swmc
+
+// ------------------------------------------------------------------------------------------------
+struct swm_cctx {
+    int N = 0, rows = 0, pitch = 0;
+    long long plane = 0, fs = 0;
+    double scale = 1, omega = 1, grav = 1, hx = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double *geo = nullptr;  // 8 fields: Jo, Jx, Jy, Jq, Qi12o, Qi11x, Qi22y, fo
+    double *hs = nullptr;
+    // gather lists: tangential vector halo, face halo, O halo; neighbour copies of edge q points
+    long long *lists = nullptr;
+    signed char *signs = nullptr;
+    int nvec = 0, nface = 0, no = 0;
+    long long *vec_dst = nullptr, *vec_src = nullptr, *face_dst = nullptr, *face_src = nullptr, *o_dst = nullptr,
+              *o_src = nullptr, *qnb = nullptr;
+    signed char *vec_sign = nullptr, *face_sign = nullptr;
+    // work fields: A, B, G (pair), KEu, KEv, KE, W, Ws, fu, fv, cup, cvp, RK stages k0..k3, y
+    double *work = nullptr;
+    hipGraphExec_t graph = nullptr, graph_multi = nullptr;
+    void *graph_state = nullptr;
+    double graph_dt = 0.0;
+
+    size_t field_bytes() const { return (size_t)fs * sizeof(double); }
+    double *wf(int k) const { return work + (size_t)k * fs; }
+};
+
+namespace {
+
+constexpr int CW_A = 0, CW_B = 1, CW_G = 2 /* and 3 */, CW_KEU = 4, CW_KEV = 5, CW_KE = 6, CW_W = 7, CW_WS = 8,
+              CW_FU = 9, CW_FV = 10, CW_CUP = 11, CW_CVP = 12, CW_K = 13 /* 4 stages x 3 fields */,
+              CW_Y = 25 /* 3 fields */, CW_COUNT = 28;
+
+swmc::CGeo make_cgeo(const swm_cctx *c) {
+    swmc::CGeo g;
+    const double *G = c->geo;
+    const size_t F = (size_t)c->fs;
+    g.Jo = G;
+    g.Jx = G + F;
+    g.Jy = G + 2 * F;
+    g.Jq = G + 3 * F;
+    g.Qi12o = G + 4 * F;
+    g.Qi11x = G + 5 * F;
+    g.Qi22y = G + 6 * F;
+    g.fo = G + 7 * F;
+    g.hs = c->hs;
+    g.N = c->N;
+    g.pitch = c->pitch;
+    g.plane = c->plane;
+    g.fs = c->fs;
+    g.hx = c->hx;
+    g.scale = c->scale;
+    g.grav = c->grav;
+    g.dx1 = 1.0 / (c->scale * c->hx);
+    return g;
+}
+
+int c_gather(swm_cctx *c, int n, const long long *dst, const long long *src, const signed char *sgn, double *base) {
+    if (n <= 0) return SWM_OK;
+    hipLaunchKernelGGL(swmc::k_gather, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, dst, src, sgn, base);
+    HIP_TRY(hipGetLastError());
+    return SWM_OK;
+}
+
+// operator_swm_t%apply on the C grid: out = RHS(y); y's u, v halos are written
+int c_enqueue_rhs(swm_cctx *c, double *y, double *out) {
+    const swmc::CGeo g = make_cgeo(c);
+    const dim3 blk(swmc::BX, swmc::BY), grd((c->N + 1 + swmc::BX - 1) / swmc::BX, (c->N + 1 + swmc::BY - 1) / swmc::BY, 6);
+    const size_t F = (size_t)c->fs;
+    double *h = y, *u = y + F, *v = y + 2 * F;
+    int rc;
+    if ((rc = c_gather(c, c->nvec, c->vec_dst, c->vec_src, c->vec_sign, u))) return rc;
+    hipLaunchKernelGGL(swmc::k_contra_o, grd, blk, 0, c->stream, g, u, v, c->wf(CW_A), c->wf(CW_B));
+    hipLaunchKernelGGL(swmc::k_flux, grd, blk, 0, c->stream, g, h, u, v, c->wf(CW_A), c->wf(CW_B), c->wf(CW_G),
+                       c->wf(CW_KEU), c->wf(CW_KEV));
+    HIP_TRY(hipGetLastError());
+    if ((rc = c_gather(c, c->nface, c->face_dst, c->face_src, c->face_sign, c->wf(CW_G)))) return rc;
+    hipLaunchKernelGGL(swmc::k_ke_div, grd, blk, 0, c->stream, g, h, c->wf(CW_KEU), c->wf(CW_KEV), c->wf(CW_G),
+                       c->wf(CW_KE), out);
+    hipLaunchKernelGGL(swmc::k_curl, grd, blk, 0, c->stream, g, u, v, c->wf(CW_W));
+    hipLaunchKernelGGL(swmc::k_qsync, grd, blk, 0, c->stream, g, c->wf(CW_W), c->qnb, c->wf(CW_WS));
+    hipLaunchKernelGGL(swmc::k_w2uv, grd, blk, 0, c->stream, g, c->wf(CW_WS), c->wf(CW_FU), c->wf(CW_FV));
+    hipLaunchKernelGGL(swmc::k_cor_o, grd, blk, 0, c->stream, g, h, c->wf(CW_FU), c->wf(CW_FV), c->wf(CW_G),
+                       c->wf(CW_CUP), c->wf(CW_CVP));
+    HIP_TRY(hipGetLastError());
+    if ((rc = c_gather(c, c->no, c->o_dst, c->o_src, nullptr, c->wf(CW_KE)))) return rc;
+    hipLaunchKernelGGL(swmc::k_mom, grd, blk, 0, c->stream, g, c->wf(CW_KE), c->wf(CW_CUP), c->wf(CW_CVP), out + F,
+                       out + 2 * F);
+    HIP_TRY(hipGetLastError());
+    return SWM_OK;
+}
+
+// rk4_t%step (rk4_mod.f90:20-42)
+int c_enqueue_rk4_step(swm_cctx *c, double *state, double dt) {
+    const swmc::CGeo g = make_cgeo(c);
+    const dim3 blk(swmc::BX, swmc::BY), grd((c->N + 1 + swmc::BX - 1) / swmc::BX, (c->N + 1 + swmc::BY - 1) / swmc::BY, 6);
+    const size_t F = (size_t)c->fs;
+    double *k[4] = {c->wf(CW_K), c->wf(CW_K + 3), c->wf(CW_K + 6), c->wf(CW_K + 9)}, *y = c->wf(CW_Y);
+    const double b[3] = {0.5 * dt, 0.5 * dt, dt};
+    int rc;
+    if ((rc = c_enqueue_rhs(c, state, k[0]))) return rc;
+    for (int s = 0; s < 3; ++s) {
+        hipLaunchKernelGGL(swmc::k_rk_axpy, grd, blk, 0, c->stream, g, y, state, b[s], k[s]);
+        HIP_TRY(hipGetLastError());
+        if ((rc = c_enqueue_rhs(c, y, k[s + 1]))) return rc;
+    }
+    hipLaunchKernelGGL(swmc::k_rk_final, grd, blk, 0, c->stream, g, state, k[0], k[1], k[2], k[3], dt / 6.0, dt / 3.0);
+    HIP_TRY(hipGetLastError());
+    (void)F;
+    return SWM_OK;
+}
+
+void c_drop_graphs(swm_cctx *c) {
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    if (c->graph_multi) { (void)hipGraphExecDestroy(c->graph_multi); c->graph_multi = nullptr; }
+    c->graph_state = nullptr;
+}
+
+void c_free(swm_cctx *c) {
+    if (!c) return;
+    c_drop_graphs(c);
+    for (void *p : {(void *)c->geo, (void *)c->hs, (void *)c->lists, (void *)c->signs, (void *)c->work})
+        if (p) (void)hipFree(p);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// host <-> device copy of one field of point type t (nx x ny points) between the host tile layout
+// (6, N+1+2hw, N+1+2hw) and the stored layout
+int c_copy_field(swm_cctx *c, int t, double *dev, double *host, int hw, bool upload) {
+    const int N = c->N, nx = N + (t == 1 || t == 3), ny = N + (t == 2 || t == 3), ld = N + 1 + 2 * hw;
+    std::vector<double> buf((size_t)c->fs, 0.0);
+    if (upload) {
+        for (int p = 0; p < 6; ++p)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i)
+                    buf[((size_t)p * c->rows + (j - 1 + swmc::H)) * c->pitch + (i - 1 + swmc::H)] =
+                        host[((size_t)p * ld + (j - 1 + hw)) * ld + (i - 1 + hw)];
+        HIP_TRY(hipMemcpyAsync(dev, buf.data(), c->field_bytes(), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(buf.data(), dev, c->field_bytes(), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        std::fill(host, host + (size_t)6 * ld * ld, 0.0);
+        for (int p = 0; p < 6; ++p)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i)
+                    host[((size_t)p * ld + (j - 1 + hw)) * ld + (i - 1 + hw)] =
+                        buf[((size_t)p * c->rows + (j - 1 + swmc::H)) * c->pitch + (i - 1 + swmc::H)];
+    }
+    return SWM_OK;
+}
+
+}  // namespace
+
+extern "C" int swm_c_create(swm_cctx **out, int N, const char *div, const char *grad, const char *curl,
+                            const char *coriolis, const char *ke, const char *co2contra, const char *massflux,
+                            const char *quad, double scale, double omega, double grav) {
+    if (!out) return fail(SWM_EARG, "ctx pointer is NULL");
+    *out = nullptr;
+    if (N < 8 || N > 8192) return fail(SWM_EARG, "N must be in [8, 8192]");
+    if (!(scale > 0) || !(grav > 0)) return fail(SWM_EARG, "scale and grav must be positive");
+    const std::pair<const char *, const char *> names[] = {
+        {div, "divergence_c_sbp42"}, {grad, "gradient_c_sbp42"}, {curl, "curl_c_sbp42"},
+        {coriolis, "coriolis_Cgrid_sbp42"}, {ke, "KE_Cgrid_sbp42"}, {co2contra, "co2contra_c_sbp42"},
+        {massflux, "massflux_c_sbp42"}, {quad, "SBP_C42_quadrature"}};
+    for (const auto &nm : names)
+        if (!nm.first || std::string(nm.first) != nm.second)
+            return fail(SWM_ENAME, std::string("unsupported C-grid operator name: ") + (nm.first ? nm.first : "(null)") +
+                                       " (expected " + nm.second + ")");
+    swm_cctx *c = new swm_cctx();
+    c->N = N;
+    c->rows = N + 1 + 2 * swmc::H;
+    c->pitch = (c->rows + 7) / 8 * 8;
+    c->plane = (long long)c->rows * c->pitch;
+    c->fs = 6LL * c->plane;
+    c->scale = scale;
+    c->omega = omega;
+    c->grav = grav;
+    c->hx = (0.25 * std::acos(-1.0) - (-0.25 * std::acos(-1.0))) / (double)N;
+    auto bail = [&](int code) { c_free(c); return code; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SWM_EHIP, "hipStreamCreate failed"));
+    c->own_stream = true;
+
+    const size_t F = (size_t)c->fs;
+    std::vector<double> geo(8 * F, 0.0);
+    const double axis[3] = {0.0, 0.0, 1.0};
+    double *G = geo.data();
+    swmh_c_geometry(N, 0, swmc::H, c->rows, c->pitch, omega, axis, G, nullptr, G + 4 * F, nullptr, G + 7 * F);
+    swmh_c_geometry(N, 1, swmc::H, c->rows, c->pitch, omega, axis, G + F, G + 5 * F, nullptr, nullptr, nullptr);
+    swmh_c_geometry(N, 2, swmc::H, c->rows, c->pitch, omega, axis, G + 2 * F, nullptr, nullptr, G + 6 * F, nullptr);
+    swmh_c_geometry(N, 3, swmc::H, c->rows, c->pitch, omega, axis, G + 3 * F, nullptr, nullptr, nullptr, nullptr);
+    if (hipMalloc((void **)&c->geo, geo.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(c->geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SWM_EHIP, "C-grid geometry upload failed"));
+
+    const int dvec = 3, dO = 3;
+    const size_t mv = (size_t)6 * 4 * (N + 1) * dvec, mf = (size_t)6 * 4 * N, mo = (size_t)6 * 4 * N * dO,
+                 mq = (size_t)6 * 4 * (N + 1);
+    std::vector<long long> vd(mv), vs(mv), fd(mf), fsrc(mf), od(mo), os(mo), qo(mq), qn(mq);
+    std::vector<signed char> vsg(mv), fsg(mf);
+    int counts[4] = {0, 0, 0, 0};
+    const int hr = swmh_c_halo_lists(N, swmc::H, c->rows, c->pitch, c->fs, dvec, dO, vd.data(), vs.data(), vsg.data(),
+                                     fd.data(), fsrc.data(), fsg.data(), od.data(), os.data(), qo.data(), qn.data(), counts);
+    if (hr != 0 || (size_t)counts[0] != mv || (size_t)counts[1] != mf || (size_t)counts[2] != mo || (size_t)counts[3] != mq)
+        return bail(fail(SWM_EARG, "C-grid halo lists do not match the cube topology"));
+    // every offset stays inside one field pair (the pair for vectors, one field otherwise)
+    for (size_t k = 0; k < mv; ++k)
+        if (vd[k] < 0 || vd[k] >= 2 * c->fs || vs[k] < 0 || vs[k] >= 2 * c->fs) return bail(fail(SWM_EARG, "vector halo offset out of range"));
+    for (size_t k = 0; k < mf; ++k)
+        if (fd[k] < 0 || fd[k] >= 2 * c->fs || fsrc[k] < 0 || fsrc[k] >= 2 * c->fs) return bail(fail(SWM_EARG, "face halo offset out of range"));
+    for (size_t k = 0; k < mo; ++k)
+        if (od[k] < 0 || od[k] >= c->fs || os[k] < 0 || os[k] >= c->fs) return bail(fail(SWM_EARG, "O halo offset out of range"));
+    for (size_t k = 0; k < mq; ++k)
+        if (qn[k] < 0 || qn[k] >= c->fs) return bail(fail(SWM_EARG, "q sync offset out of range"));
+    c->nvec = (int)mv;
+    c->nface = (int)mf;
+    c->no = (int)mo;
+    const size_t nl = 2 * mv + 2 * mf + 2 * mo + mq;
+    std::vector<long long> L;
+    L.reserve(nl);
+    for (auto *v : {&vd, &vs, &fd, &fsrc, &od, &os, &qn}) L.insert(L.end(), v->begin(), v->end());
+    std::vector<signed char> S;
+    S.insert(S.end(), vsg.begin(), vsg.end());
+    S.insert(S.end(), fsg.begin(), fsg.end());
+    if (hipMalloc((void **)&c->lists, L.size() * sizeof(long long)) != hipSuccess ||
+        hipMemcpy(c->lists, L.data(), L.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc((void **)&c->signs, S.size()) != hipSuccess ||
+        hipMemcpy(c->signs, S.data(), S.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SWM_EHIP, "C-grid halo list upload failed"));
+    long long *lp = c->lists;
+    c->vec_dst = lp; lp += mv;
+    c->vec_src = lp; lp += mv;
+    c->face_dst = lp; lp += mf;
+    c->face_src = lp; lp += mf;
+    c->o_dst = lp; lp += mo;
+    c->o_src = lp; lp += mo;
+    c->qnb = lp;
+    c->vec_sign = c->signs;
+    c->face_sign = c->signs + mv;
+
+    const size_t wb = (size_t)CW_COUNT * c->field_bytes();
+    if (hipMalloc((void **)&c->work, wb) != hipSuccess || hipMemset(c->work, 0, wb) != hipSuccess)
+        return bail(fail(SWM_EHIP, "C-grid work buffer allocation failed"));
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(SWM_EHIP, "set-up synchronisation failed"));
+    *out = c;
+    return SWM_OK;
+}
+
+extern "C" int swm_c_destroy(swm_cctx *c) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c_free(c);
+    return SWM_OK;
+}
+
+extern "C" int swm_c_info(const swm_cctx *c, int *N, int *halo, int *rows, int *pitch, long long *field_elems) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (N) *N = c->N;
+    if (halo) *halo = swmc::H;
+    if (rows) *rows = c->rows;
+    if (pitch) *pitch = c->pitch;
+    if (field_elems) *field_elems = c->fs;
+    return SWM_OK;
+}
+
+extern "C" int swm_c_synchronize(swm_cctx *c) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SWM_OK;
+}
+
+extern "C" int swm_c_state_alloc(swm_cctx *c, void **state) {
+    if (!c || !state) return fail(SWM_EARG, "NULL argument");
+    const size_t sb = 3 * c->field_bytes();
+    HIP_TRY(hipMalloc(state, sb));
+    HIP_TRY(hipMemsetAsync(*state, 0, sb, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SWM_OK;
+}
+
+extern "C" int swm_c_state_free(swm_cctx *c, void *state) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (!state) return SWM_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->graph_state == state) c_drop_graphs(c);
+    HIP_TRY(hipFree(state));
+    return SWM_OK;
+}
+
+extern "C" int swm_c_state_upload(swm_cctx *c, void *state, const double *h, const double *u, const double *v, int hw) {
+    if (!c || !state || !h || !u || !v) return fail(SWM_EARG, "NULL argument");
+    if (hw < 0 || hw > 64) return fail(SWM_EARG, "halo_width out of range");
+    const double *src[3] = {h, u, v};
+    for (int k = 0; k < 3; ++k) {
+        int rc = c_copy_field(c, k, (double *)state + (size_t)k * c->fs, const_cast<double *>(src[k]), hw, true);
+        if (rc) return rc;
+    }
+    return SWM_OK;
+}
+
+extern "C" int swm_c_state_download(swm_cctx *c, const void *state, double *h, double *u, double *v, int hw) {
+    if (!c || !state || !h || !u || !v) return fail(SWM_EARG, "NULL argument");
+    if (hw < 0 || hw > 64) return fail(SWM_EARG, "halo_width out of range");
+    double *dst[3] = {h, u, v};
+    for (int k = 0; k < 3; ++k) {
+        int rc = c_copy_field(c, k, (double *)state + (size_t)k * c->fs, dst[k], hw, false);
+        if (rc) return rc;
+    }
+    return SWM_OK;
+}
+
+extern "C" int swm_c_set_orography(swm_cctx *c, const double *hs, int hw) {
+    if (!c) return fail(SWM_EARG, "ctx is NULL");
+    if (hw < 0 || hw > 64) return fail(SWM_EARG, "halo_width out of range");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c_drop_graphs(c);
+    if (!hs) {
+        if (c->hs) HIP_TRY(hipFree(c->hs));
+        c->hs = nullptr;
+        return SWM_OK;
+    }
+    if (!c->hs) {
+        HIP_TRY(hipMalloc((void **)&c->hs, c->field_bytes()));
+        HIP_TRY(hipMemsetAsync(c->hs, 0, c->field_bytes(), c->stream));
+    }
+    return c_copy_field(c, 0, c->hs, const_cast<double *>(hs), hw, true);
+}
+
+extern "C" int swm_c_rhs(swm_cctx *c, void *vin, void *vout) {
+    if (!c || !vin || !vout) return fail(SWM_EARG, "NULL argument");
+    if (vin == vout) return fail(SWM_EARG, "vin and vout must differ");
+    return c_enqueue_rhs(c, (double *)vin, (double *)vout);
+}
+
+extern "C" int swm_c_rk4_steps(swm_cctx *c, void *state, double dt, int nsteps) {
+    if (!c || !state) return fail(SWM_EARG, "NULL argument");
+    if (nsteps < 0) return fail(SWM_EARG, "nsteps must be >= 0");
+    if (nsteps == 0) return SWM_OK;
+    if (c->graph_state != state || c->graph_dt != dt) c_drop_graphs(c);
+    const int nmulti = nsteps / RK4_GRAPH_STEPS, nsingle = nsteps - nmulti * RK4_GRAPH_STEPS;
+    for (int which = 0; which < 2; ++which) {
+        hipGraphExec_t &ge = which == 0 ? c->graph : c->graph_multi;
+        const int per = which == 0 ? 1 : RK4_GRAPH_STEPS;
+        if (ge || (which == 0 ? nsingle : nmulti) == 0) continue;
+        hipGraph_t gr = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = SWM_OK;
+        for (int k = 0; k < per && !rc; ++k) rc = c_enqueue_rk4_step(c, (double *)state, dt);
+        hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+        if (e != hipSuccess) return fail(SWM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) { ge = nullptr; return fail(SWM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+        c->graph_state = state;
+        c->graph_dt = dt;
+    }
+    for (int s = 0; s < nmulti; ++s) HIP_TRY(hipGraphLaunch(c->graph_multi, c->stream));
+    for (int s = 0; s < nsingle; ++s) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    return SWM_OK;
+}

@@ -274,6 +274,44 @@ int swm_state_copy_to_device(swm_ctx *ctx, const void *state, void *h, void *u, 
 int swm_debug_geometry(swm_ctx *ctx, int which, double *out, int halo_width);
 int swm_debug_halo_table(swm_ctx *ctx, int *table);
 
+/* ---- C grid (staggering 'C'), SBP42 vector-invariant family, FP64 ----
+ * h at cell centres (N x N per panel), u at x faces (N+1 x N), v at y faces (N x N+1).  Host
+ * arrays use the layout of the Ah grid of the same N: (6, N+1+2hw, N+1+2hw), point (i, j) of any
+ * type at [p][j-1+hw][i-1+hw]; points outside a type's range are ignored on upload and zero on
+ * download.  Device state vectors hold h, u, v as 3 fields of field_elems doubles. */
+typedef struct swm_cctx swm_cctx;
+
+/* create_domain(staggering 'C') + create_swm_operator with the &shallow_water_model_c names
+ * (namelist_swm:80-102; create_vector_invariant_swm_operator,
+ * models/shallow_water/operator/operator_swm_factory_mod.f90:43-120):
+ * div "divergence_c_sbp42" (div_factory_mod.f90:67-87), grad "gradient_c_sbp42"
+ * (grad_factory_mod.f90:73-87), curl "curl_c_sbp42" (curl_factory_mod.f90:57-88), coriolis
+ * "coriolis_Cgrid_sbp42" (coriolis_factory_mod.f90:45-84), KE "KE_Cgrid_sbp42"
+ * (KE_factory_mod.f90:33-50), co2contra "co2contra_c_sbp42" (co2contra_factory_mod.f90:39-65),
+ * massflux "massflux_c_sbp42" (massflux_factory_mod.f90:56-61), quadrature "SBP_C42_quadrature";
+ * other names return SWM_ENAME.  8 <= N <= 8192. */
+int swm_c_create(swm_cctx **out, int N, const char *div_op_name, const char *grad_op_name,
+                 const char *curl_op_name, const char *coriolis_op_name, const char *KE_op_name,
+                 const char *co2contra_op_name, const char *massflux_op_name, const char *quadrature_name,
+                 double scale, double omega, double grav);
+int swm_c_destroy(swm_cctx *ctx);
+/* N, halo width of the stored fields, rows and pitch of one panel, elements of one field */
+int swm_c_info(const swm_cctx *ctx, int *N, int *halo, int *rows, int *pitch, long long *field_elems);
+int swm_c_synchronize(swm_cctx *ctx);
+/* stvec_swm_t allocation / host copies (stvec_swm_factory_mod.f90 on mesh_p, mesh_u, mesh_v) */
+int swm_c_state_alloc(swm_cctx *ctx, void **state);
+int swm_c_state_free(swm_cctx *ctx, void *state);
+int swm_c_state_upload(swm_cctx *ctx, void *state, const double *h, const double *u, const double *v,
+                       int halo_width);
+int swm_c_state_download(swm_cctx *ctx, const void *state, double *h, double *u, double *v, int halo_width);
+/* operator_swm_t%h_surf at o points (operator_swm_mod.f90:85); NULL removes it */
+int swm_c_set_orography(swm_cctx *ctx, const double *hs, int halo_width);
+/* operator_swm_t%apply (operator_swm_mod.f90:61-113) with the C-grid operators, enqueued on the
+ * context stream; vin's u, v halos are overwritten (the reference's exchange does the same) */
+int swm_c_rhs(swm_cctx *ctx, void *vin, void *vout);
+/* nsteps of rk4_t%step (time_schemes/rk4_mod.f90:20-42) in place, HIP-graph captured */
+int swm_c_rk4_steps(swm_cctx *ctx, void *state, double dt, int nsteps);
+
 #ifdef __cplusplus
 }
 #endif
