@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--c-steps", type=int, default=200, help="C-grid RK4 steps timed")
+    ap.add_argument("--c-dt", type=float, default=37.5,
+                    help="C-grid time step (the C-grid scheme's RK4 limit is near gravity-wave Courant number 0.5)")
     ap.add_argument("--forms", default="fp32,adjoint,batched,hyperdiffusion,cgrid",
                     help="comma-separated extra measurements beside the FP64 headline: fp32, adjoint, batched, "
                          "hyperdiffusion, cgrid")
@@ -285,10 +287,13 @@ def hyperdiffusion_form(args, stream, op, x):
             "sbp63_us_per_step": us63}
 
 
-# Field passes of one C-grid RK4 step as implemented (each intermediate field read or written
-# once per kernel): RHS = 6 (contra_o) + 13 (flux) + 8 (ke_div) + 4 (curl) + 2 (qsync) +
-# 3 (w2uv) + 8 (cor_o) + 5 (mom) = 49, step = 4 RHS + 3 x 9 (stage axpy) + 18 (final) = 241.
-C_STEP_FIELD_PASSES = 241
+# Field passes of one C-grid RK4 step as implemented (fields read or written once per kernel,
+# shared metric planes not counted): RHS = 4 (contra_o: u, v; A, B) + 9 (flux: A, B, h, u, v;
+# G pair, KEu, KEv) + 6 (ke_div: KEu, KEv, h, G pair; KE) + 3 (curl: u, v; W) + 3 (w2uv: W; fu,
+# fv) + 7 (cor_o: fu, fv, G pair, h, f; cup, cvp) + 3 (mom: KE, cup, cvp) = 35, plus the
+# tendency outputs: stages 1-3 write k (3), read v0 (3) and write the next input (3), stage 4
+# reads v0, k0, k1, k2 (12) and writes v0 (3): step = 4 x 35 + 3 x 9 + 15.
+C_STEP_FIELD_PASSES = 182
 
 
 def cgrid_form(args, stream):
@@ -299,7 +304,7 @@ def cgrid_form(args, stream):
     import torch
     import cubedsphere_sbp_swm_amd as swm
     from oracle.oracle import CGrid, lib as oracle_lib
-    N, dt, steps = args.N, args.dt, args.c_steps
+    N, dt, steps = args.N, args.c_dt, args.c_steps
     op = swm.create_swm_operator(N, **swm.operator_names("c42"), scale=swm.EARTH["radius"],
                                  omega=swm.EARTH["omega"], grav=swm.EARTH["grav"])
     C = CGrid(N, swm.EARTH["radius"], swm.EARTH["omega"], swm.EARTH["grav"])

@@ -478,9 +478,15 @@ class OperatorSWMC:
         if a.shape != self.host_shape(halo_width):
             raise ValueError(f"host array shape {a.shape} != {self.host_shape(halo_width)}")
 
-    # operator_t%apply (operator_swm_mod.f90:61-113); vin's u, v halos are overwritten
+    # operator_t%apply (operator_swm_mod.f90:61-113)
     def apply(self, vout, vin):
         check(_capi.lib().swm_c_rhs(self.ctx, ctypes.c_void_p(vin.ptr), ctypes.c_void_p(vout.ptr)))
+
+    # operator_t%get_diagnostics (operator_swm_mod.f90:115-236); enstrophy is NaN on the C grid
+    def get_diagnostics(self, v):
+        out = np.zeros(7)
+        check(_capi.lib().swm_c_diagnostics(self.ctx, ctypes.c_void_p(v.ptr), _dp(out)))
+        return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))
 
     def set_orography(self, hs, halo_width=0):
         if hs is None:

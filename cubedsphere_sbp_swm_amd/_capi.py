@@ -87,6 +87,7 @@ SIGNATURES = {
     "swm_c_set_orography": (_i, [_vp, _dp, _i]),
     "swm_c_rhs": (_i, [_vp, _vp, _vp]),
     "swm_c_rk4_steps": (_i, [_vp, _vp, _d, _i]),
+    "swm_c_diagnostics": (_i, [_vp, _vp, _dp]),
 }
 
 # host set-up routines (geometry.h), exported by the same library; they need no device

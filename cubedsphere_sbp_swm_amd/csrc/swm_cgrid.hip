@@ -15,9 +15,13 @@
 //     sbp_operator_mod.f90:344-413),
 //   - the O-point halo of the kinetic-energy potential (gradient, grad_c_sbp42_mod.f90:53-139),
 //   - the corner sync of the curl (halo_Ah_scalar_sync_mod.f90:52-93),
-// each a gather through precomputed offset lists.  One RHS is eleven launches, all
-// memory-bound stencil sweeps with one thread per output point (64 x 4 thread blocks,
-// coalesced along i); the RK4 step is captured into a HIP graph.
+// each read on the fly through precomputed neighbour lists by the threads on the panel edges.  One
+// RHS is seven launches of memory-bound stencil sweeps with one thread per output point (64 x 4
+// thread blocks, coalesced along i); the RK4 stage updates are applied by the kernels that produce
+// the tendencies.  Intermediate values at another point type (the J Qi12-weighted components at o
+// points, the curl at x and y points) are stored once and read by the next kernel: recomputing
+// them inside the consumer (4-6 nested stencil rows per output) measured slower than the extra
+// field traffic.  The RK4 step is captured into a HIP graph.
 //
 // Every expression keeps the reference's operation order with no fused multiply-adds
 // (-ffp-contract=off), so results match an FP64 CPU evaluation of the same algorithm bit for bit.
@@ -82,41 +86,60 @@ constexpr double A_IF1 = 7.0 / 18.0;
 constexpr double PEN_L[3] = {(15.0 / 8.0) / (13.0 / 12.0), (-5.0 / 4.0) / (7.0 / 8.0), (3.0 / 8.0) / (25.0 / 24.0)};
 constexpr double PEN_R[3] = {(3.0 / 8.0) / (25.0 / 24.0), (-5.0 / 4.0) / (7.0 / 8.0), (15.0 / 8.0) / (13.0 / 12.0)};
 
-// one output row i (1-based) of sbp_apply along a line with f(k) = f0[k*s], k = 1..nsrc:
-// left rows sum(Ql(1:last,i) f(1:last)), inner rows sum(Qin f(i+inshift:...)), right rows the
-// left block reflected (sign SIGN), each accumulated left to right from the first product
-template <class O>
-__device__ __forceinline__ double sop(int i, int nsrc, const double *__restrict__ f0, long long s) {
+// one output row i (1-based) of sbp_apply along a line with values f(k), k = 1..nsrc: left rows
+// sum(Ql(1:last,i) f(1:last)), inner rows sum(Qin f(i+inshift:...)), right rows the left block
+// reflected (sign SIGN), each accumulated left to right from the first product
+template <class O, class F>
+__device__ __forceinline__ double sopf(int i, int nsrc, F &&f) {
     const int ntarg = nsrc + O::DNX;
     double r;
     if (i <= O::NE) {
         const int L = O::last[i - 1];
-        r = O::Ql[i - 1][0] * f0[s];
+        r = O::Ql[i - 1][0] * f(1);
 #pragma unroll
         for (int m = 2; m <= O::NW; ++m)
-            if (m <= L) r = r + O::Ql[i - 1][m - 1] * f0[(long long)m * s];
+            if (m <= L) r = r + O::Ql[i - 1][m - 1] * f(m);
     } else if (i <= ntarg - O::NE) {
-        const long long b = (long long)(i + O::INSHIFT) * s;
-        r = O::Qin[0] * f0[b];
-        r = r + O::Qin[1] * f0[b + s];
-        r = r + O::Qin[2] * f0[b + 2 * s];
-        r = r + O::Qin[3] * f0[b + 3 * s];
+        const int b = i + O::INSHIFT;
+        r = O::Qin[0] * f(b);
+        r = r + O::Qin[1] * f(b + 1);
+        r = r + O::Qin[2] * f(b + 2);
+        r = r + O::Qin[3] * f(b + 3);
     } else {
         const int l = ntarg - i + 1;  // mirrored left row
         const int L = O::last[l - 1];
-        const long long b = (long long)(nsrc + 1 - L) * s;
-        r = (O::SIGN * O::Ql[l - 1][L - 1]) * f0[b];
+        const int b = nsrc + 1 - L;
+        r = (O::SIGN * O::Ql[l - 1][L - 1]) * f(b);
 #pragma unroll
         for (int t = 1; t < O::NW; ++t)
-            if (t < L) r = r + (O::SIGN * O::Ql[l - 1][L - 1 - t]) * f0[b + (long long)t * s];
+            if (t < L) r = r + (O::SIGN * O::Ql[l - 1][L - 1 - t]) * f(b + t);
     }
     return r;
 }
 
+// the same on a strided array line, f(k) = f0[k*s]
+template <class O>
+__device__ __forceinline__ double sop(int i, int nsrc, const double *__restrict__ f0, long long s) {
+    return sopf<O>(i, nsrc, [&](int k) { return f0[(long long)k * s]; });
+}
+
+// metric terms J, Qi at the four point types depend on (alpha, beta) only: one panel plane each,
+// read for every panel at the point's offset within its panel (mid); f differs per panel.
+// Cube-edge neighbour lists (offsets in the stored layout, signs +-1), indexed by panel p, edge
+// e (0: i = 1 side, 1: i = nx side, 2: j = 1 side, 3: j = ny side), depth d and position k:
+//   vsrc/vsg  the tangential vector component (v beyond x edges, u beyond y edges), offsets from
+//             the u field (+fs for v), [((p*4+e)*3 + d-1)*(N+1) + k-1]
+//   fsrc/fsg  the neighbour's copy of a shared face (normal component), from the first field of
+//             the pair, [(p*4+e)*N + k-1]
+//   osrc      O points, [((p*4+e)*3 + d-1)*N + k-1]
+//   qnb       q points on an edge: the neighbour's copy, [(p*4+e)*(N+1) + k-1]
 struct CGeo {
     const double *Jo, *Jx, *Jy, *Jq, *Qi12o, *Qi11x, *Qi22y, *fo;
     const double *hs;  // orography at o points, or NULL
+    const long long *vsrc, *fsrc, *osrc, *qnb;
+    const signed char *vsg, *fsg;
     int N, pitch;
+    int nbx, nby, ntiles, per;  // tiles per row, row strips per panel, tiles, tiles per XCD
     long long plane, fs;
     double hx, scale, grav, dx1;
 };
@@ -125,10 +148,78 @@ __device__ __forceinline__ long long cid(const CGeo &g, int p, int i, int j) {
     return (long long)p * g.plane + (long long)(j - 1 + H) * g.pitch + (i - 1 + H);
 }
 
-#define C_POINT                                   \
-    const int i = blockIdx.x * BX + threadIdx.x + 1; \
-    const int j = blockIdx.y * BY + threadIdx.y + 1; \
-    const int p = blockIdx.z;
+// tangential vector component beyond edge e at depth d, position k (pair = u field)
+__device__ __forceinline__ double vhalo(const CGeo &g, const double *pair, int p, int e, int d, int k) {
+    const long long x = (((long long)p * 4 + e) * 3 + (d - 1)) * (g.N + 1) + (k - 1);
+    const double val = pair[g.vsrc[x]];
+    return g.vsg[x] < 0 ? -val : val;
+}
+// the neighbour's copy of the shared face beyond edge e at position k (pair = first field)
+__device__ __forceinline__ double fhalo(const CGeo &g, const double *pair, int p, int e, int k) {
+    const long long x = ((long long)p * 4 + e) * g.N + (k - 1);
+    const double val = pair[g.fsrc[x]];
+    return g.fsg[x] < 0 ? -val : val;
+}
+__device__ __forceinline__ double ohalo(const CGeo &g, const double *f, int p, int e, int d, int k) {
+    return f[g.osrc[(((long long)p * 4 + e) * 3 + (d - 1)) * g.N + (k - 1)]];
+}
+
+// the curl at q point (i, j) after halo_Ah_scalar_sync (halo_Ah_scalar_sync_mod.f90:52-93): a
+// point on a panel edge takes the mean of its copies, own and neighbours', in the reference's order
+__device__ __forceinline__ double wsync(const CGeo &g, const double *__restrict__ W, int p, int i, int j) {
+    const int n = g.N + 1;
+    const double own = W[cid(g, p, i, j)];
+    if (i > 1 && i < n && j > 1 && j < n) return own;
+    auto nb = [&](int e, int k) { return W[g.qnb[((long long)p * 4 + e) * n + (k - 1)]]; };
+    if (i == 1 && j == 1) return (own + nb(0, 1) + nb(2, 1)) / 3.0;
+    if (i == 1 && j == n) return (own + nb(0, n) + nb(3, 1)) / 3.0;
+    if (i == n && j == 1) return (own + nb(1, 1) + nb(2, n)) / 3.0;
+    if (i == n && j == n) return (own + nb(1, n) + nb(3, n)) / 3.0;
+    if (i == 1) return 0.5 * (nb(0, j) + own);
+    if (i == n) return 0.5 * (own + nb(1, j));
+    if (j == 1) return 0.5 * (own + nb(2, i));
+    return 0.5 * (own + nb(3, i));
+}
+
+// RK4 stage handling of the kernels that produce the tendencies (rk4_mod.f90:20-42):
+// RK_RHS writes the tendency k; RK_STAGE writes k and the next stage input 1.0*v0 + b*k
+// (stvec_swm_t%assign); RK_FINAL writes v0 = (v0 + c2 k0 + c3 k1) + c3 k2 + c2 k (k = the last
+// stage's tendency, not stored)
+enum { RK_RHS = 0, RK_STAGE = 1, RK_FINAL = 2 };
+struct RkArgs {
+    double *k;          // tendency out (RK_RHS, RK_STAGE)
+    const double *v0;   // stage-1 state (RK_STAGE, RK_FINAL)
+    double *yn;         // next stage input (RK_STAGE)
+    double *v0w;        // state written (RK_FINAL; the same buffer as v0)
+    const double *k0, *k1, *k2;
+    double b, c2, c3;
+};
+
+template <int RKM>
+__device__ __forceinline__ void rk_out(const RkArgs &r, long long off, double k) {
+    if (RKM == RK_RHS) {
+        r.k[off] = k;
+    } else if (RKM == RK_STAGE) {
+        r.k[off] = k;
+        r.yn[off] = 1.0 * r.v0[off] + r.b * k;
+    } else {
+        double v = r.v0[off] + r.c2 * r.k0[off] + r.c3 * r.k1[off];
+        v = v + r.c3 * r.k2[off] + r.c2 * k;
+        r.v0w[off] = v;
+    }
+}
+
+// Block -> tile mapping: tiles of BX x BY points in the order (column strip fastest, row strip,
+// panel); the hardware hands consecutive block indices to the 8 XCDs in turn, so block b takes
+// tile (b % 8) * per + b / 8 and every XCD walks a contiguous band of row strips, keeping the
+// rows its stencils share in its own L2
+#define C_POINT                                                        \
+    const int tile_ = (int)(blockIdx.x & 7) * g.per + (int)(blockIdx.x >> 3); \
+    if (tile_ >= g.ntiles) return;                                     \
+    const int tx_ = tile_ % g.nbx, ty_ = (tile_ / g.nbx) % g.nby;       \
+    const int i = tx_ * BX + threadIdx.x + 1;                          \
+    const int j = ty_ * BY + threadIdx.y + 1;                          \
+    const int p = tile_ / (g.nbx * g.nby);
 
 // co2contra_c_sbp42 first half (co2contra_Cgrid_mod.f90:361-378): at o points
 // a = J Qi12 * (v interpolated along y), b = J Qi12 * (u interpolated along x)
@@ -136,17 +227,17 @@ __global__ void __launch_bounds__(256) k_contra_o(CGeo g, const double *__restri
                                                   double *__restrict__ A, double *__restrict__ B) {
     C_POINT
     if (i > g.N || j > g.N) return;
-    const long long id = cid(g, p, i, j);
+    const long long id = cid(g, p, i, j), mid = id - (long long)p * g.plane;
     const double dy = sop<W42i2c>(j, g.N + 1, v + cid(g, p, i, 0), g.pitch);
     const double dx = sop<W42i2c>(i, g.N + 1, u + cid(g, p, 0, j), 1);
-    A[id] = g.Jo[id] * g.Qi12o[id] * dy;
-    B[id] = g.Jo[id] * g.Qi12o[id] * dx;
+    A[id] = g.Jo[mid] * g.Qi12o[mid] * dy;
+    B[id] = g.Jo[mid] * g.Qi12o[mid] * dx;
 }
 
-// second half of co2contra (:370-398), massflux_c_sbp42 (massflux_Cgrid_mod.f90:330-355),
-// the J-weighted fluxes of the divergence and Coriolis terms (div_c_sbp42_mod.f90:101-130,
-// coriolis_Cgrid_mod.f90:69-84) and the kinetic-energy halves of KE_Cgrid (KE_Cgrid_mod.f90:
-// 41-57): G = (J hu, J hv), KEu = J u ut / 2, KEv = J v vt / 2
+// at x and y points: second half of co2contra (:370-398), massflux_c_sbp42
+// (massflux_Cgrid_mod.f90:330-355), the J-weighted fluxes of the divergence and Coriolis terms
+// (div_c_sbp42_mod.f90:101-130, coriolis_Cgrid_mod.f90:69-84) and the halves of KE_Cgrid
+// (KE_Cgrid_mod.f90:41-57): G = (J hu, J hv), KEu = J u ut / 2, KEv = J v vt / 2
 __global__ void __launch_bounds__(256) k_flux(CGeo g, const double *__restrict__ h, const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ A,
                                               const double *__restrict__ B, double *__restrict__ G,
@@ -154,129 +245,109 @@ __global__ void __launch_bounds__(256) k_flux(CGeo g, const double *__restrict__
     C_POINT
     const int N = g.N;
     if (i <= N + 1 && j <= N) {
-        const long long id = cid(g, p, i, j);
+        const long long id = cid(g, p, i, j), mid = id - (long long)p * g.plane;
         const double t = sop<W42c2i>(i, N, A + cid(g, p, 0, j), 1);
-        const double ut = g.Qi11x[id] * u[id] + t / g.Jx[id];
+        const double ut = g.Qi11x[mid] * u[id] + t / g.Jx[mid];
         const double hh = sop<W42c2i>(i, N, h + cid(g, p, 0, j), 1);
         const double hu = ut * hh;
-        G[id] = g.Jx[id] * hu;
-        KEu[id] = g.Jx[id] * (u[id] * ut) / 2.0;
+        G[id] = g.Jx[mid] * hu;
+        KEu[id] = g.Jx[mid] * (u[id] * ut) / 2.0;
     }
     if (i <= N && j <= N + 1) {
-        const long long id = cid(g, p, i, j);
+        const long long id = cid(g, p, i, j), mid = id - (long long)p * g.plane;
         const double t = sop<W42c2i>(j, N, B + cid(g, p, i, 0), g.pitch);
-        const double vt = g.Qi22y[id] * v[id] + t / g.Jy[id];
+        const double vt = g.Qi22y[mid] * v[id] + t / g.Jy[mid];
         const double hh = sop<W42c2i>(j, N, h + cid(g, p, i, 0), g.pitch);
         const double hv = vt * hh;
-        G[g.fs + id] = g.Jy[id] * hv;
-        KEv[id] = g.Jy[id] * (v[id] * vt) / 2.0;
+        G[g.fs + id] = g.Jy[mid] * hv;
+        KEv[id] = g.Jy[mid] * (v[id] * vt) / 2.0;
     }
 }
 
 // at o points: KE = (I_x KEu + I_y KEv)/J + g h + g hs (KE_Cgrid_mod.f90:59-75 and the update of
 // operator_swm_mod.f90:85), and dh = -div(J hu, J hv) with the centre penalties
-// (div_c_sbp42_mod.f90:70-99, sbp_operator_mod.f90:344-413); G holds its face halo
+// (div_c_sbp42_mod.f90:70-99, sbp_operator_mod.f90:344-413), the neighbour's copies of the shared
+// faces read through fsrc
+template <int RKM>
 __global__ void __launch_bounds__(256) k_ke_div(CGeo g, const double *__restrict__ h, const double *__restrict__ KEu,
                                                 const double *__restrict__ KEv, const double *__restrict__ G,
-                                                double *__restrict__ KE, double *__restrict__ kh) {
+                                                double *__restrict__ KE, RkArgs r) {
     C_POINT
     const int N = g.N;
     if (i > N || j > N) return;
-    const long long id = cid(g, p, i, j);
+    const long long id = cid(g, p, i, j), mid = id - (long long)p * g.plane;
     const double kuh = sop<W42i2c>(i, N + 1, KEu + cid(g, p, 0, j), 1);
     const double kvh = sop<W42i2c>(j, N + 1, KEv + cid(g, p, i, 0), g.pitch);
-    double ke = (kuh + kvh) / g.Jo[id];
+    double ke = (kuh + kvh) / g.Jo[mid];
     ke = ke + g.grav * h[id] + g.grav * (g.hs ? g.hs[id] : 0.0);
     KE[id] = ke;
     const double *Gu = G, *Gv = G + g.fs;
     double dx = sop<D42i2c>(i, N + 1, Gu + cid(g, p, 0, j), 1);
     if (i <= 3) {
-        const double df = Gu[cid(g, p, 1, j)] - Gu[cid(g, p, 0, j)];
+        const double df = Gu[cid(g, p, 1, j)] - fhalo(g, G, p, 0, j);
         dx = dx + 0.5 * PEN_L[i - 1] * df;
     }
     if (i >= N - 2) {
-        const double df = Gu[cid(g, p, N + 2, j)] - Gu[cid(g, p, N + 1, j)];
+        const double df = fhalo(g, G, p, 1, j) - Gu[cid(g, p, N + 1, j)];
         dx = dx + 0.5 * PEN_R[i - N + 2] * df;
     }
     double dy = sop<D42i2c>(j, N + 1, Gv + cid(g, p, i, 0), g.pitch);
     if (j <= 3) {
-        const double df = Gv[cid(g, p, i, 1)] - Gv[cid(g, p, i, 0)];
+        const double df = Gv[cid(g, p, i, 1)] - fhalo(g, G, p, 2, i);
         dy = dy + 0.5 * PEN_L[j - 1] * df;
     }
     if (j >= N - 2) {
-        const double df = Gv[cid(g, p, i, N + 2)] - Gv[cid(g, p, i, N + 1)];
+        const double df = fhalo(g, G, p, 3, i) - Gv[cid(g, p, i, N + 1)];
         dy = dy + 0.5 * PEN_R[j - N + 2] * df;
     }
-    const double div = (dx + dy) / (g.Jo[id] * g.hx * g.scale);
-    kh[id] = -1.0 * div;
+    const double div = (dx + dy) / (g.Jo[mid] * g.hx * g.scale);
+    rk_out<RKM>(r, id, -1.0 * div);
 }
 
 // curl_c_sbp42 at q points before the corner sync (curl_c_sbp_mod.f90:73-118): staggered
-// differences of v along x and u along y with the interface penalties; u, v hold their
-// tangential halo
-__global__ void __launch_bounds__(256) k_curl(CGeo g, const double *__restrict__ u, const double *__restrict__ v,
-                                              double *__restrict__ W) {
+// differences of v along x and u along y with the interface penalties, the neighbour's
+// tangential components read through vsrc
+__global__ void __launch_bounds__(256) k_curl(CGeo g, const double *__restrict__ u, double *__restrict__ W) {
     C_POINT
     const int N = g.N, n = N + 1;
     if (i > n || j > n) return;
-    const long long id = cid(g, p, i, j);
+    const double *v = u + g.fs;
+    const long long id = cid(g, p, i, j), mid = id - (long long)p * g.plane;
     double dvx = sop<D42c2i>(i, N, v + cid(g, p, 0, j), 1);
     if (i == 1) {
         double dv = 0.0;
-        for (int k = 1; k <= 3; ++k) dv = dv + PROJ[k - 1] * (v[cid(g, p, k, j)] - v[cid(g, p, 1 - k, j)]);
+        for (int k = 1; k <= 3; ++k) dv = dv + PROJ[k - 1] * (v[cid(g, p, k, j)] - vhalo(g, u, p, 0, k, j));
         dvx = dvx + 0.5 * dv / A_IF1;
     }
     if (i == n) {
         double dv = 0.0;
-        for (int t = 0; t <= 2; ++t) dv = dv + PROJ[t] * (v[cid(g, p, n + t, j)] - v[cid(g, p, n - t - 1, j)]);
+        for (int t = 0; t <= 2; ++t) dv = dv + PROJ[t] * (vhalo(g, u, p, 1, t + 1, j) - v[cid(g, p, n - t - 1, j)]);
         dvx = dvx + 0.5 * dv / A_IF1;
     }
     double duy = sop<D42c2i>(j, N, u + cid(g, p, i, 0), g.pitch);
     if (j == 1) {
         double du = 0.0;
-        for (int k = 1; k <= 3; ++k) du = du + PROJ[k - 1] * (u[cid(g, p, i, k)] - u[cid(g, p, i, 1 - k)]);
+        for (int k = 1; k <= 3; ++k) du = du + PROJ[k - 1] * (u[cid(g, p, i, k)] - vhalo(g, u, p, 2, k, i));
         duy = duy + 0.5 * du / A_IF1;
     }
     if (j == n) {
         double du = 0.0;
-        for (int t = 0; t <= 2; ++t) du = du + PROJ[t] * (u[cid(g, p, i, n + t)] - u[cid(g, p, i, n - t - 1)]);
+        for (int t = 0; t <= 2; ++t) du = du + PROJ[t] * (vhalo(g, u, p, 3, t + 1, i) - u[cid(g, p, i, n - t - 1)]);
         duy = duy + 0.5 * du / A_IF1;
     }
-    W[id] = (dvx - duy) / (g.Jq[id] * g.hx * g.scale);
+    W[id] = (dvx - duy) / (g.Jq[mid] * g.hx * g.scale);
 }
 
-// halo_Ah_scalar_sync on the q points (halo_Ah_scalar_sync_mod.f90:52-93): a point on a panel
-// edge takes the mean of its copies, own and neighbours' in the reference's order; qnb lists
-// the neighbour's copy per (panel, edge i=1 / i=n / j=1 / j=n, position 1..n)
-__global__ void __launch_bounds__(256) k_qsync(CGeo g, const double *__restrict__ W, const long long *__restrict__ qnb,
-                                               double *__restrict__ Ws) {
-    C_POINT
-    const int n = g.N + 1;
-    if (i > n || j > n) return;
-    const long long id = cid(g, p, i, j);
-    const double own = W[id];
-    auto nb = [&](int e, int k) { return W[qnb[((long long)p * 4 + e) * n + (k - 1)]]; };
-    double r;
-    if (i == 1 && j == 1) r = (own + nb(0, 1) + nb(2, 1)) / 3.0;
-    else if (i == 1 && j == n) r = (own + nb(0, n) + nb(3, 1)) / 3.0;
-    else if (i == n && j == 1) r = (own + nb(1, 1) + nb(2, n)) / 3.0;
-    else if (i == n && j == n) r = (own + nb(1, n) + nb(3, n)) / 3.0;
-    else if (i == 1) r = 0.5 * (nb(0, j) + own);
-    else if (i == n) r = 0.5 * (own + nb(1, j));
-    else if (j == 1) r = 0.5 * (own + nb(2, i));
-    else if (j == n) r = 0.5 * (own + nb(3, i));
-    else r = own;
-    Ws[id] = r;
-}
-
-// interpolator_w2h first half (interpolator_w2h_mod.f90:50-68): the curl at x points
-// (interpolated along y) and at y points (along x)
-__global__ void __launch_bounds__(256) k_w2uv(CGeo g, const double *__restrict__ Ws, double *__restrict__ fu,
+// interpolator_w2h first half (interpolator_w2h_mod.f90:50-68): the synced curl at x points
+// (interpolated along y) and at y points (along x); the corner sync applied as the values are read
+__global__ void __launch_bounds__(256) k_w2uv(CGeo g, const double *__restrict__ W, double *__restrict__ fu,
                                               double *__restrict__ fv) {
     C_POINT
     const int N = g.N;
-    if (i <= N + 1 && j <= N) fu[cid(g, p, i, j)] = sop<W42i2c>(j, N + 1, Ws + cid(g, p, i, 0), g.pitch);
-    if (i <= N && j <= N + 1) fv[cid(g, p, i, j)] = sop<W42i2c>(i, N + 1, Ws + cid(g, p, 0, j), 1);
+    if (i <= N + 1 && j <= N)
+        fu[cid(g, p, i, j)] = sopf<W42i2c>(j, N + 1, [&](int jq) { return wsync(g, W, p, i, jq); });
+    if (i <= N && j <= N + 1)
+        fv[cid(g, p, i, j)] = sopf<W42i2c>(i, N + 1, [&](int iq) { return wsync(g, W, p, iq, j); });
 }
 
 // at o points: curl_p = (I_x fu + I_y fv)/2 (interpolator_w2h_mod.f90:40-46), the fluxes J hu, J hv
@@ -297,8 +368,9 @@ __global__ void __launch_bounds__(256) k_cor_o(CGeo g, const double *__restrict_
     cvp[id] = -(g.fo[id] + cp) * ghup / h[id];
 }
 
-// gradient_c_sbp42 along one line (grad_c_sbp42_mod.f90:61-139): F(k) = f0[k*s], k = -2..N+3
-__device__ __forceinline__ double grad_line(int i, int N, const double *__restrict__ f0, long long s, double dx1) {
+// gradient_c_sbp42 along one line (grad_c_sbp42_mod.f90:61-139), F(k) for k = -2..N+3
+template <class F>
+__device__ __forceinline__ double grad_line(int i, int N, F &&Fv, double dx1) {
     constexpr double dedge[4][5] = {{-2.0, 3.0, -1.0, 0.0, 0.0},
                                     {-1.0, 1.0, 0.0, 0.0, 0.0},
                                     {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0, 0.0},
@@ -306,34 +378,33 @@ __device__ __forceinline__ double grad_line(int i, int N, const double *__restri
     constexpr int dlast[4] = {3, 2, 4, 5};
     constexpr double din[4] = {1.0 / 24.0, -9.0 / 8.0, 9.0 / 8.0, -1.0 / 24.0};
     const int n = N + 1;
-    auto F = [&](int k) { return f0[(long long)k * s]; };
     double r;
     if (i <= 4) {
-        r = dedge[i - 1][0] * F(1);
+        r = dedge[i - 1][0] * Fv(1);
 #pragma unroll
         for (int m = 2; m <= 5; ++m)
-            if (m <= dlast[i - 1]) r = r + dedge[i - 1][m - 1] * F(m);
+            if (m <= dlast[i - 1]) r = r + dedge[i - 1][m - 1] * Fv(m);
         r = r * dx1;
         if (i == 1) {
-            const double dh = (PROJ[0] * (F(1) - F(0)) + PROJ[1] * (F(2) - F(-1)) + PROJ[2] * (F(3) - F(-2))) * dx1;
+            const double dh = (PROJ[0] * (Fv(1) - Fv(0)) + PROJ[1] * (Fv(2) - Fv(-1)) + PROJ[2] * (Fv(3) - Fv(-2))) * dx1;
             r = r + 0.5 * dh / A_IF1;
         }
     } else if (i <= N + 1 - 4) {
-        r = F(i - 2) * din[0];
-        r = r + F(i - 1) * din[1];
-        r = r + F(i) * din[2];
-        r = r + F(i + 1) * din[3];
+        r = Fv(i - 2) * din[0];
+        r = r + Fv(i - 1) * din[1];
+        r = r + Fv(i) * din[2];
+        r = r + Fv(i + 1) * din[3];
         r = r * dx1;
     } else {
         const int i2 = n - i + 1, i1 = dlast[i2 - 1];
-        double t = dedge[i2 - 1][0] * F(n - 1);
+        double t = dedge[i2 - 1][0] * Fv(n - 1);
 #pragma unroll
         for (int m = 2; m <= 5; ++m)
-            if (m <= i1) t = t + dedge[i2 - 1][m - 1] * F(n - m);
+            if (m <= i1) t = t + dedge[i2 - 1][m - 1] * Fv(n - m);
         r = -t * dx1;
         if (i == n) {
             const double dh =
-                (PROJ[0] * (F(n) - F(n - 1)) + PROJ[1] * (F(n + 1) - F(n - 2)) + PROJ[2] * (F(n + 2) - F(n - 3))) * dx1;
+                (PROJ[0] * (Fv(n) - Fv(n - 1)) + PROJ[1] * (Fv(n + 1) - Fv(n - 2)) + PROJ[2] * (Fv(n + 2) - Fv(n - 3))) * dx1;
             r = r + 0.5 * dh / A_IF1;
         }
     }
@@ -342,62 +413,134 @@ __device__ __forceinline__ double grad_line(int i, int N, const double *__restri
 
 // momentum tendencies at x and y points: du = -grad_x(KE) + cor_u, dv = -grad_y(KE) + cor_v with
 // cor interpolated from o points (interp2d_pvec2uv_C_sbp42, interpolator_p2uv_mod.f90:90-113;
-// operator_swm_mod.f90:94-95); KE holds its O halo
+// operator_swm_mod.f90:94-95); KE beyond the panel edges read through osrc
+template <int RKM>
 __global__ void __launch_bounds__(256) k_mom(CGeo g, const double *__restrict__ KE, const double *__restrict__ cup,
-                                             const double *__restrict__ cvp, double *__restrict__ ku,
-                                             double *__restrict__ kv) {
+                                             const double *__restrict__ cvp, RkArgs r) {
     C_POINT
     const int N = g.N;
     if (i <= N + 1 && j <= N) {
         const long long id = cid(g, p, i, j);
-        const double gx = grad_line(i, N, KE + cid(g, p, 0, j), 1, g.dx1);
+        const double *row = KE + cid(g, p, 0, j);
+        const double gx = grad_line(i, N, [&](int k) {
+            return k < 1 ? ohalo(g, KE, p, 0, 1 - k, j) : k > N ? ohalo(g, KE, p, 1, k - N, j) : row[k];
+        }, g.dx1);
         const double cu = sop<W42c2i>(i, N, cup + cid(g, p, 0, j), 1);
-        ku[id] = -1.0 * gx + 1.0 * cu;
+        rk_out<RKM>(r, g.fs + id, -1.0 * gx + 1.0 * cu);
     }
     if (i <= N && j <= N + 1) {
         const long long id = cid(g, p, i, j);
-        const double gy = grad_line(j, N, KE + cid(g, p, i, 0), g.pitch, g.dx1);
+        const double *col = KE + cid(g, p, i, 0);
+        const double gy = grad_line(j, N, [&](int k) {
+            return k < 1 ? ohalo(g, KE, p, 2, 1 - k, i) : k > N ? ohalo(g, KE, p, 3, k - N, i)
+                                                               : col[(long long)k * g.pitch];
+        }, g.dx1);
         const double cv = sop<W42c2i>(j, N, cvp + cid(g, p, i, 0), g.pitch);
-        kv[id] = -1.0 * gy + 1.0 * cv;
+        rk_out<RKM>(r, 2 * g.fs + id, -1.0 * gy + 1.0 * cv);
     }
 }
 
-// cube-edge halo: base[dst[k]] = +-base[src[k]] (sources are mesh points, targets halo points)
-__global__ void __launch_bounds__(256) k_gather(int n, const long long *__restrict__ dst, const long long *__restrict__ src,
-                                                const signed char *__restrict__ sgn, double *base) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double v = base[src[k]];
-    base[dst[k]] = (sgn && sgn[k] < 0) ? -v : v;
+// ---- get_diagnostics on the C grid (operator_swm_mod.f90:115-155, calc_energy :211-236) ----
+// Per-point terms f*J*hx*hy*Ax(i)*Ay(j) of SBP_C42_quadrature (sbp_quadrature_mod.f90:21-40;
+// Ax, Ay = A_centers along centre directions, A_interfaces along face directions) summed with
+// compensated (Neumaier) additions per block: mass, KE_u, KE_v, PE, then hmin, hmax.
+constexpr int CD_NQ = 4;
+
+__device__ __forceinline__ void cd_add(double &s, double &c, double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x))
+        c += (s - t) + x;
+    else
+        c += (x - t) + s;
+    s = t;
 }
 
-// y = 1.0*v0 + b*k over the points of h (o), u (x) and v (y) (stvec_swm_t%assign)
-__global__ void __launch_bounds__(256) k_rk_axpy(CGeo g, double *__restrict__ y, const double *__restrict__ v0, double b,
-                                                 const double *__restrict__ k) {
+__global__ void __launch_bounds__(256) k_cdiag_partial(CGeo g, const double *__restrict__ y,
+                                                       const double *__restrict__ Acen,
+                                                       const double *__restrict__ Aint, double *__restrict__ part) {
     C_POINT
     const int N = g.N;
-    for (int f = 0; f < 3; ++f) {
-        const int nx = N + (f == 1), ny = N + (f == 2);
-        if (i > nx || j > ny) continue;
-        const long long id = f * g.fs + cid(g, p, i, j);
-        y[id] = 1.0 * v0[id] + b * k[id];
+    const double *h = y, *u = y + g.fs, *v = y + 2 * g.fs;
+    double s[CD_NQ] = {0.0, 0.0, 0.0, 0.0}, cc[CD_NQ] = {0.0, 0.0, 0.0, 0.0};
+    double hmin = 1e300, hmax = -1e300;
+    const double hx = g.hx;
+    if (i <= N && j <= N) {
+        const long long id = cid(g, p, i, j), mid = id - (long long)p * g.plane;
+        const double hh = h[id];
+        hmin = hh;
+        hmax = hh;
+        cd_add(s[0], cc[0], hh * g.Jo[mid] * hx * hx * Acen[i] * Acen[j]);
+        const double pe = 1.0 * hh + 1.0 * (g.hs ? g.hs[id] : 0.0);
+        const double PE = 0.5 * g.grav * pe * pe;
+        cd_add(s[3], cc[3], PE * g.Jo[mid] * hx * hx * Acen[i] * Acen[j]);
     }
+    if (i <= N + 1 && j <= N) {  // KE_diag_u = 0.5 * hu * u at x points, hu = massflux(h, ut)
+        const long long id = cid(g, p, i, j), mid = id - (long long)p * g.plane;
+        const double t = sopf<W42c2i>(i, N, [&](int ip) {
+            const long long m = cid(g, 0, ip, j);
+            const double dy = sop<W42i2c>(j, N + 1, v + cid(g, p, ip, 0), g.pitch);
+            return g.Jo[m] * g.Qi12o[m] * dy;
+        });
+        const double ut = g.Qi11x[mid] * u[id] + t / g.Jx[mid];
+        const double hu = ut * sop<W42c2i>(i, N, h + cid(g, p, 0, j), 1);
+        const double ke = 0.5 * hu * u[id];
+        cd_add(s[1], cc[1], ke * g.Jx[mid] * hx * hx * Aint[i] * Acen[j]);
+    }
+    if (i <= N && j <= N + 1) {
+        const long long id = cid(g, p, i, j), mid = id - (long long)p * g.plane;
+        const double t = sopf<W42c2i>(j, N, [&](int jp) {
+            const long long m = cid(g, 0, i, jp);
+            const double dx = sop<W42i2c>(i, N + 1, u + cid(g, p, 0, jp), 1);
+            return g.Jo[m] * g.Qi12o[m] * dx;
+        });
+        const double vt = g.Qi22y[mid] * v[id] + t / g.Jy[mid];
+        const double hv = vt * sop<W42c2i>(j, N, h + cid(g, p, i, 0), g.pitch);
+        const double ke = 0.5 * hv * v[id];
+        cd_add(s[2], cc[2], ke * g.Jy[mid] * hx * hx * Acen[i] * Aint[j]);
+    }
+    __shared__ double sh[256][2 * CD_NQ + 2];
+    const int t = threadIdx.y * BX + threadIdx.x;
+    for (int q = 0; q < CD_NQ; ++q) {
+        sh[t][2 * q] = s[q];
+        sh[t][2 * q + 1] = cc[q];
+    }
+    sh[t][2 * CD_NQ] = hmin;
+    sh[t][2 * CD_NQ + 1] = hmax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int q = 0; q < CD_NQ; ++q) {
+                double a = sh[t][2 * q], ac = sh[t][2 * q + 1];
+                cd_add(a, ac, sh[t + w][2 * q]);
+                ac += sh[t + w][2 * q + 1];
+                sh[t][2 * q] = a;
+                sh[t][2 * q + 1] = ac;
+            }
+            sh[t][2 * CD_NQ] = fmin(sh[t][2 * CD_NQ], sh[t + w][2 * CD_NQ]);
+            sh[t][2 * CD_NQ + 1] = fmax(sh[t][2 * CD_NQ + 1], sh[t + w][2 * CD_NQ + 1]);
+        }
+        __syncthreads();
+    }
+    if (t < 2 * CD_NQ + 2) part[(long long)tile_ * 10 + t] = sh[0][t];
 }
 
-// v0 = v0 + c2 k0 + c3 k1, then v0 = v0 + c3 k2 + c2 k3 (rk4_mod.f90:38-39)
-__global__ void __launch_bounds__(256) k_rk_final(CGeo g, double *__restrict__ v0, const double *__restrict__ k0,
-                                                  const double *__restrict__ k1, const double *__restrict__ k2,
-                                                  const double *__restrict__ k3, double c2, double c3) {
-    C_POINT
-    const int N = g.N;
-    for (int f = 0; f < 3; ++f) {
-        const int nx = N + (f == 1), ny = N + (f == 2);
-        if (i > nx || j > ny) continue;
-        const long long id = f * g.fs + cid(g, p, i, j);
-        double r = v0[id] + c2 * k0[id] + c3 * k1[id];
-        r = r + c3 * k2[id] + c2 * k3[id];
-        v0[id] = r;
+// one thread over the block partials: {mass, KE_u, KE_v, PE, hmin, hmax}
+__global__ void __launch_bounds__(64) k_cdiag_final(const double *__restrict__ part, long long nblocks,
+                                                    double *__restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double s[CD_NQ] = {0.0, 0.0, 0.0, 0.0}, cc[CD_NQ] = {0.0, 0.0, 0.0, 0.0};
+    double hmin = 1e300, hmax = -1e300;
+    for (long long b = 0; b < nblocks; ++b) {
+        for (int q = 0; q < CD_NQ; ++q) {
+            cd_add(s[q], cc[q], part[b * 10 + 2 * q]);
+            cc[q] += part[b * 10 + 2 * q + 1];
+        }
+        hmin = fmin(hmin, part[b * 10 + 2 * CD_NQ]);
+        hmax = fmax(hmax, part[b * 10 + 2 * CD_NQ + 1]);
     }
+    for (int q = 0; q < CD_NQ; ++q) out[q] = s[q] + cc[q];
+    out[CD_NQ] = hmin;
+    out[CD_NQ + 1] = hmax;
 }
 
 #undef C_POINT
@@ -411,17 +554,20 @@ struct swm_cctx {
     double scale = 1, omega = 1, grav = 1, hx = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    double *geo = nullptr;  // 8 fields: Jo, Jx, Jy, Jq, Qi12o, Qi11x, Qi22y, fo
+    double *geo = nullptr;  // one panel plane each of Jo, Jx, Jy, Jq, Qi12o, Qi11x, Qi22y, then fo (6 panels)
     double *hs = nullptr;
-    // gather lists: tangential vector halo, face halo, O halo; neighbour copies of edge q points
+    // cube-edge neighbour lists (swmc::CGeo): vsrc, fsrc, osrc, qnb; signs vsg, fsg
     long long *lists = nullptr;
     signed char *signs = nullptr;
-    int nvec = 0, nface = 0, no = 0;
-    long long *vec_dst = nullptr, *vec_src = nullptr, *face_dst = nullptr, *face_src = nullptr, *o_dst = nullptr,
-              *o_src = nullptr, *qnb = nullptr;
-    signed char *vec_sign = nullptr, *face_sign = nullptr;
-    // work fields: A, B, G (pair), KEu, KEv, KE, W, Ws, fu, fv, cup, cvp, RK stages k0..k3, y
+    long long *vsrc = nullptr, *fsrc = nullptr, *osrc = nullptr, *qnb = nullptr;
+    signed char *vsg = nullptr, *fsg = nullptr;
+    // work fields: G (pair), KEu, KEv, KE, W, cup, cvp, RK tendencies k0..k2, stage inputs yA, yB,
+    // A, B (J Qi12-weighted components at o points), fu, fv (the synced curl at x and y points)
     double *work = nullptr;
+    // diagnostics: quadrature weights A_centers (1..N) and A_interfaces (1..N+1) per index, block
+    // partial sums (10 per block) and the 6 results
+    double *qw = nullptr, *dpart = nullptr;
+    long long dblocks = 0;
     hipGraphExec_t graph = nullptr, graph_multi = nullptr;
     void *graph_state = nullptr;
     double graph_dt = 0.0;
@@ -432,25 +578,35 @@ struct swm_cctx {
 
 namespace {
 
-constexpr int CW_A = 0, CW_B = 1, CW_G = 2 /* and 3 */, CW_KEU = 4, CW_KEV = 5, CW_KE = 6, CW_W = 7, CW_WS = 8,
-              CW_FU = 9, CW_FV = 10, CW_CUP = 11, CW_CVP = 12, CW_K = 13 /* 4 stages x 3 fields */,
-              CW_Y = 25 /* 3 fields */, CW_COUNT = 28;
+constexpr int CW_G = 0 /* and 1 */, CW_KEU = 2, CW_KEV = 3, CW_KE = 4, CW_W = 5, CW_CUP = 6, CW_CVP = 7,
+              CW_K = 8 /* 3 stages x 3 fields */, CW_YA = 17, CW_YB = 20, CW_A = 23, CW_B = 24, CW_FU = 25,
+              CW_FV = 26, CW_COUNT = 27;
 
 swmc::CGeo make_cgeo(const swm_cctx *c) {
     swmc::CGeo g;
     const double *G = c->geo;
-    const size_t F = (size_t)c->fs;
+    const size_t P = (size_t)c->plane;
     g.Jo = G;
-    g.Jx = G + F;
-    g.Jy = G + 2 * F;
-    g.Jq = G + 3 * F;
-    g.Qi12o = G + 4 * F;
-    g.Qi11x = G + 5 * F;
-    g.Qi22y = G + 6 * F;
-    g.fo = G + 7 * F;
+    g.Jx = G + P;
+    g.Jy = G + 2 * P;
+    g.Jq = G + 3 * P;
+    g.Qi12o = G + 4 * P;
+    g.Qi11x = G + 5 * P;
+    g.Qi22y = G + 6 * P;
+    g.fo = G + 7 * P;
     g.hs = c->hs;
+    g.vsrc = c->vsrc;
+    g.fsrc = c->fsrc;
+    g.osrc = c->osrc;
+    g.qnb = c->qnb;
+    g.vsg = c->vsg;
+    g.fsg = c->fsg;
     g.N = c->N;
     g.pitch = c->pitch;
+    g.nbx = (c->N + 1 + swmc::BX - 1) / swmc::BX;
+    g.nby = (c->N + 1 + swmc::BY - 1) / swmc::BY;
+    g.ntiles = g.nbx * g.nby * 6;
+    g.per = (g.ntiles + 7) / 8;
     g.plane = c->plane;
     g.fs = c->fs;
     g.hx = c->hx;
@@ -460,59 +616,53 @@ swmc::CGeo make_cgeo(const swm_cctx *c) {
     return g;
 }
 
-int c_gather(swm_cctx *c, int n, const long long *dst, const long long *src, const signed char *sgn, double *base) {
-    if (n <= 0) return SWM_OK;
-    hipLaunchKernelGGL(swmc::k_gather, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, dst, src, sgn, base);
-    HIP_TRY(hipGetLastError());
-    return SWM_OK;
-}
-
-// operator_swm_t%apply on the C grid: out = RHS(y); y's u, v halos are written
-int c_enqueue_rhs(swm_cctx *c, double *y, double *out) {
+// operator_swm_t%apply on the C grid for the input y (h, u, v), seven launches; the tendencies go
+// where RKM says (RK_RHS: r.k; RK_STAGE: r.k and the next stage input; RK_FINAL: the state)
+template <int RKM>
+int c_enqueue_rhs(swm_cctx *c, const double *y, const swmc::RkArgs &r) {
     const swmc::CGeo g = make_cgeo(c);
-    const dim3 blk(swmc::BX, swmc::BY), grd((c->N + 1 + swmc::BX - 1) / swmc::BX, (c->N + 1 + swmc::BY - 1) / swmc::BY, 6);
+    const dim3 blk(swmc::BX, swmc::BY), grd(8 * g.per);
     const size_t F = (size_t)c->fs;
-    double *h = y, *u = y + F, *v = y + 2 * F;
-    int rc;
-    if ((rc = c_gather(c, c->nvec, c->vec_dst, c->vec_src, c->vec_sign, u))) return rc;
+    const double *h = y, *u = y + F, *v = y + 2 * F;
     hipLaunchKernelGGL(swmc::k_contra_o, grd, blk, 0, c->stream, g, u, v, c->wf(CW_A), c->wf(CW_B));
     hipLaunchKernelGGL(swmc::k_flux, grd, blk, 0, c->stream, g, h, u, v, c->wf(CW_A), c->wf(CW_B), c->wf(CW_G),
                        c->wf(CW_KEU), c->wf(CW_KEV));
-    HIP_TRY(hipGetLastError());
-    if ((rc = c_gather(c, c->nface, c->face_dst, c->face_src, c->face_sign, c->wf(CW_G)))) return rc;
-    hipLaunchKernelGGL(swmc::k_ke_div, grd, blk, 0, c->stream, g, h, c->wf(CW_KEU), c->wf(CW_KEV), c->wf(CW_G),
-                       c->wf(CW_KE), out);
-    hipLaunchKernelGGL(swmc::k_curl, grd, blk, 0, c->stream, g, u, v, c->wf(CW_W));
-    hipLaunchKernelGGL(swmc::k_qsync, grd, blk, 0, c->stream, g, c->wf(CW_W), c->qnb, c->wf(CW_WS));
-    hipLaunchKernelGGL(swmc::k_w2uv, grd, blk, 0, c->stream, g, c->wf(CW_WS), c->wf(CW_FU), c->wf(CW_FV));
+    hipLaunchKernelGGL(swmc::k_ke_div<RKM>, grd, blk, 0, c->stream, g, h, c->wf(CW_KEU), c->wf(CW_KEV), c->wf(CW_G),
+                       c->wf(CW_KE), r);
+    hipLaunchKernelGGL(swmc::k_curl, grd, blk, 0, c->stream, g, u, c->wf(CW_W));
+    hipLaunchKernelGGL(swmc::k_w2uv, grd, blk, 0, c->stream, g, c->wf(CW_W), c->wf(CW_FU), c->wf(CW_FV));
     hipLaunchKernelGGL(swmc::k_cor_o, grd, blk, 0, c->stream, g, h, c->wf(CW_FU), c->wf(CW_FV), c->wf(CW_G),
                        c->wf(CW_CUP), c->wf(CW_CVP));
-    HIP_TRY(hipGetLastError());
-    if ((rc = c_gather(c, c->no, c->o_dst, c->o_src, nullptr, c->wf(CW_KE)))) return rc;
-    hipLaunchKernelGGL(swmc::k_mom, grd, blk, 0, c->stream, g, c->wf(CW_KE), c->wf(CW_CUP), c->wf(CW_CVP), out + F,
-                       out + 2 * F);
+    hipLaunchKernelGGL(swmc::k_mom<RKM>, grd, blk, 0, c->stream, g, c->wf(CW_KE), c->wf(CW_CUP), c->wf(CW_CVP), r);
     HIP_TRY(hipGetLastError());
     return SWM_OK;
 }
 
-// rk4_t%step (rk4_mod.f90:20-42)
+// rk4_t%step (rk4_mod.f90:20-42): stage inputs state -> yA -> yB -> yA, the stage updates and the
+// final combination inside the tendency kernels
 int c_enqueue_rk4_step(swm_cctx *c, double *state, double dt) {
-    const swmc::CGeo g = make_cgeo(c);
-    const dim3 blk(swmc::BX, swmc::BY), grd((c->N + 1 + swmc::BX - 1) / swmc::BX, (c->N + 1 + swmc::BY - 1) / swmc::BY, 6);
-    const size_t F = (size_t)c->fs;
-    double *k[4] = {c->wf(CW_K), c->wf(CW_K + 3), c->wf(CW_K + 6), c->wf(CW_K + 9)}, *y = c->wf(CW_Y);
+    double *k[3] = {c->wf(CW_K), c->wf(CW_K + 3), c->wf(CW_K + 6)};
+    const double *in[4] = {state, c->wf(CW_YA), c->wf(CW_YB), c->wf(CW_YA)};
+    double *next[3] = {c->wf(CW_YA), c->wf(CW_YB), c->wf(CW_YA)};
     const double b[3] = {0.5 * dt, 0.5 * dt, dt};
     int rc;
-    if ((rc = c_enqueue_rhs(c, state, k[0]))) return rc;
     for (int s = 0; s < 3; ++s) {
-        hipLaunchKernelGGL(swmc::k_rk_axpy, grd, blk, 0, c->stream, g, y, state, b[s], k[s]);
-        HIP_TRY(hipGetLastError());
-        if ((rc = c_enqueue_rhs(c, y, k[s + 1]))) return rc;
+        swmc::RkArgs r{};
+        r.k = k[s];
+        r.v0 = state;
+        r.yn = next[s];
+        r.b = b[s];
+        if ((rc = c_enqueue_rhs<swmc::RK_STAGE>(c, in[s], r))) return rc;
     }
-    hipLaunchKernelGGL(swmc::k_rk_final, grd, blk, 0, c->stream, g, state, k[0], k[1], k[2], k[3], dt / 6.0, dt / 3.0);
-    HIP_TRY(hipGetLastError());
-    (void)F;
-    return SWM_OK;
+    swmc::RkArgs r{};
+    r.v0 = state;
+    r.v0w = state;
+    r.k0 = k[0];
+    r.k1 = k[1];
+    r.k2 = k[2];
+    r.c2 = dt / 6.0;
+    r.c3 = dt / 3.0;
+    return c_enqueue_rhs<swmc::RK_FINAL>(c, in[3], r);
 }
 
 void c_drop_graphs(swm_cctx *c) {
@@ -524,7 +674,8 @@ void c_drop_graphs(swm_cctx *c) {
 void c_free(swm_cctx *c) {
     if (!c) return;
     c_drop_graphs(c);
-    for (void *p : {(void *)c->geo, (void *)c->hs, (void *)c->lists, (void *)c->signs, (void *)c->work})
+    for (void *p : {(void *)c->geo, (void *)c->hs, (void *)c->lists, (void *)c->signs, (void *)c->work, (void *)c->qw,
+                    (void *)c->dpart})
         if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -596,8 +747,20 @@ extern "C" int swm_c_create(swm_cctx **out, int N, const char *div, const char *
     swmh_c_geometry(N, 1, swmc::H, c->rows, c->pitch, omega, axis, G + F, G + 5 * F, nullptr, nullptr, nullptr);
     swmh_c_geometry(N, 2, swmc::H, c->rows, c->pitch, omega, axis, G + 2 * F, nullptr, nullptr, G + 6 * F, nullptr);
     swmh_c_geometry(N, 3, swmc::H, c->rows, c->pitch, omega, axis, G + 3 * F, nullptr, nullptr, nullptr, nullptr);
-    if (hipMalloc((void **)&c->geo, geo.size() * sizeof(double)) != hipSuccess ||
-        hipMemcpy(c->geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    // device copy: panel 0 of the seven metric planes (every panel holds the same values, checked
+    // here bit for bit), then f on all six panels
+    const size_t P = (size_t)c->plane;
+    std::vector<double> dgeo(7 * P + F);
+    for (int w = 0; w < 7; ++w) {
+        const double *m = G + (size_t)w * F;
+        for (int p = 1; p < 6; ++p)
+            if (std::memcmp(m, m + p * P, P * sizeof(double)) != 0)
+                return bail(fail(SWM_EARG, "C-grid metric differs between panels"));
+        std::copy(m, m + P, dgeo.begin() + (size_t)w * P);
+    }
+    std::copy(G + 7 * F, G + 8 * F, dgeo.begin() + 7 * P);
+    if (hipMalloc((void **)&c->geo, dgeo.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(c->geo, dgeo.data(), dgeo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(SWM_EHIP, "C-grid geometry upload failed"));
 
     const int dvec = 3, dO = 3;
@@ -619,13 +782,10 @@ extern "C" int swm_c_create(swm_cctx **out, int N, const char *div, const char *
         if (od[k] < 0 || od[k] >= c->fs || os[k] < 0 || os[k] >= c->fs) return bail(fail(SWM_EARG, "O halo offset out of range"));
     for (size_t k = 0; k < mq; ++k)
         if (qn[k] < 0 || qn[k] >= c->fs) return bail(fail(SWM_EARG, "q sync offset out of range"));
-    c->nvec = (int)mv;
-    c->nface = (int)mf;
-    c->no = (int)mo;
-    const size_t nl = 2 * mv + 2 * mf + 2 * mo + mq;
+    // device lists: sources in the order the kernels index them (swmc::CGeo)
     std::vector<long long> L;
-    L.reserve(nl);
-    for (auto *v : {&vd, &vs, &fd, &fsrc, &od, &os, &qn}) L.insert(L.end(), v->begin(), v->end());
+    L.reserve(mv + mf + mo + mq);
+    for (auto *v : {&vs, &fsrc, &os, &qn}) L.insert(L.end(), v->begin(), v->end());
     std::vector<signed char> S;
     S.insert(S.end(), vsg.begin(), vsg.end());
     S.insert(S.end(), fsg.begin(), fsg.end());
@@ -634,17 +794,30 @@ extern "C" int swm_c_create(swm_cctx **out, int N, const char *div, const char *
         hipMalloc((void **)&c->signs, S.size()) != hipSuccess ||
         hipMemcpy(c->signs, S.data(), S.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(SWM_EHIP, "C-grid halo list upload failed"));
-    long long *lp = c->lists;
-    c->vec_dst = lp; lp += mv;
-    c->vec_src = lp; lp += mv;
-    c->face_dst = lp; lp += mf;
-    c->face_src = lp; lp += mf;
-    c->o_dst = lp; lp += mo;
-    c->o_src = lp; lp += mo;
-    c->qnb = lp;
-    c->vec_sign = c->signs;
-    c->face_sign = c->signs + mv;
+    c->vsrc = c->lists;
+    c->fsrc = c->vsrc + mv;
+    c->osrc = c->fsrc + mf;
+    c->qnb = c->osrc + mo;
+    c->vsg = c->signs;
+    c->fsg = c->signs + mv;
 
+    {
+        // create_mass_coefficients (quadrature_factory_mod.f90:156-175): A_centers (13/12, 7/8,
+        // 25/24, 1, ...) and A_interfaces (7/18, 9/8, 1, 71/72, 1, ...) from both ends
+        const double Ac[4] = {13.0 / 12.0, 7.0 / 8.0, 25.0 / 24.0, 1.0};
+        const double Ai[5] = {7.0 / 18.0, 9.0 / 8.0, 1.0, 71.0 / 72.0, 1.0};
+        std::vector<double> q((size_t)2 * (N + 2), 1.0);
+        double *acen = q.data(), *aint = q.data() + (N + 2);
+        for (int i = 1; i <= std::min(4, N); ++i) acen[i] = Ac[i - 1];
+        for (int i = std::max(N - 3, 1); i <= N; ++i) acen[i] = Ac[N - i];
+        for (int i = 1; i <= std::min(5, N + 1); ++i) aint[i] = Ai[i - 1];
+        for (int i = std::max(N + 1 - 4, 1); i <= N + 1; ++i) aint[i] = Ai[N + 1 - i];
+        c->dblocks = (long long)((N + 1 + swmc::BX - 1) / swmc::BX) * ((N + 1 + swmc::BY - 1) / swmc::BY) * 6;
+        if (hipMalloc((void **)&c->qw, q.size() * sizeof(double)) != hipSuccess ||
+            hipMemcpy(c->qw, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMalloc((void **)&c->dpart, (size_t)(c->dblocks * 10 + 8) * sizeof(double)) != hipSuccess)
+            return bail(fail(SWM_EHIP, "C-grid diagnostics set-up failed"));
+    }
     const size_t wb = (size_t)CW_COUNT * c->field_bytes();
     if (hipMalloc((void **)&c->work, wb) != hipSuccess || hipMemset(c->work, 0, wb) != hipSuccess)
         return bail(fail(SWM_EHIP, "C-grid work buffer allocation failed"));
@@ -733,10 +906,12 @@ extern "C" int swm_c_set_orography(swm_cctx *c, const double *hs, int hw) {
     return c_copy_field(c, 0, c->hs, const_cast<double *>(hs), hw, true);
 }
 
-extern "C" int swm_c_rhs(swm_cctx *c, void *vin, void *vout) {
+extern "C" int swm_c_rhs(swm_cctx *c, const void *vin, void *vout) {
     if (!c || !vin || !vout) return fail(SWM_EARG, "NULL argument");
     if (vin == vout) return fail(SWM_EARG, "vin and vout must differ");
-    return c_enqueue_rhs(c, (double *)vin, (double *)vout);
+    swmc::RkArgs r{};
+    r.k = (double *)vout;
+    return c_enqueue_rhs<swmc::RK_RHS>(c, (const double *)vin, r);
 }
 
 extern "C" int swm_c_rk4_steps(swm_cctx *c, void *state, double dt, int nsteps) {
@@ -764,5 +939,29 @@ extern "C" int swm_c_rk4_steps(swm_cctx *c, void *state, double dt, int nsteps) 
     }
     for (int s = 0; s < nmulti; ++s) HIP_TRY(hipGraphLaunch(c->graph_multi, c->stream));
     for (int s = 0; s < nsingle; ++s) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    return SWM_OK;
+}
+
+extern "C" int swm_c_diagnostics(swm_cctx *c, const void *state, double out[7]) {
+    if (!c || !state || !out) return fail(SWM_EARG, "NULL argument");
+    const swmc::CGeo g = make_cgeo(c);
+    const dim3 blk(swmc::BX, swmc::BY), grd(8 * g.per);
+    double *res = c->dpart + c->dblocks * 10;
+    hipLaunchKernelGGL(swmc::k_cdiag_partial, grd, blk, 0, c->stream, g, (const double *)state, c->qw,
+                       c->qw + (c->N + 2), c->dpart);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(swmc::k_cdiag_final, dim3(1), dim3(64), 0, c->stream, c->dpart, c->dblocks, res);
+    HIP_TRY(hipGetLastError());
+    double r[6];
+    HIP_TRY(hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double ke = r[1] + r[2];
+    out[0] = r[4];
+    out[1] = r[5];
+    out[2] = r[0];
+    out[3] = ke + r[3];
+    out[4] = ke;
+    out[5] = r[3];
+    out[6] = std::nan("");
     return SWM_OK;
 }

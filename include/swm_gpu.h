@@ -306,11 +306,16 @@ int swm_c_state_upload(swm_cctx *ctx, void *state, const double *h, const double
 int swm_c_state_download(swm_cctx *ctx, const void *state, double *h, double *u, double *v, int halo_width);
 /* operator_swm_t%h_surf at o points (operator_swm_mod.f90:85); NULL removes it */
 int swm_c_set_orography(swm_cctx *ctx, const double *hs, int halo_width);
-/* operator_swm_t%apply (operator_swm_mod.f90:61-113) with the C-grid operators, enqueued on the
- * context stream; vin's u, v halos are overwritten (the reference's exchange does the same) */
-int swm_c_rhs(swm_cctx *ctx, void *vin, void *vout);
+/* operator_swm_t%apply (operator_swm_mod.f90:61-113) with the C-grid operators: vout = RHS(vin),
+ * enqueued on the context stream */
+int swm_c_rhs(swm_cctx *ctx, const void *vin, void *vout);
 /* nsteps of rk4_t%step (time_schemes/rk4_mod.f90:20-42) in place, HIP-graph captured */
 int swm_c_rk4_steps(swm_cctx *ctx, void *state, double dt, int nsteps);
+/* operator_t%get_diagnostics (operator_swm_mod.f90:115-236) with SBP_C42_quadrature
+ * (quadrature_factory_mod.f90:88-175): out = {hmin, hmax, mass, TE, KE, PE, enstrophy} with
+ * compensated FP64 sums; enstrophy is NaN (calc_enstrophy, :238-260, divides the curl at corner
+ * points by h at the same indices and is marked in the reference as not working for the C grid) */
+int swm_c_diagnostics(swm_cctx *ctx, const void *state, double out[7]);
 
 #ifdef __cplusplus
 }

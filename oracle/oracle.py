@@ -99,6 +99,9 @@ def lib():
         L.ora_c_l2norm.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.ora_c_maxabs.restype = ctypes.c_double
         L.ora_c_maxabs.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.ora_c_quad_mass.restype = ctypes.c_double
+        L.ora_c_quad_mass.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.ora_c_diagnostics.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -458,3 +461,11 @@ class CGrid:
 
     def maxabs(self, t, f):
         return lib().ora_c_maxabs(self.h, C_TYPES[t], _p(f))
+
+    def quad_mass(self, t, f):
+        return lib().ora_c_quad_mass(self.h, C_TYPES[t], _p(f))
+
+    def diagnostics(self, h, u, v, hs=None):
+        out = np.zeros(7)
+        lib().ora_c_diagnostics(self.h, _p(h), _p(u), _p(v), _pn(hs), _p(out))
+        return dict(zip(("hmin", "hmax", "mass", "TE", "KE", "PE", "Enstrophy"), out.tolist()))

@@ -2444,3 +2444,75 @@ double ora_c_maxabs(const ora_cgrid *c, int t, const double *f) {
             }
     return m;
 }
+
+/* quadrature_t%mass with SBP_C42_quadrature on the mesh of point type t (quadrature_factory_mod.f90:
+ * 88-153 create_C_sbp_quadrature: D42_A_interfaces along face directions, D42_A_centers along
+ * centre directions; sbp_quadrature_mod.f90:21-40) */
+double ora_c_quad_mass(const ora_cgrid *c, int t, const double *f) {
+    const ora_grid *g = c->g;
+    const int N = c->N, nx = c_nx(N, t), ny = c_ny(N, t);
+    static const double Ai[5] = {7.0 / 18.0, 9.0 / 8.0, 1.0, 71.0 / 72.0, 1.0};
+    static const double Ac[4] = {13.0 / 12.0, 7.0 / 8.0, 25.0 / 24.0, 1.0};
+    const int facex = (t == 1 || t == 3), facey = (t == 2 || t == 3);
+    double *Ax = malloc(sizeof(double) * (nx + 1)), *Ay = malloc(sizeof(double) * (ny + 1));
+    /* create_mass_coefficients (:156-175): 1 inside, the edge weights from both ends */
+    for (int pass = 0; pass < 2; pass++) {
+        double *A = pass == 0 ? Ax : Ay;
+        const int n = pass == 0 ? nx : ny, face = pass == 0 ? facex : facey;
+        const double *E = face ? Ai : Ac;
+        const int ne = face ? 5 : 4;
+        for (int i = 1; i <= n; i++) A[i] = 1.0;
+        for (int i = 1; i <= (ne < n ? ne : n); i++) A[i] = E[i - 1];
+        for (int i = (n - ne + 1 > 1 ? n - ne + 1 : 1); i <= n; i++) A[i] = E[n - i];
+    }
+    double mass = 0.0;
+    for (int p = 0; p < 6; p++) {
+        double out = 0.0;
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const long id = IDX(g, p, i, j);
+                out = out + f[id] * c->J[t][id] * g->hx * g->hx * Ax[i] * Ay[j];
+            }
+        mass = mass + out;
+    }
+    free(Ax); free(Ay);
+    return mass;
+}
+
+/* operator_swm_t%get_diagnostics (operator_swm_mod.f90:115-155) with calc_energy (:211-236) on the
+ * C grid: hmin, hmax, mass, TE, KE, PE; the enstrophy (:238-260) divides the curl at q points by h
+ * at the same indices, which the reference itself marks as not working for the C grid: NaN */
+void ora_c_diagnostics(const ora_cgrid *c, double *h, double *u, double *v, const double *hs, double out[7]) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    double hmin = 1e300, hmax = -1e300;
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const double x = h[IDX(g, p, i, j)];
+                if (x < hmin) hmin = x;
+                if (x > hmax) hmax = x;
+            }
+    double *ut = c_field(c), *vt = c_field(c), *hu = c_field(c), *hv = c_field(c), *PE = c_field(c);
+    c_co2contra(c, u, v, ut, vt);
+    c_massflux(c, h, ut, vt, hu, hv);
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) { const long id = IDX(g, p, i, j); hu[id] = 0.5 * hu[id] * u[id]; }
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) { const long id = IDX(g, p, i, j); hv[id] = 0.5 * hv[id] * v[id]; }
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) {
+                const long id = IDX(g, p, i, j);
+                const double pe = 1.0 * h[id] + 1.0 * (hs ? hs[id] : 0.0);
+                PE[id] = 0.5 * g->grav * pe * pe;
+            }
+    }
+    const double ke = ora_c_quad_mass(c, 1, hu) + ora_c_quad_mass(c, 2, hv);
+    const double pe = ora_c_quad_mass(c, 0, PE);
+    out[0] = hmin; out[1] = hmax;
+    out[2] = ora_c_quad_mass(c, 0, h);
+    out[3] = ke + pe; out[4] = ke; out[5] = pe;
+    out[6] = NAN;
+    free(ut); free(vt); free(hu); free(hv); free(PE);
+}

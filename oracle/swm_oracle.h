@@ -141,6 +141,10 @@ void ora_c_rhs(const ora_cgrid *c, double *h, double *u, double *v, const double
                double *dh, double *du, double *dv);
 void ora_c_rk4(const ora_cgrid *c, double *h, double *u, double *v, const double *hs, double dt, int nsteps);
 double ora_c_l2norm(const ora_cgrid *c, int t, const double *f);
+/* SBP_C42_quadrature mass on the mesh of point type t, and the diagnostics hmin, hmax, mass, TE,
+ * KE, PE, enstrophy (NaN: the reference's enstrophy does not apply to the C grid) */
+double ora_c_quad_mass(const ora_cgrid *c, int t, const double *f);
+void ora_c_diagnostics(const ora_cgrid *c, double *h, double *u, double *v, const double *hs, double out[7]);
 double ora_c_maxabs(const ora_cgrid *c, int t, const double *f);
 
 #ifdef __cplusplus

@@ -119,3 +119,43 @@ def test_c_rk4_multi_step_graph_bitwise(oracle_lib):
     ref = [a.copy() for a in (h, u, v)]
     C.rk4(*ref, dt, 55)
     _compare(C, x.download(C.hw), ref, "RK4 55 steps")
+
+
+@pytest.mark.parametrize("N,with_hs", [(24, False), (64, True)])
+def test_c_diagnostics_match_oracle(oracle_lib, N, with_hs):
+    """hmin, hmax exactly; mass, TE, KE, PE within 1e-12 relative (compensated device sums against
+    the oracle's serial sums of the same per-point terms); enstrophy NaN on both sides."""
+    print(f"[c diagnostics N={N} hs={with_hs}]", flush=True)
+    swm, op = _op(N)
+    C = oracle_lib.CGrid(N, A, OM, G0)
+    h, u, v = _state(oracle_lib, C, seed=7)
+    hs = None
+    if with_hs:
+        xo, yo, zo = C.geometry("o", "r")
+        hs = C.embed("o", C.interior("o", 300.0 * np.sin(2 * np.arctan2(yo, xo)) * (1 - zo ** 2)))
+        op.set_orography(hs, C.hw)
+    x = swm.create_stvec_swm(op)
+    x.upload(h, u, v, C.hw)
+    got = op.get_diagnostics(x)
+    ref = C.diagnostics(h, u, v, hs)
+    assert got["hmin"] == ref["hmin"] and got["hmax"] == ref["hmax"]
+    for k in ("mass", "TE", "KE", "PE"):
+        assert abs(got[k] - ref[k]) <= 1e-12 * abs(ref[k]), (k, got[k], ref[k])
+    assert np.isnan(got["Enstrophy"]) and np.isnan(ref["Enstrophy"])
+    print(f"  diagnostics agree: {got}")
+
+
+def test_c384_cgrid_two_steps_bitwise(oracle_lib):
+    """The C-grid measurement's size: C384, 2 RK4 steps at dt = 37.5 s (gravity-wave Courant number
+    0.45; the C-grid scheme's RK4 limit lies near 0.5), every point against the oracle."""
+    N = 384
+    print(f"[c384 c-grid]", flush=True)
+    swm, op = _op(N)
+    C = oracle_lib.CGrid(N, A, OM, G0)
+    h, u, v = _state(oracle_lib, C, seed=11, noise=1.0)
+    x = swm.create_stvec_swm(op)
+    x.upload(h, u, v, C.hw)
+    swm.create_timescheme(op, "rk4").steps(x, 37.5, 2)
+    ref = [a.copy() for a in (h, u, v)]
+    C.rk4(*ref, 37.5, 2)
+    _compare(C, x.download(C.hw), ref, "C384 RK4 x2")
