@@ -15,8 +15,8 @@
 //     sbp_operator_mod.f90:344-413),
 //   - the O-point halo of the kinetic-energy potential (gradient, grad_c_sbp42_mod.f90:53-139),
 //   - the corner sync of the curl (halo_Ah_scalar_sync_mod.f90:52-93),
-// each read on the fly through precomputed neighbour lists by the threads on the panel edges.  One
-// RHS is seven launches of memory-bound stencil sweeps with one thread per output point (64 x 4
+// the first three read on the fly through precomputed neighbour lists by the threads on the panel
+// edges, the sync one small launch over the shared points.  One RHS is eight launches of memory-bound stencil sweeps with one thread per output point (64 x 4
 // thread blocks, coalesced along i); the RK4 stage updates are applied by the kernels that produce
 // the tendencies.  Intermediate values at another point type (the J Qi12-weighted components at o
 // points, the curl at x and y points) are stored once and read by the next kernel: recomputing
@@ -162,23 +162,6 @@ __device__ __forceinline__ double fhalo(const CGeo &g, const double *pair, int p
 }
 __device__ __forceinline__ double ohalo(const CGeo &g, const double *f, int p, int e, int d, int k) {
     return f[g.osrc[(((long long)p * 4 + e) * 3 + (d - 1)) * g.N + (k - 1)]];
-}
-
-// the curl at q point (i, j) after halo_Ah_scalar_sync (halo_Ah_scalar_sync_mod.f90:52-93): a
-// point on a panel edge takes the mean of its copies, own and neighbours', in the reference's order
-__device__ __forceinline__ double wsync(const CGeo &g, const double *__restrict__ W, int p, int i, int j) {
-    const int n = g.N + 1;
-    const double own = W[cid(g, p, i, j)];
-    if (i > 1 && i < n && j > 1 && j < n) return own;
-    auto nb = [&](int e, int k) { return W[g.qnb[((long long)p * 4 + e) * n + (k - 1)]]; };
-    if (i == 1 && j == 1) return (own + nb(0, 1) + nb(2, 1)) / 3.0;
-    if (i == 1 && j == n) return (own + nb(0, n) + nb(3, 1)) / 3.0;
-    if (i == n && j == 1) return (own + nb(1, 1) + nb(2, n)) / 3.0;
-    if (i == n && j == n) return (own + nb(1, n) + nb(3, n)) / 3.0;
-    if (i == 1) return 0.5 * (nb(0, j) + own);
-    if (i == n) return 0.5 * (own + nb(1, j));
-    if (j == 1) return 0.5 * (own + nb(2, i));
-    return 0.5 * (own + nb(3, i));
 }
 
 // RK4 stage handling of the kernels that produce the tendencies (rk4_mod.f90:20-42):
@@ -338,16 +321,41 @@ __global__ void __launch_bounds__(256) k_curl(CGeo g, const double *__restrict__
     W[id] = (dvx - duy) / (g.Jq[mid] * g.hx * g.scale);
 }
 
+// halo_Ah_scalar_sync of the curl at q points (halo_Ah_scalar_sync_mod.f90:52-93), in place on
+// the panel-edge points only.  Every shared point is one group handled by one thread: a point on
+// a cube edge has two copies whose mean 0.5 (a + b) is the same value on both panels; a cube
+// corner has three copies, copy c = (own + first neighbour + second neighbour) / 3 in the
+// order of its own panel's formula.  pairs[2k], pairs[2k+1]: offsets of the two copies;
+// corners[6g..6g+2]: offsets of the three copies, corners[6g+3+c]: the neighbour order of copy
+// c packed as 3 * first + second.
+__global__ void __launch_bounds__(256) k_qsync_edges(int npairs, int ncorners, const long long *__restrict__ pairs,
+                                                     const long long *__restrict__ corners, double *W) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < npairs) {
+        const long long a = pairs[2 * k], b = pairs[2 * k + 1];
+        const double v = 0.5 * (W[a] + W[b]);
+        W[a] = v;
+        W[b] = v;
+    } else if (k < npairs + ncorners) {
+        const long long *cg = corners + 6 * (k - npairs);
+        const double r[3] = {W[cg[0]], W[cg[1]], W[cg[2]]};
+        double v[3];
+        for (int c = 0; c < 3; ++c) {
+            const int ord = (int)cg[3 + c];
+            v[c] = (r[c] + r[ord / 3] + r[ord % 3]) / 3.0;
+        }
+        for (int c = 0; c < 3; ++c) W[cg[c]] = v[c];
+    }
+}
+
 // interpolator_w2h first half (interpolator_w2h_mod.f90:50-68): the synced curl at x points
-// (interpolated along y) and at y points (along x); the corner sync applied as the values are read
+// (interpolated along y) and at y points (along x)
 __global__ void __launch_bounds__(256) k_w2uv(CGeo g, const double *__restrict__ W, double *__restrict__ fu,
                                               double *__restrict__ fv) {
     C_POINT
     const int N = g.N;
-    if (i <= N + 1 && j <= N)
-        fu[cid(g, p, i, j)] = sopf<W42i2c>(j, N + 1, [&](int jq) { return wsync(g, W, p, i, jq); });
-    if (i <= N && j <= N + 1)
-        fv[cid(g, p, i, j)] = sopf<W42i2c>(i, N + 1, [&](int iq) { return wsync(g, W, p, iq, j); });
+    if (i <= N + 1 && j <= N) fu[cid(g, p, i, j)] = sop<W42i2c>(j, N + 1, W + cid(g, p, i, 0), g.pitch);
+    if (i <= N && j <= N + 1) fv[cid(g, p, i, j)] = sop<W42i2c>(i, N + 1, W + cid(g, p, 0, j), 1);
 }
 
 // at o points: curl_p = (I_x fu + I_y fv)/2 (interpolator_w2h_mod.f90:40-46), the fluxes J hu, J hv
@@ -561,6 +569,9 @@ struct swm_cctx {
     signed char *signs = nullptr;
     long long *vsrc = nullptr, *fsrc = nullptr, *osrc = nullptr, *qnb = nullptr;
     signed char *vsg = nullptr, *fsg = nullptr;
+    // groups of the q-point edge sync (k_qsync_edges)
+    long long *qpairs = nullptr, *qcorners = nullptr;
+    int nqpairs = 0, nqcorners = 0;
     // work fields: G (pair), KEu, KEv, KE, W, cup, cvp, RK tendencies k0..k2, stage inputs yA, yB,
     // A, B (J Qi12-weighted components at o points), fu, fv (the synced curl at x and y points)
     double *work = nullptr;
@@ -616,7 +627,7 @@ swmc::CGeo make_cgeo(const swm_cctx *c) {
     return g;
 }
 
-// operator_swm_t%apply on the C grid for the input y (h, u, v), seven launches; the tendencies go
+// operator_swm_t%apply on the C grid for the input y (h, u, v), eight launches; the tendencies go
 // where RKM says (RK_RHS: r.k; RK_STAGE: r.k and the next stage input; RK_FINAL: the state)
 template <int RKM>
 int c_enqueue_rhs(swm_cctx *c, const double *y, const swmc::RkArgs &r) {
@@ -630,6 +641,9 @@ int c_enqueue_rhs(swm_cctx *c, const double *y, const swmc::RkArgs &r) {
     hipLaunchKernelGGL(swmc::k_ke_div<RKM>, grd, blk, 0, c->stream, g, h, c->wf(CW_KEU), c->wf(CW_KEV), c->wf(CW_G),
                        c->wf(CW_KE), r);
     hipLaunchKernelGGL(swmc::k_curl, grd, blk, 0, c->stream, g, u, c->wf(CW_W));
+    const int nsync = c->nqpairs + c->nqcorners;
+    hipLaunchKernelGGL(swmc::k_qsync_edges, dim3((nsync + 255) / 256), dim3(256), 0, c->stream, c->nqpairs,
+                       c->nqcorners, c->qpairs, c->qcorners, c->wf(CW_W));
     hipLaunchKernelGGL(swmc::k_w2uv, grd, blk, 0, c->stream, g, c->wf(CW_W), c->wf(CW_FU), c->wf(CW_FV));
     hipLaunchKernelGGL(swmc::k_cor_o, grd, blk, 0, c->stream, g, h, c->wf(CW_FU), c->wf(CW_FV), c->wf(CW_G),
                        c->wf(CW_CUP), c->wf(CW_CVP));
@@ -675,7 +689,7 @@ void c_free(swm_cctx *c) {
     if (!c) return;
     c_drop_graphs(c);
     for (void *p : {(void *)c->geo, (void *)c->hs, (void *)c->lists, (void *)c->signs, (void *)c->work, (void *)c->qw,
-                    (void *)c->dpart})
+                    (void *)c->dpart, (void *)c->qpairs})
         if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -800,6 +814,67 @@ extern "C" int swm_c_create(swm_cctx **out, int N, const char *div, const char *
     c->qnb = c->osrc + mo;
     c->vsg = c->signs;
     c->fsg = c->signs + mv;
+    {
+        // q-point sync groups from the neighbour copies qn[(p*4+e)*(N+1) + k-1] of edge point k of
+        // edge e (0: i = 1, 1: i = N+1, 2: j = 1, 3: j = N+1) of panel p
+        const int n = N + 1;
+        auto own = [&](int p, int i, int j) { return ((long long)p * c->rows + (j - 1 + swmc::H)) * c->pitch + (i - 1 + swmc::H); };
+        auto nb = [&](int p, int e, int k) { return qn[((size_t)p * 4 + e) * n + (k - 1)]; };
+        std::vector<long long> pairs, corners;
+        for (int p = 0; p < 6; ++p)
+            for (int e = 0; e < 4; ++e)
+                for (int k = 2; k <= n - 1; ++k) {
+                    const int i = e == 0 ? 1 : e == 1 ? n : k, j = e == 2 ? 1 : e == 3 ? n : k;
+                    const long long a = own(p, i, j), b = nb(p, e, k);
+                    if (a < b) { pairs.push_back(a); pairs.push_back(b); }
+                }
+        // corner copies: (own, first, second) in the order of halo_Ah_scalar_sync_mod.f90:56-72
+        struct Copy { long long o, a, b; };
+        std::vector<Copy> cps;
+        for (int p = 0; p < 6; ++p) {
+            cps.push_back({own(p, 1, 1), nb(p, 0, 1), nb(p, 2, 1)});
+            cps.push_back({own(p, 1, n), nb(p, 0, n), nb(p, 3, 1)});
+            cps.push_back({own(p, n, 1), nb(p, 1, 1), nb(p, 2, n)});
+            cps.push_back({own(p, n, n), nb(p, 1, n), nb(p, 3, n)});
+        }
+        std::vector<bool> used(cps.size(), false);
+        for (size_t x = 0; x < cps.size(); ++x) {
+            if (used[x]) continue;
+            long long g3[3] = {cps[x].o, cps[x].a, cps[x].b};
+            size_t idx[3] = {x, 0, 0};
+            for (int m = 1; m < 3; ++m) {
+                idx[m] = cps.size();
+                for (size_t y = 0; y < cps.size(); ++y)
+                    if (cps[y].o == g3[m]) idx[m] = y;
+                if (idx[m] == cps.size()) return bail(fail(SWM_EARG, "cube corner copies do not form a group"));
+            }
+            long long ords[3];
+            for (int m = 0; m < 3; ++m) {
+                const Copy &cp = cps[idx[m]];
+                int fa = -1, fb = -1;
+                for (int t = 0; t < 3; ++t) {
+                    if (g3[t] == cp.a) fa = t;
+                    if (g3[t] == cp.b) fb = t;
+                }
+                if (fa < 0 || fb < 0 || fa == m || fb == m || fa == fb)
+                    return bail(fail(SWM_EARG, "cube corner copies do not form a group"));
+                used[idx[m]] = true;
+                ords[m] = 3 * fa + fb;
+            }
+            for (int m = 0; m < 3; ++m) corners.push_back(g3[m]);
+            for (int m = 0; m < 3; ++m) corners.push_back(ords[m]);
+        }
+        if (pairs.size() != (size_t)2 * 12 * (n - 2) || corners.size() != (size_t)6 * 8)
+            return bail(fail(SWM_EARG, "q-point sync groups do not match the cube topology"));
+        c->nqpairs = (int)(pairs.size() / 2);
+        c->nqcorners = (int)(corners.size() / 6);
+        std::vector<long long> Q(pairs);
+        Q.insert(Q.end(), corners.begin(), corners.end());
+        if (hipMalloc((void **)&c->qpairs, Q.size() * sizeof(long long)) != hipSuccess ||
+            hipMemcpy(c->qpairs, Q.data(), Q.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(SWM_EHIP, "q-point sync group upload failed"));
+        c->qcorners = c->qpairs + pairs.size();
+    }
 
     {
         // create_mass_coefficients (quadrature_factory_mod.f90:156-175): A_centers (13/12, 7/8,
