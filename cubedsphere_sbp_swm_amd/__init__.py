@@ -204,8 +204,8 @@ class OperatorSWM:
 
     def laplace(self, out, x, laplace_name="laplace_ah_sbp42_narrow"):
         """h plane of `out` = narrow-stencil SBP Laplacian of the h plane of `x`
-        (laplace_Ah_sbp42_narrow_t / laplace_Ah_sbp63_narrow_t %calc_laplace, picked by the
-        reference's names laplace_ah_sbp42_narrow / laplace_ah_sbp63_narrow); FP64,
+        (laplace_Ah_sbp{21,42,63}_narrow_t %calc_laplace, picked by the reference's names
+        laplace_ah_sbp21_narrow / laplace_ah_sbp42_narrow / laplace_ah_sbp63_narrow); FP64,
         N+1 >= 12 (sbp42) or 18 (sbp63)."""
         check(_capi.lib().swm_laplace(self.ctx, laplace_name.encode(), ctypes.c_void_p(x.ptr),
                                       ctypes.c_void_p(out.ptr)))

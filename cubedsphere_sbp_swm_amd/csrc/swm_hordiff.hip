@@ -1,5 +1,6 @@
-// Hyperdiffusion of the Ah state with the narrow-stencil SBP Laplacians sbp42 and sbp63, FP64
-// (included into swm_capi.hip).
+// Hyperdiffusion of the Ah state with the narrow-stencil SBP Laplacians sbp21, sbp42 and sbp63,
+// FP64 (included into swm_capi.hip).  sbp21 (three-point rows, laplace_Ah_sbp21_narrow_mod.f90)
+// runs as one point per thread (hd_lap21); the table-driven sbp42 / sbp63 on LDS tiles.
 //
 // Laplacian, laplace_Ah_sbp42_narrow_t%calc_laplace (differential_operators/horizontal/
 // laplace/laplace_Ah_sbp42_narrow_mod.f90:23-763) and laplace_Ah_sbp63_narrow_t%calc_laplace
@@ -356,6 +357,69 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
         Wout[k * g.fs + o] = v;
 }
 
+// The sbp21 narrow Laplacian (laplace_Ah_sbp21_narrow_mod.f90:22-245) of field k = blockIdx.y,
+// one thread per point, before the edge sync (the output convention of hd_lap_tile).  Three-point
+// rows: D2(b) with b = Qi11 J along x, Qi22 J along y, edge rows -(b0+b1) f0 + (b0+b1) f1,
+// interior (bp+bc)/2 f+ - (bm+2bc+bp)/2 f + (bm+bc)/2 f-; the cross-term penalties with
+// coefficient 2, all into d2x; first derivatives d21 times J Qi12, formed where they are read.
+__global__ void __launch_bounds__(256) hd_lap21(HdGeo g, const double *__restrict__ W, double *__restrict__ Wout,
+                                                double *__restrict__ E) {
+    const int n = g.n;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= 6LL * n * n) return;
+    const int k = blockIdx.y;
+    const int i = (int)(t % n);
+    const long long pj = t / n;
+    const int j = (int)(pj % n), p = (int)(pj / n);
+    const long long pb = (long long)p * n * g.pitch;
+    const double *f = W + k * g.fs + pb;
+    auto at = [&](int a, int b) { return (long long)b * g.pitch + a; };
+    auto d1x = [&](int a, int b) {
+        const double d = sbp_row<21, double>(a, n, [&](int m) { return f[at(m, b)]; });
+        return d * g.J[pb + at(a, b)] * g.Qi12[pb + at(a, b)];
+    };
+    auto d1y = [&](int a, int b) {
+        const double d = sbp_row<21, double>(b, n, [&](int m) { return f[at(a, m)]; });
+        return d * g.J[pb + at(a, b)] * g.Qi12[pb + at(a, b)];
+    };
+    auto d2 = [&](int t0, auto bget, auto fget) {
+        if (t0 == 0) {
+            const double bc = bget(0), bp = bget(1);
+            const double cc = -(bc + bp), cp = +(bc + bp);
+            return cc * fget(0) + cp * fget(1);
+        }
+        if (t0 == n - 1) {
+            const double bc = bget(n - 1), bm = bget(n - 2);
+            const double cc = -(bc + bm), cm = +(bc + bm);
+            return cm * fget(n - 2) + cc * fget(n - 1);
+        }
+        const double bm = bget(t0 - 1), bc = bget(t0), bp = bget(t0 + 1);
+        const double cm = (bm + bc) / 2, cc = -(bm + 2 * bc + bp) / 2, cp = (bp + bc) / 2;
+        return cp * fget(t0 + 1) + cc * fget(t0) + cm * fget(t0 - 1);
+    };
+    double d2x = d2(i, [&](int a) { return g.Qi11[pb + at(a, j)] * g.J[pb + at(a, j)]; },
+                    [&](int a) { return f[at(a, j)]; });
+    const double d2y = d2(j, [&](int b) { return g.Qi22[pb + at(i, b)] * g.J[pb + at(i, b)]; },
+                          [&](int b) { return f[at(i, b)]; });
+    // laplace_Ah_sbp21_narrow_mod.f90:189-215, x edges first
+    constexpr double pc = 2.0;
+    if (i == 0) d2x = d2x - 0.5 * (pc * (-2.0 * d1y(i, j)));
+    if (i == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * d1y(i, j)));
+    if (j == 0) d2x = d2x - 0.5 * (pc * (-2.0 * d1x(i, j)));
+    if (j == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * d1x(i, j)));
+    const double d2yx = sbp_row<21, double>(j, n, [&](int b) { return d1x(i, b); });
+    const double d2xy = sbp_row<21, double>(i, n, [&](int a) { return d1y(a, j); });
+    double v = 1.0 * d2x + 1.0 * d2y;
+    v = v + 1.0 * d2xy + 1.0 * d2yx;
+    v = v / (g.hs * g.hs);
+    const long long o = pb + at(i, j);
+    v = v / g.J[o];
+    if (i == 0 || j == 0 || i == n - 1 || j == n - 1)
+        E[(long long)k * 6 * g.es + p * g.es + edge_slot(n, i, j)] = v;
+    else
+        Wout[k * g.fs + o] = v;
+}
+
 // scalar edge sync (halo_Ah_scalar_sync_mod.f90:52-93) of field blockIdx.y from E into Wout
 __global__ void hd_sync(HdGeo g, const double *__restrict__ E, double *__restrict__ Wout) {
     const int n = g.n, ns = 6 * g.es;
@@ -469,8 +533,9 @@ struct HdWork {
     double *E = nullptr, *E2 = nullptr;           // 4 x ns, 2 x ns edge values
 };
 
-// closure rows of both panel edges disjoint: N+1 >= 12 (sbp42), 18 (sbp63)
-int hd_min_n(int order) { return 2 * (order == 63 ? swm::Ln<63>::NB : swm::Ln<42>::NB); }
+// closure rows of both panel edges disjoint: N+1 >= 12 (sbp42), 18 (sbp63); sbp21 has one-point
+// closures (any context size)
+int hd_min_n(int order) { return order == 21 ? 3 : 2 * (order == 63 ? swm::Ln<63>::NB : swm::Ln<42>::NB); }
 
 int ensure_hd(swm_ctx *c, int order) {
     if (c->prec != 64) return fail(SWM_EARG, "the hyperdiffusion runs in FP64 contexts only");
@@ -510,7 +575,7 @@ swm::HdGeo hd_geo(const swm_ctx *c) {
 }
 
 // fields k0 .. k0+nf-1 of `in` (4-field work layout) -> L of each in the same fields of `out`,
-// synced; order 42 or 63 picks the narrow Laplacian
+// synced; order 21, 42 or 63 picks the narrow Laplacian
 int enqueue_laplace(swm_ctx *c, int order, int k0, int nf, const double *in, double *out) {
     HdWork *w = c->hd;
     const swm::HdGeo g = hd_geo(c);
@@ -520,7 +585,10 @@ int enqueue_laplace(swm_ctx *c, int order, int k0, int nf, const double *in, dou
     const int ltiles = 6 * ((c->n + LTX - 1) / LTX) * ((c->n + LTY - 1) / LTY);
     const double *ik = in + (long long)k0 * c->fs;
     double *ok = out + (long long)k0 * c->fs, *ek = w->E + (long long)k0 * ns;
-    if (order == 63)
+    if (order == 21)
+        hipLaunchKernelGGL(swm::hd_lap21, dim3((unsigned)((6LL * c->n * c->n + 255) / 256), nf), blk, 0, c->stream, g,
+                           ik, ok, ek);
+    else if (order == 63)
         hipLaunchKernelGGL((swm::hd_lap_tile<63, LTX, LTY>), dim3(ltiles * nf), dim3(LTX * LTY), 0, c->stream, g, ik,
                            ok, ek, nf);
     else
@@ -534,13 +602,13 @@ int enqueue_laplace(swm_ctx *c, int order, int k0, int nf, const double *in, dou
 // hordiff_factory_mod.f90:25-50: the narrow-Laplacian hyperdiffusion names and their families
 int hordiff_order(const char *name, const char *prefix, int &order) {
     const std::string s(name);
-    for (int o : {42, 63})
+    for (int o : {21, 42, 63})
         if (s == prefix + std::to_string(o) + "_narrow") {
             order = o;
             return SWM_OK;
         }
-    return fail(SWM_ENAME, std::string("unsupported hyperdiffusion: ") + name +
-                               " (supported: " + prefix + "42_narrow, " + prefix + "63_narrow)");
+    return fail(SWM_ENAME, std::string("unsupported hyperdiffusion: ") + name + " (supported: " + prefix +
+                               "21_narrow, " + prefix + "42_narrow, " + prefix + "63_narrow)");
 }
 
 int parse_hordiff(const char *uv_name, const char *h_name, int &uv_order, int &h_order) {
@@ -554,10 +622,11 @@ int parse_hordiff(const char *uv_name, const char *h_name, int &uv_order, int &h
 int parse_laplace(const char *name, int &order) {
     if (!name) return fail(SWM_EARG, "NULL Laplacian name");
     const std::string s(name);
-    if (s == "laplace_ah_sbp42_narrow") order = 42;
+    if (s == "laplace_ah_sbp21_narrow") order = 21;
+    else if (s == "laplace_ah_sbp42_narrow") order = 42;
     else if (s == "laplace_ah_sbp63_narrow") order = 63;
-    else return fail(SWM_ENAME, "unsupported Laplacian: " + s +
-                                    " (supported: laplace_ah_sbp42_narrow, laplace_ah_sbp63_narrow)");
+    else return fail(SWM_ENAME, "unsupported Laplacian: " + s + " (supported: laplace_ah_sbp21_narrow, "
+                                    "laplace_ah_sbp42_narrow, laplace_ah_sbp63_narrow)");
     return SWM_OK;
 }
 

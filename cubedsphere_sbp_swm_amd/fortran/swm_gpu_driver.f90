@@ -8,7 +8,7 @@
 ! variable names, plus &swm_gpu_run for what the reference takes from its test-case modules: the
 ! seeded spherical-harmonic initial state, gravity, arithmetic precision and the output file.
 ! Settings the device path does not implement (another staggering, topology, metric, rotated
-! frame, operator family, or diffusion other than the sbp42 / sbp63 narrow-stencil hyperdiffusion
+! frame, operator family, or diffusion other than the sbp21 / sbp42 / sbp63 narrow-stencil hyperdiffusion
 ! with explicit_Eul1) stop the run with a message.
 !
 ! Loop as in run_ts2 (ts2_mod.f90:141-156): an rk4 step, then, when a diffusion coefficient is
@@ -99,12 +99,14 @@ if (trim(swm_op_type) /= "vector_invariant" .or. trim(v_components_type) /= "cov
 use_diff = uv_diff_coeff /= 0.0_8 .or. h_diff_coeff /= 0.0_8
 if (use_diff .and. trim(diff_time_scheme) /= "explicit_Eul1") &
     call fail("device path implements diff_time_scheme = explicit_Eul1 only")
-if (use_diff .and. ((trim(hordiff_uv_name) /= "hordiff_vec_xyz_Ah_sbp_42_narrow" .and. &
+if (use_diff .and. ((trim(hordiff_uv_name) /= "hordiff_vec_xyz_Ah_sbp_21_narrow" .and. &
+                     trim(hordiff_uv_name) /= "hordiff_vec_xyz_Ah_sbp_42_narrow" .and. &
                      trim(hordiff_uv_name) /= "hordiff_vec_xyz_Ah_sbp_63_narrow") .or. &
-                    (trim(hordiff_h_name) /= "hordiff_scalar_Ah_sbp_42_narrow" .and. &
+                    (trim(hordiff_h_name) /= "hordiff_scalar_Ah_sbp_21_narrow" .and. &
+                     trim(hordiff_h_name) /= "hordiff_scalar_Ah_sbp_42_narrow" .and. &
                      trim(hordiff_h_name) /= "hordiff_scalar_Ah_sbp_63_narrow"))) &
-    call fail("device path implements the hyperdiffusion names hordiff_vec_xyz_Ah_sbp_{42,63}_narrow / " // &
-              "hordiff_scalar_Ah_sbp_{42,63}_narrow only")
+    call fail("device path implements the hyperdiffusion names hordiff_vec_xyz_Ah_sbp_{21,42,63}_narrow / " // &
+              "hordiff_scalar_Ah_sbp_{21,42,63}_narrow only")
 
 simulation_time = simulation_time_days*86400.0_8 + simulation_time_hours*3600.0_8 + &
                   simulation_time_min*60.0_8 + simulation_time_sec

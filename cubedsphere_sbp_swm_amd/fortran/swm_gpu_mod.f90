@@ -398,7 +398,7 @@ end subroutine steps_rk4_gpu
 
 ! create_swm_diff_operator (operator_swm_diff_factory_mod.f90:11-34) for the names of the
 ! &shallow_water_model namelist; the device implements the narrow-stencil names
-! "hordiff_vec_xyz_Ah_sbp_{42,63}_narrow" and "hordiff_scalar_Ah_sbp_{42,63}_narrow"
+! "hordiff_vec_xyz_Ah_sbp_{21,42,63}_narrow" and "hordiff_scalar_Ah_sbp_{21,42,63}_narrow"
 subroutine create_swm_diff_gpu_operator(op_diff, op, hordiff_uv_name, uv_diff_coeff, hordiff_h_name, h_diff_coeff)
     type(operator_swm_diff_gpu_t), intent(out) :: op_diff
     type(operator_swm_gpu_t),      intent(in)  :: op

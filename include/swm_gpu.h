@@ -105,18 +105,19 @@ int swm_ic_spherical_harmonics(swm_ctx *ctx, unsigned long long seed, int lmax, 
                                int halo_width);
 
 /* ---- Hyperdiffusion (FP64; N+1 >= 12 for the sbp42 family, >= 18 for sbp63) ----
- * swm_laplace: laplace_Ah_sbp42_narrow_t%calc_laplace (differential_operators/horizontal/
- * laplace/laplace_Ah_sbp42_narrow_mod.f90:23-58) or laplace_Ah_sbp63_narrow_t%calc_laplace
+ * swm_laplace: laplace_Ah_sbp21_narrow_t%calc_laplace (differential_operators/horizontal/
+ * laplace/laplace_Ah_sbp21_narrow_mod.f90:22-56), laplace_Ah_sbp42_narrow_t%calc_laplace
+ * (laplace_Ah_sbp42_narrow_mod.f90:23-58) or laplace_Ah_sbp63_narrow_t%calc_laplace
  * (laplace_Ah_sbp63_narrow_mod.f90:494-529), picked by the name create_laplace_operator takes
- * (laplace_factory_mod.f90:11-33): "laplace_ah_sbp42_narrow" or "laplace_ah_sbp63_narrow"
- * (other names return SWM_ENAME).  out = narrow-stencil SBP Laplacian of the field f (one field
+ * (laplace_factory_mod.f90:11-33): "laplace_ah_sbp21_narrow", "laplace_ah_sbp42_narrow" or
+ * "laplace_ah_sbp63_narrow" (other names return SWM_ENAME).  out = narrow-stencil SBP Laplacian of the field f (one field
  * of the context layout: field_elems elements; out distinct from f), edge-synced.
  * swm_diff_apply: operator_swm_diff_t%apply (models/shallow_water/operator/
  * operator_swm_diff_mod.f90:25-46) as built by create_swm_diff_operator
  * (operator_swm_diff_factory_mod.f90:11-34): vout = hyperdiffusion tendency of vin (distinct
  * state vectors).  Names as in the &shallow_water_model namelist (hordiff_factory_mod.f90:
- * 11-51): hordiff_uv_name "hordiff_vec_xyz_Ah_sbp_42_narrow" or "..._sbp_63_narrow",
- * hordiff_h_name "hordiff_scalar_Ah_sbp_42_narrow" or "..._sbp_63_narrow", each independently;
+ * 11-51): hordiff_uv_name "hordiff_vec_xyz_Ah_sbp_{21,42,63}_narrow", hordiff_h_name
+ * "hordiff_scalar_Ah_sbp_{21,42,63}_narrow", each independently;
  * other names return SWM_ENAME.
  * swm_diff_step: timescheme_diff%step with diff_time_scheme "explicit_Eul1"
  * (time_schemes/explicit_Eul1_mod.f90:20-30) on that operator: state += dt * diff(state),

@@ -233,10 +233,11 @@ class Grid:
     def rk4(self, h, u, v, dt, nsteps, hs=None):
         lib().ora_rk4(self.h, _p(h), _p(u), _p(v), _pn(hs), dt, nsteps)
 
-    # narrow-stencil SBP Laplacians (order 42 or 63) and hyperdiffusion; N+1 >= 12 (42), 18 (63)
+    # narrow-stencil SBP Laplacians (order 21, 42 or 63) and hyperdiffusion; N+1 >= 2 (21), 12 (42),
+    # 18 (63)
     def _check_narrow(self, order):
-        if order not in (42, 63) or self.n < lib().ora_narrow_min_n(order):
-            raise ValueError(f"narrow Laplacian of order {order} needs order 42/63 and N+1 large enough")
+        if order not in (21, 42, 63) or self.n < lib().ora_narrow_min_n(order):
+            raise ValueError(f"narrow Laplacian of order {order} needs order 21/42/63 and N+1 large enough")
 
     def laplace(self, f, order=42):
         self._check_narrow(order)

@@ -900,7 +900,9 @@ void ora_rk4(const ora_grid *g, double *h, double *u, double *v, const double *h
     for (int c = 0; c < 3; c++) free(y[c]);
 }
 
-/* ---- narrow-stencil SBP Laplacians, laplace_Ah_sbp42_narrow_t and laplace_Ah_sbp63_narrow_t
+/* ---- narrow-stencil SBP Laplacians, laplace_Ah_sbp21_narrow_t (laplace_Ah_sbp21_narrow_mod.f90:
+ * calc_laplace :22-56, calc_tile_d1x_d1y :58-92, calc_laplace_tile :93-245, first derivative d21,
+ * laplace_factory_mod.f90:143-178), laplace_Ah_sbp42_narrow_t and laplace_Ah_sbp63_narrow_t
  * (differential_operators/horizontal/laplace/laplace_Ah_sbp42_narrow_mod.f90: calc_laplace
  * :23-58, calc_tile_d1x_d1y :60-93, calc_laplace_tile :94-763; laplace_Ah_sbp63_narrow_mod.f90:
  * calc_laplace :494-529, calc_tile_d1x_d1y :531-564, calc_laplace_tile :565-1717; the
@@ -922,7 +924,11 @@ static const narrow_tab NARROW63 = {63, LN63_NB, LN63_NCOL, LN63_R, LN63_ROW_NEG
                                     LN63_ICOEF, LN63_NCOLS, &LN63_MSTART[0][0], LN63_TSIGN, LN63_TB,
                                     LN63_ISTART, LN63_ISIGN, LN63_IB, LN63_PC};
 
+/* sbp21 (laplace_Ah_sbp21_narrow_mod.f90): three-point rows written out in narrow_d2, penalty 2 */
+static const narrow_tab NARROW21 = {21, 1, 0, 1, 0, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, 2.0};
+
 static const narrow_tab *narrow_table(int order) {
+    if (order == 21) return &NARROW21;
     if (order == 42) return &NARROW42;
     if (order == 63) return &NARROW63;
     fprintf(stderr, "oracle: no narrow Laplacian of order %d\n", order);
@@ -968,6 +974,23 @@ static double narrow_d2(const narrow_tab *T, const ora_grid *g, int p, int i, in
     const double *J = g->geo[0], *Q = dir == 'x' ? g->geo[1] : g->geo[3];
     const int t = dir == 'x' ? i : j;
 #define AT(a) (dir == 'x' ? IDX(g, p, (a), j) : IDX(g, p, i, (a)))
+    if (T->order == 21) {
+        /* laplace_Ah_sbp21_narrow_mod.f90:117-187: b = Qi(1 or 3)*J; edge rows -(bc+b_in) f_edge +
+         * (bc+b_in) f_next, interior (bp+bc)/2 f(t+1) - (bm+2bc+bp)/2 f(t) + (bm+bc)/2 f(t-1) */
+        if (t == 1) {
+            const double bc = Q[AT(1)] * J[AT(1)], bp = Q[AT(2)] * J[AT(2)];
+            const double cc = -(bc + bp), cp = +(bc + bp);
+            return cc * f[AT(1)] + cp * f[AT(2)];
+        }
+        if (t == n) {
+            const double bc = Q[AT(n)] * J[AT(n)], bm = Q[AT(n - 1)] * J[AT(n - 1)];
+            const double cc = -(bc + bm), cm = +(bc + bm);
+            return cm * f[AT(n - 1)] + cc * f[AT(n)];
+        }
+        const double bm = Q[AT(t - 1)] * J[AT(t - 1)], bc = Q[AT(t)] * J[AT(t)], bp = Q[AT(t + 1)] * J[AT(t + 1)];
+        const double cm = (bm + bc) / 2, cc = -(bm + 2 * bc + bp) / 2, cp = (bp + bc) / 2;
+        return cp * f[AT(t + 1)] + cc * f[AT(t)] + cm * f[AT(t - 1)];
+    }
     double b[16], fv[16];
     if (t <= T->nb || t >= n + 1 - T->nb) {
         const int left = t <= T->nb;

@@ -117,9 +117,9 @@ def test_driver_equals_python_path(tmp_path, N, order, nsteps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("order", [42, 63])
+@pytest.mark.parametrize("order", [21, 42, 63])
 def test_driver_with_hyperdiffusion_equals_python_path(tmp_path, order):
-    """&shallow_water_model with the sbp42 / sbp63 narrow-stencil hyperdiffusion (the operator
+    """&shallow_water_model with the sbp21 / sbp42 / sbp63 narrow-stencil hyperdiffusion (the operator
     family of the same order): the driver's RK4 + explicit_Eul1 loop equals the Python path step
     for step, bit for bit."""
     _need_driver()
@@ -184,7 +184,7 @@ def test_driver_latlon_output_equals_python_outputers(tmp_path):
 def test_driver_rejects_other_hyperdiffusion(tmp_path):
     _need_driver()
     nml, _ = _write_nml(tmp_path, 16, 42, 600.0, 2, 1, uvc=0.03, hc=0.01,
-                        uvn="hordiff_vec_xyz_Ah_sbp_21_narrow", hn="hordiff_scalar_Ah_sbp_42_narrow")
+                        uvn="hordiff_c_biharm_curl_div", hn="hordiff_scalar_Ah_sbp_42_narrow")
     r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=60, cwd=tmp_path)
     assert r.returncode != 0
     assert "hyperdiffusion" in r.stdout

@@ -1,9 +1,10 @@
 """Hyperdiffusion on the device (swm_laplace, swm_diff_apply, swm_diff_step) against the oracle's
-restatement of laplace_Ah_sbp42_narrow_t / laplace_Ah_sbp63_narrow_t, hordiff_scalar_t /
+restatement of laplace_Ah_sbp{21,42,63}_narrow_t, hordiff_scalar_t /
 hordiff_colocated_xyz_t and explicit_Eul1: FP64, the same coefficient tables and operation
 order, so every point is expected bit for bit; the tests allow 1e-13 relative (max abs
 difference / max abs value).  Panel sizes from the smallest the closures allow (N+1 = 12 for
-sbp42, 18 for sbp63; 23-25 put one tile over the closures of both edges for sbp63) to 49;
+sbp42, 18 for sbp63; 23-25 put one tile over the closures of both edges for sbp63; sbp21 from
+the smallest context, N = 8) to 49;
 seeded spherical-harmonic states.  The diffusion step is checked inside an RK4 loop as the
 reference drivers call it (ts2_mod.f90:145-146), also with different families for the velocity
 and the height."""
@@ -30,8 +31,8 @@ def _rel(G, a, b):
     return float(np.max(np.abs(ai - bi)) / max(np.max(np.abs(bi)), 1e-300))
 
 
-LAPLACE_CASES = [(42, 11), (42, 16), (42, 33), (42, 48), (63, 17), (63, 22), (63, 23), (63, 24), (63, 33),
-                 (63, 48)]
+LAPLACE_CASES = [(21, 8), (21, 16), (21, 33), (42, 11), (42, 16), (42, 33), (42, 48), (63, 17), (63, 22),
+                 (63, 23), (63, 24), (63, 33), (63, 48)]
 
 
 @pytest.mark.parametrize("order,N", LAPLACE_CASES)
@@ -52,8 +53,8 @@ def _names(uv_order, h_order):
     return f"hordiff_vec_xyz_Ah_sbp_{uv_order}_narrow", f"hordiff_scalar_Ah_sbp_{h_order}_narrow"
 
 
-@pytest.mark.parametrize("uv_order,h_order,N", [(42, 42, 16), (42, 42, 33), (63, 63, 17), (63, 63, 33),
-                                                (63, 42, 24)])
+@pytest.mark.parametrize("uv_order,h_order,N", [(21, 21, 16), (21, 63, 24), (42, 42, 16), (42, 42, 33),
+                                                (63, 63, 17), (63, 63, 33), (63, 42, 24)])
 def test_diff_apply_matches_oracle(oracle_lib, uv_order, h_order, N):
     swm, op, G, h, u, v = _setup(oracle_lib, N, seed=2 * N)
     print(f"[diff_apply] uv={uv_order} h={h_order} N={N}", flush=True)
@@ -68,7 +69,7 @@ def test_diff_apply_matches_oracle(oracle_lib, uv_order, h_order, N):
     assert max(errs) < TOL
 
 
-@pytest.mark.parametrize("uv_order,h_order", [(42, 42), (63, 63), (42, 63)])
+@pytest.mark.parametrize("uv_order,h_order", [(21, 21), (42, 42), (63, 63), (42, 63)])
 def test_rk4_with_diff_steps_matches_oracle(oracle_lib, uv_order, h_order):
     """Three steps of RK4 (sbp42 operators) followed by the explicit hyperdiffusion step, C24."""
     N, dt, steps = 24, 600.0, 3
@@ -92,9 +93,9 @@ def test_diff_rejects_unsupported():
     op = swm.create_swm_operator(16, **swm.operator_names(42))
     x, t = swm.create_stvec_swm(op), swm.create_stvec_swm(op)
     with pytest.raises(swm.SwmError):
-        op.diff_apply(t, x, 0.03, 0.01, hordiff_uv_name="hordiff_vec_xyz_Ah_sbp_21_narrow")
+        op.diff_apply(t, x, 0.03, 0.01, hordiff_uv_name="hordiff_vec_xyz_Ah_sbp_43_narrow")
     with pytest.raises(swm.SwmError):
-        op.laplace(t, x, "laplace_ah_sbp21_narrow")
+        op.laplace(t, x, "laplace_ah_sbp43_narrow")
     with pytest.raises(swm.SwmError):   # N+1 = 17 < 18: too small for the sbp63 closures
         op.diff_step(x, 1.0, 0.03, 0.01, hordiff_h_name="hordiff_scalar_Ah_sbp_63_narrow")
     with pytest.raises(swm.SwmError):
