@@ -496,6 +496,73 @@ class OperatorSWMC:
         self._check_host(hs, halo_width)
         check(_capi.lib().swm_c_set_orography(self.ctx, _dp(hs), int(halo_width)))
 
+    # ---- ECS halos of the C grid and the operators on them (host arrays in, host arrays out) ----
+    def _device_buffer(self, n):
+        import torch
+        return torch.zeros(int(n), dtype=torch.float64, device="cuda")
+
+    def _from_stored(self, buf, halo_width):
+        """A stored field (field_elems doubles) as a host array (6, N+1+2hw, N+1+2hw)."""
+        H, n = self.halo, self.n
+        st = buf.reshape(6, self.rows, self.pitch)
+        ld = n + 2 * halo_width
+        out = np.zeros((6, ld, ld))
+        m = min(halo_width, H)
+        out[:, halo_width - m:halo_width + n + m, halo_width - m:halo_width + n + m] = st[:, H - m:H + n + m,
+                                                                                        H - m:H + n + m]
+        return out
+
+    def _state_of(self, h, u, v, halo_width):
+        x = StvecSWMC(self)
+        z = np.zeros(self.host_shape(halo_width))
+        x.upload(z if h is None else h, z if u is None else u, z if v is None else v, halo_width)
+        return x
+
+    def ecs_halo_vector(self, u, v, halo_width, procedure="ecs_C_vec", halo_width_in=0):
+        """get_halo_vector of the "ecs_C_vec" (contravariant) or "ecs_C_vec_covariant" halo
+        procedure for host u (x points), v (y points): arrays (6, N+2hw, N+1+2hw) and
+        (6, N+1+2hw, N+2hw) for i, j from 1-hw, NaN in the corner squares."""
+        import torch
+        N, hw = self.N, int(halo_width)
+        x = self._state_of(None, u, v, halo_width_in)
+        n = 6 * (N + 2 * hw) * (N + 1 + 2 * hw)
+        ou, ov = self._device_buffer(n), self._device_buffer(n)
+        torch.cuda.current_stream().synchronize()
+        check(_capi.lib().swm_c_ecs_halo_vec(self.ctx, procedure.encode(), ctypes.c_void_p(x.ptr), hw,
+                                             ctypes.c_void_p(ou.data_ptr()), ctypes.c_void_p(ov.data_ptr())))
+        self.synchronize()
+        x.free()
+        return (ou.cpu().numpy().reshape(6, N + 2 * hw, N + 1 + 2 * hw),
+                ov.cpu().numpy().reshape(6, N + 1 + 2 * hw, N + 2 * hw))
+
+    def calc_grad(self, f, grad_op_name="gradient_c2_ecs", halo_width=1):
+        """calc_grad of "gradient_c2_ecs" for a host o-point field f: (gx, gy) host arrays with
+        halo halo_width (gx also on the rows j = 0, N+1 and gy on the columns i = 0, N+1)."""
+        import torch
+        x = self._state_of(f, None, None, halo_width)
+        gx, gy = self._device_buffer(self.field_elems), self._device_buffer(self.field_elems)
+        torch.cuda.current_stream().synchronize()
+        check(_capi.lib().swm_c_grad(self.ctx, grad_op_name.encode(), ctypes.c_void_p(x.ptr),
+                                     ctypes.c_void_p(gx.data_ptr()), ctypes.c_void_p(gy.data_ptr())))
+        self.synchronize()
+        x.free()
+        return self._from_stored(gx.cpu().numpy(), halo_width), self._from_stored(gy.cpu().numpy(), halo_width)
+
+    def calc_massflux(self, f, u, v, massflux_op_name="massflux_c_up4", halo_width=0):
+        """calc_massflux of "massflux_c_up4": (fx, fy) host arrays for host f (o points), u, v."""
+        import torch
+        x = self._state_of(f, u, v, halo_width)
+        fs = self.field_elems
+        fx, fy = self._device_buffer(fs), self._device_buffer(fs)
+        torch.cuda.current_stream().synchronize()
+        base = x.ptr
+        check(_capi.lib().swm_c_massflux(self.ctx, massflux_op_name.encode(), ctypes.c_void_p(base),
+                                         ctypes.c_void_p(base + 8 * fs), ctypes.c_void_p(base + 16 * fs),
+                                         ctypes.c_void_p(fx.data_ptr()), ctypes.c_void_p(fy.data_ptr())))
+        self.synchronize()
+        x.free()
+        return self._from_stored(fx.cpu().numpy(), halo_width), self._from_stored(fy.cpu().numpy(), halo_width)
+
 
 class StvecSWMC:
     """Device-resident C-grid state vector (h at o points, u at x faces, v at y faces)."""

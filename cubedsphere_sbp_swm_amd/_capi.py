@@ -88,6 +88,9 @@ SIGNATURES = {
     "swm_c_rhs": (_i, [_vp, _vp, _vp]),
     "swm_c_rk4_steps": (_i, [_vp, _vp, _d, _i]),
     "swm_c_diagnostics": (_i, [_vp, _vp, _dp]),
+    "swm_c_ecs_halo_vec": (_i, [_vp, ctypes.c_char_p, _vp, _i, _vp, _vp]),
+    "swm_c_grad": (_i, [_vp, ctypes.c_char_p, _vp, _vp, _vp]),
+    "swm_c_massflux": (_i, [_vp, ctypes.c_char_p, _vp, _vp, _vp, _vp, _vp]),
 }
 
 # host set-up routines (geometry.h), exported by the same library; they need no device
@@ -108,6 +111,10 @@ HOST_SIGNATURES = {
                           [ctypes.POINTER(ctypes.c_longlong)] * 2 + [ctypes.c_void_p] +
                           [ctypes.POINTER(ctypes.c_longlong)] * 2 + [ctypes.c_void_p] +
                           [ctypes.POINTER(ctypes.c_longlong)] * 4 + [_ip]),
+    "swmh_c_ecs_vec_tables": (_i, [_i, _i, _i, _dp, _ip, _dp, _ip, _dp, _ip, _dp, _dp, _dp]),
+    "swmh_c_ecs_strip_sources": (_i, [_i, _i, _i, _i, ctypes.c_longlong, _i] +
+                                 [ctypes.POINTER(ctypes.c_longlong), ctypes.c_void_p] * 2 +
+                                 [ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 _lib = None

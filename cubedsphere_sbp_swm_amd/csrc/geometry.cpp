@@ -825,3 +825,136 @@ extern "C" int swmh_c_halo_lists(int N, int H, int rows, int pitch, long long fs
     counts[0] = nv; counts[1] = nf; counts[2] = no; counts[3] = nq;
     return 0;
 }
+
+namespace {
+// init_1d_tile_interp (cubic_lag, ecs_halo_vec_c_factory_mod.f90:272-341) for positions
+// 1..npos and halo rows 1..hw: source grid shift sx, target shift (tx, ty), stencil base clamped
+// to [2, nx-2]
+void c_ecs_interp(int N, int npos, int hw, int nx, double sx, double tx, double ty, double *w, int *ind) {
+    const double pi = pi_(), hx = grid_hx(N);
+    for (int j = 1; j <= hw; ++j) {
+        const double beta = 0.25 * pi + ((double)j + ty) * hx;
+        for (int i = 1; i <= npos; ++i) {
+            double alpha = -0.25 * pi + ((double)i - 1.0 + tx) * hx;
+            double x = std::tan(alpha), y = std::tan(beta);
+            const double z = 1.0 / std::sqrt(1.0 + x * x + y * y);
+            x = x * z;
+            y = y * z;
+            alpha = std::atan(x / y);
+            double xi = (alpha + 0.25 * pi - sx * hx) / hx;
+            const int id = std::max(2, std::min((int)xi + 1, nx - 2));
+            xi = xi - id + 1;
+            const int o = (i - 1) * hw + (j - 1);
+            ind[o] = id;
+            w[4 * o + 0] = -xi * (xi - 1.0) * (xi - 2.0) / 6.0;
+            w[4 * o + 1] = (xi + 1.0) * (xi - 1.0) * (xi - 2.0) / 2.0;
+            w[4 * o + 2] = -(xi + 1.0) * xi * (xi - 2.0) / 2.0;
+            w[4 * o + 3] = (xi + 1.0) * xi * (xi - 1.0) / 6.0;
+        }
+    }
+}
+// the halo point's coordinate along the edge in the neighbour's frame
+double c_ecs_along(double alpha, double beta) {
+    double x = std::tan(alpha), y = std::tan(beta);
+    const double z = 1.0 / std::sqrt(1.0 + x * x + y * y);
+    x = x * z;
+    y = y * z;
+    return std::atan(x / y);
+}
+}  // namespace
+
+extern "C" int swmh_c_ecs_vec_tables(int N, int hw, int covariant, double *w0, int *i0, double *w1, int *i1,
+                                     double *w2, int *i2, double *q1, double *q2, double *ct) {
+    if (hw < 1 || hw > 5 || N < 8) return -1;
+    const double pi = pi_(), hx = grid_hx(N);
+    c_ecs_interp(N, N + 1, hw, N + 1, 0.0, 0.0, -0.5, w0, i0);   // tangential, "x" -> "x"
+    c_ecs_interp(N, N, hw, N, 0.5, 0.5, 0.0, w1, i1);            // normal, "y" -> "y"
+    if (covariant) c_ecs_interp(N, N + 1, hw, N, 0.0, 0.5, 0.0, w2, i2);   // "x" -> "y"
+    else c_ecs_interp(N, N + 1, hw, N, 0.5, 0.0, -0.5, w2, i2);            // "y" -> "x"
+    for (int j = 1; j <= hw; ++j)
+        for (int i = 1; i <= N + 1; ++i) {
+            const int o = (i - 1) * hw + (j - 1);
+            double v[3], a[3], b[3], e[3];
+            if (!covariant) {
+                // init_basis_transform (:343-380): qtt, qtn
+                const double beta = 0.25 * pi + ((double)j - 0.5) * hx, alpha = -0.25 * pi + ((double)i - 1.0) * hx;
+                const double al1 = c_ecs_along(alpha, beta), be1 = -0.25 * pi + ((double)j - 0.5) * hx;
+                b1_proto(alpha, beta, b);
+                a1_proto(al1, be1, v);
+                a[0] = v[0], a[1] = v[2], a[2] = -v[1];
+                a2_proto(al1, be1, v);
+                e[0] = v[0], e[1] = v[2], e[2] = -v[1];
+                q1[o] = dot3(a, b);
+                q2[o] = dot3(e, b);
+            } else {
+                // init_basis_transform_cov (:382-419): qnn first, then qnt
+                const double beta = 0.25 * pi + (double)j * hx, alpha = -0.25 * pi + ((double)i - 0.5) * hx;
+                const double al1 = c_ecs_along(alpha, beta), be1 = -0.25 * pi + (double)j * hx;
+                a2_proto(alpha, beta, a);
+                b1_proto(al1, be1, v);
+                b[0] = v[0], b[1] = v[2], b[2] = -v[1];
+                b2_proto(al1, be1, v);
+                e[0] = v[0], e[1] = v[2], e[2] = -v[1];
+                q1[o] = dot3(a, e);
+                q2[o] = dot3(a, b);
+                // init_tangent_comp_stretch (:421-449)
+                const double bt = 0.25 * pi + ((double)j - 0.5) * hx, at = -0.25 * pi + ((double)i - 1.0) * hx;
+                const double at1 = c_ecs_along(at, bt), bt1 = -0.25 * pi + ((double)j - 0.5) * hx;
+                double tv[3], sv[3];
+                a1_proto(at, bt, tv);
+                a1_proto(at1, bt1, sv);
+                ct[o] = std::sqrt((tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]) /
+                                  (sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]));
+            }
+        }
+    return 0;
+}
+
+extern "C" int swmh_c_ecs_strip_sources(int N, int H, int rows, int pitch, long long fs, int depth,
+                                        long long *nsrc, signed char *nsign, long long *tsrc, signed char *tsign,
+                                        long long *osrc) {
+    const int N2 = 2 * N, n2 = N2 + 1, nd = 2 * depth;
+    if (depth < 1 || nd > n2) return -1;
+    std::vector<int> tab((size_t)6 * 4 * n2 * nd * 4);
+    swmh_halo_table(N2, nd, tab.data());
+    auto entry = [&](int p, int e, int k2, int d2) { return &tab[((((size_t)p * 4 + e) * n2 + k2) * nd + d2) * 4]; };
+    for (int p = 0; p < 6; ++p)
+        for (int e = 0; e < 4; ++e) {
+            const int normal = e < 2 ? 0 : 1;   // the component normal to the edge: u at x edges
+            for (int D = 0; D < depth; ++D) {
+                // normal component: faces at depth D (0 = the neighbour's copy of the shared face)
+                for (int k = 1; k <= N; ++k) {
+                    const int *t = entry(p, e, 2 * k - 1, 2 * D);
+                    const int q = t[0], a = t[1], b = t[2], code = t[3];
+                    const int qc = (code & 1) ? 1 - normal : normal;
+                    const bool neg = normal == 0 ? (code & 2) : (code & 4);
+                    if (qc == 0 ? (a % 2 != 0 || b % 2 != 1) : (a % 2 != 1 || b % 2 != 0)) return -2;
+                    const int qi = qc == 0 ? a / 2 + 1 : (a + 1) / 2, qj = qc == 0 ? (b + 1) / 2 : b / 2 + 1;
+                    const size_t o = (((size_t)p * 4 + e) * depth + D) * N + (k - 1);
+                    nsrc[o] = qc * fs + c_off(H, rows, pitch, q, qi, qj);
+                    nsign[o] = neg ? -1 : 1;
+                }
+                // tangential component: centre lines at depth D
+                const int tc = 1 - normal;
+                for (int k = 1; k <= N + 1; ++k) {
+                    const int *t = entry(p, e, 2 * (k - 1), 2 * D + 1);
+                    const int q = t[0], a = t[1], b = t[2], code = t[3];
+                    const int qc = (code & 1) ? 1 - tc : tc;
+                    const bool neg = tc == 0 ? (code & 2) : (code & 4);
+                    if (qc == 0 ? (a % 2 != 0 || b % 2 != 1) : (a % 2 != 1 || b % 2 != 0)) return -3;
+                    const int qi = qc == 0 ? a / 2 + 1 : (a + 1) / 2, qj = qc == 0 ? (b + 1) / 2 : b / 2 + 1;
+                    const size_t o = (((size_t)p * 4 + e) * depth + D) * (N + 1) + (k - 1);
+                    tsrc[o] = qc * fs + c_off(H, rows, pitch, q, qi, qj);
+                    tsign[o] = neg ? -1 : 1;
+                }
+                // cell centres at depth D
+                for (int k = 1; k <= N; ++k) {
+                    const int *t = entry(p, e, 2 * k - 1, 2 * D + 1);
+                    const int q = t[0], a = t[1], b = t[2];
+                    if (a % 2 != 1 || b % 2 != 1) return -4;
+                    osrc[(((size_t)p * 4 + e) * depth + D) * N + (k - 1)] = c_off(H, rows, pitch, q, (a + 1) / 2, (b + 1) / 2);
+                }
+            }
+        }
+    return 0;
+}

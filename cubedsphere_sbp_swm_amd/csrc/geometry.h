@@ -95,6 +95,23 @@ int swmh_c_halo_lists(int N, int H, int rows, int pitch, long long fs, int depth
                       long long *o_dst, long long *o_src, long long *q_own, long long *q_nb,
                       int *counts);
 
+// C grid ECS vector halo (create_ecs_C_vec_halo_procedure / ..._covariant_..., ecs_halo_vec_c_factory_mod.f90:
+// 18-270), one table set for every edge: w0/i0 tangential interpolation (positions 1..N+1), w1/i1
+// normal interpolation (1..N), w2/i2 interpolation of the other component to the transformed
+// component's positions (1..N+1); q1 = the transformed component's own coefficient (qtt or qnn),
+// q2 = the other's (qtn or qnt), ct the covariant tangential stretch.  Entry (i, j) of position i
+// and halo row j at (i-1)*hw + j-1 (weights x4).  1 <= hw <= 5, N >= 8; returns 0.
+int swmh_c_ecs_vec_tables(int N, int hw, int covariant, double *w0, int *i0, double *w1, int *i1, double *w2,
+                          int *i2, double *q1, double *q2, double *ct);
+
+// Sources of the strips beyond each panel edge to `depth` rows, as element offsets in the stored C
+// layout from the u field (v at +fs) or the o field: normal component at face depth D (0 = the
+// neighbour's copy of the shared face) [p][e][D][k < N], tangential component at centre depth D
+// [p][e][D][k < N+1] with signs, cell centres at depth D [p][e][D][k < N]; edges e = 0 (i < 1),
+// 1 (i > N+1 / N), 2 (j < 1), 3 (j > ...).  Returns 0, or a negative value on a topology mismatch.
+int swmh_c_ecs_strip_sources(int N, int H, int rows, int pitch, long long fs, int depth, long long *nsrc,
+                             signed char *nsign, long long *tsrc, signed char *tsign, long long *osrc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -675,3 +675,4 @@ extern "C" int swm_debug_halo_table(swm_ctx *c, int *table) {
 #include "swm_hordiff.hip"
 #include "swm_output.hip"
 #include "swm_cgrid.hip"
+#include "swm_cgrid_ecs.hip"

@@ -582,6 +582,20 @@ struct swm_cctx {
     hipGraphExec_t graph = nullptr, graph_multi = nullptr;
     void *graph_state = nullptr;
     double graph_dt = 0.0;
+    // ECS halos of the C grid (the vector halo and the ECS_O halo of gradient_c2_ecs and
+    // massflux_c_up4), set up on first use: strip sources (normal, tangential, cells; element
+    // offsets) and signs, the gathered strips, ECS_O weights, the vector tables of one (width,
+    // kind) and the extended o-point array
+    long long *ecs_src = nullptr;
+    signed char *ecs_sign = nullptr;
+    double *ecs_strips = nullptr;
+    double *ecs_ow = nullptr;
+    int *ecs_oind = nullptr;
+    int ecs_orows = 0;
+    double *ecs_vt = nullptr;
+    int *ecs_vti = nullptr;
+    int ecs_vt_hw = 0, ecs_vt_cov = -1;
+    double *ecs_ext = nullptr;
 
     size_t field_bytes() const { return (size_t)fs * sizeof(double); }
     double *wf(int k) const { return work + (size_t)k * fs; }
@@ -689,7 +703,9 @@ void c_free(swm_cctx *c) {
     if (!c) return;
     c_drop_graphs(c);
     for (void *p : {(void *)c->geo, (void *)c->hs, (void *)c->lists, (void *)c->signs, (void *)c->work, (void *)c->qw,
-                    (void *)c->dpart, (void *)c->qpairs})
+                    (void *)c->dpart, (void *)c->qpairs, (void *)c->ecs_src, (void *)c->ecs_sign,
+                    (void *)c->ecs_strips, (void *)c->ecs_ow, (void *)c->ecs_oind, (void *)c->ecs_vt,
+                    (void *)c->ecs_vti, (void *)c->ecs_ext})
         if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

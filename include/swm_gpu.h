@@ -318,6 +318,30 @@ int swm_c_rk4_steps(swm_cctx *ctx, void *state, double dt, int nsteps);
  * points by h at the same indices and is marked in the reference as not working for the C grid) */
 int swm_c_diagnostics(swm_cctx *ctx, const void *state, double out[7]);
 
+/* ECS vector halo of the state's (u, v): create_vector_halo_procedure(halo, domain, halo_width, name)
+ * + get_halo_vector(u, v, domain, halo_width) with name "ecs_C_vec" (contravariant components,
+ * ecs_halo_vec_c_t) or "ecs_C_vec_covariant" (ecs_halo_vec_c_cov_t) (halo_factory_mod.f90,
+ * equiang_cs/ecs_halo_vec_c_mod.f90:76-595, ecs_halo_vec_c_factory_mod.f90:18-449).  Device
+ * outputs out_u (6, N+2hw, N+1+2hw) and out_v (6, N+1+2hw, N+2hw) doubles, i fastest, for i, j from
+ * 1-hw: the panel's points (the normal component's edge faces averaged with the neighbour's
+ * copies for "ecs_C_vec"), the halo strips of width hw beyond the four edges, NaN in the corner
+ * squares (not defined by the procedure).  1 <= halo_width <= 5; other names return SWM_ENAME.
+ * Enqueued on the context stream. */
+int swm_c_ecs_halo_vec(swm_cctx *ctx, const char *halo_procedure_name, const void *state, int halo_width,
+                       void *out_u, void *out_v);
+/* create_grad_operator(domain, "gradient_c2_ecs") + calc_grad(gx, gy, f, domain)
+ * (grad_c2_ecs_mod.f90:33-51,98-139): the ECS_O halo of width 1 of f (o points), then
+ * (f(i) - f(i-1)) / (hx scale) at x points i = 1..N+1, j = 0..N+1 (gx) and likewise at y points
+ * i = 0..N+1, j = 1..N+1 (gy).  f, gx, gy: device fields of field_elems doubles in the stored
+ * layout (the h, u, v slots of a state vector will do); other names return SWM_ENAME. */
+int swm_c_grad(swm_cctx *ctx, const char *grad_op_name, const void *f, void *gx, void *gy);
+/* create_massflux_operator(domain, "massflux_c_up4") + calc_massflux(fx, fy, f, u, v, domain)
+ * (massflux_Cgrid_mod.f90:87-104,166-207): the ECS_O halo of width 3 of f, then the 4-point
+ * upwind-biased face value times u at x points (fx) and v at y points (fy).  Device fields in the
+ * stored layout; other names return SWM_ENAME. */
+int swm_c_massflux(swm_cctx *ctx, const char *massflux_op_name, const void *f, const void *u, const void *v,
+                   void *fx, void *fy);
+
 #ifdef __cplusplus
 }
 #endif

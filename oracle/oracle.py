@@ -102,6 +102,14 @@ def lib():
         L.ora_c_quad_mass.restype = ctypes.c_double
         L.ora_c_quad_mass.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.ora_c_diagnostics.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.ora_c_ecs_vec_tables.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _ip, _dp, _ip, _dp,
+                                           _ip, _dp, _dp, _dp]
+        L.ora_c_ecs_halo_vec.restype = ctypes.c_int
+        L.ora_c_ecs_halo_vec.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.ora_c_grad_c2_ecs.restype = ctypes.c_int
+        L.ora_c_grad_c2_ecs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.ora_c_massflux_up4.restype = ctypes.c_int
+        L.ora_c_massflux_up4.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -113,6 +121,11 @@ def _p(a):
 
 def _pn(a):
     return None if a is None else _p(a)
+
+
+def _pi(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
 
 
 GEO = {"J": 0, "Qi11": 1, "Qi12": 2, "Qi22": 3, "f": 4, "r": 5, "a1": 8, "a2": 11,
@@ -465,6 +478,41 @@ class CGrid:
 
     def quad_mass(self, t, f):
         return lib().ora_c_quad_mass(self.h, C_TYPES[t], _p(f))
+
+    def ecs_vec_tables(self, hw, covariant):
+        """Interpolation tables of the C-grid ECS vector halo: dict of w0, i0 (tangential), w1, i1
+        (normal), w2, i2 (cross-component), q1, q2 (basis transform), ct (covariant stretch)."""
+        N = self.N
+        np_ = (N + 1) * hw
+        t = {k: np.zeros(4 * np_) for k in ("w0", "w1", "w2")}
+        t.update({k: np.zeros(np_, dtype=np.int32) for k in ("i0", "i1", "i2")})
+        t.update({k: np.zeros(np_) for k in ("q1", "q2", "ct")})
+        lib().ora_c_ecs_vec_tables(self.h, int(hw), int(bool(covariant)), _p(t["w0"]), _pi(t["i0"]), _p(t["w1"]),
+                                   _pi(t["i1"]), _p(t["w2"]), _pi(t["i2"]), _p(t["q1"]), _p(t["q2"]), _p(t["ct"]))
+        return t
+
+    def ecs_halo_vec(self, u, v, hw, covariant=False):
+        """ECS vector halo of (u, v) (the reference's ecs_C_vec or ecs_C_vec_covariant procedure):
+        arrays (6, N+2hw, N+1+2hw) for u and (6, N+1+2hw, N+2hw) for v, i and j from 1-hw."""
+        N = self.N
+        ou = np.zeros((6, N + 2 * hw, N + 1 + 2 * hw))
+        ov = np.zeros((6, N + 1 + 2 * hw, N + 2 * hw))
+        rc = lib().ora_c_ecs_halo_vec(self.h, _p(u), _p(v), int(hw), int(bool(covariant)), _p(ou), _p(ov))
+        if rc:
+            raise ValueError("ora_c_ecs_halo_vec: 1 <= hw <= 5 and N >= 8 required")
+        return ou, ov
+
+    def grad_c2_ecs(self, f):
+        gx, gy = self.zeros(), self.zeros()
+        if lib().ora_c_grad_c2_ecs(self.h, _p(f), _p(gx), _p(gy)):
+            raise ValueError("ora_c_grad_c2_ecs: N >= 8 required")
+        return gx, gy
+
+    def massflux_up4(self, f, u, v):
+        fx, fy = self.zeros(), self.zeros()
+        if lib().ora_c_massflux_up4(self.h, _p(f), _p(u), _p(v), _p(fx), _p(fy)):
+            raise ValueError("ora_c_massflux_up4: N >= 8 required")
+        return fx, fy
 
     def diagnostics(self, h, u, v, hs=None):
         out = np.zeros(7)

@@ -2539,3 +2539,361 @@ void ora_c_diagnostics(const ora_cgrid *c, double *h, double *u, double *v, cons
     out[6] = NAN;
     free(ut); free(vt); free(hu); free(hv); free(PE);
 }
+
+/* =====================================================================================
+ * ECS vector halo on the C grid (create_ecs_C_vec_halo_procedure and
+ * create_ecs_C_vec_covariant_halo_procedure, equiang_cs/ecs_halo_vec_c_factory_mod.f90:18-270;
+ * get_halo_vector of ecs_halo_vec_c_t / ecs_halo_vec_c_cov_t, ecs_halo_vec_c_mod.f90:76-595)
+ * and its consumers gradient_c2_ecs (grad_c2_ecs_mod.f90:33-51,98-139) and massflux_c_up4
+ * (massflux_Cgrid_mod.f90:87-104,166-207, ECS_O halo of width 3).
+ * Tables (one set serves every edge and panel, as every tile is a whole panel):
+ *   kind 0 tangential interpolation  init_1d_tile_interp(1..N+1, nx = N+1, "x" -> "x")
+ *   kind 1 normal interpolation      init_1d_tile_interp(1..N,   nx = N,   "y" -> "y")
+ *   kind 2 contravariant: interpolation of the normal component to the tangential component's
+ *          positions ("y" -> "x", nx = N); covariant: of the tangential to the normal's
+ *          positions ("x" -> "y", nx = N); positions 1..N+1 in both
+ *   w[((i-1)*hw + j-1)*4 + s+1], ind[(i-1)*hw + j-1] (1-based base index) for position i, row j;
+ *   q1, q2 = qtt, qtn (init_basis_transform) or qnn, qnt (init_basis_transform_cov): the target
+ *   component's own coefficient first, then that of the other component, ct
+ *   (init_tangent_comp_stretch, covariant only): [(i-1)*hw + j-1], i = 1..N+1.
+ * ===================================================================================== */
+static void c_ecs_interp_table(int npos, int hw, int nx, double hx, double sx, double tx, double ty, double *w,
+                               int *ind) {
+    const double pi = pi_();
+    for (int j = 1; j <= hw; j++) {
+        const double beta = 0.25 * pi + ((double)j + ty) * hx;
+        for (int i = 1; i <= npos; i++) {
+            double alpha = -0.25 * pi + ((double)i - 1.0 + tx) * hx;
+            double x = tan(alpha), y = tan(beta);
+            const double z = 1.0 / sqrt(1.0 + x * x + y * y);
+            x = x * z;
+            y = y * z;
+            alpha = atan(x / y);
+            double xi = (alpha + 0.25 * pi - sx * hx) / hx;
+            int id = (int)xi + 1;
+            if (id > nx - 2) id = nx - 2;
+            if (id < 2) id = 2;
+            xi = xi - id + 1;
+            const int o = (i - 1) * hw + (j - 1);
+            ind[o] = id;
+            w[4 * o + 0] = -xi * (xi - 1.0) * (xi - 2.0) / 6.0;
+            w[4 * o + 1] = (xi + 1.0) * (xi - 1.0) * (xi - 2.0) / 2.0;
+            w[4 * o + 2] = -(xi + 1.0) * xi * (xi - 2.0) / 2.0;
+            w[4 * o + 3] = (xi + 1.0) * xi * (xi - 1.0) / 6.0;
+        }
+    }
+}
+
+/* halo point (alpha, beta) of the prototype face beyond beta = pi/4 and the same point in the
+ * neighbour's frame: alpha1 along the edge, beta1 = -pi/4 + depth */
+static double c_ecs_alpha1(double alpha, double beta) {
+    double x = tan(alpha), y = tan(beta);
+    const double z = 1.0 / sqrt(1.0 + x * x + y * y);
+    x = x * z;
+    y = y * z;
+    return atan(x / y);
+}
+
+void ora_c_ecs_vec_tables(const ora_cgrid *c, int hw, int covariant, double *w0, int *i0, double *w1, int *i1,
+                          double *w2, int *i2, double *q1, double *q2, double *ct) {
+    const int N = c->N;
+    const double hx = c->g->hx, pi = pi_();
+    c_ecs_interp_table(N + 1, hw, N + 1, hx, 0.0, 0.0, -0.5, w0, i0);
+    c_ecs_interp_table(N, hw, N, hx, 0.5, 0.5, 0.0, w1, i1);
+    if (!covariant) c_ecs_interp_table(N + 1, hw, N, hx, 0.5, 0.0, -0.5, w2, i2);
+    else c_ecs_interp_table(N + 1, hw, N, hx, 0.0, 0.5, 0.0, w2, i2);
+    for (int j = 1; j <= hw; j++)
+        for (int i = 1; i <= N + 1; i++) {
+            const int o = (i - 1) * hw + (j - 1);
+            double v[3], a[3], b[3], e[3];
+            if (!covariant) {
+                /* init_basis_transform, ecs_halo_vec_c_factory_mod.f90:343-380 */
+                const double beta = 0.25 * pi + ((double)j - 0.5) * hx, alpha = -0.25 * pi + ((double)i - 1.0) * hx;
+                const double alpha1 = c_ecs_alpha1(alpha, beta), beta1 = -0.25 * pi + ((double)j - 0.5) * hx;
+                ecs_b1_proto(alpha, beta, b);
+                ecs_a1_proto(alpha1, beta1, v);
+                a[0] = v[0]; a[1] = v[2]; a[2] = -v[1];
+                ecs_a2_proto(alpha1, beta1, v);
+                e[0] = v[0]; e[1] = v[2]; e[2] = -v[1];
+                q1[o] = dot3(a, b);
+                q2[o] = dot3(e, b);
+            } else {
+                /* init_basis_transform_cov, :382-419 */
+                const double beta = 0.25 * pi + (double)j * hx, alpha = -0.25 * pi + ((double)i - 0.5) * hx;
+                const double alpha1 = c_ecs_alpha1(alpha, beta), beta1 = -0.25 * pi + (double)j * hx;
+                ecs_a2_proto(alpha, beta, a);
+                ecs_b1_proto(alpha1, beta1, v);
+                b[0] = v[0]; b[1] = v[2]; b[2] = -v[1];
+                ecs_b2_proto(alpha1, beta1, v);
+                e[0] = v[0]; e[1] = v[2]; e[2] = -v[1];
+                q1[o] = dot3(a, e);   /* qnn: the normal component's own coefficient */
+                q2[o] = dot3(a, b);   /* qnt */
+                /* init_tangent_comp_stretch, :421-449 */
+                const double bt = 0.25 * pi + ((double)j - 0.5) * hx, at = -0.25 * pi + ((double)i - 1.0) * hx;
+                const double at1 = c_ecs_alpha1(at, bt), bt1 = -0.25 * pi + ((double)j - 0.5) * hx;
+                double t3[3], s3[3];
+                ecs_a1_proto(at, bt, t3);
+                ecs_a1_proto(at1, bt1, s3);
+                ct[o] = sqrt((t3[0] * t3[0] + t3[1] * t3[1] + t3[2] * t3[2]) /
+                             (s3[0] * s3[0] + s3[1] * s3[1] + s3[2] * s3[2]));
+            }
+        }
+}
+
+/* local extended arrays of one panel: U (x points) over i = 1-CE..N+1+CE, j = 1-CE..N+CE and V
+ * (y points) over i = 1-CE..N+CE, j = 1-CE..N+1+CE */
+#define CE 8
+typedef struct {
+    int N, hw, LUx, LUy, LVx, LVy;
+    double *U, *V;
+    const double *wt, *wn, *wx, *q1, *q2, *ct;
+    const int *it, *in, *ix;
+} c_ecs_loc;
+#define UL(L, i, j) (L)->U[((long)(j) - 1 + CE) * (L)->LUx + (i) - 1 + CE]
+#define VL(L, i, j) (L)->V[((long)(j) - 1 + CE) * (L)->LVx + (i) - 1 + CE]
+
+/* sum(w(-1:2,i,j) * row(ind-1 : ind+2)) in the reference's order */
+static double c_ecs_dot(const double *w, int base, const double *row, long stride) {
+    double s = w[0] * row[(long)(base - 1) * stride];
+    s = s + w[1] * row[(long)base * stride];
+    s = s + w[2] * row[(long)(base + 1) * stride];
+    s = s + w[3] * row[(long)(base + 2) * stride];
+    return s;
+}
+/* value of the extended array A (U or V) at row r of an edge and stencil position k */
+static double *c_ecs_rowptr(c_ecs_loc *L, int isU, int vertical, int r, long *stride) {
+    /* vertical: the edge is an x edge (left / right): rows are columns i = r, positions along j */
+    if (isU) {
+        *stride = vertical ? L->LUx : 1;
+        return vertical ? &UL(L, r, 0) : &UL(L, 0, r);
+    }
+    *stride = vertical ? L->LVx : 1;
+    return vertical ? &VL(L, r, 0) : &VL(L, 0, r);
+}
+/* interp_edge (ecs_halo_vec_c_mod.f90:352-394): row r(j) = rbase + rdir*j for j = 1..hw,
+ * positions 1..npos, in place through a temporary row */
+static void c_ecs_interp_edge(c_ecs_loc *L, int isU, int vertical, const double *w, const int *ind, int npos,
+                              int rfirst, int rdir) {
+    const int hw = L->hw;
+    double *tmp = malloc(sizeof(double) * (npos + 1));
+    for (int j = 1; j <= hw; j++) {
+        const int r = rfirst + rdir * (j - 1);
+        long st;
+        double *row = c_ecs_rowptr(L, isU, vertical, r, &st);
+        for (int i = 1; i <= npos; i++) {
+            const int o = (i - 1) * hw + (j - 1);
+            tmp[i] = c_ecs_dot(w + 4 * o, ind[o], row, st);
+        }
+        for (int i = 1; i <= npos; i++) row[(long)i * st] = tmp[i];
+    }
+    free(tmp);
+}
+/* tangent_comp_transform (:396-489) when !cov, normal_comp_transform (:491-595) when cov: target
+ * component T (isU_T) rows r(j), source component S rows around it; sgn -1 on the low edges */
+static void c_ecs_transform(c_ecs_loc *L, int cov, int isU_T, int vertical, int low, int nh) {
+    const int hw = L->hw, npos = L->N + 1;
+    for (int j = 1; j <= hw; j++) {
+        /* target row and the four source rows (the reference's row numbers) */
+        int rt, rs[4];
+        if (!cov) {
+            if (low) { rt = 1 - j; if (j == 1) { rs[0] = 1; rs[1] = 0; rs[2] = -1; rs[3] = -2; }
+                                   else { rs[0] = 3 - j; rs[1] = 2 - j; rs[2] = 1 - j; rs[3] = -j; } }
+            else { rt = nh + j; if (j == 1) { rs[0] = nh + 1; rs[1] = nh + 2; rs[2] = nh + 3; rs[3] = nh + 4; }
+                                else { rs[0] = nh + j - 1; rs[1] = nh + j; rs[2] = nh + j + 1; rs[3] = nh + j + 2; } }
+        } else {
+            if (low) { rt = 1 - j; if (j == 1) { rs[0] = 0; rs[1] = -1; rs[2] = -2; rs[3] = -3; }
+                                   else { rs[0] = 2 - j; rs[1] = 1 - j; rs[2] = -j; rs[3] = -j - 1; } }
+            else { rt = nh + j + 1; if (j == 1) { rs[0] = nh + 1; rs[1] = nh + 2; rs[2] = nh + 3; rs[3] = nh + 4; }
+                                    else { rs[0] = nh + j - 1; rs[1] = nh + j; rs[2] = nh + j + 1; rs[3] = nh + j + 2; } }
+        }
+        long stT, stS[4];
+        double *rowT = c_ecs_rowptr(L, isU_T, vertical, rt, &stT);
+        double *rowS[4];
+        for (int k = 0; k < 4; k++) rowS[k] = c_ecs_rowptr(L, !isU_T, vertical, rs[k], &stS[k]);
+        for (int i = 1; i <= npos; i++) {
+            const int o = (i - 1) * hw + (j - 1);
+            double s1 = c_ecs_dot(L->wx + 4 * o, L->ix[o], rowS[0], stS[0]);
+            double s2 = c_ecs_dot(L->wx + 4 * o, L->ix[o], rowS[1], stS[1]);
+            double s3 = c_ecs_dot(L->wx + 4 * o, L->ix[o], rowS[2], stS[2]);
+            double s4 = c_ecs_dot(L->wx + 4 * o, L->ix[o], rowS[3], stS[3]);
+            const double s0 = j == 1 ? (s4 - 5.0 * s3 + 15.0 * s2 + 5.0 * s1) / 16.0
+                                     : (-s1 + 9.0 * s2 + 9.0 * s3 - s4) / 16.0;
+            double *t = &rowT[(long)i * stT];
+            if (low) *t = L->q1[o] * *t - L->q2[o] * s0;
+            else *t = L->q1[o] * *t + L->q2[o] * s0;
+        }
+    }
+}
+
+int ora_c_ecs_halo_vec(const ora_cgrid *c, const double *u, const double *v, int hw, int covariant, double *out_u,
+                       double *out_v) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    if (hw < 1 || hw > 5 || N < 8) return -1;
+    const int np = (N + 1) * hw;
+    double *wt = malloc(sizeof(double) * 4 * np), *wn = malloc(sizeof(double) * 4 * np),
+           *wx = malloc(sizeof(double) * 4 * np), *q1 = malloc(sizeof(double) * np), *q2 = malloc(sizeof(double) * np),
+           *ct = malloc(sizeof(double) * np);
+    int *it = malloc(sizeof(int) * np), *in = malloc(sizeof(int) * np), *ix = malloc(sizeof(int) * np);
+    ora_c_ecs_vec_tables(c, hw, covariant, wt, it, wn, in, wx, ix, q1, q2, ct);
+    c_ecs_loc L = {N, hw, N + 1 + 2 * CE, N + 2 * CE, N + 2 * CE, N + 1 + 2 * CE, NULL, NULL, wt, wn, wx, q1, q2, ct,
+                   it, in, ix};
+    L.U = malloc(sizeof(double) * L.LUx * L.LUy);
+    L.V = malloc(sizeof(double) * L.LVx * L.LVy);
+    const int Lux = N + 1 + 2 * hw, Luy = N + 2 * hw, Lvx = N + 2 * hw, Lvy = N + 1 + 2 * hw;
+    for (int p = 0; p < 6; p++) {
+        for (long k = 0; k < (long)L.LUx * L.LUy; k++) L.U[k] = NAN;
+        for (long k = 0; k < (long)L.LVx * L.LVy; k++) L.V[k] = NAN;
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) UL(&L, i, j) = u[IDX(g, p, i, j)];
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) VL(&L, i, j) = v[IDX(g, p, i, j)];
+        /* create_symmetric_halo_vec_exchange_C(..., 7, 'full'): the strips beyond each edge to
+         * depth 7, as in c_exchange_vec (the procedure reads down to depth max(hw + 2, 4)) */
+        const double *fld[2] = {u, v};
+        for (int comp = 0; comp < 2; comp++) {
+            const int t = comp == 0 ? 1 : 2, nx = c_nx(N, t), ny = c_ny(N, t);
+            for (int e = 0; e < 4; e++) {
+                const int xn = e < 2, faces = (xn && t == 1) || (!xn && t == 2), ntan = xn ? ny : nx;
+                for (int d = 1; d <= 7; d++)
+                    for (int k = 1; k <= ntan; k++) {
+                        const int d2 = faces ? 2 * (d - 1) : 2 * d - 1;
+                        const int T2 = xn ? (t == 1 ? 2 * k - 1 : 2 * (k - 1)) : (t == 1 ? 2 * (k - 1) : 2 * k - 1);
+                        int i, j, q, a, b, sgn, qaxis;
+                        if (e == 0) { i = 1 - d; j = k; }
+                        else if (e == 1) { i = nx + d; j = k; }
+                        else if (e == 2) { i = k; j = 1 - d; }
+                        else { i = k; j = ny + d; }
+                        c_fold(N, p, e, d2, T2, &q, &a, &b);
+                        c_axis_map(p, e, comp, q, &sgn, &qaxis);
+                        if (qaxis == 0 ? (a % 2 != 0 || b % 2 != 1) : (a % 2 != 1 || b % 2 != 0)) abort();
+                        const int qi = qaxis == 0 ? a / 2 + 1 : (a + 1) / 2, qj = qaxis == 0 ? (b + 1) / 2 : b / 2 + 1;
+                        const double val = fld[qaxis][IDX(g, q, qi, qj)];
+                        if (comp == 0) UL(&L, i, j) = sgn < 0 ? -val : val;
+                        else VL(&L, i, j) = sgn < 0 ? -val : val;
+                    }
+            }
+        }
+        /* left and right edges (the normal component is u), then bottom and top (v) */
+        for (int e = 0; e < 4; e++) {
+            const int vertical = e < 2, low = e == 0 || e == 2;
+            const int isUn = vertical;  /* the normal component is u at x edges, v at y edges */
+            /* the normal component's edge line: averaged (contravariant only) and its rows
+             * shifted by one towards the panel */
+            const int nedge = N + 1;  /* index of the far face line of the normal component */
+            for (int k = 1; k <= N; k++) {
+#define NRM(ii, kk) (*(isUn ? &UL(&L, (ii), (kk)) : &VL(&L, (kk), (ii))))
+                if (!covariant) {
+                    if (low) NRM(1, k) = 0.5 * (NRM(1, k) + NRM(0, k));
+                    else NRM(nedge, k) = 0.5 * (NRM(nedge, k) + NRM(nedge + 1, k));
+                }
+                for (int s = 1; s <= hw + 1; s++) {
+                    if (low) NRM(1 - s, k) = NRM(-s, k);
+                    else NRM(nedge + s, k) = NRM(nedge + s + 1, k);
+                }
+#undef NRM
+            }
+            if (!covariant) {
+                /* tangential: interp_edge, tangent_comp_transform; then the normal: interp_edge */
+                c_ecs_interp_edge(&L, !isUn, vertical, wt, it, N + 1, low ? 0 : N + 1, low ? -1 : 1);
+                c_ecs_transform(&L, 0, !isUn, vertical, low, N);
+                c_ecs_interp_edge(&L, isUn, vertical, wn, in, N, low ? 0 : N + 2, low ? -1 : 1);
+            } else {
+                /* normal: interp_edge, normal_comp_transform; tangential: interp_edge, stretch */
+                c_ecs_interp_edge(&L, isUn, vertical, wn, in, N, low ? 0 : N + 2, low ? -1 : 1);
+                c_ecs_transform(&L, 1, isUn, vertical, low, N);
+                c_ecs_interp_edge(&L, !isUn, vertical, wt, it, N + 1, low ? 0 : N + 1, low ? -1 : 1);
+                for (int j = 1; j <= hw; j++) {
+                    long st;
+                    double *row = c_ecs_rowptr(&L, !isUn, vertical, low ? 1 - j : N + j, &st);
+                    for (int i = 1; i <= N + 1; i++) row[(long)i * st] = row[(long)i * st] * ct[(i - 1) * hw + j - 1];
+                }
+            }
+        }
+        /* windows; the corner squares are not defined by the procedure */
+        for (int jj = 0; jj < Luy; jj++)
+            for (int ii = 0; ii < Lux; ii++) {
+                const int i = ii + 1 - hw, j = jj + 1 - hw;
+                const int in_i = i >= 1 && i <= N + 1, in_j = j >= 1 && j <= N;
+                out_u[((long)p * Luy + jj) * Lux + ii] = (in_i || in_j) ? UL(&L, i, j) : NAN;
+            }
+        for (int jj = 0; jj < Lvy; jj++)
+            for (int ii = 0; ii < Lvx; ii++) {
+                const int i = ii + 1 - hw, j = jj + 1 - hw;
+                const int in_i = i >= 1 && i <= N, in_j = j >= 1 && j <= N + 1;
+                out_v[((long)p * Lvy + jj) * Lvx + ii] = (in_i || in_j) ? VL(&L, i, j) : NAN;
+            }
+    }
+    free(L.U); free(L.V);
+    free(wt); free(wn); free(wx); free(q1); free(q2); free(ct); free(it); free(in); free(ix);
+    return 0;
+}
+#undef UL
+#undef VL
+#undef CE
+
+/* ECS_O halo of an o-point field stored in the C layout: out[(p*L + jj)*L + ii], L = N + 2hw */
+static int c_ecs_o(const ora_cgrid *c, const double *f, int hw, double *out) {
+    const ora_grid *g = c->g;
+    const int N = c->N;
+    double *fc = malloc(sizeof(double) * 6 * N * N);
+    for (int p = 0; p < 6; p++)
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N; i++) fc[((long)p * N + j - 1) * N + i - 1] = f[IDX(g, p, i, j)];
+    const int rc = ora_ecs_halo_o(g, fc, hw, out);
+    free(fc);
+    return rc;
+}
+
+/* gradient_c2_ecs: the ECS_O halo of width 1, then one-sided differences at x points i = 1..N+1,
+ * j = 0..N+1 and y points i = 0..N+1, j = 1..N+1 (calc_grad_on_tile, grad_c2_ecs_mod.f90:98-139) */
+int ora_c_grad_c2_ecs(const ora_cgrid *c, const double *f, double *gx, double *gy) {
+    const ora_grid *g = c->g;
+    const int N = c->N, hw = 1, L = N + 2 * hw;
+    double *F = malloc(sizeof(double) * 6 * L * L);
+    if (c_ecs_o(c, f, hw, F)) { free(F); return -1; }
+    const double d = g->hx * g->scale;
+#define FE(p, i, j) F[((long)(p) * L + (j) - 1 + hw) * L + (i) - 1 + hw]
+    for (int p = 0; p < 6; p++) {
+        for (int j = 0; j <= N + 1; j++)
+            for (int i = 1; i <= N + 1; i++) gx[IDX(g, p, i, j)] = (FE(p, i, j) - FE(p, i - 1, j)) / d;
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 0; i <= N + 1; i++) gy[IDX(g, p, i, j)] = (FE(p, i, j) - FE(p, i, j - 1)) / d;
+    }
+#undef FE
+    free(F);
+    return 0;
+}
+
+/* massflux_c_up4: the ECS_O halo of width 3, then the 4-point upwind-biased face values times the
+ * velocity at x points (i = 1..N+1, j = 1..N) and y points (calc_up4_massflux_tile,
+ * massflux_Cgrid_mod.f90:166-207); zl = 0.5 + sign(0.5, u) with the sign of a negative zero */
+int ora_c_massflux_up4(const ora_cgrid *c, const double *f, const double *u, const double *v, double *fx,
+                       double *fy) {
+    const ora_grid *g = c->g;
+    const int N = c->N, hw = 3, L = N + 2 * hw;
+    double *F = malloc(sizeof(double) * 6 * L * L);
+    if (c_ecs_o(c, f, hw, F)) { free(F); return -1; }
+#define FE(p, i, j) F[((long)(p) * L + (j) - 1 + hw) * L + (i) - 1 + hw]
+    for (int p = 0; p < 6; p++) {
+        for (int j = 1; j <= N; j++)
+            for (int i = 1; i <= N + 1; i++) {
+                const double w = u[IDX(g, p, i, j)];
+                const double zl = 0.5 + copysign(0.5, w), zr = 1.0 - zl;
+                const double A = 5.0 * FE(p, i, j) + 15.0 * FE(p, i - 1, j) - 5.0 * FE(p, i - 2, j) + FE(p, i - 3, j);
+                const double B = 5.0 * FE(p, i - 1, j) + 15.0 * FE(p, i, j) - 5.0 * FE(p, i + 1, j) + FE(p, i + 2, j);
+                fx[IDX(g, p, i, j)] = w * (zl * A + zr * B) / 16.0;
+            }
+        for (int j = 1; j <= N + 1; j++)
+            for (int i = 1; i <= N; i++) {
+                const double w = v[IDX(g, p, i, j)];
+                const double zl = 0.5 + copysign(0.5, w), zr = 1.0 - zl;
+                const double A = 5.0 * FE(p, i, j) + 15.0 * FE(p, i, j - 1) - 5.0 * FE(p, i, j - 2) + FE(p, i, j - 3);
+                const double B = 5.0 * FE(p, i, j - 1) + 15.0 * FE(p, i, j) - 5.0 * FE(p, i, j + 1) + FE(p, i, j + 2);
+                fy[IDX(g, p, i, j)] = w * (zl * A + zr * B) / 16.0;
+            }
+    }
+#undef FE
+    free(F);
+    return 0;
+}

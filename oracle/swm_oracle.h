@@ -146,6 +146,18 @@ double ora_c_l2norm(const ora_cgrid *c, int t, const double *f);
 double ora_c_quad_mass(const ora_cgrid *c, int t, const double *f);
 void ora_c_diagnostics(const ora_cgrid *c, double *h, double *u, double *v, const double *hs, double out[7]);
 double ora_c_maxabs(const ora_cgrid *c, int t, const double *f);
+/* ECS vector halo on the C grid (the reference's "ecs_C_vec" / "ecs_C_vec_covariant" halo procedures),
+ * 1 <= hw <= 5, N >= 8: interpolation tables (see swm_oracle.c) and the halo of u (x points), v (y
+ * points) into out_u (6, N+2hw, N+1+2hw), out_v (6, N+1+2hw, N+2hw) with i, j from 1-hw; NaN in
+ * the corner squares, which the procedure does not define */
+void ora_c_ecs_vec_tables(const ora_cgrid *c, int hw, int covariant, double *w0, int *i0, double *w1, int *i1,
+                          double *w2, int *i2, double *q1, double *q2, double *ct);
+int ora_c_ecs_halo_vec(const ora_cgrid *c, const double *u, const double *v, int hw, int covariant, double *out_u,
+                       double *out_v);
+/* gradient_c2_ecs (gx at x points j = 0..N+1, gy at y points i = 0..N+1) and massflux_c_up4 */
+int ora_c_grad_c2_ecs(const ora_cgrid *c, const double *f, double *gx, double *gy);
+int ora_c_massflux_up4(const ora_cgrid *c, const double *f, const double *u, const double *v, double *fx,
+                       double *fy);
 
 #ifdef __cplusplus
 }

@@ -118,3 +118,74 @@ def test_c_q_sync_neighbours_are_the_ah_exchange_copies(oracle_lib):
                 s = src[p, j - 1 + H, i - 1 + H]
                 assert s >= 0 and conv(int(s)) == L["qn"][k], (p, e, t)
                 k += 1
+
+
+@pytest.mark.parametrize("hw", [1, 2, 3, 5])
+@pytest.mark.parametrize("cov", [0, 1])
+def test_c_ecs_vec_tables_bitwise(cgrid, hw, cov):
+    """Tables of the ECS vector halo on the C grid (interpolation weights and bases, basis
+    transforms, covariant stretch) computed by the host code = the oracle's, bit for bit."""
+    ref = cgrid.ecs_vec_tables(hw, cov)
+    n = (N + 1) * hw
+    d = {k: np.zeros(4 * n) for k in ("w0", "w1", "w2")}
+    d.update({k: np.zeros(n) for k in ("q1", "q2", "ct")})
+    k = {x: np.zeros(n, dtype=np.int32) for x in ("i0", "i1", "i2")}
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    assert _lib().swmh_c_ecs_vec_tables(N, hw, cov, dp(d["w0"]), ip(k["i0"]), dp(d["w1"]), ip(k["i1"]), dp(d["w2"]),
+                                        ip(k["i2"]), dp(d["q1"]), dp(d["q2"]), dp(d["ct"])) == 0
+    nn = N * hw   # the normal component's tables cover positions 1..N
+    for name, m in (("w0", 4 * n), ("w1", 4 * nn), ("w2", 4 * n), ("q1", n), ("q2", n), ("i0", n), ("i1", nn),
+                    ("i2", n)) + ((("ct", n),) if cov else ()):
+        ours = d[name] if name in d else k[name]
+        assert np.array_equal(ours[:m], ref[name][:m]), name
+    assert _lib().swmh_c_ecs_vec_tables(N, 6, cov, dp(d["w0"]), ip(k["i0"]), dp(d["w1"]), ip(k["i1"]), dp(d["w2"]),
+                                        ip(k["i2"]), dp(d["q1"]), dp(d["q2"]), dp(d["ct"])) != 0
+
+
+def test_c_ecs_strip_sources_match_oracle_exchange(cgrid):
+    """The rows gathered beyond each edge for the ECS halos (normal component at face depth D,
+    D = 0 the neighbour's copy of the shared face; tangential at centre depth D; cells at depth D)
+    are the values the oracle's C-grid exchanges put there, signs included."""
+    rows, pitch, F = _layout()
+    DE = 8
+    nn, nt = 24 * DE * N, 24 * DE * (N + 1)
+    src = np.zeros(nn + nt + nn, dtype=np.int64)
+    sg = np.zeros(nn + nt, dtype=np.int8)
+    lp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+    s_n, s_t, s_o = src[:nn], src[nn:nn + nt], src[nn + nt:]
+    assert _lib().swmh_c_ecs_strip_sources(N, H, rows, pitch, F, DE, lp(s_n), sg[:nn].ctypes.data_as(ctypes.c_void_p),
+                                           lp(s_t), sg[nn:].ctypes.data_as(ctypes.c_void_p), lp(s_o)) == 0
+    rng = np.random.default_rng(17)
+    u = cgrid.embed("x", rng.standard_normal((6, N, N + 1)))
+    v = cgrid.embed("y", rng.standard_normal((6, N + 1, N)))
+    f = cgrid.embed("o", rng.standard_normal((6, N, N)))
+    uv = np.concatenate([_from_oracle(u, rows, pitch), _from_oracle(v, rows, pitch)])
+    sgn = np.where(sg < 0, -1.0, 1.0)
+    En = (uv[s_n] * sgn[:nn]).reshape(6, 4, DE, N)
+    Et = (uv[s_t] * sgn[nn:]).reshape(6, 4, DE, N + 1)
+    Eo = _from_oracle(f, rows, pitch)[s_o].reshape(6, 4, DE, N)
+    ue, ve, fe = u.copy(), v.copy(), f.copy()
+    cgrid.exchange_vec(ue, ve, 3)
+    cgrid.exchange_o(fe, 3)
+    hw = cgrid.hw
+    at = lambda a, p, i, j: a[p, j - 1 + hw, i - 1 + hw]
+    for p in range(6):
+        for D in range(3):
+            for kk in range(1, N + 1):
+                assert En[p, 0, D, kk - 1] == at(ue, p, -D, kk)
+                assert En[p, 1, D, kk - 1] == at(ue, p, N + 2 + D, kk)
+                assert En[p, 2, D, kk - 1] == at(ve, p, kk, -D)
+                assert En[p, 3, D, kk - 1] == at(ve, p, kk, N + 2 + D)
+                assert Eo[p, 0, D, kk - 1] == at(fe, p, -D, kk)
+                assert Eo[p, 1, D, kk - 1] == at(fe, p, N + 1 + D, kk)
+                assert Eo[p, 2, D, kk - 1] == at(fe, p, kk, -D)
+                assert Eo[p, 3, D, kk - 1] == at(fe, p, kk, N + 1 + D)
+            for kk in range(1, N + 2):
+                assert Et[p, 0, D, kk - 1] == at(ve, p, -D, kk)
+                assert Et[p, 1, D, kk - 1] == at(ve, p, N + 1 + D, kk)
+                assert Et[p, 2, D, kk - 1] == at(ue, p, kk, -D)
+                assert Et[p, 3, D, kk - 1] == at(ue, p, kk, N + 1 + D)
+    # every value gathered is a distinct interior value of the neighbour (no halo, no repeats
+    # along a row)
+    assert np.all(np.abs(En) > 0) and np.all(np.abs(Et) > 0) and np.all(np.abs(Eo) > 0)
