@@ -296,6 +296,48 @@ def hyperdiffusion_form(args, stream, op, x):
 C_STEP_FIELD_PASSES = 182
 
 
+def cgrid_ecs_timings(args, op, xs, reps=200):
+    """The C-grid ECS halo operators on the device state (h, u, v) at the form's N: the vector halo
+    of width 3 (both kinds), gradient_c2_ecs and massflux_c_up4; time per call over `reps` calls
+    enqueued back to back (launch included), and the operator's minimum bytes (its inputs read
+    once, its outputs written once) per call time."""
+    import ctypes
+    import torch
+    from cubedsphere_sbp_swm_amd import _capi
+    L, N, fs, hw = _capi.lib(), op.N, op.field_elems, 3
+    nvec = 6 * (N + 2 * hw) * (N + 1 + 2 * hw)
+    ou, ov = (torch.empty(nvec, dtype=torch.float64, device="cuda") for _ in range(2))
+    gx, gy = (torch.empty(fs, dtype=torch.float64, device="cuda") for _ in range(2))
+    torch.cuda.synchronize()
+    vp = ctypes.c_void_p
+    h_, u_, v_ = vp(xs.ptr), vp(xs.ptr + 8 * fs), vp(xs.ptr + 16 * fs)
+    calls = {
+        "ecs_C_vec_hw3": (lambda: L.swm_c_ecs_halo_vec(op.ctx, b"ecs_C_vec", vp(xs.ptr), hw, vp(ou.data_ptr()),
+                                                       vp(ov.data_ptr())),
+                          2 * 6 * N * (N + 1) + 2 * nvec),
+        "ecs_C_vec_covariant_hw3": (lambda: L.swm_c_ecs_halo_vec(op.ctx, b"ecs_C_vec_covariant", vp(xs.ptr), hw,
+                                                                 vp(ou.data_ptr()), vp(ov.data_ptr())),
+                                    2 * 6 * N * (N + 1) + 2 * nvec),
+        "gradient_c2_ecs": (lambda: L.swm_c_grad(op.ctx, b"gradient_c2_ecs", h_, vp(gx.data_ptr()), vp(gy.data_ptr())),
+                            6 * N * N + 2 * 6 * (N + 1) * (N + 2)),
+        "massflux_c_up4": (lambda: L.swm_c_massflux(op.ctx, b"massflux_c_up4", h_, u_, v_, vp(gx.data_ptr()),
+                                                    vp(gy.data_ptr())),
+                           6 * N * N + 4 * 6 * N * (N + 1)),
+    }
+    res = {}
+    for name, (fn, values) in calls.items():
+        for _ in range(5):
+            _capi.check(fn())
+        op.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _capi.check(fn())
+        op.synchronize()
+        us = (time.perf_counter() - t0) * 1e6 / reps
+        res[name] = {"us_per_call": us, "min_bytes": 8 * values, "min_bytes_GBs": 8 * values / (us * 1e-6) / 1e9}
+    return res
+
+
 def cgrid_form(args, stream):
     """C-grid SBP42 family (divergence_c_sbp42, gradient_c_sbp42, curl_c_sbp42,
     coriolis_Cgrid_sbp42, KE_Cgrid_sbp42, co2contra_c_sbp42, massflux_c_sbp42), FP64 RK4 on the
@@ -342,6 +384,7 @@ def cgrid_form(args, stream):
            "fused_minimum_GBs_equiv": STEP_VALUES * 8 * points / (secs / steps) / 1e9}
     hd, _, _ = xs.download(C.hw)
     out["h_range_after"] = [float(C.interior("o", hd).min()), float(C.interior("o", hd).max())]
+    out["ecs"] = cgrid_ecs_timings(args, op, xs)
     if not args.no_cpu:
         hc, uc, vc = h.copy(), u.copy(), v.copy()
         t0 = time.perf_counter()

@@ -129,14 +129,17 @@ __device__ double c_vec_strip(const CVecArgs &g, int p, int e, int jr, int k, bo
 // one thread per output point: u (6, N+2hw, N+1+2hw) first, then v (6, N+1+2hw, N+2hw)
 template <bool COV>
 __global__ void k_cvec_fill(CVecArgs g) {
-    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= g.nu + g.nv) return;
+    // 32-bit index arithmetic (nu + nv < 2^32 for every panel size the library accepts)
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)(g.nu + g.nv)) return;
     const int N = g.N, hw = g.hw;
-    const bool isu = t < g.nu;
-    if (!isu) t -= g.nu;
+    const bool isu = t < (unsigned)g.nu;
+    if (!isu) t -= (unsigned)g.nu;
     const int nx = isu ? N + 1 : N, ny = isu ? N : N + 1;
     const int Lx = nx + 2 * hw, Ly = ny + 2 * hw;
-    const int ii = (int)(t % Lx), jj = (int)((t / Lx) % Ly), p = (int)(t / ((long long)Lx * Ly));
+    const unsigned per = (unsigned)Lx * (unsigned)Ly;
+    const int p = (int)(t / per), r = (int)(t - (unsigned)p * per);
+    const int jj = r / Lx, ii = r - jj * Lx;
     const int i = ii + 1 - hw, j = jj + 1 - hw;
     const bool in_i = i >= 1 && i <= nx, in_j = j >= 1 && j <= ny;
     double val;
@@ -164,11 +167,11 @@ __global__ void k_cvec_fill(CVecArgs g) {
 // then y points i = 0..N+1, j = 1..N+1
 __global__ void k_grad_c2(const double *__restrict__ F, int N, int rows, int pitch, double d, double *__restrict__ gx,
                           double *__restrict__ gy) {
-    const long long per = (long long)(N + 1) * (N + 2), t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t0 >= 12 * per) return;
-    const bool isx = t0 < 6 * per;
-    const long long t = isx ? t0 : t0 - 6 * per;
-    const int p = (int)(t / per), r = (int)(t % per);
+    const unsigned per = (unsigned)(N + 1) * (unsigned)(N + 2), t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 >= 12u * per) return;
+    const bool isx = t0 < 6u * per;
+    const unsigned t = isx ? t0 : t0 - 6u * per;
+    const int p = (int)(t / per), r = (int)(t - (unsigned)p * per);
     const int L = N + 2;
     int i, j;
     if (isx) { i = r % (N + 1) + 1; j = r / (N + 1); }
@@ -182,11 +185,11 @@ __global__ void k_grad_c2(const double *__restrict__ F, int N, int rows, int pit
 // massflux_c_up4 from the extended o field F (6, N+6, N+6): x points, then y points
 __global__ void k_up4(const double *__restrict__ F, const double *__restrict__ u, const double *__restrict__ v, int N,
                       int rows, int pitch, double *__restrict__ fx, double *__restrict__ fy) {
-    const long long per = (long long)(N + 1) * N, t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t0 >= 12 * per) return;
-    const bool isx = t0 < 6 * per;
-    const long long t = isx ? t0 : t0 - 6 * per;
-    const int p = (int)(t / per), r = (int)(t % per);
+    const unsigned per = (unsigned)(N + 1) * (unsigned)N, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 >= 12u * per) return;
+    const bool isx = t0 < 6u * per;
+    const unsigned t = isx ? t0 : t0 - 6u * per;
+    const int p = (int)(t / per), r = (int)(t - (unsigned)p * per);
     const int hw = 3, L = N + 2 * hw;
     int i, j;
     if (isx) { i = r % (N + 1) + 1; j = r / (N + 1) + 1; }

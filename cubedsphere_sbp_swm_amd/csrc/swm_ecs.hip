@@ -182,9 +182,11 @@ struct EcsOVec {
     __device__ T W(int s, int t, int r) const { return w[((r - 1) * (N + 4) + t + 1) * 4 + s + 1]; }
 };
 
-// special value k (0..5) of corner c (0..3: (1,1), (N,1), (N,N), (1,N) sides), ecs_halo_1corner
+// special value k (0..5) of corner c (0..3: (1,1), (N,1), (N,N), (1,N) sides), ecs_halo_1corner.
+// Inlined: a call would make every thread of ecs_o_fill copy its argument struct to scratch
+// memory to pass it by reference.
 template <typename S>
-__device__ typename S::value_type ecs_o_special(const S &o, int p, int c, int k) {
+__device__ __forceinline__ typename S::value_type ecs_o_special(const S &o, int p, int c, int k) {
     using T = typename S::value_type;
     const int N = o.N;
     const int ist = (c == 1 || c == 2) ? N + 1 : 0, idir = (c == 1 || c == 2) ? -1 : 1;
@@ -230,10 +232,12 @@ template <typename S>
 __global__ void ecs_o_fill(S o, typename S::value_type *__restrict__ out) {
     using T = typename S::value_type;
     const int N = o.N, hw = o.hw, L = N + 2 * hw;
-    const long long total = 6LL * L * L;
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int ii = (int)(t % L), jj = (int)((t / L) % L), p = (int)(t / ((long long)L * L));
+    // 32-bit index arithmetic: 6 L^2 < 2^32 for every panel size the library accepts
+    const unsigned LL = (unsigned)L * (unsigned)L;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6u * LL) return;
+    const int p = (int)(t / LL), r = (int)(t - (unsigned)p * LL);
+    const int jj = r / L, ii = r - jj * L;
     const int i = ii + 1 - hw, j = jj + 1 - hw;
     const bool in_i = i >= 1 && i <= N, in_j = j >= 1 && j <= N;
     // corner and special index of a point next to a cube corner (-1 if not special)
