@@ -200,8 +200,6 @@ struct HdTile {
     static constexpr int one(int t) { return t + R + B - 1 > NR ? t + R + B - 1 : NR; }
     static constexpr int span(int t) { return one(t) > both(t) ? one(t) : both(t); }
     static constexpr int LW = span(TX), LH = span(TY), NT = TX * TY;
-    static constexpr int KF = (LH * LW + NT - 1) / NT;              // box positions per thread
-    static constexpr int KX = (TY * LW + LH * TX + NT - 1) / NT;    // cross positions per thread
 };
 
 template <int ORD>
@@ -241,83 +239,87 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
     int lx, hx, ly, hy;
     hd_rows<ORD>(x0, xe, n, lx, hx);
     hd_rows<ORD>(y0, ye, n, ly, hy);
-    const double *f = W + k * g.fs;
     const long long pb = (long long)p * n;
-    const int wa = hx - lx, nbox = (hy - ly) * wa;
-    const int nA = (ye - y0) * wa, nc = xe - x0, nB = (hy - ly) * nc;
-    const int tid = threadIdx.x;
+    const int pitch = g.pitch;
+    // panel bases; offsets within a panel are 32-bit (the 24-bit multiply: rows, pitch < 2^24)
+    const double *fb = W + k * g.fs + pb * pitch;
+    const double *Jb = g.J + pb * pitch, *Q11b = g.Qi11 + pb * pitch, *Q12b = g.Qi12 + pb * pitch,
+                 *Q22b = g.Qi22 + pb * pitch;
+    const int wa = hx - lx, hb = hy - ly, nc = xe - x0, nr = ye - y0;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    constexpr int RB = (C::LH + TY - 1) / TY, CB = (C::LW + TX - 1) / TX;
 
-    // ---- loads: f on the box; Qi, J, Qi12 at the cross positions (tile rows x line range:
-    // kind 1; line range x tile columns: kind 2) ----
-    double fv[C::KF];
+    // ---- loads, each thread on rows ty + a*TY and columns tx + b*TX of its ranges: f on the box;
+    // Qi, J, Qi12 at the cross positions (tile rows x line range: kind 1, line range x tile
+    // columns: kind 2) ----
+    double fv[RB][CB];
 #pragma unroll
-    for (int u = 0; u < C::KF; ++u) {
-        const int q = tid + u * C::NT;
-        if (q < nbox) {
-            const int r = q / wa;
-            fv[u] = f[(pb + ly + r) * g.pitch + lx + (q - r * wa)];
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            const int r = ty + a * TY, c = tx + b * TX;
+            if (r < hb && c < wa) fv[a][b] = fb[__umul24(ly + r, pitch) + lx + c];
+        }
+    double q1v[CB], j1v[CB], q121[CB];   // kind 1: row y0 + ty, column lx + tx + b*TX
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+        const int c = tx + b * TX;
+        if (ty < nr && c < wa) {
+            const int o = __umul24(y0 + ty, pitch) + lx + c;
+            q1v[b] = Q11b[o];
+            j1v[b] = Jb[o];
+            q121[b] = Q12b[o];
         }
     }
-    double qv[C::KX], jv[C::KX], q12[C::KX];
-    int kind[C::KX], px[C::KX], py[C::KX];
+    double q2v[RB], j2v[RB], q122[RB];   // kind 2: row ly + ty + a*TY, column x0 + tx
 #pragma unroll
-    for (int u = 0; u < C::KX; ++u) {
-        const int q = tid + u * C::NT;
-        kind[u] = 0;
-        px[u] = 0;
-        py[u] = 0;
-        if (q < nA) {
-            const int r = q / wa;
-            py[u] = y0 + r;
-            px[u] = lx + (q - r * wa);
-            kind[u] = 1;
-        } else if (q < nA + nB) {
-            const int q2 = q - nA, r = q2 / nc;
-            py[u] = ly + r;
-            px[u] = x0 + (q2 - r * nc);
-            kind[u] = 2;
-        }
-        if (kind[u]) {
-            const long long o = (pb + py[u]) * g.pitch + px[u];
-            qv[u] = kind[u] == 1 ? g.Qi11[o] : g.Qi22[o];
-            jv[u] = g.J[o];
-            q12[u] = g.Qi12[o];
+    for (int a = 0; a < RB; ++a) {
+        const int r = ty + a * TY;
+        if (r < hb && tx < nc) {
+            const int o = __umul24(ly + r, pitch) + x0 + tx;
+            q2v[a] = Q22b[o];
+            j2v[a] = Jb[o];
+            q122[a] = Q12b[o];
         }
     }
 #pragma unroll
-    for (int u = 0; u < C::KF; ++u) {
-        const int q = tid + u * C::NT;
-        if (q < nbox) {
-            const int r = q / wa;
-            sf[r][q - r * wa] = fv[u];
-        }
-    }
+    for (int a = 0; a < RB; ++a)
 #pragma unroll
-    for (int u = 0; u < C::KX; ++u) {
-        if (kind[u] == 1) sbx[py[u] - y0][px[u] - lx] = qv[u] * jv[u];
-        else if (kind[u] == 2) sby[py[u] - ly][px[u] - x0] = qv[u] * jv[u];
-    }
+        for (int b = 0; b < CB; ++b) {
+            const int r = ty + a * TY, c = tx + b * TX;
+            if (r < hb && c < wa) sf[r][c] = fv[a][b];
+        }
+#pragma unroll
+    for (int b = 0; b < CB; ++b)
+        if (ty < nr && tx + b * TX < wa) sbx[ty][tx + b * TX] = q1v[b] * j1v[b];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+        if (ty + a * TY < hb && tx < nc) sby[ty + a * TY][tx] = q2v[a] * j2v[a];
     __syncthreads();
 
     // ---- first derivatives at the cross positions (calc_tile_d1x_d1y: sbp42 :60-93,
     // sbp63 :531-564) ----
 #pragma unroll
-    for (int u = 0; u < C::KX; ++u) {
-        if (kind[u] == 1) {
-            const int c = px[u] - lx;
-            const double d = sbp_row<ORD, double>(py[u], n, [&](int a) { return sf[a - ly][c]; });
-            sdy[py[u] - y0][c] = d * jv[u] * q12[u];
-        } else if (kind[u] == 2) {
-            const int r = py[u] - ly;
-            const double d = sbp_row<ORD, double>(px[u], n, [&](int a) { return sf[r][a - lx]; });
-            sdx[r][px[u] - x0] = d * jv[u] * q12[u];
+    for (int b = 0; b < CB; ++b) {
+        const int c = tx + b * TX;
+        if (ty < nr && c < wa) {
+            const double d = sbp_row<ORD, double>(y0 + ty, n, [&](int a) { return sf[a - ly][c]; });
+            sdy[ty][c] = d * j1v[b] * q121[b];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+        const int r = ty + a * TY;
+        if (r < hb && tx < nc) {
+            const double d = sbp_row<ORD, double>(x0 + tx, n, [&](int b) { return sf[r][b - lx]; });
+            sdx[r][tx] = d * j2v[a] * q122[a];
         }
     }
     __syncthreads();
 
-    const int tx = tid % TX, ty = tid / TX, i = x0 + tx, j = y0 + ty;
+    const int i = x0 + tx, j = y0 + ty;
     if (i >= n || j >= n) return;
-    const long long o = (pb + j) * g.pitch + i;
+    const int o = __umul24(j, pitch) + i;
     auto d2 = [&](int t, const auto &bget, const auto &fget) {
         double bb[L::NCOL], ff[L::NCOL];
         if (t < L::NB || t >= n - L::NB) {
@@ -350,11 +352,11 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
     double v = 1.0 * d2x + 1.0 * d2y;
     v = v + 1.0 * d2xy + 1.0 * d2yx;
     v = v / (g.hs * g.hs);
-    v = v / g.J[o];
+    v = v / Jb[o];
     if (i == 0 || j == 0 || i == n - 1 || j == n - 1)
         E[(long long)k * 6 * g.es + p * g.es + edge_slot(n, i, j)] = v;
     else
-        Wout[k * g.fs + o] = v;
+        (Wout + k * g.fs + pb * pitch)[o] = v;
 }
 
 // The sbp21 narrow Laplacian (laplace_Ah_sbp21_narrow_mod.f90:22-245) of field k = blockIdx.y,
