@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--adj-dt", type=float, default=150.0)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-fd", type=int, default=4, help="finite-difference directions checked")
+    ap.add_argument("--adj-full", type=int, default=1,
+                    help="also time the gradient with one segment (every stage input stored, no recomputation)")
     ap.add_argument("--adj-tiling", type=int, default=16, choices=(0, 16, 32),
                     help="transposed RHS: fused launch on tiles of this width x 16, or two launches (0)")
     # batched form: 16 members each at C48, C96, C144, C192, sbp42 FP64, 200 RK4 steps at
@@ -211,18 +213,22 @@ def adjoint_form(args, stream):
     bwd()
     # one gradient = checkpointed forward then reverse sweep, timed as a pair (the forward
     # records the last segment, which the reverse sweep then does not recompute)
-    t_tape_fwd, t_bwd = 1e30, 1e30
-    for _ in range(3):
-        e0, e1, e2 = ev(), ev(), ev()
-        torch.cuda.synchronize()
-        e0.record(stream)
-        fwd()
-        e1.record(stream)
-        bwd()
-        e2.record(stream)
-        torch.cuda.synchronize()
-        if e0.elapsed_time(e2) < (t_tape_fwd + t_bwd) * 1e3:
-            t_tape_fwd, t_bwd = e0.elapsed_time(e1) / 1e3, e1.elapsed_time(e2) / 1e3
+    def timed_pair(fwd, bwd):
+        t_f, t_b = 1e30, 1e30
+        for _ in range(3):
+            e0, e1, e2 = ev(), ev(), ev()
+            torch.cuda.synchronize()
+            e0.record(stream)
+            fwd()
+            e1.record(stream)
+            bwd()
+            e2.record(stream)
+            torch.cuda.synchronize()
+            if e0.elapsed_time(e2) < (t_f + t_b) * 1e3:
+                t_f, t_b = e0.elapsed_time(e1) / 1e3, e1.elapsed_time(e2) / 1e3
+        return t_f, t_b
+
+    t_tape_fwd, t_bwd = timed_pair(fwd, bwd)
     grads = g0.download()
 
     def loss_at(y):
@@ -239,6 +245,9 @@ def adjoint_form(args, stream):
                     loss_at([a - eps * dd for a, dd in zip(y0, d)])) / (2 * eps))
     ads, fds = np.array(ads), np.array(fds)
     t_grad = t_tape_fwd + t_bwd
+    full = _adjoint_single_segment(args, op, tape, y0, lam, grads, timed_pair, stream) if args.adj_full else None
+    if full:
+        full["adjoint_forward_time_ratio"] = full["gradient_ms"] / (t_fwd * 1e3)
     return {"workload": f"C{N} sbp42 FP64 RK4 dt={dt:g}s x {steps} steps, checkpoints every {every}, "
                         "loss 0.5*sum(w (h_T - h_target)^2)",
             "transposed_rhs": f"fused, {args.adj_tiling} x 16 tiles" if args.adj_tiling else "two launches",
@@ -247,7 +256,38 @@ def adjoint_form(args, stream):
             "checkpointed_forward_ms": t_tape_fwd * 1e3, "reverse_sweep_ms": t_bwd * 1e3,
             "forward_cell_steps_per_s": cells * steps / t_fwd, "loss": loss,
             "fd_check": {"directions": int(args.adj_fd), "step": eps,
-                         "max_rel_err": float(np.max(np.abs(ads - fds) / np.abs(fds))) if len(fds) else None}}
+                         "max_rel_err": float(np.max(np.abs(ads - fds) / np.abs(fds))) if len(fds) else None},
+            "single_segment": full}
+
+
+def _adjoint_single_segment(args, op, tape, y0, lam, grads, timed_pair, stream):
+    """The same gradient with one checkpoint segment: the forward records every RK4 stage input
+    of the whole trajectory, so the reverse sweep recomputes nothing. Reports its device memory
+    (free memory before and after the tape is made), its time ratio and whether its gradient is
+    bit for bit the checkpointed one."""
+    import ctypes
+    import numpy as np
+    import torch
+    import cubedsphere_sbp_swm_amd as swm
+    from cubedsphere_sbp_swm_amd import _capi
+    from cubedsphere_sbp_swm_amd.autograd import AdjointTape
+    L = _capi.lib()
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    full = AdjointTape(op, tape.nsteps, tape.dt, tape.nsteps)
+    torch.cuda.synchronize()
+    tape_bytes = free0 - torch.cuda.mem_get_info()[0]
+    xT, g0 = swm.create_stvec_swm(op), swm.create_stvec_swm(op)
+    full.x0.upload(*y0)
+    fwd = lambda: _capi.check(L.swm_adj_forward(full.tape, ctypes.c_void_p(full.x0.ptr), ctypes.c_void_p(xT.ptr)))
+    bwd = lambda: _capi.check(L.swm_adj_backward(full.tape, ctypes.c_void_p(lam.ptr), ctypes.c_void_p(g0.ptr)))
+    fwd()
+    bwd()
+    t_f, t_b = timed_pair(fwd, bwd)
+    same = all(np.array_equal(a, b) for a, b in zip(g0.download(), grads))
+    full.close()
+    return {"tape_bytes": int(tape_bytes), "forward_with_tape_ms": t_f * 1e3, "reverse_sweep_ms": t_b * 1e3,
+            "gradient_ms": (t_f + t_b) * 1e3, "gradient_bitwise_equal_to_checkpointed": bool(same)}
 
 
 # algorithmic FP64 values per point of one explicit hyperdiffusion step (4 fields: h and the
