@@ -483,12 +483,13 @@ def batched_form(args, stream):
     torch.cuda.synchronize()
     t_ens = e0.elapsed_time(e1) / 1e3
     finals = [x.download() for x in xs]
-    # dominant kernel: the ensemble stage-2/3 launch
-    ens.bench_stage(3)
+    # dominant kernel: the ensemble stage-2/3 launch, after 40 untimed launches (about 10 ms),
+    # which bring the clocks back up after the host-side downloads left the GPU idle
+    ens.bench_stage(40)
     torch.cuda.synchronize()
     k0, k1 = ev(), ev()
     k0.record(stream)
-    reps = 20
+    reps = 40
     ens.bench_stage(reps)
     k1.record(stream)
     torch.cuda.synchronize()

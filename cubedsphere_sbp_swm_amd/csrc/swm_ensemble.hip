@@ -1,10 +1,10 @@
 // Batched ensemble form (included into swm_capi.hip): cubes of different resolutions, each
 // with its own context (geometry, work buffers) and time step, advanced together.  One RK4
-// step of the whole ensemble = 4 persistent stage launches (ens_stage_kernel), each walking
-// the concatenated tile list of all members; the launches of one step and of RK4_GRAPH_STEPS
-// steps are captured once in hipGraphs and replayed.  The tile height, the on-the-fly metric and the size of the
-// persistent grid are chosen by swm_ens_autotune, which times the stage-2 launch of the
-// whole ensemble.
+// step of the whole ensemble = 4 stage launches (ens_stage_kernel), each walking the
+// concatenated tile list of all members; the launches of one step and of RK4_GRAPH_STEPS steps
+// are captured once in hipGraphs and replayed.  The tile shape, the on-the-fly metric and the
+// grid are chosen by swm_ens_autotune, which times the stage-2 launch of the whole ensemble
+// (persistent grids of 1 or 4 blocks per resident slot, or one block per task).
 
 struct swm_ens {
     std::vector<swm_ctx *> m;
@@ -12,7 +12,7 @@ struct swm_ens {
     int prec = 64, order = 42;
     hipStream_t stream = nullptr;
     int num_cu = 0;
-    int tx = 64, ty = 8, bps = 1;  // tile shape (swm::tile_shape_ok), persistent blocks per resident slot
+    int tx = 64, ty = 8, bps = 1;  // tile shape (swm::tile_shape_ok), blocks per resident slot (0: one per task)
     bool otf = false;
     void *tables[4] = {nullptr, nullptr, nullptr, nullptr};  // EnsMember<T>[m] per RK stage
     int2 *tasks = nullptr;  // per task of the persistent loop: (member, task index within the member)
@@ -208,7 +208,8 @@ extern "C" int swm_ens_set_tile_shape(swm_ens *e, int tile_cols, int tile_rows, 
     if (!swm::tile_shape_ok(tile_cols, tile_rows))
         return fail(SWM_EARG, "tile shape must be 64 x 8, 32 x 16, 16 x 32 or 32 x 8 points");
     if (metric_on_the_fly && e->prec != 64) return fail(SWM_EARG, "metric on the fly is FP64 only");
-    if (blocks_per_slot < 1 || blocks_per_slot > 64) return fail(SWM_EARG, "blocks_per_slot must be in [1, 64]");
+    if (blocks_per_slot < 0 || blocks_per_slot > 64)
+        return fail(SWM_EARG, "blocks_per_slot must be in [0, 64] (0: one block per task)");
     for (swm_ctx *c : e->m)
         if (c->n <= tile_rows) return fail(SWM_EARG, "tile_rows must be smaller than every member's N+1");
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -288,9 +289,8 @@ extern "C" int swm_ens_rk4_steps(swm_ens *e, const double *dt, int nsteps) {
 }
 
 // Autotuner: times `reps` stage-2 launches of the whole ensemble for every tile shape (64 x 8,
-// 32 x 16, 16 x 32),
-// metric source (FP64: read or on the fly) and persistent-grid size (1, 2 or 4 blocks per
-// resident slot); keeps the fastest.  The stage-2 launch reads the work buffers and writes
+// 32 x 16, 16 x 32, 32 x 8), metric source (FP64: read or on the fly) and grid (a persistent
+// grid of 1 or 4 blocks per resident slot, or one block per task); keeps the fastest.  The stage-2 launch reads the work buffers and writes
 // yB / acc of every member, so the states are unchanged.
 extern "C" int swm_ens_autotune(swm_ens *e, int reps, double *best_us) {
     if (!e || reps < 1) return fail(SWM_EARG, "bad argument");
@@ -305,7 +305,7 @@ extern "C" int swm_ens_autotune(swm_ens *e, int reps, double *best_us) {
     for (auto sh : {std::make_pair(64, 8), std::make_pair(32, 16), std::make_pair(16, 32), std::make_pair(32, 8)}) {
         if (min_n <= sh.second) continue;  // a tile may not span both y-edges of a panel
         for (int otf = 0; otf < (e->prec == 64 ? 2 : 1); ++otf)
-            for (int bps : {1, 2, 4}) cands.push_back({sh.first, sh.second, otf != 0, bps});
+            for (int bps : {1, 4, 0}) cands.push_back({sh.first, sh.second, otf != 0, bps});
     }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));

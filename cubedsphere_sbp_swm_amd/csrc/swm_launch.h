@@ -22,8 +22,8 @@ struct StageLaunch {
 };
 
 // ensemble stage launch: member table (EnsMember<T>[nmembers]) and task table (member, task
-// index within the member) on the device; the persistent grid is num_cu x resident blocks per
-// CU x blocks_per_slot
+// index within the member) on the device; the grid is num_cu x resident blocks per CU x
+// blocks_per_slot (a persistent grid), or one block per task for blocks_per_slot = 0
 struct EnsLaunch {
     int prec, mode, tx, ty;
     bool otf;

@@ -492,9 +492,9 @@ hipError_t launch_ens_mode(const EnsLaunch &r) {
         if (e != hipSuccess) return e;
         per_cu = nb > 0 ? nb : 1;
     }
-    // resident blocks x the requested waves of blocks; a multiple of 8 keeps a block's tiles
-    // on its XCD
-    long long grid = (long long)r.num_cu * per_cu * (r.blocks_per_slot > 0 ? r.blocks_per_slot : 1);
+    // resident blocks x the requested waves of blocks (0: one block per task); a multiple of 8
+    // keeps a block's tiles on its XCD
+    long long grid = r.blocks_per_slot > 0 ? (long long)r.num_cu * per_cu * r.blocks_per_slot : (long long)r.ntiles;
     grid = (grid + 7) / 8 * 8;
     if (grid > r.ntiles) grid = (r.ntiles + 7) / 8 * 8;
     if (grid < 8) grid = 8;

@@ -211,12 +211,15 @@ int swm_adj_set_tiling(swm_ctx *ctx, int tile_x);
 /* ---- Batched ensemble form (no counterpart in the reference) ----
  * Cubes of different resolutions, one context and one state vector each (same precision and
  * SBP family 42/43/63, N >= 16), advanced together: every RK4 stage of the whole ensemble is
- * one persistent kernel launch over all members' tiles, and one ensemble step (4 launches) and a
- * run of steps are captured once in hipGraphs.  dt holds one time step per member.  The ensemble
- * runs on the first member's stream; the states stay owned by the caller.  swm_ens_autotune
- * times the stage-2 launch over the tile shapes of swm_ctx_set_tile_shape, the metric source
- * (FP64: read or on the fly) and the persistent grid size (1, 2, 4 blocks per resident slot) and
- * keeps the fastest; swm_ens_bench_stage runs `reps` stage-2 launches (measurement helper). */
+ * one kernel launch over all members' tiles, and one ensemble step (4 launches) and a run of
+ * steps are captured once in hipGraphs.  dt holds one time step per member.  The ensemble runs
+ * on the first member's stream; the states stay owned by the caller.  The launch's blocks walk
+ * the task list of all members: blocks_per_slot = 1..64 sizes a persistent grid of that many
+ * blocks per resident block slot, 0 launches one block per task (the hardware dispatcher
+ * hands out the tasks).  swm_ens_autotune times the stage-2 launch over the tile shapes of
+ * swm_ctx_set_tile_shape, the metric source (FP64: read or on the fly) and the grid (1 or 4
+ * blocks per resident slot, or one block per task) and keeps the fastest; swm_ens_bench_stage
+ * runs `reps` stage-2 launches (measurement helper). */
 typedef struct swm_ens swm_ens;
 int swm_ens_create(swm_ctx *const *members, void *const *states, int nmembers, swm_ens **ens);
 int swm_ens_destroy(swm_ens *ens);

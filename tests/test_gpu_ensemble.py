@@ -1,5 +1,5 @@
 """Batched ensemble form: cubes of different resolutions and time steps advanced by one
-persistent launch per RK stage (hipGraph per step) give, member by member, the same states as
+launch per RK stage (a persistent grid, or one block per task; hipGraph per step) give, member by member, the same states as
 the single-cube path (which equals the oracle bit for bit): bitwise equality asserted."""
 import numpy as np
 import pytest
@@ -29,7 +29,7 @@ def _dt(N):
 
 
 @pytest.mark.parametrize("tiling", [(8, False, 1), (16, True, 2), (16, False, 4), (8, False, 2, 32), (8, True, 1, 32),
-                                    "auto"])
+                                    (8, True, 0, 32), (16, False, 0), "auto"])
 def test_ensemble_equals_single_cube_path(tiling):
     Ns = [16, 24, 33, 48]
     swm, ops, xs, ics = _members(Ns, 42)
