@@ -16,7 +16,7 @@
 //   - the O-point halo of the kinetic-energy potential (gradient, grad_c_sbp42_mod.f90:53-139),
 //   - the corner sync of the curl (halo_Ah_scalar_sync_mod.f90:52-93),
 // the first three read on the fly through precomputed neighbour lists by the threads on the panel
-// edges, the sync one small launch over the shared points.  One RHS is eight launches of memory-bound stencil sweeps with one thread per output point (64 x 4
+// edges, the sync one small launch over the shared points.  One RHS is eight launches of memory-bound stencil sweeps with one thread per output point (16 x 16
 // thread blocks, coalesced along i); the RK4 stage updates are applied by the kernels that produce
 // the tendencies.  Intermediate values at another point type (the J Qi12-weighted components at o
 // points, the curl at x and y points) are stored once and read by the next kernel: recomputing
@@ -29,7 +29,7 @@
 namespace swmc {
 
 constexpr int H = 4;
-constexpr int BX = 64, BY = 4;
+constexpr int BX = 16, BY = 16;
 
 // ---- staggered SBP42 operators, sbp_operators_collection_mod.f90:110-191 ----
 // W42 interpolation interfaces -> centres, optimised weights (W42_staggered_i2c_opt1)
