@@ -219,35 +219,56 @@ __device__ __forceinline__ void hd_rows(int a0, int ae, int n, int &lo, int &hi)
     }
 }
 
-template <int ORD, int TX, int TY>
-__global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__restrict__ W,
-                                                       double *__restrict__ Wout, double *__restrict__ E, int nf) {
+// The interior row of the first-derivative SBP operator (sbp_row away from the closures).
+template <int ORD, typename G>
+__device__ __forceinline__ double sbp_row_in(int t, const G &get) {
+    using S = Sbp<ORD>;
+    double s = S::Qin[0] * get(t + S::INSHIFT);
+#pragma unroll
+    for (int k = 1; k < S::WIN; ++k) s = s + S::Qin[k] * get(t + S::INSHIFT + k);
+    return s;
+}
+
+// One tile of hd_lap_tile.  INNER: the tile holds no closure row and no panel-edge point in
+// either direction (B <= x0, x0 + TX <= n - B, the same in y), so its box is the tile +-R,
+// every row of it is an interior row and its guards and branches are known at compile time.
+template <int ORD, int TX, int TY, bool INNER>
+__device__ __forceinline__ void hd_lap_run(const HdGeo &g, const double *__restrict__ W, double *__restrict__ Wout,
+                                           double *__restrict__ E, int k, int p, int x0, int y0,
+                                           double (&sf)[HdTile<ORD, TX, TY>::LH][HdTile<ORD, TX, TY>::LW],
+                                           double (&sbx)[TY][HdTile<ORD, TX, TY>::LW],
+                                           double (&sdy)[TY][HdTile<ORD, TX, TY>::LW],
+                                           double (&sby)[HdTile<ORD, TX, TY>::LH][TX],
+                                           double (&sdx)[HdTile<ORD, TX, TY>::LH][TX]) {
     using C = HdTile<ORD, TX, TY>;
     using L = Ln<ORD>;
-    __shared__ double sf[C::LH][C::LW];                  // f on the box
-    __shared__ double sbx[TY][C::LW], sdy[TY][C::LW];    // tile rows: Qi11 J, d1y
-    __shared__ double sby[C::LH][TX], sdx[C::LH][TX];    // tile columns: Qi22 J, d1x
     const int n = g.n;
-    const int ntx = (n + TX - 1) / TX, nty = (n + TY - 1) / TY;
-    // one flat block index over (tile, field), field fastest: the nf blocks of a tile sit in the
-    // same XCD's contiguous range, so the metric they all read is fetched into one L2
-    const int u = xcd_tile(blockIdx.x, 6 * ntx * nty * nf);
-    const int tile = u / nf, k = u - tile * nf;
-    const int p = tile / (ntx * nty), rem = tile - p * ntx * nty;
-    const int x0 = (rem % ntx) * TX, y0 = (rem / ntx) * TY;
     const int xe = x0 + TX < n ? x0 + TX : n, ye = y0 + TY < n ? y0 + TY : n;
     int lx, hx, ly, hy;
-    hd_rows<ORD>(x0, xe, n, lx, hx);
-    hd_rows<ORD>(y0, ye, n, ly, hy);
+    if constexpr (INNER) {
+        lx = x0 - C::R; hx = x0 + TX + C::R;
+        ly = y0 - C::R; hy = y0 + TY + C::R;
+    } else {
+        hd_rows<ORD>(x0, xe, n, lx, hx);
+        hd_rows<ORD>(y0, ye, n, ly, hy);
+    }
     const long long pb = (long long)p * n;
     const int pitch = g.pitch;
     // panel bases; offsets within a panel are 32-bit (the 24-bit multiply: rows, pitch < 2^24)
     const double *fb = W + k * g.fs + pb * pitch;
     const double *Jb = g.J + pb * pitch, *Q11b = g.Qi11 + pb * pitch, *Q12b = g.Qi12 + pb * pitch,
                  *Q22b = g.Qi22 + pb * pitch;
-    const int wa = hx - lx, hb = hy - ly, nc = xe - x0, nr = ye - y0;
+    // box extents (compile-time in an inner tile)
+    constexpr int WAI = TX + 2 * C::R, HBI = TY + 2 * C::R;
+    const int wa = INNER ? WAI : hx - lx, hb = INNER ? HBI : hy - ly;
+    const int nc = INNER ? TX : xe - x0, nr = INNER ? TY : ye - y0;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     constexpr int RB = (C::LH + TY - 1) / TY, CB = (C::LW + TX - 1) / TX;
+    // row ty + a*TY / column tx + b*TX inside the box: always so in an inner tile when the whole
+    // pass fits (after unrolling these are constants)
+    auto row_in = [&](int a) { return (INNER && (a + 1) * TY <= HBI) || ty + a * TY < hb; };
+    auto col_in = [&](int b) { return (INNER && (b + 1) * TX <= WAI) || tx + b * TX < wa; };
+    const bool own_r = INNER || ty < nr, own_c = INNER || tx < nc;
 
     // ---- loads, each thread on rows ty + a*TY and columns tx + b*TX of its ranges: f on the box;
     // Qi, J, Qi12 at the cross positions (tile rows x line range: kind 1, line range x tile
@@ -258,13 +279,13 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
 #pragma unroll
         for (int b = 0; b < CB; ++b) {
             const int r = ty + a * TY, c = tx + b * TX;
-            if (r < hb && c < wa) fv[a][b] = fb[__umul24(ly + r, pitch) + lx + c];
+            if (row_in(a) && col_in(b)) fv[a][b] = fb[__umul24(ly + r, pitch) + lx + c];
         }
     double q1v[CB], j1v[CB], q121[CB];   // kind 1: row y0 + ty, column lx + tx + b*TX
 #pragma unroll
     for (int b = 0; b < CB; ++b) {
         const int c = tx + b * TX;
-        if (ty < nr && c < wa) {
+        if (own_r && col_in(b)) {
             const int o = __umul24(y0 + ty, pitch) + lx + c;
             q1v[b] = Q11b[o];
             j1v[b] = Jb[o];
@@ -275,7 +296,7 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
 #pragma unroll
     for (int a = 0; a < RB; ++a) {
         const int r = ty + a * TY;
-        if (r < hb && tx < nc) {
+        if (row_in(a) && own_c) {
             const int o = __umul24(ly + r, pitch) + x0 + tx;
             q2v[a] = Q22b[o];
             j2v[a] = Jb[o];
@@ -287,42 +308,46 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
 #pragma unroll
         for (int b = 0; b < CB; ++b) {
             const int r = ty + a * TY, c = tx + b * TX;
-            if (r < hb && c < wa) sf[r][c] = fv[a][b];
+            if (row_in(a) && col_in(b)) sf[r][c] = fv[a][b];
         }
 #pragma unroll
     for (int b = 0; b < CB; ++b)
-        if (ty < nr && tx + b * TX < wa) sbx[ty][tx + b * TX] = q1v[b] * j1v[b];
+        if (own_r && col_in(b)) sbx[ty][tx + b * TX] = q1v[b] * j1v[b];
 #pragma unroll
     for (int a = 0; a < RB; ++a)
-        if (ty + a * TY < hb && tx < nc) sby[ty + a * TY][tx] = q2v[a] * j2v[a];
+        if (row_in(a) && own_c) sby[ty + a * TY][tx] = q2v[a] * j2v[a];
     __syncthreads();
 
     // ---- first derivatives at the cross positions (calc_tile_d1x_d1y: sbp42 :60-93,
     // sbp63 :531-564) ----
+    auto d1 = [&](int t, const auto &get) {
+        if constexpr (INNER) return sbp_row_in<ORD>(t, get);
+        else return sbp_row<ORD, double>(t, n, get);
+    };
 #pragma unroll
     for (int b = 0; b < CB; ++b) {
         const int c = tx + b * TX;
-        if (ty < nr && c < wa) {
-            const double d = sbp_row<ORD, double>(y0 + ty, n, [&](int a) { return sf[a - ly][c]; });
+        if (own_r && col_in(b)) {
+            const double d = d1(y0 + ty, [&](int a) { return sf[a - ly][c]; });
             sdy[ty][c] = d * j1v[b] * q121[b];
         }
     }
 #pragma unroll
     for (int a = 0; a < RB; ++a) {
         const int r = ty + a * TY;
-        if (r < hb && tx < nc) {
-            const double d = sbp_row<ORD, double>(x0 + tx, n, [&](int b) { return sf[r][b - lx]; });
+        if (row_in(a) && own_c) {
+            const double d = d1(x0 + tx, [&](int b) { return sf[r][b - lx]; });
             sdx[r][tx] = d * j2v[a] * q122[a];
         }
     }
     __syncthreads();
 
     const int i = x0 + tx, j = y0 + ty;
-    if (i >= n || j >= n) return;
+    if (!INNER && (i >= n || j >= n)) return;
     const int o = __umul24(j, pitch) + i;
     auto d2 = [&](int t, const auto &bget, const auto &fget) {
         double bb[L::NCOL], ff[L::NCOL];
-        if (t < L::NB || t >= n - L::NB) {
+        if (!INNER && (t < L::NB || t >= n - L::NB)) {
             const bool left = t < L::NB;
 #pragma unroll
             for (int q = 0; q < L::NCOL; ++q) {
@@ -341,22 +366,46 @@ __global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__
     };
     double d2x = d2(i, [&](int a) { return sbx[ty][a - lx]; }, [&](int a) { return sf[j - ly][a - lx]; });
     const double d2y = d2(j, [&](int a) { return sby[a - ly][tx]; }, [&](int a) { return sf[a - ly][i - lx]; });
-    // penalties of the cross terms (sbp42 :716-742, sbp63 :1671-1697), x edges first
-    constexpr double pc = L::PC;
-    if (i == 0) d2x = d2x - 0.5 * (pc * (-2.0 * sdy[ty][i - lx]));
-    if (i == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * sdy[ty][i - lx]));
-    if (j == 0) d2x = d2x - 0.5 * (pc * (-2.0 * sdx[j - ly][tx]));
-    if (j == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * sdx[j - ly][tx]));
-    const double d2yx = sbp_row<ORD, double>(j, n, [&](int a) { return sdx[a - ly][tx]; });
-    const double d2xy = sbp_row<ORD, double>(i, n, [&](int a) { return sdy[ty][a - lx]; });
+    if constexpr (!INNER) {
+        // penalties of the cross terms (sbp42 :716-742, sbp63 :1671-1697), x edges first
+        constexpr double pc = L::PC;
+        if (i == 0) d2x = d2x - 0.5 * (pc * (-2.0 * sdy[ty][i - lx]));
+        if (i == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * sdy[ty][i - lx]));
+        if (j == 0) d2x = d2x - 0.5 * (pc * (-2.0 * sdx[j - ly][tx]));
+        if (j == n - 1) d2x = d2x - 0.5 * (pc * (2.0 * sdx[j - ly][tx]));
+    }
+    const double d2yx = d1(j, [&](int a) { return sdx[a - ly][tx]; });
+    const double d2xy = d1(i, [&](int a) { return sdy[ty][a - lx]; });
     double v = 1.0 * d2x + 1.0 * d2y;
     v = v + 1.0 * d2xy + 1.0 * d2yx;
     v = v / (g.hs * g.hs);
     v = v / Jb[o];
-    if (i == 0 || j == 0 || i == n - 1 || j == n - 1)
+    if (!INNER && (i == 0 || j == 0 || i == n - 1 || j == n - 1))
         E[(long long)k * 6 * g.es + p * g.es + edge_slot(n, i, j)] = v;
     else
         (Wout + k * g.fs + pb * pitch)[o] = v;
+}
+
+template <int ORD, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) hd_lap_tile(HdGeo g, const double *__restrict__ W,
+                                                       double *__restrict__ Wout, double *__restrict__ E, int nf) {
+    using C = HdTile<ORD, TX, TY>;
+    __shared__ double sf[C::LH][C::LW];                  // f on the box
+    __shared__ double sbx[TY][C::LW], sdy[TY][C::LW];    // tile rows: Qi11 J, d1y
+    __shared__ double sby[C::LH][TX], sdx[C::LH][TX];    // tile columns: Qi22 J, d1x
+    const int n = g.n;
+    const int ntx = (n + TX - 1) / TX, nty = (n + TY - 1) / TY;
+    // one flat block index over (tile, field), field fastest: the nf blocks of a tile sit in the
+    // same XCD's contiguous range, so the metric they all read is fetched into one L2
+    const int u = xcd_tile(blockIdx.x, 6 * ntx * nty * nf);
+    const int tile = u / nf, k = u - tile * nf;
+    const int p = tile / (ntx * nty), rem = tile - p * ntx * nty;
+    const int x0 = (rem % ntx) * TX, y0 = (rem / ntx) * TY;
+    const bool inner = x0 >= C::B && x0 + TX <= n - C::B && y0 >= C::B && y0 + TY <= n - C::B;
+    if (inner)
+        hd_lap_run<ORD, TX, TY, true>(g, W, Wout, E, k, p, x0, y0, sf, sbx, sdy, sby, sdx);
+    else
+        hd_lap_run<ORD, TX, TY, false>(g, W, Wout, E, k, p, x0, y0, sf, sbx, sdy, sby, sdx);
 }
 
 // The sbp21 narrow Laplacian (laplace_Ah_sbp21_narrow_mod.f90:22-245) of field k = blockIdx.y,
