@@ -219,16 +219,6 @@ __device__ __forceinline__ void hd_rows(int a0, int ae, int n, int &lo, int &hi)
     }
 }
 
-// The interior row of the first-derivative SBP operator (sbp_row away from the closures).
-template <int ORD, typename G>
-__device__ __forceinline__ double sbp_row_in(int t, const G &get) {
-    using S = Sbp<ORD>;
-    double s = S::Qin[0] * get(t + S::INSHIFT);
-#pragma unroll
-    for (int k = 1; k < S::WIN; ++k) s = s + S::Qin[k] * get(t + S::INSHIFT + k);
-    return s;
-}
-
 // One tile of hd_lap_tile.  INNER: the tile holds no closure row and no panel-edge point in
 // either direction (B <= x0, x0 + TX <= n - B, the same in y), so its box is the tile +-R,
 // every row of it is an interior row and its guards and branches are known at compile time.
@@ -321,7 +311,7 @@ __device__ __forceinline__ void hd_lap_run(const HdGeo &g, const double *__restr
     // ---- first derivatives at the cross positions (calc_tile_d1x_d1y: sbp42 :60-93,
     // sbp63 :531-564) ----
     auto d1 = [&](int t, const auto &get) {
-        if constexpr (INNER) return sbp_row_in<ORD>(t, get);
+        if constexpr (INNER) return sbp_row_in<ORD, double>(t, get);
         else return sbp_row<ORD, double>(t, n, get);
     };
 #pragma unroll

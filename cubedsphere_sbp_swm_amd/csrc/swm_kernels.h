@@ -239,15 +239,20 @@ __device__ __forceinline__ T sbp_right_dispatch(int ir, int n, const G &get, std
     return r;
 }
 
+// interior row of the SBP first derivative at t (NE <= t < n - NE)
+template <int ORD, typename T, typename G>
+__device__ __forceinline__ T sbp_row_in(int t, const G &get) {
+    using S = Sbp<ORD>;
+    T s = T(S::Qin[0]) * get(t + S::INSHIFT);
+#pragma unroll
+    for (int k = 1; k < S::WIN; ++k) s = s + T(S::Qin[k]) * get(t + S::INSHIFT + k);
+    return s;
+}
+
 template <int ORD, typename T, typename G>
 __device__ __forceinline__ T sbp_row(int t, int n, const G &get) {
     using S = Sbp<ORD>;
-    if (t >= S::NE && t < n - S::NE) {
-        T s = T(S::Qin[0]) * get(t + S::INSHIFT);
-#pragma unroll
-        for (int k = 1; k < S::WIN; ++k) s = s + T(S::Qin[k]) * get(t + S::INSHIFT + k);
-        return s;
-    }
+    if (t >= S::NE && t < n - S::NE) return sbp_row_in<ORD, T>(t, get);
     if (t < S::NE) return sbp_left_dispatch<ORD, T>(t, get, std::make_integer_sequence<int, S::NE>{});
     return sbp_right_dispatch<ORD, T>(t - (n - S::NE), n, get, std::make_integer_sequence<int, S::NE>{});
 }

@@ -162,7 +162,10 @@ __device__ __forceinline__ void finish_point(const DevGeo<T> &g, const StageArgs
 
 // One TX x TY tile of panel p starting at (tx0, ty0) >= (1, 1): the whole RHS + RK stage update
 // of its points off the panel edges (the panel-edge points belong to the edge segments).
-template <int ORD, typename T, int MODE, int TX, int TY, bool OTF, bool PERSIST = false>
+// INNER: the tile lies inside NE <= i, j < n - NE (inner_tile), so every point of it and of its
+// halo exists, none is on a panel edge and every derivative row is an interior one; its guards
+// and closure branches are then known at compile time.
+template <int ORD, typename T, int MODE, int TX, int TY, bool OTF, bool PERSIST = false, bool INNER = false>
 __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T> &a, int p, int tx0, int ty0,
                                            TileSmem<ORD, T, TX, TY> &sm) {
     using C = Tile<ORD, T, TX, TY>;
@@ -183,7 +186,7 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     constexpr int NXH = (2 * R * TY + NT - 1) / NT, NYH = (2 * R * TX + NT - 1) / NT;
     const int tx = tid % TX, ty = tid / TX;
     const int i = tx0 + tx, j = ty0 + ty;
-    const bool own_ok = i < n && j < n;
+    const bool own_ok = INNER || (i < n && j < n);
     RawPoint<T> own{};
     if (own_ok) own = load_point<OTF>(g, y, a.hs, p, i, j);
     RawPoint<T> rx[NXH], ry[NYH];
@@ -194,7 +197,7 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         const int r = c / (2 * R), q = c - r * (2 * R);
         const int lx = q < R ? q : q + TX;
         const int xi = tx0 - R + lx, xj = ty0 + r;
-        okx[k] = c < 2 * R * TY && xi >= 0 && xi < n && xj < n;
+        okx[k] = c < 2 * R * TY && (INNER || (xi >= 0 && xi < n && xj < n));
         if (okx[k]) rx[k] = load_point<OTF>(g, y, a.hs, p, xi, xj);
     }
 #pragma unroll
@@ -203,7 +206,7 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
         const int r = c / TX, cx = c - r * TX;
         const int ly = r < R ? r : r + TY;
         const int yi = tx0 + cx, yj = ty0 - R + ly;
-        oky[k] = c < 2 * R * TX && yi < n && yj >= 0 && yj < n;
+        oky[k] = c < 2 * R * TX && (INNER || (yi < n && yj >= 0 && yj < n));
         if (oky[k]) ry[k] = load_point<OTF>(g, y, a.hs, p, yi, yj);
     }
     // of the own point only h, u, v, J and the mass fluxes h*ut, h*vt stay live for the output
@@ -244,18 +247,22 @@ __device__ __forceinline__ void stage_tile(const DevGeo<T> &g, const StageArgs<T
     // the output point's Coriolis parameter and RK vectors: issued after the load phase (issued
     // with it, they raised its register peak from 74 to 88 VGPRs at FP64, one wave per SIMD
     // less), in flight across the barrier and the stencil phase
-    if (i > 0 && j > 0 && i < n - 1 && j < n - 1) pre = prefetch_point<T, MODE>(g, a, pidx(g, p, i, j));
+    if (INNER || (i > 0 && j > 0 && i < n - 1 && j < n - 1)) pre = prefetch_point<T, MODE>(g, a, pidx(g, p, i, j));
     __syncthreads();
 
     // ---- output phase: the thread's own point ----
-    if (i <= 0 || j <= 0 || i >= n - 1 || j >= n - 1) return;
+    if (!INNER && (i <= 0 || j <= 0 || i >= n - 1 || j >= n - 1)) return;
     const int lx = tx + R, ly = ty + R;
-    const T dxK = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.K[ly][ai - tx0 + R]; });
-    const T dyK = sbp_row<ORD, T>(j, n, [&](int aj) { return sm.K[aj - ty0 + R][lx]; });
-    const T dxD = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.Gxd[ty][ai - tx0 + R]; });
-    const T dyD = sbp_row<ORD, T>(j, n, [&](int aj) { return sm.Gyd[aj - ty0 + R][tx]; });
-    const T dxC = sbp_row<ORD, T>(i, n, [&](int ai) { return sm.Gxc[ty][ai - tx0 + R]; });
-    const T dyC = sbp_row<ORD, T>(j, n, [&](int aj) { return sm.Gyc[aj - ty0 + R][tx]; });
+    auto d1 = [&](int t, const auto &get) {
+        if constexpr (INNER) return sbp_row_in<ORD, T>(t, get);
+        else return sbp_row<ORD, T>(t, n, get);
+    };
+    const T dxK = d1(i, [&](int ai) { return sm.K[ly][ai - tx0 + R]; });
+    const T dyK = d1(j, [&](int aj) { return sm.K[aj - ty0 + R][lx]; });
+    const T dxD = d1(i, [&](int ai) { return sm.Gxd[ty][ai - tx0 + R]; });
+    const T dyD = d1(j, [&](int aj) { return sm.Gyd[aj - ty0 + R][tx]; });
+    const T dxC = d1(i, [&](int ai) { return sm.Gxc[ty][ai - tx0 + R]; });
+    const T dyC = d1(j, [&](int aj) { return sm.Gyc[aj - ty0 + R][tx]; });
     const long long id = pidx(g, p, i, j);
     const T Jd = own.J * g.hx * g.scale;
     const T gx = dxK / g.hxs;
@@ -403,6 +410,14 @@ __device__ __forceinline__ void stage_edge(const DevGeo<T> &g, const StageArgs<T
     finish_point<T, MODE>(g, a, id, y[id], y[id + g.fs], y[id + 2 * g.fs], Jv, q11, q12, q22, gx, gy, dv, cu, pre);
 }
 
+// a tile starting at (tx0, ty0) whose derivative rows are all interior ones (NE <= i, j < n - NE)
+// and whose halo of REACH points lies inside the panel: M <= i, j < n - M with M the larger
+template <int ORD, int TX, int TY>
+__device__ __forceinline__ bool inner_tile(int tx0, int ty0, int n) {
+    constexpr int NE = Sbp<ORD>::NE, RE = Sbp<ORD>::REACH, M = NE > RE ? NE : RE;
+    return tx0 >= M && tx0 + TX <= n - M && ty0 >= M && ty0 + TY <= n - M;
+}
+
 // edge blocks first (a multiple of 8, so that the tiles keep their XCD-contiguous mapping)
 SWM_HD constexpr int edge_blocks(int n) { return (edge_segs(n) + 7) / 8 * 8; }
 
@@ -418,7 +433,11 @@ __global__ void __launch_bounds__(TX * TY) stage_kernel(DevGeo<T> g, StageArgs<T
     const int tile = xcd_tile(blockIdx.x - ne, 6 * ntx * nty);
     const int p = tile / (ntx * nty);
     const int rem = tile - p * ntx * nty;
-    stage_tile<ORD, T, MODE, TX, TY, OTF>(g, a, p, 1 + (rem % ntx) * TX, 1 + (rem / ntx) * TY, sm.t);
+    const int tx0 = 1 + (rem % ntx) * TX, ty0 = 1 + (rem / ntx) * TY;
+    if (inner_tile<ORD, TX, TY>(tx0, ty0, g.n))
+        stage_tile<ORD, T, MODE, TX, TY, OTF, false, true>(g, a, p, tx0, ty0, sm.t);
+    else
+        stage_tile<ORD, T, MODE, TX, TY, OTF>(g, a, p, tx0, ty0, sm.t);
 }
 
 // Copy of a block-uniform struct into scalar registers (word by word through readfirstlane):
@@ -459,7 +478,11 @@ __global__ void __launch_bounds__(TX * TY) ens_stage_kernel(const EnsMember<T> *
         }
         const int p = lt / (ntx * nty);
         const int rem = lt - p * ntx * nty;
-        stage_tile<ORD, T, MODE, TX, TY, OTF, true>(g, a, p, 1 + (rem % ntx) * TX, 1 + (rem / ntx) * TY, sm.t);
+        const int tx0 = 1 + (rem % ntx) * TX, ty0 = 1 + (rem / ntx) * TY;
+        if (inner_tile<ORD, TX, TY>(tx0, ty0, g.n))
+            stage_tile<ORD, T, MODE, TX, TY, OTF, true, true>(g, a, p, tx0, ty0, sm.t);
+        else
+            stage_tile<ORD, T, MODE, TX, TY, OTF, true>(g, a, p, tx0, ty0, sm.t);
     }
 }
 
