@@ -79,14 +79,30 @@ int ens_build_tables(swm_ens *e) {
             HIP_TRY(hipMemcpy(e->tables[s], tab.data(), M * sizeof(tab[0]), hipMemcpyHostToDevice));
         }
     }
-    // task table for the current tile height: each member's tiles, then its edge segments
-    std::vector<int2> tk;
+    // task table for the current tile shape: each member's tiles, then its edge segments.  The
+    // launch gives XCD x the x-th eighth of the table (xcd_tile), so the members are dealt to 8
+    // bins, largest task count first into the bin with the fewest tasks, and the bins are
+    // concatenated: every XCD then gets about the same mix of member sizes (the same for an
+    // ensemble of 8 k members of each size) instead of the smallest members on the first XCDs
+    // and the largest on the last.
+    std::vector<int> cnt(M), order(M);
     for (size_t k = 0; k < M; ++k) {
         const int n = e->m[k]->n;
-        const int tx = e->tx;
-        const int cnt = 6 * swm::tiles_along(n, tx) * swm::tiles_along(n, e->ty) + swm::edge_segs(n, true);
-        for (int t = 0; t < cnt; ++t) tk.push_back(make_int2((int)k, t));
+        cnt[k] = 6 * swm::tiles_along(n, e->tx) * swm::tiles_along(n, e->ty) + swm::edge_segs(n, true);
+        order[k] = (int)k;
     }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+    std::vector<std::vector<int>> bins(8);
+    std::vector<long long> load(8, 0);
+    for (int k : order) {
+        const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        bins[b].push_back(k);
+        load[b] += cnt[k];
+    }
+    std::vector<int2> tk;
+    for (const auto &bin : bins)
+        for (int k : bin)
+            for (int t = 0; t < cnt[k]; ++t) tk.push_back(make_int2(k, t));
     if (e->tasks && (int)tk.size() > e->ntiles_cap) { HIP_TRY(hipFree(e->tasks)); e->tasks = nullptr; }
     if (!e->tasks) {
         HIP_TRY(hipMalloc((void **)&e->tasks, tk.size() * sizeof(int2)));
