@@ -444,7 +444,7 @@ def cgrid_form(args, stream):
 
 
 def batched_form(args, stream):
-    """One persistent launch per RK stage over all members (hipGraph per step) vs the plain
+    """One launch per RK stage over all members (hipGraph per step) vs the plain
     single-cube path run member after member; same states bit for bit."""
     import numpy as np
     import torch

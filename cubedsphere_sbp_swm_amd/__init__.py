@@ -376,7 +376,8 @@ class RK4:
 
 class Ensemble:
     """Cubes of different resolutions (one operator and one state each) advanced together:
-    one persistent launch per RK stage over all members, one captured hipGraph per step."""
+    one launch per RK stage over all members (a persistent grid or one block per task, as the
+    autotuner picks), one captured hipGraph per step."""
 
     def __init__(self, ops, states):
         if len(ops) != len(states) or not ops:

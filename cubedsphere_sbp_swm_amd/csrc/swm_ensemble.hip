@@ -15,7 +15,7 @@ struct swm_ens {
     int tx = 64, ty = 8, bps = 1;  // tile shape (swm::tile_shape_ok), blocks per resident slot (0: one per task)
     bool otf = false;
     void *tables[4] = {nullptr, nullptr, nullptr, nullptr};  // EnsMember<T>[m] per RK stage
-    int2 *tasks = nullptr;  // per task of the persistent loop: (member, task index within the member)
+    int2 *tasks = nullptr;  // per task of the launch: (member, task index within the member)
     int ntiles = 0, ntiles_cap = 0;
     std::vector<double> dt;
     std::vector<unsigned> hs_gen;  // the members' orography generations the tables were built with

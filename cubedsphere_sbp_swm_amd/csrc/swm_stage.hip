@@ -454,9 +454,10 @@ __device__ __forceinline__ S uniform_copy(const S *src) {
     return out;
 }
 
-// Ensemble of cubes of different sizes in one persistent launch: the blocks walk the
-// concatenated task list of all members (per member its 6*ntx*nty tiles (tiles_along), then its
-// edge_segs(n, true) edge segments), XCD-contiguous.  tasks[t] = (member, task index within the
+// Ensemble of cubes of different sizes in one launch: the blocks walk the concatenated task list
+// of all members (per member its 6*ntx*nty tiles (tiles_along), then its edge_segs(n, true) edge
+// segments), XCD-contiguous; a persistent grid loops over it, a grid of one block per task runs
+// each block once.  tasks[t] = (member, task index within the
 // member): one load gives a block its member, whose geometry and stage arguments it then takes
 // from the member table.
 template <int ORD, typename T, int MODE, int TX, int TY, bool OTF>
